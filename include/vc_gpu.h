@@ -1,0 +1,86 @@
+/* vc_gpu.h — MI355X extensions to the vc.h API: device-resident decode,
+ * batched atom decode, explicit-q encode, stage timing hooks and the seeded
+ * synthetic-volume generator used by the benchmarks.
+ *
+ * None of these has a counterpart in the reference (SuperOptimizer/
+ * volume-compressor ships a CPU-only vc.h); they are the batching/serving
+ * surface layered beside the drop-in API.  All device pointers are plain
+ * `void *` HBM addresses; all work is enqueued on the calling host thread's
+ * own HIP stream (vcx_stream), so calls from different host threads overlap.
+ */
+#ifndef VC_GPU_H
+#define VC_GPU_H
+
+#include "vc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Select/initialise the HIP device (idempotent).  VC_ERR_IO if no usable GPU. */
+vc_status vcx_device_init(void);
+
+/* The calling thread's HIP stream (a hipStream_t), created on first use. */
+void *vcx_stream(void);
+
+/* Device memory helpers (hipMalloc / hipFree / synchronous copies). */
+void     *vcx_dev_alloc(size_t bytes);
+void      vcx_dev_free(void *p);
+vc_status vcx_h2d(void *dst_dev, const void *src_host, size_t bytes);
+vc_status vcx_d2h(void *dst_host, const void *src_dev, size_t bytes);
+
+/* Decode a whole LOD / a voxel box into device memory (raster of the LOD /
+ * of the box).  sync=1: wait and return the status like vc_decode_lod.
+ * sync=0: enqueue only and return VC_OK; vcx_sync() later returns the status
+ * accumulated since the previous vcx_sync(). */
+vc_status vcx_decode_lod_device(vc_archive *a, int lod, void *dev_out, int sync);
+vc_status vcx_decode_region_device(vc_archive *a, int lod, vc_box box, void *dev_out, int sync);
+vc_status vcx_sync(void);
+
+/* Batched random-access decode: n atoms given as (lod, ax, ay, az) quads;
+ * out receives n full 32^3 atoms (host memory, atom i at i*VC_ATOM3), the
+ * same bytes vc_decode_atom returns.  per_atom (optional) gets each status.
+ * Returns VC_ERR_RANGE if any atom is out of range (none decoded), else the
+ * first non-OK per-atom status. */
+vc_status vcx_decode_atoms(vc_archive *a, size_t n, const int32_t *lod_ax_ay_az,
+                           uint8_t *out, vc_status *per_atom);
+
+/* Stage events: while set, each decode call records events[0] before its
+ * entropy kernel, events[1] between the entropy and transform kernels and
+ * events[2] after the transform kernel, on the calling thread's stream.
+ * events are hipEvent_t handles owned by the caller; NULL disables. */
+void vcx_set_stage_events(void *events[3]);
+
+/* Whole-volume encode with explicit per-LOD base_q (clamped to [0.05,4096]
+ * as vc_set_base_q does) instead of rate control.  Same archive layout as
+ * vc_encode. */
+vc_status vcx_encode_q(const uint8_t *vol, vc_dims dims, float target_ratio,
+                       const float base_q[VC_NLOD], uint8_t **out_archive, size_t *out_len);
+/* vc_encode from a volume already resident in device memory. */
+vc_status vcx_encode_device(const void *dev_vol, vc_dims dims, float target_ratio,
+                            uint8_t **out_archive, size_t *out_len);
+/* The LOD0 base_q that vc_encode's rate control picks for this volume. */
+float     vcx_rate_control(const uint8_t *vol, vc_dims dims, float target_ratio);
+
+/* Batched single-atom codec (no container): encode n 32^3 atoms at base_q
+ * into payloads ([dc][range-coder bytes], the bytes a v1/v2 slot points at).
+ * payload i is written at out + i*cap_per_atom, its length to lens[i]
+ * (0 = all-zero atom, stored as KNOWN_ZERO).  VC_ERR_RANGE if a payload
+ * would exceed cap_per_atom or the reference coder's 131136-byte bound. */
+vc_status vcx_encode_atoms(size_t n, const uint8_t *vox, float base_q,
+                           uint8_t *out, size_t cap_per_atom, uint32_t *lens);
+/* Decode n payloads (lens[i] bytes at pay + offs[i]) at base_q into n atoms. */
+vc_status vcx_decode_payloads(size_t n, const uint8_t *pay, const uint64_t *offs,
+                              const uint32_t *lens, float base_q, uint8_t *out,
+                              vc_status *per_atom);
+
+/* Seeded synthetic scroll-like volume (background ~N(12,6), nsheets warped
+ * sheets of intensity 110..190 and thickness 3..6, 7-tap sigma=1 blur,
+ * additive ~N(0,8) noise), integer-exact.  dev_out: nx*ny*nz bytes of device
+ * memory. */
+vc_status vcx_synth_device(uint64_t seed, vc_dims dims, int nsheets, void *dev_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VC_GPU_H */

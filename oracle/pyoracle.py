@@ -1,0 +1,407 @@
+"""ctypes bindings for the CPU oracle (oracle/liboracle.so) and for the
+reference builds under oracle/_ref/ (made by oracle/ref.mk).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, tests/golden/, __graft_entry__
+.smoke() and bench.py's cpu_baseline leg, as the checker.  Never imported by
+the product library.
+"""
+import ctypes as C
+import os
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(HERE, "_ref")
+A3 = 32768
+
+_u8p = C.POINTER(C.c_uint8)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_u8p)
+
+
+class Oracle:
+    def __init__(self, path=None):
+        path = path or os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
+        L = self.L = C.CDLL(path)
+        vp, sz = C.c_void_p, C.c_size_t
+        L.vco_tables.argtypes = [vp, vp, vp, vp]
+        L.vco_dq_tables.argtypes = [C.c_float, vp, vp]
+        L.vco_fdct.argtypes = [vp, C.c_int32, vp]
+        L.vco_quantize.argtypes = [vp, C.c_float, vp]
+        L.vco_dequantize.argtypes = [vp, C.c_float, vp]
+        L.vco_quant_levels.argtypes = [C.c_float, C.c_int, vp, sz, vp]
+        L.vco_idct.argtypes = [vp, C.c_int32, vp]
+        L.vco_entropy_encode.argtypes = [vp, vp, sz]
+        L.vco_entropy_encode.restype = C.c_long
+        L.vco_entropy_decode.argtypes = [vp, sz, vp, C.POINTER(C.c_uint32)]
+        L.vco_encode_atom.argtypes = [vp, C.c_float, vp, sz, C.POINTER(C.c_int)]
+        L.vco_encode_atom.restype = C.c_long
+        L.vco_decode_payload.argtypes = [vp, C.c_uint32, C.c_float, vp]
+        L.vco_downsample2x.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, vp]
+        L.vco_nlod.argtypes = [C.c_uint32] * 3
+        L.vco_gather_atom.argtypes = [vp] + [C.c_uint32] * 6 + [vp]
+        L.vco_encode_q.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, vp,
+                                   C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]
+        L.vco_encode.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
+                                 C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]
+        L.vco_rate_control.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float]
+        L.vco_rate_control.restype = C.c_float
+        L.vco_lod_schedule.argtypes = [C.c_float, C.c_int, vp]
+        L.vco_open.argtypes = [vp, sz]
+        L.vco_open.restype = vp
+        L.vco_close.argtypes = [vp]
+        L.vco_decode_atom.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]
+        L.vco_coverage.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
+                                   C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]
+        L.vco_decode_region.argtypes = [vp, C.c_int, vp, vp]
+        L.vco_decode_lod.argtypes = [vp, C.c_int, vp, vp]
+        L.vco_decode_atoms_omp.argtypes = [vp, C.c_int, vp, sz, C.c_int, vp]
+        L.vco_decode_atoms_omp.restype = C.c_double
+        L.vco_synth.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, vp]
+        self.libc = C.CDLL("libc.so.6")
+        self.libc.free.argtypes = [C.c_void_p]
+
+    # ---- tables ----
+    def tables(self):
+        cm = np.zeros((32, 32), np.int32)
+        gs = np.zeros(94, np.float32)
+        scan = np.zeros(A3, np.uint16)
+        band = np.zeros(A3, np.uint8)
+        self.L.vco_tables(cm.ctypes.data, gs.ctypes.data, scan.ctypes.data, band.ctypes.data)
+        return cm, gs, scan, band
+
+    def dq_tables(self, q):
+        st = np.zeros(94, np.int32)
+        bi = np.zeros(94, np.int32)
+        self.L.vco_dq_tables(C.c_float(q), st.ctypes.data, bi.ctypes.data)
+        return st, bi
+
+    # ---- stages ----
+    def fdct(self, vox, dc):
+        vox = np.ascontiguousarray(vox, np.uint8).ravel()
+        out = np.zeros(A3, np.int32)
+        self.L.vco_fdct(vox.ctypes.data, int(dc), out.ctypes.data)
+        return out
+
+    def quantize(self, coef, q):
+        coef = np.ascontiguousarray(coef, np.int32).ravel()
+        out = np.zeros(A3, np.int16)
+        self.L.vco_quantize(coef.ctypes.data, C.c_float(q), out.ctypes.data)
+        return out
+
+    def quant_levels(self, q, freq, coef):
+        coef = np.ascontiguousarray(coef, np.int32)
+        out = np.zeros(coef.size, np.int32)
+        self.L.vco_quant_levels(C.c_float(q), freq, coef.ctypes.data, coef.size, out.ctypes.data)
+        return out
+
+    def dequantize(self, lv, q):
+        lv = np.ascontiguousarray(lv, np.int16).ravel()
+        out = np.zeros(A3, np.int32)
+        self.L.vco_dequantize(lv.ctypes.data, C.c_float(q), out.ctypes.data)
+        return out
+
+    def idct(self, coef, dc):
+        coef = np.ascontiguousarray(coef, np.int32).ravel()
+        out = np.zeros(A3, np.uint8)
+        self.L.vco_idct(coef.ctypes.data, int(dc), out.ctypes.data)
+        return out
+
+    def entropy_encode(self, lv):
+        lv = np.ascontiguousarray(lv, np.int16).ravel()
+        buf = np.zeros(A3 * 4 + 64, np.uint8)
+        n = self.L.vco_entropy_encode(lv.ctypes.data, buf.ctypes.data, buf.size)
+        return None if n < 0 else buf[:n].tobytes()
+
+    def entropy_decode(self, rc):
+        b = np.frombuffer(rc, np.uint8) if len(rc) else np.zeros(1, np.uint8)
+        out = np.zeros(A3, np.int16)
+        eob = C.c_uint32(0)
+        st = self.L.vco_entropy_decode(b.ctypes.data, len(rc), out.ctypes.data, C.byref(eob))
+        return st, out, eob.value
+
+    def encode_atom(self, vox, q):
+        vox = np.ascontiguousarray(vox, np.uint8).ravel()
+        buf = np.zeros(A3 * 4 + 65, np.uint8)
+        z = C.c_int(0)
+        n = self.L.vco_encode_atom(vox.ctypes.data, C.c_float(q), buf.ctypes.data, buf.size, C.byref(z))
+        if z.value:
+            return b""
+        if n < 0:
+            raise OverflowError("range coder overflow")
+        return buf[:n].tobytes()
+
+    def decode_payload(self, pay, q):
+        b = np.frombuffer(pay, np.uint8)
+        out = np.zeros(A3, np.uint8)
+        st = self.L.vco_decode_payload(b.ctypes.data, len(pay), C.c_float(q), out.ctypes.data)
+        return st, out
+
+    # ---- volumes ----
+    def downsample2x(self, vol):
+        nz, ny, nx = vol.shape
+        vol = np.ascontiguousarray(vol, np.uint8)
+        out = np.zeros(((nz + 1) // 2, (ny + 1) // 2, (nx + 1) // 2), np.uint8)
+        self.L.vco_downsample2x(vol.ctypes.data, nx, ny, nz, out.ctypes.data)
+        return out
+
+    def nlod(self, nx, ny, nz):
+        return self.L.vco_nlod(nx, ny, nz)
+
+    def synth(self, seed, nx, ny, nz, nsheets):
+        out = np.zeros((nz, ny, nx), np.uint8)
+        self.L.vco_synth(C.c_uint64(seed), nx, ny, nz, nsheets, out.ctypes.data)
+        return out
+
+    def _take(self, p, n):
+        b = C.string_at(p, n)
+        self.libc.free(p)
+        return b
+
+    def encode_q(self, vol, target_ratio, qs):
+        nz, ny, nx = vol.shape
+        vol = np.ascontiguousarray(vol, np.uint8)
+        q8 = np.zeros(8, np.float32)
+        q8[:len(qs)] = qs
+        p, n = C.c_void_p(), C.c_size_t()
+        st = self.L.vco_encode_q(vol.ctypes.data, nx, ny, nz, C.c_float(target_ratio),
+                                 q8.ctypes.data, C.byref(p), C.byref(n))
+        return st, (self._take(p, n.value) if st == 0 else None)
+
+    def encode(self, vol, target_ratio):
+        nz, ny, nx = vol.shape
+        vol = np.ascontiguousarray(vol, np.uint8)
+        p, n = C.c_void_p(), C.c_size_t()
+        st = self.L.vco_encode(vol.ctypes.data, nx, ny, nz, C.c_float(target_ratio),
+                               C.byref(p), C.byref(n))
+        return st, (self._take(p, n.value) if st == 0 else None)
+
+    def rate_control(self, vol, target_ratio):
+        nz, ny, nx = vol.shape
+        vol = np.ascontiguousarray(vol, np.uint8)
+        return self.L.vco_rate_control(vol.ctypes.data, nx, ny, nz, C.c_float(target_ratio))
+
+    def lod_schedule(self, q0, nlod):
+        qs = np.zeros(8, np.float32)
+        self.L.vco_lod_schedule(C.c_float(q0), nlod, qs.ctypes.data)
+        return qs
+
+    def open(self, buf):
+        return OracleArchive(self, buf)
+
+
+class OracleArchive:
+    def __init__(self, o, buf):
+        self.o = o
+        self.arr = np.frombuffer(bytes(buf), np.uint8).copy()
+        self.h = o.L.vco_open(self.arr.ctypes.data, self.arr.size)
+        if not self.h:
+            raise ValueError("vco_open rejected the archive")
+
+    def close(self):
+        if self.h:
+            self.o.L.vco_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def decode_atom(self, lod, ax, ay, az):
+        out = np.zeros(A3, np.uint8)
+        st = self.o.L.vco_decode_atom(self.h, lod, ax, ay, az, out.ctypes.data)
+        return st, out.reshape(32, 32, 32)
+
+    def coverage(self, lod, az, ay, ax):
+        off, ln = C.c_uint64(), C.c_uint32()
+        c = self.o.L.vco_coverage(self.h, lod, az, ay, ax, C.byref(off), C.byref(ln))
+        return c, off.value, ln.value
+
+    def decode_region(self, lod, box):
+        x0, y0, z0, x1, y1, z1 = box
+        out = np.zeros((z1 - z0, y1 - y0, x1 - x0), np.uint8)
+        b = np.array(box, np.uint32)
+        st = self.o.L.vco_decode_region(self.h, lod, b.ctypes.data, out.ctypes.data)
+        return st, out
+
+    def decode_lod(self, lod, dims):
+        nx, ny, nz = dims
+        out = np.zeros((nz, ny, nx), np.uint8)
+        d = np.zeros(3, np.uint32)
+        st = self.o.L.vco_decode_lod(self.h, lod, out.ctypes.data, d.ctypes.data)
+        return st, out
+
+    def decode_atoms_omp(self, lod, atoms_zyx, nthreads, keep=False):
+        atoms = np.ascontiguousarray(atoms_zyx, np.uint32)
+        n = atoms.shape[0]
+        out = np.zeros((n, A3), np.uint8) if keep else None
+        secs = self.o.L.vco_decode_atoms_omp(self.h, lod, atoms.ctypes.data, n, nthreads,
+                                             out.ctypes.data if keep else None)
+        return secs, out
+
+
+# ---------------------------------------------------------------------------
+# Reference build (oracle/_ref): the reference's own vc.c, compiled in place.
+# ---------------------------------------------------------------------------
+def ref_variants():
+    if not os.path.isdir(REF_DIR):
+        return {}
+    return {f[len("libvcref_"):-3]: os.path.join(REF_DIR, f)
+            for f in sorted(os.listdir(REF_DIR)) if f.startswith("libvcref_") and f.endswith(".so")}
+
+
+def host_has_avx512():
+    try:
+        with open("/proc/cpuinfo") as f:
+            return " avx512f " in f.read()
+    except OSError:
+        return False
+
+
+class _Dims(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32)]
+
+
+class _Box(C.Structure):
+    _fields_ = [("x0", C.c_uint32), ("y0", C.c_uint32), ("z0", C.c_uint32),
+                ("x1", C.c_uint32), ("y1", C.c_uint32), ("z1", C.c_uint32)]
+
+
+class Ref:
+    """The reference library (one build variant)."""
+
+    def __init__(self, variant="strict"):
+        v = ref_variants()
+        if variant not in v:
+            raise FileNotFoundError(f"reference variant {variant!r} not built (make -f oracle/ref.mk)")
+        self.variant = variant
+        L = self.L = C.CDLL(v[variant])
+        vp = C.c_void_p
+        L.vc_create.argtypes = [C.c_char_p, _Dims, C.c_float]
+        L.vc_create.restype = vp
+        L.vc_set_base_q.argtypes = [vp, C.c_int, C.c_float]
+        L.vc_append_atom.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, vp]
+        L.vc_mark_zero_atom.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.vc_writer_close.argtypes = [vp]
+        L.vc_open.argtypes = [vp, C.c_size_t]
+        L.vc_open.restype = vp
+        L.vc_close.argtypes = [vp]
+        L.vc_decode_atom.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]
+        L.vc_decode_region.argtypes = [vp, C.c_int, _Box, vp]
+        L.vc_atom_payload_range.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
+                                            C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]
+        L.vc_atom_coverage.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.vc_lod_dims.argtypes = [vp, C.c_int, C.POINTER(_Dims)]
+        L.vc_archive_version.argtypes = [vp]
+        L.vc_region_blob_range.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
+                                           C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.vc_test_atom_rt.argtypes = [vp, C.c_float, vp]
+        L.vc_test_atom_rt.restype = C.c_uint32
+        L.vc_test_downsample2x.argtypes = [vp, _Dims, C.POINTER(_Dims)]
+        L.vc_test_downsample2x.restype = vp
+        self.libc = C.CDLL("libc.so.6")
+        self.libc.free.argtypes = [vp]
+
+    def atom_rt(self, vox, q):
+        vox = np.ascontiguousarray(vox, np.uint8).ravel()
+        rec = np.zeros(A3, np.uint8)
+        n = self.L.vc_test_atom_rt(vox.ctypes.data, C.c_float(q), rec.ctypes.data)
+        return n, rec
+
+    def downsample2x(self, vol):
+        nz, ny, nx = vol.shape
+        vol = np.ascontiguousarray(vol, np.uint8)
+        od = _Dims()
+        p = self.L.vc_test_downsample2x(vol.ctypes.data, _Dims(nx, ny, nz), C.byref(od))
+        out = np.frombuffer(C.string_at(p, od.nx * od.ny * od.nz), np.uint8).reshape(od.nz, od.ny, od.nx).copy()
+        self.libc.free(p)
+        return out
+
+    def write_v1(self, path, dims, target_ratio, qs, atoms):
+        """atoms: iterable of (lod, az, ay, ax, vox or None for mark-zero)."""
+        nx, ny, nz = dims
+        w = self.L.vc_create(path.encode(), _Dims(nx, ny, nz), C.c_float(target_ratio))
+        if not w:
+            raise RuntimeError("vc_create failed")
+        for l, q in enumerate(qs):
+            self.L.vc_set_base_q(w, l, C.c_float(q))
+        for lod, az, ay, ax, vox in atoms:
+            if vox is None:
+                st = self.L.vc_mark_zero_atom(w, lod, az, ay, ax)
+            else:
+                v = np.ascontiguousarray(vox, np.uint8).ravel()
+                st = self.L.vc_append_atom(w, lod, az, ay, ax, v.ctypes.data)
+            if st != 0:
+                self.L.vc_writer_close(w)
+                raise RuntimeError(f"append failed {st}")
+        self.L.vc_writer_close(w)
+
+    def payloads(self, vox_list, q):
+        """Exact payload bytes of encode_atom via a one-LOD v1 archive in /tmp."""
+        n = len(vox_list)
+        fd, path = tempfile.mkstemp(suffix=".vca")
+        os.close(fd)
+        try:
+            self.write_v1(path, (32 * n, 32, 32), 10.0, [q] * 8,
+                          [(0, 0, 0, i, v) for i, v in enumerate(vox_list)])
+            buf = open(path, "rb").read()
+        finally:
+            os.unlink(path)
+        arc = RefArchive(self, buf)
+        out = []
+        for i in range(n):
+            c, off, ln = arc.payload_range(0, 0, 0, i)
+            out.append(buf[off:off + ln] if c == 2 else b"")
+        arc.close()
+        return out
+
+    def open(self, buf):
+        return RefArchive(self, buf)
+
+
+class RefArchive:
+    def __init__(self, ref, buf):
+        self.ref = ref
+        self.arr = np.frombuffer(bytes(buf), np.uint8).copy()
+        self.h = ref.L.vc_open(self.arr.ctypes.data, self.arr.size)
+        if not self.h:
+            raise ValueError("reference vc_open rejected the archive")
+
+    def close(self):
+        if self.h:
+            self.ref.L.vc_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def version(self):
+        return self.ref.L.vc_archive_version(self.h)
+
+    def lod_dims(self, lod):
+        d = _Dims()
+        st = self.ref.L.vc_lod_dims(self.h, lod, C.byref(d))
+        return st, (d.nx, d.ny, d.nz)
+
+    def decode_atom(self, lod, ax, ay, az):
+        out = np.zeros(A3, np.uint8)
+        st = self.ref.L.vc_decode_atom(self.h, lod, ax, ay, az, out.ctypes.data)
+        return st, out.reshape(32, 32, 32)
+
+    def decode_region(self, lod, box):
+        x0, y0, z0, x1, y1, z1 = box
+        out = np.zeros((z1 - z0, y1 - y0, x1 - x0), np.uint8)
+        st = self.ref.L.vc_decode_region(self.h, lod, _Box(*box), out.ctypes.data)
+        return st, out
+
+    def payload_range(self, lod, az, ay, ax):
+        off, ln = C.c_uint64(), C.c_uint32()
+        c = self.ref.L.vc_atom_payload_range(self.h, lod, az, ay, ax, C.byref(off), C.byref(ln))
+        return c, off.value, ln.value
+
+    def coverage(self, lod, az, ay, ax):
+        return self.ref.L.vc_atom_coverage(self.h, lod, az, ay, ax)

@@ -1,0 +1,153 @@
+// vcg_runtime.hip — device selection, per-thread streams and scratch, device
+// memory helpers and the device-resident archive.
+#include <hip/hip_runtime.h>
+#include <mutex>
+#include <vector>
+#include <stdio.h>
+#include <string.h>
+#include "vcg_dev.hpp"
+
+vcg_dtab *g_dtab = nullptr;
+static std::once_flag g_init_once;
+static int g_init_rc = -1;
+
+void vcg_fail(const char *what, hipError_t e, const char *file, int line) {
+    char msg[512];
+    snprintf(msg, sizeof msg, "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+    vc__panic(msg);
+    __builtin_unreachable();
+}
+
+extern "C" int vcg_init(const vcg_tables *t) {
+    std::call_once(g_init_once, [t] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) { g_init_rc = -1; return; }
+        VCG_CHECK(hipSetDevice(0));
+        vcg_dtab h;
+        memcpy(h.cmat, t->cmat, sizeof h.cmat);
+        memcpy(h.gstep, t->gstep, sizeof h.gstep);
+        memcpy(h.band_start, t->band_start, sizeof h.band_start);
+        memcpy(h.scan, t->scan, sizeof h.scan);
+        VCG_CHECK(hipMalloc(&g_dtab, sizeof(vcg_dtab)));
+        VCG_CHECK(hipMemcpy(g_dtab, &h, sizeof h, hipMemcpyHostToDevice));
+        g_init_rc = 0;
+    });
+    return g_init_rc;
+}
+
+// ---- per-thread contexts: created on first use, recycled when the thread exits
+static std::mutex g_pool_mu;
+static std::vector<vcg_ctx *> g_pool;
+
+struct ctx_holder {
+    vcg_ctx *c = nullptr;
+    ~ctx_holder() {
+        if (c) { std::lock_guard<std::mutex> g(g_pool_mu); g_pool.push_back(c); }
+    }
+};
+static thread_local ctx_holder tl_ctx;
+
+vcg_ctx *vcg_get_ctx() {
+    if (tl_ctx.c) return tl_ctx.c;
+    {
+        std::lock_guard<std::mutex> g(g_pool_mu);
+        if (!g_pool.empty()) { tl_ctx.c = g_pool.back(); g_pool.pop_back(); return tl_ctx.c; }
+    }
+    vcg_ctx *c = new vcg_ctx();
+    VCG_CHECK(hipSetDevice(0));
+    VCG_CHECK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    VCG_CHECK(hipMalloc(&c->status_d, 4));
+    VCG_CHECK(hipHostMalloc((void **)&c->status_h, 4, hipHostMallocDefault));
+    VCG_CHECK(hipMemsetAsync(c->status_d, 0, 4, c->s));
+    VCG_CHECK(hipStreamSynchronize(c->s));
+    tl_ctx.c = c;
+    return c;
+}
+
+// Grow a device buffer to hold `need` elements (contents are not preserved).
+void vcg_grow(void **p, size_t *cap, size_t need, size_t elem) {
+    if (need <= *cap) return;
+    vcg_ctx *c = vcg_get_ctx();
+    if (*p) {
+        VCG_CHECK(hipStreamSynchronize(c->s));
+        VCG_CHECK(hipFree(*p));
+        *p = nullptr;
+    }
+    size_t n = need + need / 4;
+    if (hipMalloc(p, n * elem) != hipSuccess) {
+        (void)hipGetLastError();
+        *cap = 0; *p = nullptr;
+        vc__panic("out of device memory (scratch)");
+    }
+    *cap = n;
+}
+
+extern "C" void *vcg_stream(void) { return (void *)vcg_get_ctx()->s; }
+
+extern "C" void *vcg_alloc(size_t bytes) {
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+    return p;
+}
+extern "C" void vcg_free(void *p) { if (p) VCG_CHECK(hipFree(p)); }
+
+extern "C" int vcg_copy(void *dst, const void *src, size_t bytes, int kind) {
+    vcg_ctx *c = vcg_get_ctx();
+    hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    VCG_CHECK(hipMemcpyAsync(dst, src, bytes, k, c->s));
+    VCG_CHECK(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+extern "C" void vcg_set_stage_events(void *ev[3]) {
+    vcg_ctx *c = vcg_get_ctx();
+    for (int i = 0; i < 3; ++i) c->ev[i] = ev ? ev[i] : nullptr;
+}
+
+__global__ void k_reset_word(uint32_t *w) { *w = 0; }
+
+extern "C" int vcg_sync(void) {
+    vcg_ctx *c = vcg_get_ctx();
+    VCG_CHECK(hipMemcpyAsync(c->status_h, c->status_d, 4, hipMemcpyDeviceToHost, c->s));
+    hipLaunchKernelGGL(k_reset_word, dim3(1), dim3(1), 0, c->s, c->status_d);
+    VCG_CHECK(hipGetLastError());
+    VCG_CHECK(hipStreamSynchronize(c->s));
+    return (*c->status_h & 1u) ? 1 : 0;
+}
+
+// ---- device-resident archive
+struct vcg_arc {
+    uint8_t  *buf = nullptr;   // archive bytes in HBM
+    vcg_geom *geom = nullptr;  // geometry + dequant tables in HBM
+    vcg_geom  host;            // host copy (launch bounds)
+};
+
+extern "C" vcg_arc *vcg_arc_create(const uint8_t *buf, const vcg_geom *g) {
+    vcg_ctx *c = vcg_get_ctx();
+    vcg_arc *a = new vcg_arc();
+    a->host = *g;
+    if (hipMalloc(&a->buf, g->len ? g->len : 1) != hipSuccess ||
+        hipMalloc(&a->geom, sizeof(vcg_geom)) != hipSuccess) {
+        (void)hipGetLastError();
+        if (a->buf) (void)hipFree(a->buf);
+        delete a;
+        return nullptr;
+    }
+    VCG_CHECK(hipMemcpyAsync(a->buf, buf, g->len, hipMemcpyHostToDevice, c->s));
+    VCG_CHECK(hipMemcpyAsync(a->geom, g, sizeof(vcg_geom), hipMemcpyHostToDevice, c->s));
+    VCG_CHECK(hipStreamSynchronize(c->s));
+    return a;
+}
+
+extern "C" void vcg_arc_destroy(vcg_arc *a) {
+    if (!a) return;
+    VCG_CHECK(hipDeviceSynchronize());
+    VCG_CHECK(hipFree(a->buf));
+    VCG_CHECK(hipFree(a->geom));
+    delete a;
+}
+
+// accessors for the other translation units
+const uint8_t *vcg_arc_buf(const vcg_arc *a) { return a->buf; }
+const vcg_geom *vcg_arc_geom_dev(const vcg_arc *a) { return a->geom; }
+const vcg_geom *vcg_arc_geom_host(const vcg_arc *a) { return &a->host; }

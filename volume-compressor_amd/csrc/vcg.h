@@ -1,0 +1,110 @@
+/* vcg.h — the thin C ABI between the plain-C host library (csrc/host) and the
+ * HIP kernels (csrc/hip).  Host C owns the archive format, rate control and
+ * layout; the vcg_* functions own device memory, streams and kernel launches.
+ * Plain C types only.
+ */
+#ifndef VCG_H
+#define VCG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VCG_A3        32768u
+#define VCG_NFREQ     94
+#define VCG_NLOD      8
+#define VCG_RC_CAP    (32768u * 4u + 64u)   /* range-coder bound, src/vc/vc.c:611 */
+#define VCG_PAY_CAP   (VCG_RC_CAP + 1u)     /* + dc byte */
+
+/* Codec tables, computed once on the host (libm) and uploaded. */
+typedef struct {
+    int32_t  cmat[32][32];      /* Q14 DCT basis CMATA[k][n] */
+    float    gstep[VCG_NFREQ];  /* (1+freq)^0.65f */
+    uint16_t scan[VCG_A3];      /* scan position -> raster index */
+    uint32_t band_start[9];     /* first scan position of each sig band; [8] = 32768 */
+} vcg_tables;
+
+/* Archive geometry, parsed and validated on the host. */
+typedef struct {
+    uint64_t len;                        /* archive bytes */
+    uint64_t l1_off, l1_cap;             /* v1 */
+    uint64_t dir_off;                    /* v2 */
+    uint64_t region_prefix[VCG_NLOD + 1];/* v2 region index base per LOD */
+    uint32_t nrx[VCG_NLOD], nry[VCG_NLOD], nrz[VCG_NLOD]; /* region grid */
+    uint32_t acx[VCG_NLOD], acy[VCG_NLOD], acz[VCG_NLOD]; /* atom grid */
+    uint32_t lnx[VCG_NLOD], lny[VCG_NLOD], lnz[VCG_NLOD]; /* LOD dims */
+    int32_t  version, nlod;
+    int32_t  dq_step[VCG_NLOD][VCG_NFREQ];  /* Q8 dequant per LOD (host libm) */
+    int32_t  dq_bias[VCG_NLOD][VCG_NFREQ];
+} vcg_geom;
+
+typedef struct { uint32_t x0, y0, z0, x1, y1, z1; } vcg_box;
+
+/* ---- runtime ---- */
+int   vcg_init(const vcg_tables *t);        /* 0 ok, -1 no device */
+void *vcg_stream(void);                     /* calling thread's hipStream_t */
+void *vcg_alloc(size_t bytes);              /* NULL on failure */
+void  vcg_free(void *p);
+int   vcg_copy(void *dst, const void *src, size_t bytes, int kind); /* 1 h2d, 2 d2h, 3 d2d */
+void  vcg_set_stage_events(void *ev[3]);
+int   vcg_sync(void);                       /* wait; returns accumulated vc status */
+
+/* ---- device-resident archive ---- */
+typedef struct vcg_arc vcg_arc;
+vcg_arc *vcg_arc_create(const uint8_t *buf, const vcg_geom *g);   /* copies buf to HBM */
+void     vcg_arc_destroy(vcg_arc *a);
+
+/* Decode every atom overlapping `box` (voxels of LOD `lod`) into out (box
+ * raster).  out_host: out is host memory (staged through HBM).  sync: wait
+ * and return the status (0 OK / 1 FORMAT); otherwise enqueue and return 0. */
+int vcg_decode_box(vcg_arc *a, int lod, const vcg_box *box, uint8_t *out, int out_host, int sync);
+/* Decode n whole atoms given as (lod,ax,ay,az) quads into host out (n*32768).
+ * Coordinates must already be range-checked. per_item: status per atom. */
+int vcg_decode_list(vcg_arc *a, size_t n, const int32_t *quads, uint8_t *out, int32_t *per_item);
+/* Decode n raw payloads (no archive) at one base_q. */
+int vcg_decode_payloads(size_t n, const uint8_t *pay, const uint64_t *offs, const uint32_t *lens,
+                        const int32_t dq_step[VCG_NFREQ], const int32_t dq_bias[VCG_NFREQ],
+                        uint8_t *out, int32_t *per_item);
+
+/* ---- encode ---- */
+typedef struct vcg_enc vcg_enc;
+/* Upload (or adopt) a LOD0 volume and build the nlod-level pyramid on device. */
+vcg_enc *vcg_enc_begin(const uint8_t *vol, int vol_on_device, const uint32_t dims[3], int nlod);
+/* Rate-control probe: total payload bytes of the listed LOD0 atoms (zyx
+ * triples) at base_q.  First call caches their coefficients.  Returns 0, or 1
+ * if any atom overflowed the coder bound.  zero_mask[i] = 1 for all-zero atoms. */
+int  vcg_enc_sample(vcg_enc *e, const uint32_t *atoms_zyx, int n, float base_q,
+                    uint64_t *bytes, uint8_t *zero_mask);
+/* Encode every atom of every LOD (LOD-major, raster) at qs[lod]; lens gets
+ * each payload length (0 = all-zero atom).  Returns 0, or 1 on coder overflow. */
+int  vcg_enc_all(vcg_enc *e, const float qs[VCG_NLOD], uint32_t *lens, uint64_t total_atoms);
+/* Assemble the archive image: head (header + directory, head_len bytes) at 0,
+ * then for atom i with lens[i] > 0 its payload at pay_off[i] and, for every
+ * atom with slot_off[i] != 0, its 16-byte slot at slot_off[i] (flags 1 with
+ * off/len/dc for payloads, flags 2 for zero atoms).  Writes total_len bytes
+ * to host out. */
+int  vcg_enc_assemble(vcg_enc *e, const uint8_t *head, size_t head_len, const uint64_t *pay_off,
+                      const uint64_t *slot_off, const uint32_t *lens, uint64_t total_atoms,
+                      uint8_t *out, uint64_t total_len);
+void vcg_enc_end(vcg_enc *e);
+
+/* Batched atom encode without a container (vcx_encode_atoms). */
+int vcg_encode_atoms(size_t n, const uint8_t *vox, float base_q, uint8_t *out, size_t cap,
+                     uint32_t *lens);
+/* Device downsample for a volume already on device (LOD l -> l+1). */
+int vcg_downsample(const void *dev_in, const uint32_t dims[3], void *dev_out);
+
+/* ---- synthetic generator ---- */
+typedef struct { int32_t axis, c0, tu, tv, a1, a2, w1, w2, p1, p2, half, inten; } vcg_sheet;
+int vcg_synth(uint64_t seed, const uint32_t dims[3], const vcg_sheet *sheets, int nsheets, void *dev_out);
+
+/* host-side fail-fast (implemented in host C: panic hook + abort) */
+void vc__panic(const char *msg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VCG_H */

@@ -1,0 +1,360 @@
+"""ctypes binding of libvc.so (the C ABI in include/vc.h, vc_gpu.h,
+vc_testhooks.h).  Thin: every call goes straight to the C entry point; there
+is no Python or CPU codec behind it.  Raises if libvc.so is not built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvc.so")
+A3 = 32768
+VC_OK, VC_ERR_FORMAT, VC_ERR_RANGE, VC_ERR_OOM, VC_ERR_IO = range(5)
+
+
+class Dims(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32)]
+
+
+class Box(C.Structure):
+    _fields_ = [("x0", C.c_uint32), ("y0", C.c_uint32), ("z0", C.c_uint32),
+                ("x1", C.c_uint32), ("y1", C.c_uint32), ("z1", C.c_uint32)]
+
+
+READ_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_uint8))
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
+                           "(make -C volume-compressor_amd)")
+    L = C.CDLL(LIB_PATH)
+    vp, sz, u32, u64, i32, f32 = C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint64, C.c_int, C.c_float
+    P = C.POINTER
+    sig = {
+        "vc_encode": (i32, [vp, Dims, f32, P(vp), P(sz)]),
+        "vc_open": (vp, [vp, sz]),
+        "vc_open_streaming": (vp, [READ_FN, vp, u64]),
+        "vc_close": (None, [vp]),
+        "vc_lod_dims": (i32, [vp, i32, P(Dims)]),
+        "vc_decode_lod": (i32, [vp, i32, vp, P(Dims)]),
+        "vc_decode_atom": (i32, [vp, i32, i32, i32, i32, vp]),
+        "vc_decode_region": (i32, [vp, i32, Box, vp]),
+        "vc_atom_coverage": (i32, [vp, i32, u32, u32, u32]),
+        "vc_atom_payload_range": (i32, [vp, i32, u32, u32, u32, P(u64), P(u32)]),
+        "vc_archive_version": (i32, [vp]),
+        "vc_region_blob_range": (i32, [vp, i32, u32, u32, u32, P(u64), P(u64)]),
+        "vc_region_atoms": (u32, []),
+        "vc_create": (vp, [C.c_char_p, Dims, f32]),
+        "vc_set_base_q": (i32, [vp, i32, f32]),
+        "vc_append_atom": (i32, [vp, i32, u32, u32, u32, vp]),
+        "vc_append_box": (i32, [vp, i32, Box, vp]),
+        "vc_mark_zero_atom": (i32, [vp, i32, u32, u32, u32]),
+        "vc_mark_zero_region": (i32, [vp, i32, u32, u32, u32]),
+        "vc_writer_decode_atom": (i32, [vp, i32, u32, u32, u32, vp]),
+        "vc_writer_coverage": (i32, [vp, i32, u32, u32, u32]),
+        "vc_writer_decode_2x2x2": (i32, [vp, i32, u32, u32, u32, vp, P(i32)]),
+        "vc_writer_close": (None, [vp]),
+        "vc_test_downsample2x": (vp, [vp, Dims, P(Dims)]),
+        "vc_test_atom_rt": (u32, [vp, f32, vp]),
+        "vcx_device_init": (i32, []),
+        "vcx_stream": (vp, []),
+        "vcx_dev_alloc": (vp, [sz]),
+        "vcx_dev_free": (None, [vp]),
+        "vcx_h2d": (i32, [vp, vp, sz]),
+        "vcx_d2h": (i32, [vp, vp, sz]),
+        "vcx_decode_lod_device": (i32, [vp, i32, vp, i32]),
+        "vcx_decode_region_device": (i32, [vp, i32, Box, vp, i32]),
+        "vcx_sync": (i32, []),
+        "vcx_decode_atoms": (i32, [vp, sz, vp, vp, vp]),
+        "vcx_set_stage_events": (None, [vp]),
+        "vcx_encode_q": (i32, [vp, Dims, f32, vp, P(vp), P(sz)]),
+        "vcx_encode_device": (i32, [vp, Dims, f32, P(vp), P(sz)]),
+        "vcx_rate_control": (f32, [vp, Dims, f32]),
+        "vcx_encode_atoms": (i32, [sz, vp, f32, vp, sz, vp]),
+        "vcx_decode_payloads": (i32, [sz, vp, vp, vp, f32, vp, vp]),
+        "vcx_synth_device": (i32, [u64, Dims, i32, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+_libc = C.CDLL("libc.so.6")
+_libc.free.argtypes = [C.c_void_p]
+
+
+def _take(p, n):
+    b = C.string_at(p, n)
+    _libc.free(p)
+    return b
+
+
+def _u8(a):
+    return np.ascontiguousarray(a, np.uint8)
+
+
+def dims_of(vol):
+    nz, ny, nx = vol.shape
+    return Dims(nx, ny, nz)
+
+
+# ---- encode -----------------------------------------------------------------
+def encode(vol, target_ratio):
+    vol = _u8(vol)
+    p, n = C.c_void_p(), C.c_size_t()
+    st = lib().vc_encode(vol.ctypes.data, dims_of(vol), C.c_float(target_ratio), C.byref(p), C.byref(n))
+    return st, (_take(p, n.value) if st == VC_OK else None)
+
+
+def encode_q(vol, target_ratio, qs):
+    vol = _u8(vol)
+    q8 = np.zeros(8, np.float32)
+    q8[:len(qs)] = qs
+    p, n = C.c_void_p(), C.c_size_t()
+    st = lib().vcx_encode_q(vol.ctypes.data, dims_of(vol), C.c_float(target_ratio), q8.ctypes.data,
+                            C.byref(p), C.byref(n))
+    return st, (_take(p, n.value) if st == VC_OK else None)
+
+
+def encode_device(dev_ptr, dims, target_ratio):
+    p, n = C.c_void_p(), C.c_size_t()
+    st = lib().vcx_encode_device(dev_ptr, Dims(*dims), C.c_float(target_ratio), C.byref(p), C.byref(n))
+    return st, (_take(p, n.value) if st == VC_OK else None)
+
+
+def rate_control(vol, target_ratio):
+    vol = _u8(vol)
+    return lib().vcx_rate_control(vol.ctypes.data, dims_of(vol), C.c_float(target_ratio))
+
+
+def encode_atoms(atoms, q, cap=A3 * 4 + 65):
+    atoms = _u8(atoms).reshape(-1, A3)
+    n = atoms.shape[0]
+    out = np.zeros((n, cap), np.uint8)
+    lens = np.zeros(n, np.uint32)
+    st = lib().vcx_encode_atoms(n, atoms.ctypes.data, C.c_float(q), out.ctypes.data, cap, lens.ctypes.data)
+    return st, [out[i, :lens[i]].tobytes() for i in range(n)]
+
+
+def decode_payloads(payloads, q):
+    n = len(payloads)
+    lens = np.array([len(p) for p in payloads], np.uint32)
+    offs = np.zeros(n, np.uint64)
+    if n:
+        offs[1:] = np.cumsum(lens[:-1].astype(np.uint64))
+    pool = np.frombuffer(b"".join(payloads) or b"\0", np.uint8).copy()
+    out = np.zeros((n, A3), np.uint8)
+    per = np.zeros(n, np.int32)
+    st = lib().vcx_decode_payloads(n, pool.ctypes.data, offs.ctypes.data, lens.ctypes.data, C.c_float(q),
+                                   out.ctypes.data, per.ctypes.data)
+    return st, out, per
+
+
+# ---- reader -------------------------------------------------------------------
+class Archive:
+    """vc_open over an in-memory archive (the numpy copy keeps the borrowed
+    buffer alive for the handle's lifetime)."""
+
+    def __init__(self, buf):
+        self.L = lib()
+        self.arr = np.frombuffer(bytes(buf), np.uint8).copy()
+        self.h = self.L.vc_open(self.arr.ctypes.data, self.arr.size)
+        if not self.h:
+            raise ValueError("vc_open rejected the archive")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.vc_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def version(self):
+        return self.L.vc_archive_version(self.h)
+
+    def lod_dims(self, lod):
+        d = Dims()
+        st = self.L.vc_lod_dims(self.h, lod, C.byref(d))
+        return st, (d.nx, d.ny, d.nz)
+
+    def coverage(self, lod, az, ay, ax):
+        return self.L.vc_atom_coverage(self.h, lod, az, ay, ax)
+
+    def payload_range(self, lod, az, ay, ax):
+        o, n = C.c_uint64(), C.c_uint32()
+        c = self.L.vc_atom_payload_range(self.h, lod, az, ay, ax, C.byref(o), C.byref(n))
+        return c, o.value, n.value
+
+    def blob_range(self, lod, rz, ry, rx):
+        o, n = C.c_uint64(), C.c_uint64()
+        c = self.L.vc_region_blob_range(self.h, lod, rz, ry, rx, C.byref(o), C.byref(n))
+        return c, o.value, n.value
+
+    def decode_atom(self, lod, ax, ay, az):
+        out = np.zeros(A3, np.uint8)
+        st = self.L.vc_decode_atom(self.h, lod, ax, ay, az, out.ctypes.data)
+        return st, out.reshape(32, 32, 32)
+
+    def decode_atoms(self, quads):
+        q = np.ascontiguousarray(quads, np.int32).reshape(-1, 4)
+        out = np.zeros((q.shape[0], A3), np.uint8)
+        per = np.zeros(q.shape[0], np.int32)
+        st = self.L.vcx_decode_atoms(self.h, q.shape[0], q.ctypes.data, out.ctypes.data, per.ctypes.data)
+        return st, out, per
+
+    def decode_region(self, lod, box):
+        x0, y0, z0, x1, y1, z1 = box
+        shape = (max(z1 - z0, 0), max(y1 - y0, 0), max(x1 - x0, 0))
+        out = np.zeros(shape if all(shape) else (1,), np.uint8)
+        st = self.L.vc_decode_region(self.h, lod, Box(*box), out.ctypes.data)
+        return st, out
+
+    def decode_lod(self, lod):
+        st, (nx, ny, nz) = self.lod_dims(lod)
+        if st:
+            return st, None
+        out = np.zeros((nz, ny, nx), np.uint8)
+        d = Dims()
+        st = self.L.vc_decode_lod(self.h, lod, out.ctypes.data, C.byref(d))
+        return st, out
+
+    def decode_lod_device(self, lod, dev_out, sync=1):
+        return self.L.vcx_decode_lod_device(self.h, lod, dev_out, sync)
+
+
+class StreamingArchive(Archive):
+    """vc_open_streaming over a Python byte source (counts callback reads)."""
+
+    def __init__(self, buf, fail_after=None):
+        self.L = lib()
+        self.src = bytes(buf)
+        self.reads = 0
+        self.fail_after = fail_after
+
+        def cb(ud, off, ln, dst):
+            self.reads += 1
+            if self.fail_after is not None and self.reads > self.fail_after:
+                return VC_ERR_IO
+            if off + ln > len(self.src):
+                return VC_ERR_FORMAT
+            C.memmove(dst, self.src[off:off + ln], ln)
+            return VC_OK
+
+        self._cb = READ_FN(cb)
+        self.h = self.L.vc_open_streaming(self._cb, None, len(self.src))
+        if not self.h:
+            raise ValueError("vc_open_streaming rejected the archive")
+
+
+# ---- writer -------------------------------------------------------------------
+class Writer:
+    def __init__(self, path, dims, target_ratio):
+        self.L = lib()
+        self.h = self.L.vc_create(path.encode(), Dims(*dims), C.c_float(target_ratio))
+        if not self.h:
+            raise ValueError("vc_create failed")
+
+    def set_base_q(self, lod, q):
+        return self.L.vc_set_base_q(self.h, lod, C.c_float(q))
+
+    def append_atom(self, lod, az, ay, ax, vox):
+        v = _u8(vox).ravel()
+        return self.L.vc_append_atom(self.h, lod, az, ay, ax, v.ctypes.data)
+
+    def append_box(self, lod, box, voxels):
+        v = _u8(voxels)
+        return self.L.vc_append_box(self.h, lod, Box(*box), v.ctypes.data)
+
+    def mark_zero_atom(self, lod, az, ay, ax):
+        return self.L.vc_mark_zero_atom(self.h, lod, az, ay, ax)
+
+    def mark_zero_region(self, lod, rz, ry, rx):
+        return self.L.vc_mark_zero_region(self.h, lod, rz, ry, rx)
+
+    def coverage(self, lod, az, ay, ax):
+        return self.L.vc_writer_coverage(self.h, lod, az, ay, ax)
+
+    def decode_atom(self, lod, az, ay, ax):
+        out = np.zeros(A3, np.uint8)
+        st = self.L.vc_writer_decode_atom(self.h, lod, az, ay, ax, out.ctypes.data)
+        return st, out.reshape(32, 32, 32)
+
+    def decode_2x2x2(self, lod, az0, ay0, ax0):
+        out = np.zeros((8, A3), np.uint8)
+        anyp = C.c_int(0)
+        st = self.L.vc_writer_decode_2x2x2(self.h, lod, az0, ay0, ax0, out.ctypes.data, C.byref(anyp))
+        return st, out, anyp.value
+
+    def close(self):
+        if self.h:
+            self.L.vc_writer_close(self.h)
+            self.h = None
+
+
+# ---- device helpers -------------------------------------------------------------
+def device_init():
+    return lib().vcx_device_init()
+
+
+def dev_alloc(n):
+    p = lib().vcx_dev_alloc(n)
+    if not p:
+        raise MemoryError("vcx_dev_alloc failed")
+    return p
+
+
+def dev_free(p):
+    lib().vcx_dev_free(p)
+
+
+def d2h(dev, n):
+    out = np.zeros(n, np.uint8)
+    lib().vcx_d2h(out.ctypes.data, dev, n)
+    return out
+
+
+def h2d(dev, arr):
+    a = _u8(arr)
+    lib().vcx_h2d(dev, a.ctypes.data, a.size)
+
+
+def synth_device(seed, dims, nsheets, dev_out):
+    return lib().vcx_synth_device(seed, Dims(*dims), nsheets, dev_out)
+
+
+def synth(seed, dims, nsheets):
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    d = dev_alloc(n)
+    try:
+        st = synth_device(seed, dims, nsheets, d)
+        if st:
+            raise RuntimeError(f"vcx_synth_device failed {st}")
+        return d2h(d, n).reshape(nz, ny, nx)
+    finally:
+        dev_free(d)
+
+
+def test_downsample2x(vol):
+    vol = _u8(vol)
+    od = Dims()
+    p = lib().vc_test_downsample2x(vol.ctypes.data, dims_of(vol), C.byref(od))
+    return np.frombuffer(_take(p, od.nx * od.ny * od.nz), np.uint8).reshape(od.nz, od.ny, od.nx)
+
+
+def test_atom_rt(vox, q):
+    v = _u8(vox).ravel()
+    rec = np.zeros(A3, np.uint8)
+    n = lib().vc_test_atom_rt(v.ctypes.data, C.c_float(q), rec.ctypes.data)
+    return n, rec
