@@ -1,0 +1,73 @@
+// CPU build of the device codec code (volume-compressor_amd/csrc/hip/
+// vcg_codec.hpp, vcg_xform.hpp): the per-atom range coder and the transform
+// kernels' phases, run thread by thread between the kernels' barriers, so the
+// exact kernel logic is checked against the oracle on the host.  Test
+// infrastructure.
+#include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
+#include "vcg_xform.hpp"
+
+static const int NT = 256;   // threads per workgroup in the kernels
+
+extern "C" {
+uint32_t kc_encode(const int16_t *L, uint32_t eob, const uint32_t *band_start, uint8_t *out, uint32_t cap,
+                   int *ovf) {
+    return vcg_rc_encode_atom(L, eob, band_start, out, cap, ovf);
+}
+int kc_decode(const uint8_t *s, uint32_t n, const uint32_t *band_start, uint32_t *out, uint32_t *cnt) {
+    return vcg_rc_decode_atom(s, n, band_start, out, cnt);
+}
+void kc_quant(const int32_t *c, size_t n, float q, float g, int32_t *out) {
+    for (size_t i = 0; i < n; ++i) out[i] = vcg_quant1(c[i], q, g);
+}
+
+// k_fdct on one atom of an (nx,ny,nz) volume: returns 1 for an all-zero atom
+int kc_fdct(const uint8_t *vol, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t ax, uint32_t ay, uint32_t az,
+            const int32_t cmat[32][32], const uint16_t *scan, const float *gstep, float q, int16_t *lv,
+            int32_t *coef_raster, uint32_t *eob, uint32_t *dc_out) {
+    vcg_fwd_tab F;
+    for (int n = 0; n < 16; ++n)
+        for (int j = 0; j < 16; ++j) { F.e[n][j] = cmat[2 * j][n]; F.o[n][j] = cmat[2 * j + 1][n]; }
+    int *t = (int *)calloc(VCG_TILE_INTS, sizeof(int));
+    uint32_t sum = 0;
+    for (int tid = 0; tid < NT; ++tid) sum += vcg_gather(t, vol, nx, ny, nz, ax, ay, az, tid, NT);
+    if (sum == 0) { free(t); *eob = 0; *dc_out = 0; return 1; }
+    const int dc = (int)((sum + 16384u) / 32768u);
+    for (int tid = 0; tid < NT; ++tid) vcg_level_shift(t, dc, tid, NT);
+    for (int tid = 0; tid < NT; ++tid) vcg_fwd_pass_x(t, &F, tid, NT);
+    for (int tid = 0; tid < NT; ++tid) vcg_fwd_pass_y(t, &F, tid, NT);
+    for (int tid = 0; tid < NT; ++tid) vcg_fwd_pass_z(t, &F, tid, NT);
+    if (coef_raster)
+        for (int k = 0; k < 32768; ++k) coef_raster[k] = t[vcg_tidx(k >> 10, (k >> 5) & 31, k & 31)];
+    int last = 0;
+    for (int tid = 0; tid < NT; ++tid) { int l = vcg_quant_scan(t, scan, gstep, q, lv, tid, NT); if (l > last) last = l; }
+    *eob = (uint32_t)last; *dc_out = (uint32_t)dc;
+    free(t);
+    return 0;
+}
+
+// k_idct on one atom's entries: force = 0 picks the path like the kernel,
+// 1 forces the 24-bit path, 2 the full 32-bit path; returns the path used.
+int kc_idct(const uint32_t *L, uint32_t cnt, int dc, const int32_t *st, const int32_t *bi,
+            const int32_t cmat[32][32], const uint16_t *scan, int force, uint8_t *out) {
+    int *t = (int *)malloc(VCG_TILE_INTS * sizeof(int));
+    for (int tid = 0; tid < NT; ++tid) vcg_tile_zero(t, tid, NT);
+    const int fmax = cnt ? vcg_freq(scan[L[cnt - 1] >> 16]) : -1;
+    uint64_t l1 = 0;
+    for (int tid = 0; tid < NT; ++tid) l1 += vcg_scatter(t, L, cnt, scan, st, bi, tid, NT);
+    const bool fast = force == 1 ? true : force == 2 ? false : l1 <= 8000000ull;
+    auto emit = [&](int z, int y, int x, uint8_t v) { out[(z * 32 + y) * 32 + x] = v; };
+    if (fast) {
+        for (int tid = 0; tid < NT; ++tid) vcg_inv_pass_x<true>(t, cmat, fmax, tid, NT);
+        for (int tid = 0; tid < NT; ++tid) vcg_inv_pass_y<true>(t, cmat, fmax, tid, NT);
+        for (int tid = 0; tid < NT; ++tid) vcg_inv_pass_z<true>(t, cmat, fmax, dc, tid, NT, emit);
+    } else {
+        for (int tid = 0; tid < NT; ++tid) vcg_inv_pass_x<false>(t, cmat, fmax, tid, NT);
+        for (int tid = 0; tid < NT; ++tid) vcg_inv_pass_y<false>(t, cmat, fmax, tid, NT);
+        for (int tid = 0; tid < NT; ++tid) vcg_inv_pass_z<false>(t, cmat, fmax, dc, tid, NT, emit);
+    }
+    free(t);
+    return fast ? 1 : 2;
+}
+}

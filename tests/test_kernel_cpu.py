@@ -1,0 +1,163 @@
+"""The device codec code (vcg_codec.hpp, vcg_xform.hpp) compiled for the CPU
+and run phase by phase with the kernels' thread layout, against the
+reference's golden payloads/reconstructions and the oracle.  Catches kernel
+logic errors (indexing, strides, pruning, overflow paths) without a GPU."""
+import ctypes as C
+import hashlib
+import os
+import subprocess
+import time
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, payloads_of
+
+KDIR = os.path.join(ROOT, "tests", "kernel_cpu")
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module")
+def kc():
+    subprocess.check_call(["make", "-s", "-C", KDIR])
+    L = C.CDLL(os.path.join(KDIR, "libkernel_cpu.so"))
+    vp = C.c_void_p
+    L.kc_encode.restype = C.c_uint32
+    L.kc_encode.argtypes = [vp, C.c_uint32, vp, vp, C.c_uint32, C.POINTER(C.c_int)]
+    L.kc_decode.argtypes = [vp, C.c_uint32, vp, vp, C.POINTER(C.c_uint32)]
+    L.kc_fdct.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                          vp, vp, vp, C.c_float, vp, vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    L.kc_idct.argtypes = [vp, C.c_uint32, C.c_int, vp, vp, vp, vp, C.c_int, vp]
+    L.kc_quant.argtypes = [vp, C.c_size_t, C.c_float, C.c_float, vp]
+    return L
+
+
+@pytest.fixture(scope="module")
+def tabs(oracle):
+    cm, gs, scan, band = oracle.tables()
+    bs = np.zeros(9, np.uint32)
+    for b in range(1, 8):
+        bs[b] = np.argmax(band >= b)
+    bs[8] = 32768
+    return cm, gs, scan, bs
+
+
+def k_fdct(kc, tabs, vol, ax, ay, az, q):
+    cm, gs, scan, bs = tabs
+    vol = np.ascontiguousarray(vol, np.uint8)
+    nz, ny, nx = vol.shape
+    lv = np.zeros(32768, np.int16)
+    co = np.zeros(32768, np.int32)
+    eob, dc = C.c_uint32(), C.c_uint32()
+    z = kc.kc_fdct(vol.ctypes.data, nx, ny, nz, ax, ay, az, cm.ctypes.data, scan.ctypes.data, gs.ctypes.data,
+                   C.c_float(q), lv.ctypes.data, co.ctypes.data, C.byref(eob), C.byref(dc))
+    return z, lv, co, eob.value, dc.value
+
+
+def k_payload(kc, tabs, atom, q):
+    cm, gs, scan, bs = tabs
+    z, lv, co, eob, dc = k_fdct(kc, tabs, atom, 0, 0, 0, q)
+    if z:
+        return b""
+    out = np.zeros(32768 * 4 + 64, np.uint8)
+    ovf = C.c_int()
+    n = kc.kc_encode(lv.ctypes.data, eob, bs.ctypes.data, out.ctypes.data, out.size, C.byref(ovf))
+    assert not ovf.value
+    return bytes([dc]) + out[:n].tobytes()
+
+
+def k_decode(kc, tabs, pay, q, oracle, force=0):
+    cm, gs, scan, bs = tabs
+    rc = np.frombuffer(pay[1:], np.uint8) if len(pay) > 1 else np.zeros(1, np.uint8)
+    ent = np.zeros(32768, np.uint32)
+    cnt = C.c_uint32()
+    st = kc.kc_decode(rc.ctypes.data, len(pay) - 1, bs.ctypes.data, ent.ctypes.data, C.byref(cnt))
+    if st:
+        return st, np.zeros(32768, np.uint8), 0
+    stp, bia = oracle.dq_tables(q)
+    out = np.zeros(32768, np.uint8)
+    path = kc.kc_idct(ent.ctypes.data, cnt.value, pay[0], stp.ctypes.data, bia.ctypes.data, cm.ctypes.data,
+                      scan.ctypes.data, force, out.ctypes.data)
+    return 0, out, path
+
+
+def test_fdct_phases_match_oracle(kc, tabs, oracle, golden_atoms):
+    atoms = golden_atoms["atoms"]
+    for i in range(len(atoms)):
+        z, lv, co, eob, dc = k_fdct(kc, tabs, atoms[i], 0, 0, 0, 1.0)
+        s = int(atoms[i].astype(np.int64).sum())
+        if s == 0:
+            assert z == 1
+            continue
+        odc = (s + 16384) // 32768
+        assert dc == odc
+        assert np.array_equal(co, oracle.fdct(atoms[i], odc)), i
+
+
+def test_fdct_gather_edges(kc, tabs, oracle):
+    vol = oracle.synth(5, 70, 45, 33, 3)
+    for (ax, ay, az) in [(0, 0, 0), (2, 1, 1), (1, 0, 1)]:
+        z, lv, co, eob, dc = k_fdct(kc, tabs, vol, ax, ay, az, 2.0)
+        at = np.zeros((32, 32, 32), np.uint8)
+        s = vol[az * 32:az * 32 + 32, ay * 32:ay * 32 + 32, ax * 32:ax * 32 + 32]
+        at[:s.shape[0], :s.shape[1], :s.shape[2]] = s
+        odc = (int(at.astype(np.int64).sum()) + 16384) // 32768
+        assert np.array_equal(co, oracle.fdct(at, odc))
+
+
+def test_kernel_payloads_match_reference(kc, tabs, golden_atoms):
+    atoms, qs = golden_atoms["atoms"], golden_atoms["qs"]
+    t0 = time.time()
+    for ai, qi, pay, _ in payloads_of(golden_atoms):
+        assert k_payload(kc, tabs, atoms[ai], float(qs[qi])) == pay, (ai, qi)
+    assert time.time() - t0 < 120
+
+
+def test_kernel_decode_matches_reference(kc, tabs, oracle, golden_atoms):
+    qs = golden_atoms["qs"]
+    for ai, qi, pay, rsha in payloads_of(golden_atoms):
+        if not pay:
+            continue
+        st, out, path = k_decode(kc, tabs, pay, float(qs[qi]), oracle)
+        assert st == 0 and sha(out) == rsha, (ai, qi, path)
+
+
+def test_kernel_decode_paths_agree(kc, tabs, oracle, golden_atoms):
+    """The 24-bit path is taken only under the L1 bound; both paths agree there."""
+    qs = golden_atoms["qs"]
+    for ai, qi, pay, rsha in payloads_of(golden_atoms):
+        if not pay:
+            continue
+        st, a, path = k_decode(kc, tabs, pay, float(qs[qi]), oracle, 0)
+        st2, b, _ = k_decode(kc, tabs, pay, float(qs[qi]), oracle, 2)
+        assert np.array_equal(a, b)
+
+
+def test_kernel_decode_damaged(kc, tabs, oracle, golden_corrupt):
+    g = golden_corrupt
+    blob = g["blob"].tobytes()
+    for k in range(len(g["lens"])):
+        p = blob[int(g["offs"][k]):int(g["offs"][k]) + int(g["lens"][k])]
+        st, out, _ = k_decode(kc, tabs, p, float(g["qs"][k]), oracle)
+        assert st == 0 and sha(out) == str(g["rec_sha"][k])
+
+
+def test_kernel_eg_cap(kc, tabs, oracle):
+    from test_oracle import eg_prefix_stream
+    st, out, _ = k_decode(kc, tabs, bytes([9]) + eg_prefix_stream(70), 1.0, oracle)
+    assert st == 1
+    st, out, _ = k_decode(kc, tabs, bytes([9]) + eg_prefix_stream(20), 1.0, oracle)
+    assert st == 0
+
+
+def test_kernel_quantizer_exhaustive(kc, tabs, oracle):
+    cm, gs, scan, bs = tabs
+    c = np.arange(-65535, 65536, dtype=np.int32)
+    out = np.zeros(c.size, np.int32)
+    for q in (0.05, 0.77, 13.3, 4096.0):
+        for f in range(94):
+            kc.kc_quant(c.ctypes.data, c.size, C.c_float(q), C.c_float(float(gs[f])), out.ctypes.data)
+            assert np.array_equal(out, oracle.quant_levels(q, f, c)), (q, f)

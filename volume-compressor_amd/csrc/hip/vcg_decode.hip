@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 #include "vcg_dev.hpp"
+#include "vcg_xform.hpp"
 
 const uint8_t *vcg_arc_buf(const vcg_arc *a);
 const vcg_geom *vcg_arc_geom_dev(const vcg_arc *a);
@@ -55,38 +56,6 @@ __device__ __forceinline__ void item_atom(const dec_job &j, uint32_t gi, int &lo
     }
 }
 
-// ---------------------------------------------------------------------------
-// range decoder (src/vc/vc.c:403-441).  One renormalisation step per bin is
-// exact: after a context bin range >= 2^24*31/4096 and after a bypass
-// range >= 2^23, so a single byte shift restores range >= 2^24.
-// ---------------------------------------------------------------------------
-struct rdec {
-    const uint8_t *s;
-    uint32_t n, pos, code, range;
-};
-__device__ __forceinline__ uint32_t rd_next(rdec &d) {
-    uint32_t b = d.pos < d.n ? (uint32_t)d.s[d.pos] : 0u;
-    d.pos++;
-    return b;
-}
-__device__ __forceinline__ int rd_bit(rdec &d, uint32_t &p) {
-    uint32_t r0 = (uint32_t)(((uint64_t)d.range * p) >> 12);
-    int b;
-    if (d.code < r0) { d.range = r0; p += (4096u - p) >> 5; b = 0; }
-    else { d.code -= r0; d.range -= r0; p -= p >> 5; b = 1; }
-    if (d.range < (1u << 24)) { d.code = (d.code << 8) | rd_next(d); d.range <<= 8; }
-    return b;
-}
-__device__ __forceinline__ int rd_bypass(rdec &d) {
-    d.range >>= 1;
-    int b = d.code >= d.range;
-    if (b) d.code -= d.range;
-    if (d.range < (1u << 24)) { d.code = (d.code << 8) | rd_next(d); d.range <<= 8; }
-    return b;
-}
-
-#define EG_PREFIX_CAP 64u
-
 __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__restrict__ T,
                                                 uint32_t *__restrict__ lists,
                                                 vcg_item *__restrict__ items,
@@ -116,54 +85,8 @@ __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__res
     }
     const uint8_t *p = j.arc + off;
     it.dc = p[0];
-    rdec d;
-    d.s = p + 1; d.n = len - 1; d.pos = 0; d.code = 0; d.range = 0xFFFFFFFFu;
-    for (int k = 0; k < 5; ++k) d.code = (d.code << 8) | rd_next(d);     // vc.c:406-409
-    uint32_t eob = 0;
-    for (int k = 0; k < 16; ++k) eob = (eob << 1) | (uint32_t)rd_bypass(d);   // vc.c:520-525
-    if (eob > VCG_A3) eob = VCG_A3;                                          // vc.c:549
-    uint32_t *out = lists + (size_t)i * VCG_LIST_CAP;
-    uint32_t psig = 2048u;
-    uint32_t m0 = 2048u, m1 = 2048u, m2 = 2048u, m3 = 2048u, m4 = 2048u, m5 = 2048u,
-             m6 = 2048u, m7 = 2048u, m8 = 2048u, m9 = 2048u, m10 = 2048u, m11 = 2048u;
-    int band = 0;
-    uint32_t nextb = T->band_start[1];
     uint32_t cnt = 0;
-    int corrupt = 0;
-    for (uint32_t pp = 0; pp < eob; ++pp) {
-        // sig contexts are used band by band in increasing scan position, so
-        // one live context suffices: a new band starts from the reset value.
-        while (pp >= nextb) { ++band; nextb = T->band_start[band + 1]; psig = 2048u; }
-        if (!rd_bit(d, psig)) continue;
-        // magnitude: unary over mag[0..10], mag[11] escape, EG0 bypass (vc.c:491-506)
-        uint32_t m;
-        if (!rd_bit(d, m0)) m = 1;
-        else if (!rd_bit(d, m1)) m = 2;
-        else if (!rd_bit(d, m2)) m = 3;
-        else if (!rd_bit(d, m3)) m = 4;
-        else if (!rd_bit(d, m4)) m = 5;
-        else if (!rd_bit(d, m5)) m = 6;
-        else if (!rd_bit(d, m6)) m = 7;
-        else if (!rd_bit(d, m7)) m = 8;
-        else if (!rd_bit(d, m8)) m = 9;
-        else if (!rd_bit(d, m9)) m = 10;
-        else if (!rd_bit(d, m10)) m = 11;
-        else if (!rd_bit(d, m11)) m = 12;
-        else {
-            uint32_t nb = 0;
-            while (rd_bypass(d)) {
-                if (++nb > EG_PREFIX_CAP) { corrupt = 1; break; }
-            }
-            if (corrupt) break;
-            uint32_t x = 1;
-            for (uint32_t k = 0; k < nb; ++k) x = (x << 1) | (uint32_t)rd_bypass(d);
-            m = 11u + (x - 1u) + 1u;
-        }
-        int neg = rd_bypass(d);
-        int32_t lv = neg ? -(int32_t)m : (int32_t)m;
-        out[cnt++] = (pp << 16) | ((uint32_t)lv & 0xFFFFu);
-    }
-    if (corrupt) {
+    if (vcg_rc_decode_atom(p + 1, len - 1, T->band_start, lists + (size_t)i * VCG_LIST_CAP, &cnt)) {
         it.status = 2; it.cnt = 0;
         atomicOr(status, 1u);
     } else {
@@ -173,90 +96,9 @@ __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__res
 }
 
 // ---------------------------------------------------------------------------
-// inverse transform
+// inverse transform: phases from vcg_xform.hpp separated by barriers
 // ---------------------------------------------------------------------------
-#define PADW 33
-#define LDS_INTS (32 * 32 * PADW)
 #define IDCT_THREADS 256
-
-__device__ __forceinline__ int pad_idx(int z, int y, int x) { return (z * 32 + y) * PADW + x; }
-
-// FAST: every operand is known to fit in signed 24 bits (see the L1 bound in
-// k_idct), so v_mad_i32_i24 gives the exact low 32 bits of each product.
-template <bool FAST>
-__device__ __forceinline__ int mac(int a, int c) {
-    if constexpr (FAST) return __mul24(a, c);
-    else return (int)((uint32_t)a * (uint32_t)c);
-}
-
-// One inverse line read from LDS (element k at src[k*stride]) with only the
-// first nk coefficients nonzero (nk even, wave-uniform):
-//   evn[n] = sum_{k even} C[k][n] v[k],  odd[n] = sum_{k odd} C[k][n] v[k]
-//   o[n] = (evn+odd+8192)>>14,  o[31-n] = (evn-odd+8192)>>14      (i32 wrap)
-// src/vc/vc.c:178-184.  The k loop is kept rolled so each step loads one
-// basis row pair into SGPRs instead of keeping all 512 constants live.
-template <bool FAST>
-__device__ __forceinline__ void inv_line(const int32_t (*__restrict__ C)[32], const int *src, int stride,
-                                         int nk, int *o) {
-    int evn[16], odd[16];
-#pragma unroll
-    for (int n = 0; n < 16; ++n) { evn[n] = 0; odd[n] = 0; }
-#pragma unroll 1
-    for (int k = 0; k < nk; k += 2) {
-        const int ve = src[k * stride], vo = src[(k + 1) * stride];
-#pragma unroll
-        for (int n = 0; n < 16; ++n) {
-            evn[n] += mac<FAST>(C[k][n], ve);
-            odd[n] += mac<FAST>(C[k + 1][n], vo);
-        }
-    }
-#pragma unroll
-    for (int n = 0; n < 16; ++n) {
-        o[n] = (int)((uint32_t)evn[n] + (uint32_t)odd[n] + 8192u) >> 14;
-        o[31 - n] = (int)((uint32_t)evn[n] - (uint32_t)odd[n] + 8192u) >> 14;
-    }
-}
-
-// wave-uniform even term count covering every lane's nonzero range [0, kmax]
-__device__ __forceinline__ int uniform_nk(int kmax) {
-    int m = wave_max_i32(kmax);
-    m = m < 0 ? 0 : (m > 31 ? 32 : ((m + 2) & ~1));
-    return __builtin_amdgcn_readfirstlane(m);
-}
-
-template <bool FAST>
-__device__ void idct_passes(int *buf, const int32_t (*__restrict__ C)[32], int fmax, int tid) {
-    // pass 1: along x, lines (kz, ky); coefficients beyond kx = fmax-kz-ky are zero
-    for (int line = tid; line < 1024; line += IDCT_THREADS) {
-        int cz = line >> 5, cy = line & 31;
-        int kmax = fmax - cz - cy;
-        int nk = uniform_nk(kmax);
-        if (nk == 0) continue;
-        int *row = buf + pad_idx(cz, cy, 0);
-        int o[32];
-        inv_line<FAST>(C, row, 1, nk, o);
-        if (kmax >= 0) {
-#pragma unroll
-            for (int k = 0; k < 32; ++k) row[k] = o[k];
-        }
-    }
-    __syncthreads();
-    // pass 2: along y, lines (kz, x); ky > fmax-kz are zero
-    for (int line = tid; line < 1024; line += IDCT_THREADS) {
-        int cz = line >> 5, x = line & 31;
-        int kmax = fmax - cz;
-        int nk = uniform_nk(kmax);
-        if (nk == 0) continue;
-        int *col = buf + pad_idx(cz, 0, x);
-        int o[32];
-        inv_line<FAST>(C, col, 32 * PADW, nk, o);
-        if (kmax >= 0) {
-#pragma unroll
-            for (int k = 0; k < 32; ++k) col[k * 32 * PADW] = o[k];
-        }
-    }
-    __syncthreads();
-}
 
 __device__ __forceinline__ void store_voxel(const dec_job &j, uint32_t gi, uint32_t ax, uint32_t ay,
                                             uint32_t az, int z, int y, int x, uint8_t v) {
@@ -287,60 +129,45 @@ __global__ __launch_bounds__(IDCT_THREADS) void k_idct(dec_job j, const vcg_dtab
             store_voxel(j, gi, ax, ay, az, v >> 10, (v >> 5) & 31, v & 31, 0);
         return;
     }
-    for (int k = tid; k < LDS_INTS / 4; k += IDCT_THREADS) ((int4 *)buf)[k] = make_int4(0, 0, 0, 0);
+    vcg_tile_zero(buf, tid, IDCT_THREADS);
     if (tid == 0) l1sum = 0;
     __syncthreads();
     const uint32_t cnt = it.cnt;
     const uint32_t *L = lists + (size_t)i * VCG_LIST_CAP;
-    int fmax = -1;
-    if (cnt) {
-        uint32_t idx = T->scan[L[cnt - 1] >> 16];
-        fmax = (int)((idx >> 10) + ((idx >> 5) & 31) + (idx & 31));
-    }
+    // entries are in increasing scan position, so the last has the highest frequency
+    const int fmax = cnt ? vcg_freq(T->scan[L[cnt - 1] >> 16]) : -1;
     const int32_t *st = j.dqs + (j.mode == MODE_PAYLOAD ? 0 : (int)it.lod * VCG_NFREQ);
     const int32_t *bi = j.dqb + (j.mode == MODE_PAYLOAD ? 0 : (int)it.lod * VCG_NFREQ);
-    unsigned long long mine = 0;
-    for (uint32_t e = tid; e < cnt; e += IDCT_THREADS) {
-        uint32_t v = L[e];
-        int lv = (int)(int16_t)(v & 0xFFFFu);
-        uint32_t idx = T->scan[v >> 16];
-        int f = (int)((idx >> 10) + ((idx >> 5) & 31) + (idx & 31));
-        uint32_t mag = (uint32_t)(lv < 0 ? -lv : lv);
-        int r = (int)(mag * (uint32_t)st[f] + (uint32_t)bi[f] + 128u) >> 8;   // vc.c:334
-        r = lv < 0 ? (int)(0u - (uint32_t)r) : r;
-        buf[pad_idx(idx >> 10, (idx >> 5) & 31, idx & 31)] = r;
-        mine += (unsigned long long)(r < 0 ? -(long long)r : (long long)r);
-    }
+    const unsigned long long mine = vcg_scatter(buf, L, cnt, T->scan, st, bi, tid, IDCT_THREADS);
     if (mine) atomicAdd(&l1sum, mine);
     __syncthreads();
     // Every intermediate element is bounded by the atom's L2 norm (each 1-D
     // pass is orthonormal up to Q14 rounding, sigma_max <= 1.001, plus <= 0.5
     // rounding per element), and L2 <= L1.  With L1 <= 8e6 all MAC operands
-    // stay below 8.02e6 < 2^23.
+    // stay below 8.02e6 < 2^23, so the 24-bit multiply path is exact.
     const bool fast = l1sum <= 8000000ull;
     const int32_t (*C)[32] = T->cmat;
-    if (fast) idct_passes<true>(buf, C, fmax, tid);
-    else idct_passes<false>(buf, C, fmax, tid);
-    // pass 3: along z, lines (y, x); kz > fmax are zero; add dc, clamp, store
+    auto emit = [&](int z, int y, int x, uint8_t v) { store_voxel(j, gi, ax, ay, az, z, y, x, v); };
     const int dc = (int)it.dc;
-    const int nk3 = fmax < 0 ? 0 : (fmax > 31 ? 32 : ((fmax + 2) & ~1));
-    for (int line = tid; line < 1024; line += IDCT_THREADS) {
-        int y = line >> 5, x = line & 31;
-        int o[32];
-        if (fast) inv_line<true>(C, buf + pad_idx(0, y, x), 32 * 32 * PADW, nk3, o);
-        else inv_line<false>(C, buf + pad_idx(0, y, x), 32 * 32 * PADW, nk3, o);
-#pragma unroll
-        for (int z = 0; z < 32; ++z) {
-            int t = o[z] + dc;
-            store_voxel(j, gi, ax, ay, az, z, y, x, (uint8_t)(t < 0 ? 0 : t > 255 ? 255 : t));
-        }
+    if (fast) {
+        vcg_inv_pass_x<true>(buf, C, fmax, tid, IDCT_THREADS);
+        __syncthreads();
+        vcg_inv_pass_y<true>(buf, C, fmax, tid, IDCT_THREADS);
+        __syncthreads();
+        vcg_inv_pass_z<true>(buf, C, fmax, dc, tid, IDCT_THREADS, emit);
+    } else {
+        vcg_inv_pass_x<false>(buf, C, fmax, tid, IDCT_THREADS);
+        __syncthreads();
+        vcg_inv_pass_y<false>(buf, C, fmax, tid, IDCT_THREADS);
+        __syncthreads();
+        vcg_inv_pass_z<false>(buf, C, fmax, dc, tid, IDCT_THREADS, emit);
     }
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-static const size_t LDS_BYTES = (size_t)LDS_INTS * 4;
+static const size_t LDS_BYTES = (size_t)VCG_TILE_INTS * 4;
 static const uint32_t MAX_BATCH = 32768;   // items per launch pair (scratch: 4 GiB)
 
 static void ensure_attr() {

@@ -20,13 +20,10 @@
 #include <string.h>
 #include <vector>
 #include "vcg_dev.hpp"
+#include "vcg_xform.hpp"
 
-#define PADW 33
-#define LDS_INTS (32 * 32 * PADW)
 #define FDCT_THREADS 256
 #define ENC_BATCH 32768u
-
-__device__ __forceinline__ int pidx(int z, int y, int x) { return (z * 32 + y) * PADW + x; }
 
 struct enc_src {
     const uint8_t *lod[VCG_NLOD];
@@ -87,54 +84,10 @@ __global__ void k_downsample(const uint8_t *__restrict__ in, uint32_t nx, uint32
     }
 }
 
-// ---------------------------------------------------------------------------
-// forward transform: one line, elements at src[n*stride].
-//   s[n] = v[n]+v[31-n], d[n] = v[n]-v[31-n]  (n < 16)
-//   out[2j] = (sum_n C[2j][n] s[n] + 8192) >> 14, out[2j+1] likewise with d
-// All operands stay below 2^16 in magnitude (|v| <= 255 in pass 1, <= 2039 in
-// pass 2, <= 16293 in pass 3), so 24-bit multiplies are exact.
-struct fwd_tab { int32_t e[16][16]; int32_t o[16][16]; };   // [n][j] = C[2j][n], C[2j+1][n]
-
-__device__ __forceinline__ void fwd_line(const fwd_tab *__restrict__ F, int *src, int stride) {
-    int ae[16], ao[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) { ae[j] = 0; ao[j] = 0; }
-#pragma unroll 1
-    for (int n = 0; n < 16; ++n) {
-        const int a = src[n * stride], b = src[(31 - n) * stride];
-        const int s = a + b, d = a - b;
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            ae[j] += __mul24(F->e[n][j], s);
-            ao[j] += __mul24(F->o[n][j], d);
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        src[(2 * j) * stride] = (ae[j] + 8192) >> 14;
-        src[(2 * j + 1) * stride] = (ao[j] + 8192) >> 14;
-    }
-}
-
-// dead-zone quantizer, src/vc/vc.c:293-306, evaluated in source order with no
-// contraction (step = base_q*g; dz = 0.8f*step; level = (int)((a-dz)/step+1)).
-__device__ __forceinline__ int quant1(int c, float base_q, float g) {
-#pragma clang fp contract(off)
-    float step = base_q * g;
-    float dz = 0.80f * step;
-    float a = (float)(c < 0 ? -c : c);
-    int level = 0;
-    if (a >= dz) level = (int)((a - dz) / step + 1.0f);
-    if (level > 32767) level = 32767;
-    return c < 0 ? -level : level;
-}
-
-__device__ __forceinline__ int freq_of(uint32_t idx) { return (int)((idx >> 10) + ((idx >> 5) & 31) + (idx & 31)); }
-
 enum { FDCT_STORE_COEF = 0, FDCT_QUANT = 1 };
 
 __global__ __launch_bounds__(FDCT_THREADS) void k_fdct(enc_src s, int mode, const vcg_dtab *__restrict__ T,
-                                                       const fwd_tab *__restrict__ F,
+                                                       const vcg_fwd_tab *__restrict__ F,
                                                        int32_t *__restrict__ coef_out,
                                                        int16_t *__restrict__ lv_out,
                                                        enc_item *__restrict__ items) {
@@ -146,24 +99,9 @@ __global__ __launch_bounds__(FDCT_THREADS) void k_fdct(enc_src s, int mode, cons
     const uint64_t gi = s.base + i;
     int lod; uint32_t ax, ay, az;
     enc_atom_of(s, gi, lod, ax, ay, az);
-    const uint8_t *vol = s.lod[lod];
-    const uint32_t nx = s.nx[lod], ny = s.ny[lod], nz = s.nz[lod];
     if (tid == 0) { ssum = 0; seob = 0; }
     __syncthreads();
-    // gather (zero padded) and sum; thread handles 4 consecutive x of a row
-    uint32_t part = 0;
-    for (int r = tid; r < 32 * 32 * 8; r += FDCT_THREADS) {
-        int z = r >> 8, y = (r >> 3) & 31, x4 = (r & 7) * 4;
-        uint32_t vz = az * 32 + z, vy = ay * 32 + y, vx = ax * 32 + x4;
-        int v[4] = {0, 0, 0, 0};
-        if (vz < nz && vy < ny) {
-            const uint8_t *row = vol + ((uint64_t)vz * ny + vy) * nx;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) if (vx + q < nx) v[q] = row[vx + q];
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) { buf[pidx(z, y, x4 + q)] = v[q]; part += v[q]; }
-    }
+    const uint32_t part = vcg_gather(buf, s.lod[lod], s.nx[lod], s.ny[lod], s.nz[lod], ax, ay, az, tid, FDCT_THREADS);
     if (part) atomicAdd(&ssum, part);
     __syncthreads();
     const uint32_t sum = ssum;
@@ -172,33 +110,21 @@ __global__ __launch_bounds__(FDCT_THREADS) void k_fdct(enc_src s, int mode, cons
         return;
     }
     const int dc = (int)((sum + 16384u) / 32768u);
-    for (int k = tid; k < 32768; k += FDCT_THREADS) buf[pidx(k >> 10, (k >> 5) & 31, k & 31)] -= dc;
+    vcg_level_shift(buf, dc, tid, FDCT_THREADS);
     __syncthreads();
-    for (int line = tid; line < 1024; line += FDCT_THREADS)            // along x
-        fwd_line(F, buf + pidx(line >> 5, line & 31, 0), 1);
+    vcg_fwd_pass_x(buf, F, tid, FDCT_THREADS);
     __syncthreads();
-    for (int line = tid; line < 1024; line += FDCT_THREADS)            // along y
-        fwd_line(F, buf + pidx(line >> 5, 0, line & 31), 32 * PADW);
+    vcg_fwd_pass_y(buf, F, tid, FDCT_THREADS);
     __syncthreads();
-    for (int line = tid; line < 1024; line += FDCT_THREADS)            // along z
-        fwd_line(F, buf + pidx(0, line >> 5, line & 31), 32 * 32 * PADW);
+    vcg_fwd_pass_z(buf, F, tid, FDCT_THREADS);
     __syncthreads();
     if (mode == FDCT_STORE_COEF) {
         int32_t *co = coef_out + (size_t)i * VCG_A3;
-        for (int k = tid; k < 32768; k += FDCT_THREADS) co[k] = buf[pidx(k >> 10, (k >> 5) & 31, k & 31)];
+        for (int k = tid; k < 32768; k += FDCT_THREADS) co[k] = buf[vcg_tidx(k >> 10, (k >> 5) & 31, k & 31)];
         if (tid == 0) { enc_item it = {0, (uint32_t)dc, 0, (uint32_t)lod}; items[i] = it; }
         return;
     }
-    const float q = s.q[lod];
-    int16_t *lo = lv_out + (size_t)i * VCG_A3;
-    int last = 0;
-    for (int p = tid; p < 32768; p += FDCT_THREADS) {
-        uint32_t idx = T->scan[p];
-        int c = buf[pidx(idx >> 10, (idx >> 5) & 31, idx & 31)];
-        int l = quant1(c, q, T->gstep[freq_of(idx)]);
-        lo[p] = (int16_t)l;
-        if (l) last = p + 1;
-    }
+    const int last = vcg_quant_scan(buf, T->scan, T->gstep, s.q[lod], lv_out + (size_t)i * VCG_A3, tid, FDCT_THREADS);
     if (last) atomicMax(&seob, last);
     __syncthreads();
     if (tid == 0) { enc_item it = {(uint32_t)seob, (uint32_t)dc, 0, (uint32_t)lod}; items[i] = it; }
@@ -219,7 +145,7 @@ __global__ __launch_bounds__(FDCT_THREADS) void k_quant(const int32_t *__restric
     int last = 0;
     for (int p = tid; p < 32768; p += FDCT_THREADS) {
         uint32_t idx = T->scan[p];
-        int l = quant1(co[idx], q, T->gstep[freq_of(idx)]);
+        int l = vcg_quant1(co[idx], q, T->gstep[vcg_freq(idx)]);
         lo[p] = (int16_t)l;
         if (l) last = p + 1;
     }
@@ -229,41 +155,7 @@ __global__ __launch_bounds__(FDCT_THREADS) void k_quant(const int32_t *__restric
 }
 
 // ---------------------------------------------------------------------------
-// range encoder (src/vc/vc.c:360-401)
-struct renc {
-    uint8_t *out;
-    uint32_t pos, cap;
-    uint64_t low;
-    uint32_t range;
-    uint32_t cache;
-    uint32_t csize;
-    int ovf;
-};
-__device__ __forceinline__ void re_put(renc &e, uint32_t b) {
-    if (e.pos < e.cap) e.out[e.pos] = (uint8_t)b; else e.ovf = 1;
-    e.pos++;
-}
-__device__ __forceinline__ void re_shift_low(renc &e) {
-    if ((uint32_t)(e.low >> 32) != 0u || e.low < 0xFF000000ull) {
-        uint32_t carry = (uint32_t)(e.low >> 32);
-        do { re_put(e, (e.cache + carry) & 0xFFu); e.cache = 0xFFu; } while (--e.csize);
-        e.cache = (uint32_t)(e.low >> 24) & 0xFFu;
-    }
-    e.csize++;
-    e.low = (e.low << 8) & 0xFFFFFFFFull;
-}
-__device__ __forceinline__ void re_bit(renc &e, uint32_t &p, int bit) {
-    uint32_t r0 = (uint32_t)(((uint64_t)e.range * p) >> 12);
-    if (!bit) { e.range = r0; p += (4096u - p) >> 5; }
-    else { e.low += r0; e.range -= r0; p -= p >> 5; }
-    while (e.range < (1u << 24)) { re_shift_low(e); e.range <<= 8; }
-}
-__device__ __forceinline__ void re_bypass(renc &e, int bit) {
-    e.range >>= 1;
-    if (bit) e.low += e.range;
-    while (e.range < (1u << 24)) { re_shift_low(e); e.range <<= 8; }
-}
-
+// one lane per atom: payload = [dc][range-coder bytes]
 __global__ __launch_bounds__(64) void k_rc_encode(uint32_t n, const int16_t *__restrict__ lv,
                                                   const enc_item *__restrict__ items,
                                                   const vcg_dtab *__restrict__ T,
@@ -275,45 +167,10 @@ __global__ __launch_bounds__(64) void k_rc_encode(uint32_t n, const int16_t *__r
     if (it.zero) { lens[i] = 0; return; }
     uint8_t *out = pay + (size_t)i * pay_cap;
     out[0] = (uint8_t)it.dc;
-    renc e;
-    e.out = out + 1; e.pos = 0; e.cap = pay_cap - 1 < VCG_RC_CAP ? pay_cap - 1 : VCG_RC_CAP;
-    e.low = 0; e.range = 0xFFFFFFFFu; e.cache = 0; e.csize = 1; e.ovf = 0;
-    const int16_t *L = lv + (size_t)i * VCG_A3;
-    const uint32_t eob = it.eob;
-    for (int b = 15; b >= 0; --b) re_bypass(e, (eob >> b) & 1);          // vc.c:513-519
-    uint32_t mg[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) mg[k] = 2048u;
-    uint32_t psig = 2048u;
-    int band = 0;
-    uint32_t nextb = T->band_start[1];
-    for (uint32_t p = 0; p < eob; ++p) {
-        while (p >= nextb) { ++band; nextb = T->band_start[band + 1]; psig = 2048u; }
-        const int v = L[p];
-        if (v == 0) { re_bit(e, psig, 0); continue; }
-        re_bit(e, psig, 1);
-        const uint32_t m = (uint32_t)(v < 0 ? -v : v);
-        const uint32_t mm = m - 1;        // ones before the terminating zero (vc.c:471-490)
-#pragma unroll
-        for (int k = 0; k < 11; ++k) {
-            if (mm > (uint32_t)k) re_bit(e, mg[k], 1);
-            else { re_bit(e, mg[k], 0); break; }
-        }
-        if (mm >= 11) {
-            if (mm == 11) re_bit(e, mg[11], 0);
-            else {
-                re_bit(e, mg[11], 1);
-                const uint32_t t = mm - 11 + 1;           // EG0 of r = m-12
-                const int nb = 31 - __clz((int)t);
-                for (int b = 0; b < nb; ++b) re_bypass(e, 1);
-                re_bypass(e, 0);
-                for (int b = nb - 1; b >= 0; --b) re_bypass(e, (t >> b) & 1);
-            }
-        }
-        re_bypass(e, v < 0);
-    }
-    for (int k = 0; k < 5; ++k) re_shift_low(e);                        // vc.c:399-401
-    lens[i] = e.ovf ? 0xFFFFFFFFu : e.pos + 1;
+    const uint32_t cap = pay_cap - 1 < VCG_RC_CAP ? pay_cap - 1 : VCG_RC_CAP;
+    int ovf = 0;
+    uint32_t len = vcg_rc_encode_atom(lv + (size_t)i * VCG_A3, it.eob, T->band_start, out + 1, cap, &ovf);
+    lens[i] = ovf ? 0xFFFFFFFFu : len + 1;
 }
 
 // copy payload i (len at lens[i]) from batch scratch into the pool at pool_off[i]
@@ -357,15 +214,15 @@ __global__ void k_assemble(const uint8_t *__restrict__ pool, const uint64_t *__r
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-static fwd_tab *g_ftab = nullptr;
-static const size_t LDS_BYTES = (size_t)LDS_INTS * 4;
+static vcg_fwd_tab *g_ftab = nullptr;
+static const size_t LDS_BYTES = (size_t)VCG_TILE_INTS * 4;
 
 static void enc_setup() {
     static bool done = false;
     if (done) return;
     vcg_dtab h;
     VCG_CHECK(hipMemcpy(&h, g_dtab, sizeof h, hipMemcpyDeviceToHost));
-    fwd_tab f;
+    vcg_fwd_tab f;
     for (int n = 0; n < 16; ++n)
         for (int j = 0; j < 16; ++j) { f.e[n][j] = h.cmat[2 * j][n]; f.o[n][j] = h.cmat[2 * j + 1][n]; }
     VCG_CHECK(hipMalloc(&g_ftab, sizeof f));

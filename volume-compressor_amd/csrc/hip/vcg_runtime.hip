@@ -4,6 +4,7 @@
 #include <mutex>
 #include <vector>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 #include "vcg_dev.hpp"
 
@@ -23,6 +24,22 @@ extern "C" int vcg_init(const vcg_tables *t) {
         int n = 0;
         if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) { g_init_rc = -1; return; }
         VCG_CHECK(hipSetDevice(0));
+        // the transform kernels stage one atom as a 32x32x33 i32 tile (132 KiB + a
+        // few shared words); gfx950 provides 160 KiB of LDS per workgroup
+        int lds_block = 0, lds_optin = 0, lds_cu = 0;
+        VCG_CHECK(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, 0));
+        (void)hipDeviceGetAttribute(&lds_optin, hipDeviceAttributeSharedMemPerBlockOptin, 0);
+        (void)hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, 0);
+        (void)hipGetLastError();
+        const int need = 32 * 32 * 33 * 4 + 64;
+        if (getenv("VC_VERBOSE"))
+            fprintf(stderr, "vcg: LDS per block %d, opt-in %d, per CU %d, need %d\n", lds_block, lds_optin, lds_cu, need);
+        if ((lds_block > lds_optin ? lds_block : lds_optin) < need) {
+            char msg[160];
+            snprintf(msg, sizeof msg, "device LDS per workgroup %d/%d bytes < %d required by the transform kernels",
+                     lds_block, lds_optin, need);
+            vc__panic(msg);
+        }
         vcg_dtab h;
         memcpy(h.cmat, t->cmat, sizeof h.cmat);
         memcpy(h.gstep, t->gstep, sizeof h.gstep);

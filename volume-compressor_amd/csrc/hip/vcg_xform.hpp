@@ -1,0 +1,230 @@
+// vcg_xform.hpp — the per-workgroup phases of the forward and inverse DCT-32
+// kernels, written as __host__ __device__ functions of (tid, nthreads) over a
+// padded 32x32x(32+1) i32 tile.  The HIP kernels call them between barriers;
+// the CPU build (tests/kernel_cpu) runs the same phases thread by thread, so
+// the exact kernel logic is checked against the oracle on the host.
+//
+// Tile layout: element (z, y, x) at (z*32 + y)*33 + x (x stride 1, y stride 33,
+// z stride 1056); the +1 pad spreads the y/z-strided line accesses across LDS
+// banks.
+#pragma once
+#include <stdint.h>
+#include "vcg_codec.hpp"
+
+#define VCG_PADW 33
+#define VCG_TILE_INTS (32 * 32 * VCG_PADW)
+#define VCG_SX 1
+#define VCG_SY VCG_PADW
+#define VCG_SZ (32 * VCG_PADW)
+
+VCG_HD int vcg_tidx(int z, int y, int x) { return (z * 32 + y) * VCG_PADW + x; }
+VCG_HD int vcg_freq(uint32_t idx) { return (int)((idx >> 10) + ((idx >> 5) & 31) + (idx & 31)); }
+
+#if defined(__HIP_DEVICE_COMPILE__)
+VCG_HD int vcg_mul24(int a, int b) { return __mul24(a, b); }
+// even term count covering every lane of the wave, made provably uniform
+VCG_HD int vcg_nk(int kmax) {
+    int m = kmax;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+    m = m < 0 ? 0 : (m > 31 ? 32 : ((m + 2) & ~1));
+    return __builtin_amdgcn_readfirstlane(m);
+}
+#else
+// signed 24-bit multiply: low 32 bits of sext24(a) * sext24(b)
+VCG_HD int vcg_mul24(int a, int b) {
+    int64_t x = (int64_t)(int32_t)((uint32_t)a << 8) >> 8, y = (int64_t)(int32_t)((uint32_t)b << 8) >> 8;
+    return (int)(uint32_t)(uint64_t)(x * y);
+}
+VCG_HD int vcg_nk(int kmax) { return kmax < 0 ? 0 : (kmax > 31 ? 32 : ((kmax + 2) & ~1)); }
+#endif
+
+template <bool FAST>
+VCG_HD int vcg_mac(int a, int c) {
+    if constexpr (FAST) return vcg_mul24(a, c);
+    else return (int)((uint32_t)a * (uint32_t)c);
+}
+
+// ===========================================================================
+// inverse
+// ===========================================================================
+// One inverse line read from the tile (element k at src[k*stride]) with only
+// the first nk coefficients possibly nonzero (nk even):
+//   evn[n] = sum_{k even} C[k][n] v[k],  odd[n] = sum_{k odd} C[k][n] v[k]
+//   o[n] = (evn+odd+8192)>>14,  o[31-n] = (evn-odd+8192)>>14     (i32 wrap)
+// src/vc/vc.c:178-184.  The k loop stays rolled so each step loads one basis
+// row pair instead of keeping all 512 constants live.
+template <bool FAST>
+VCG_HD void vcg_inv_line(const int32_t (*C)[32], const int *src, int stride, int nk, int *o) {
+    int evn[16], odd[16];
+#pragma unroll
+    for (int n = 0; n < 16; ++n) { evn[n] = 0; odd[n] = 0; }
+#pragma unroll 1
+    for (int k = 0; k < nk; k += 2) {
+        const int ve = src[k * stride], vo = src[(k + 1) * stride];
+#pragma unroll
+        for (int n = 0; n < 16; ++n) {
+            evn[n] += vcg_mac<FAST>(C[k][n], ve);
+            odd[n] += vcg_mac<FAST>(C[k + 1][n], vo);
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < 16; ++n) {
+        o[n] = (int)((uint32_t)evn[n] + (uint32_t)odd[n] + 8192u) >> 14;
+        o[31 - n] = (int)((uint32_t)evn[n] - (uint32_t)odd[n] + 8192u) >> 14;
+    }
+}
+
+VCG_HD void vcg_tile_zero(int *t, int tid, int nthr) {
+    for (int k = tid; k < VCG_TILE_INTS; k += nthr) t[k] = 0;
+}
+
+// Dequantize entries [tid, cnt) step nthr into the tile (src/vc/vc.c:327-338);
+// returns this thread's sum of |coef| (for the 24-bit bound).
+VCG_HD uint64_t vcg_scatter(int *t, const uint32_t *L, uint32_t cnt, const uint16_t *scan, const int32_t *st,
+                            const int32_t *bi, int tid, int nthr) {
+    uint64_t mine = 0;
+    for (uint32_t e = (uint32_t)tid; e < cnt; e += (uint32_t)nthr) {
+        const uint32_t v = L[e];
+        const int lv = (int)(int16_t)(v & 0xFFFFu);
+        const uint32_t idx = scan[v >> 16];
+        const int f = vcg_freq(idx);
+        const uint32_t mag = (uint32_t)(lv < 0 ? -lv : lv);
+        int r = (int)(mag * (uint32_t)st[f] + (uint32_t)bi[f] + 128u) >> 8;
+        r = lv < 0 ? (int)(0u - (uint32_t)r) : r;
+        t[vcg_tidx((int)(idx >> 10), (int)((idx >> 5) & 31), (int)(idx & 31))] = r;
+        mine += (uint64_t)(r < 0 ? -(int64_t)r : (int64_t)r);
+    }
+    return mine;
+}
+
+// pass 1 (along x): lines (kz, ky); coefficients beyond kx = fmax-kz-ky are zero
+template <bool FAST>
+VCG_HD void vcg_inv_pass_x(int *t, const int32_t (*C)[32], int fmax, int tid, int nthr) {
+    for (int line = tid; line < 1024; line += nthr) {
+        const int cz = line >> 5, cy = line & 31;
+        const int kmax = fmax - cz - cy;
+        const int nk = vcg_nk(kmax);
+        if (nk == 0) continue;
+        int *row = t + vcg_tidx(cz, cy, 0);
+        int o[32];
+        vcg_inv_line<FAST>(C, row, VCG_SX, nk, o);
+        if (kmax >= 0) {
+#pragma unroll
+            for (int k = 0; k < 32; ++k) row[k * VCG_SX] = o[k];
+        }
+    }
+}
+// pass 2 (along y): lines (kz, x); ky beyond fmax-kz are zero
+template <bool FAST>
+VCG_HD void vcg_inv_pass_y(int *t, const int32_t (*C)[32], int fmax, int tid, int nthr) {
+    for (int line = tid; line < 1024; line += nthr) {
+        const int cz = line >> 5, x = line & 31;
+        const int kmax = fmax - cz;
+        const int nk = vcg_nk(kmax);
+        if (nk == 0) continue;
+        int *col = t + vcg_tidx(cz, 0, x);
+        int o[32];
+        vcg_inv_line<FAST>(C, col, VCG_SY, nk, o);
+        if (kmax >= 0) {
+#pragma unroll
+            for (int k = 0; k < 32; ++k) col[k * VCG_SY] = o[k];
+        }
+    }
+}
+// pass 3 (along z): line (y, x) -> 32 output voxels (dc added, clamped) via emit(z, y, x, v)
+template <bool FAST, class Emit>
+VCG_HD void vcg_inv_pass_z(const int *t, const int32_t (*C)[32], int fmax, int dc, int tid, int nthr, Emit emit) {
+    const int nk = fmax < 0 ? 0 : (fmax > 31 ? 32 : ((fmax + 2) & ~1));
+    for (int line = tid; line < 1024; line += nthr) {
+        const int y = line >> 5, x = line & 31;
+        int o[32];
+        vcg_inv_line<FAST>(C, t + vcg_tidx(0, y, x), VCG_SZ, nk, o);
+#pragma unroll
+        for (int z = 0; z < 32; ++z) {
+            const int v = o[z] + dc;
+            emit(z, y, x, (uint8_t)(v < 0 ? 0 : v > 255 ? 255 : v));
+        }
+    }
+}
+
+// ===========================================================================
+// forward
+// ===========================================================================
+// Basis in forward order: E[n][j] = C[2j][n], O[n][j] = C[2j+1][n].
+struct vcg_fwd_tab { int32_t e[16][16]; int32_t o[16][16]; };
+
+// One forward line (element n at src[n*stride]):
+//   s[n] = v[n]+v[31-n], d[n] = v[n]-v[31-n]  (n < 16)
+//   out[2j] = (sum_n C[2j][n] s[n] + 8192) >> 14, out[2j+1] likewise with d
+// (src/vc/vc.c:148-169).  Operands stay below 2^16 in magnitude (|v| <= 255 in
+// pass 1, <= 2039 in pass 2, <= 16293 in pass 3), so 24-bit multiplies are exact.
+VCG_HD void vcg_fwd_line(const vcg_fwd_tab *F, int *src, int stride) {
+    int ae[16], ao[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) { ae[j] = 0; ao[j] = 0; }
+#pragma unroll 1
+    for (int n = 0; n < 16; ++n) {
+        const int a = src[n * stride], b = src[(31 - n) * stride];
+        const int s = a + b, d = a - b;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            ae[j] += vcg_mul24(F->e[n][j], s);
+            ao[j] += vcg_mul24(F->o[n][j], d);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        src[(2 * j) * stride] = (int)((uint32_t)ae[j] + 8192u) >> 14;
+        src[(2 * j + 1) * stride] = (int)((uint32_t)ao[j] + 8192u) >> 14;
+    }
+}
+
+VCG_HD void vcg_fwd_pass_x(int *t, const vcg_fwd_tab *F, int tid, int nthr) {
+    for (int line = tid; line < 1024; line += nthr) vcg_fwd_line(F, t + vcg_tidx(line >> 5, line & 31, 0), VCG_SX);
+}
+VCG_HD void vcg_fwd_pass_y(int *t, const vcg_fwd_tab *F, int tid, int nthr) {
+    for (int line = tid; line < 1024; line += nthr) vcg_fwd_line(F, t + vcg_tidx(line >> 5, 0, line & 31), VCG_SY);
+}
+VCG_HD void vcg_fwd_pass_z(int *t, const vcg_fwd_tab *F, int tid, int nthr) {
+    for (int line = tid; line < 1024; line += nthr) vcg_fwd_line(F, t + vcg_tidx(0, line >> 5, line & 31), VCG_SZ);
+}
+
+// gather one zero-padded atom of an (nx,ny,nz) volume into the tile; returns
+// this thread's voxel sum.  Thread handles runs of 4 consecutive x.
+VCG_HD uint32_t vcg_gather(int *t, const uint8_t *vol, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t ax,
+                           uint32_t ay, uint32_t az, int tid, int nthr) {
+    uint32_t part = 0;
+    for (int r = tid; r < 32 * 32 * 8; r += nthr) {
+        const int z = r >> 8, y = (r >> 3) & 31, x4 = (r & 7) * 4;
+        const uint32_t vz = az * 32 + z, vy = ay * 32 + y, vx = ax * 32 + x4;
+        int v[4] = {0, 0, 0, 0};
+        if (vz < nz && vy < ny) {
+            const uint8_t *row = vol + ((uint64_t)vz * ny + vy) * nx;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) if (vx + q < nx) v[q] = row[vx + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { t[vcg_tidx(z, y, x4 + q)] = v[q]; part += (uint32_t)v[q]; }
+    }
+    return part;
+}
+
+VCG_HD void vcg_level_shift(int *t, int dc, int tid, int nthr) {
+    for (int k = tid; k < 32768; k += nthr) t[vcg_tidx(k >> 10, (k >> 5) & 31, k & 31)] -= dc;
+}
+
+// quantize the tile's coefficients in scan order into lv[p]; returns this
+// thread's max (p+1) over nonzero levels (the EOB candidate)
+VCG_HD int vcg_quant_scan(const int *t, const uint16_t *scan, const float *gstep, float q, int16_t *lv, int tid,
+                          int nthr) {
+    int last = 0;
+    for (int p = tid; p < 32768; p += nthr) {
+        const uint32_t idx = scan[p];
+        const int c = t[vcg_tidx((int)(idx >> 10), (int)((idx >> 5) & 31), (int)(idx & 31))];
+        const int l = vcg_quant1(c, q, gstep[vcg_freq(idx)]);
+        lv[p] = (int16_t)l;
+        if (l) last = p + 1;
+    }
+    return last;
+}
