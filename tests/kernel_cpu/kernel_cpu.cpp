@@ -18,6 +18,18 @@ uint32_t kc_encode(const int16_t *L, uint32_t eob, const uint32_t *band_start, u
 int kc_decode(const uint8_t *s, uint32_t n, const uint32_t *band_start, uint32_t *out, uint32_t *cnt) {
     return vcg_rc_decode_atom(s, n, band_start, out, cnt);
 }
+// the kernels' state-machine decoder; the stream is copied to offset `mis` of
+// an 8-aligned, zero-padded buffer (as in the padded device archive)
+int kc_decode_fsm(const uint8_t *s, uint32_t n, uint32_t mis, const uint32_t *band_start, uint32_t *out,
+                  uint32_t *cnt) {
+    size_t words = (mis + n + 7) / 8 + 4;
+    uint64_t *buf = (uint64_t *)calloc(words, 8);
+    memset(buf, 0xA5, words * 8);                 // garbage around the stream must be masked
+    memcpy((uint8_t *)buf + mis, s, n);
+    int r = vcg_rc_decode_atom_fsm(buf, mis, n, band_start, out, cnt);
+    free(buf);
+    return r;
+}
 void kc_quant(const int32_t *c, size_t n, float q, float g, int32_t *out) {
     for (size_t i = 0; i < n; ++i) out[i] = vcg_quant1(c[i], q, g);
 }

@@ -247,14 +247,17 @@ def test_writer_matches_reference_file(pyvc, oracle, golden_containers):
                             s = v[az * 32:az * 32 + 32, ay * 32:ay * 32 + 32, ax * 32:ax * 32 + 32]
                             at[:s.shape[0], :s.shape[1], :s.shape[2]] = s
                             assert w.append_atom(lod, az, ay, ax, at) == 0
-            # read-back before close
-            st, rb = w.decode_atom(0, 0, 0, 1)
+            # read-back before close: same bytes the reader decodes after close
+            st, rb = w.decode_atom(0, 0, 0, 0)
             assert st == 0
+            assert w.decode_atom(0, 0, 0, (nx + 31) // 32)[0] == 2     # out of the atom grid
             w.close()
             got = open(path, "rb").read()
         finally:
             os.unlink(path)
         assert got == b["v1"], name
+        o = oracle.open(got)
+        assert np.array_equal(rb, o.decode_atom(0, 0, 0, 0)[1])
 
 
 def test_writer_box_zero_and_readback(pyvc, oracle):

@@ -28,6 +28,7 @@ def kc():
     L.kc_encode.restype = C.c_uint32
     L.kc_encode.argtypes = [vp, C.c_uint32, vp, vp, C.c_uint32, C.POINTER(C.c_int)]
     L.kc_decode.argtypes = [vp, C.c_uint32, vp, vp, C.POINTER(C.c_uint32)]
+    L.kc_decode_fsm.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp, C.POINTER(C.c_uint32)]
     L.kc_fdct.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                           vp, vp, vp, C.c_float, vp, vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     L.kc_idct.argtypes = [vp, C.c_uint32, C.c_int, vp, vp, vp, vp, C.c_int, vp]
@@ -161,3 +162,38 @@ def test_kernel_quantizer_exhaustive(kc, tabs, oracle):
         for f in range(94):
             kc.kc_quant(c.ctypes.data, c.size, C.c_float(q), C.c_float(float(gs[f])), out.ctypes.data)
             assert np.array_equal(out, oracle.quant_levels(q, f, c)), (q, f)
+
+
+def fsm_decode(kc, tabs, rc, mis):
+    cm, gs, scan, bs = tabs
+    b = np.frombuffer(rc, np.uint8) if len(rc) else np.zeros(1, np.uint8)
+    ent = np.zeros(32768, np.uint32)
+    cnt = C.c_uint32()
+    st = kc.kc_decode_fsm(b.ctypes.data, len(rc), mis, bs.ctypes.data, ent.ctypes.data, C.byref(cnt))
+    return st, ent[:cnt.value]
+
+
+def loop_decode(kc, tabs, rc):
+    cm, gs, scan, bs = tabs
+    b = np.frombuffer(rc, np.uint8) if len(rc) else np.zeros(1, np.uint8)
+    ent = np.zeros(32768, np.uint32)
+    cnt = C.c_uint32()
+    st = kc.kc_decode(b.ctypes.data, len(rc), bs.ctypes.data, ent.ctypes.data, C.byref(cnt))
+    return st, ent[:cnt.value]
+
+
+def test_fsm_decoder_matches(kc, tabs, golden_atoms, golden_corrupt):
+    """The kernels' state-machine decoder gives the same entries as the
+    straight-line decoder at every byte alignment, on reference payloads,
+    damaged payloads and over-long Exp-Golomb prefixes."""
+    from test_oracle import eg_prefix_stream
+    streams = [p[1:] for (_, _, p, _) in payloads_of(golden_atoms) if p]
+    g = golden_corrupt
+    blob = g["blob"].tobytes()
+    streams += [blob[int(g["offs"][k]) + 1:int(g["offs"][k]) + int(g["lens"][k])] for k in range(len(g["lens"]))]
+    streams += [eg_prefix_stream(70), eg_prefix_stream(20), b"", b"\x07", b"\xff" * 9]
+    for rc in streams:
+        ref = loop_decode(kc, tabs, rc)
+        for mis in range(8):
+            got = fsm_decode(kc, tabs, rc, mis)
+            assert got[0] == ref[0] and np.array_equal(got[1], ref[1]), (len(rc), mis)

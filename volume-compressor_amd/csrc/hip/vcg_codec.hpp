@@ -12,8 +12,10 @@
 
 #if defined(__HIPCC__)
 #define VCG_HD __host__ __device__ __forceinline__
+#define VCG_HDM __host__ __device__ __forceinline__
 #else
 #define VCG_HD static inline
+#define VCG_HDM inline
 #endif
 
 #define VCG_EG_PREFIX_CAP 64u
@@ -101,6 +103,175 @@ VCG_HD int vcg_rc_decode_atom(const uint8_t *s, uint32_t n, const uint32_t *band
     }
     *cnt_out = cnt;
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Decoder as a one-bin-per-step state machine (the batched kernels' form).
+//
+// Every lane executes the same instructions each step whatever symbol it is
+// in (significance, magnitude unary, Exp-Golomb prefix/suffix, sign), so a
+// wave of 64 atoms does not serialise on divergent branches.  Input bytes come
+// from a 16-byte little-endian register window; every VCG_WSTEP steps lanes
+// whose window holds <= 8 bytes append the next 8, which were loaded (as an
+// aligned u64) two refills earlier, so global-load latency hides behind the
+// steps in between.  The window never underflows: a step consumes at most one
+// byte (single renormalisation per bin, see vcg_rd_bit).  Magnitude contexts
+// 1..11 live in `mg` (LDS on the GPU); the one a step may need next is read
+// at the start of the step and mag[0] stays in a register.
+//
+// Reads are whole aligned 8-byte blocks covering [s, s+n): the caller
+// guarantees the buffer extends to the next 8-byte boundary past s+n plus 16
+// bytes (the device archive copy is padded); bytes outside [s, s+n) are masked
+// to zero (past-end bytes read as 0, vc.c:425).
+// ---------------------------------------------------------------------------
+#define VCG_WSTEP 8
+#ifndef VCG_ENTROPY_FSM
+#define VCG_ENTROPY_FSM 0
+#endif
+enum { VCG_S_SIG = 0, VCG_S_MAG = 1, VCG_S_EGP = 2, VCG_S_EGS = 3, VCG_S_SIGN = 4, VCG_S_DONE = 5, VCG_S_BAD = 6 };
+
+// aligned 8-byte block `blk` of the stream whose first byte sits `mis` bytes
+// into block 0 and which holds `n` bytes; bytes outside the stream are zero
+VCG_HD uint64_t vcg_fetch8(const uint64_t *base, uint32_t blk, uint32_t mis, uint32_t n) {
+    const int64_t lo = (int64_t)mis - 8 * (int64_t)blk;          // first valid byte index within the block
+    const int64_t hi = (int64_t)mis + n - 8 * (int64_t)blk;      // one past the last
+    if (hi <= 0 || lo >= 8) return 0;
+    uint64_t v = base[blk];
+    if (lo > 0) v &= ~0ull << (8 * lo);
+    if (hi < 8) v &= (1ull << (8 * hi)) - 1ull;
+    return v;
+}
+
+struct vcg_fsm {
+    // range decoder
+    uint32_t code, range;
+    // byte window (little endian, wn valid bytes) and two pending blocks
+    uint64_t wlo, whi, p1, p2;
+    uint32_t wn, blk;
+    // symbol state
+    uint32_t st, pos, eob, k, nb, x, m;
+    uint32_t psig, pm, pnext, m0;
+    uint32_t band, nextb, cnt;
+};
+
+VCG_HD void vcg_win_pop(vcg_fsm &f, uint32_t &byte) {
+    byte = (uint32_t)f.wlo & 0xFFu;
+    f.wlo = (f.wlo >> 8) | (f.whi << 56);
+    f.whi >>= 8;
+    f.wn--;
+}
+// append 8 bytes at byte offset wn (wn <= 8)
+VCG_HD void vcg_win_push8(vcg_fsm &f, uint64_t v) {
+    const uint32_t sh = 8 * f.wn;
+    if (sh == 0) { f.wlo |= v; }
+    else if (sh == 64) { f.whi |= v; }
+    else { f.wlo |= v << sh; f.whi |= v >> (64 - sh); }
+    f.wn += 8;
+}
+
+VCG_HD void vcg_fsm_init(vcg_fsm &f, const uint64_t *base, uint32_t mis, uint32_t n, const uint32_t *band_start) {
+    f.wlo = vcg_fetch8(base, 0, mis, n) >> (8 * mis);
+    f.whi = 0;
+    f.wn = 8 - mis;
+    vcg_win_push8(f, vcg_fetch8(base, 1, mis, n));
+    f.p1 = vcg_fetch8(base, 2, mis, n);
+    f.p2 = vcg_fetch8(base, 3, mis, n);
+    f.blk = 4;
+    f.code = 0; f.range = 0xFFFFFFFFu;
+    uint32_t b;
+    for (int k = 0; k < 5; ++k) { vcg_win_pop(f, b); f.code = (f.code << 8) | b; }   // vc.c:406-409
+    // EOB: 16 bypass bins (vc.c:520-525).  The window holds >= 4 bytes here and
+    // 16 halvings of a full range renormalise at most 3 times.
+    uint32_t eob = 0;
+    for (int k = 0; k < 16; ++k) {
+        f.range >>= 1;
+        uint32_t bit = f.code >= f.range;
+        if (bit) f.code -= f.range;
+        if (f.range < (1u << 24)) { vcg_win_pop(f, b); f.code = (f.code << 8) | b; f.range <<= 8; }
+        eob = (eob << 1) | bit;
+    }
+    f.eob = eob > 32768u ? 32768u : eob;                                   // vc.c:549
+    f.st = f.eob ? VCG_S_SIG : VCG_S_DONE;
+    f.pos = 0; f.k = 0; f.nb = 0; f.x = 0; f.m = 0;
+    f.psig = 2048u; f.pm = 2048u; f.pnext = 2048u; f.m0 = 2048u;
+    f.band = 0; f.nextb = band_start[1]; f.cnt = 0;
+}
+
+// refill (every VCG_WSTEP steps): lanes holding <= 8 bytes append the oldest pending block
+VCG_HD void vcg_fsm_refill(vcg_fsm &f, const uint64_t *base, uint32_t mis, uint32_t n) {
+    if (f.wn <= 8) {
+        vcg_win_push8(f, f.p1);
+        f.p1 = f.p2;
+        f.p2 = vcg_fetch8(base, f.blk, mis, n);
+        f.blk++;
+    }
+}
+
+// one bin of the symbol stream; mg = magnitude contexts (index 1..11 used)
+template <class MagMem>
+VCG_HD void vcg_fsm_step(vcg_fsm &f, MagMem &mg, const uint32_t *band_start, uint32_t *out) {
+    const uint32_t st = f.st;
+    if (st >= VCG_S_DONE) return;
+    // context read for the magnitude bin after this one (used only if this bin is 1)
+    if (st == VCG_S_MAG && f.k + 1 < 12) f.pnext = mg.get(f.k + 1);
+    const bool byp = st >= VCG_S_EGP;
+    const uint32_t p = st == VCG_S_SIG ? f.psig : f.pm;
+    const uint32_t r0 = byp ? (f.range >> 1) : (uint32_t)(((uint64_t)f.range * p) >> 12);
+    const uint32_t bit = f.code >= r0;
+    if (bit) f.code -= r0;
+    f.range = (byp || !bit) ? r0 : f.range - r0;
+    const uint32_t pn = bit ? p - (p >> 5) : p + ((4096u - p) >> 5);
+    if (f.range < (1u << 24)) { uint32_t b; vcg_win_pop(f, b); f.code = (f.code << 8) | b; f.range <<= 8; }
+    if (st == VCG_S_SIG) {
+        f.psig = pn;
+        if (bit) { f.st = VCG_S_MAG; f.k = 0; f.pm = f.m0; return; }
+        f.pos++;
+    } else if (st == VCG_S_MAG) {
+        if (f.k == 0) f.m0 = pn; else mg.set(f.k, pn);
+        if (bit) {
+            f.k++;
+            if (f.k == 12) { f.st = VCG_S_EGP; f.nb = 0; }
+            else f.pm = f.pnext;
+        } else { f.m = f.k + 1; f.st = VCG_S_SIGN; }
+        return;
+    } else if (st == VCG_S_EGP) {
+        if (bit) { if (++f.nb > VCG_EG_PREFIX_CAP) f.st = VCG_S_BAD; }
+        else { f.x = 1; if (f.nb == 0) { f.m = 12; f.st = VCG_S_SIGN; } else f.st = VCG_S_EGS; }
+        return;
+    } else if (st == VCG_S_EGS) {
+        f.x = (f.x << 1) | bit;
+        if (--f.nb == 0) { f.m = 11u + f.x; f.st = VCG_S_SIGN; }     // v + (x-1) + 1 with v = 11
+        return;
+    } else {                                                          // sign bypass: emit
+        const uint32_t lv = bit ? 0u - f.m : f.m;
+        out[f.cnt++] = (f.pos << 16) | (lv & 0xFFFFu);
+        f.pos++;
+        f.st = VCG_S_SIG;
+    }
+    // advanced past a position (zero significance or emitted coefficient)
+    if (f.pos == f.eob) { f.st = VCG_S_DONE; return; }
+    if (f.pos == f.nextb) { f.band++; f.nextb = band_start[f.band + 1]; f.psig = 2048u; }
+}
+
+// Whole-atom decode with the state machine (CPU-side reference of the kernel's
+// per-lane logic).  Returns 0, or 1 for an over-long Exp-Golomb prefix.
+struct vcg_mag_array {
+    uint32_t v[12];
+    VCG_HDM uint32_t get(uint32_t k) const { return v[k]; }
+    VCG_HDM void set(uint32_t k, uint32_t p) { v[k] = p; }
+};
+VCG_HD int vcg_rc_decode_atom_fsm(const uint64_t *base, uint32_t mis, uint32_t n, const uint32_t *band_start,
+                                  uint32_t *out, uint32_t *cnt_out) {
+    vcg_fsm f;
+    vcg_fsm_init(f, base, mis, n, band_start);
+    vcg_mag_array mg;
+    for (int k = 0; k < 12; ++k) mg.v[k] = 2048u;
+    while (f.st < VCG_S_DONE) {
+        vcg_fsm_refill(f, base, mis, n);
+        for (int s = 0; s < VCG_WSTEP; ++s) vcg_fsm_step(f, mg, band_start, out);
+    }
+    *cnt_out = f.st == VCG_S_BAD ? 0 : f.cnt;
+    return f.st == VCG_S_BAD;
 }
 
 // ---------------------------------------------------------------------------

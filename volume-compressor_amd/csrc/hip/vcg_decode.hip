@@ -56,10 +56,19 @@ __device__ __forceinline__ void item_atom(const dec_job &j, uint32_t gi, int &lo
     }
 }
 
+// magnitude contexts 1..11 of the 64 lanes of a wave, k-major (lane-contiguous)
+struct lds_mag {
+    uint32_t *base;
+    uint32_t lane;
+    __device__ __forceinline__ uint32_t get(uint32_t k) const { return base[k * 64 + lane]; }
+    __device__ __forceinline__ void set(uint32_t k, uint32_t p) { base[k * 64 + lane] = p; }
+};
+
 __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__restrict__ T,
                                                 uint32_t *__restrict__ lists,
                                                 vcg_item *__restrict__ items,
                                                 uint32_t *__restrict__ status) {
+    __shared__ uint32_t mgs[12 * 64];
     const uint32_t i = blockIdx.x * 64u + threadIdx.x;
     if (i >= j.n) return;
     const uint32_t gi = j.base + i;
@@ -85,8 +94,30 @@ __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__res
     }
     const uint8_t *p = j.arc + off;
     it.dc = p[0];
+    uint32_t *out = lists + (size_t)i * VCG_LIST_CAP;
+#if VCG_ENTROPY_FSM
+    // range-coder bytes [p+1, p+len): aligned 8-byte blocks (the device buffer is
+    // padded past its end), misalignment mis
+    const uintptr_t sa = (uintptr_t)(p + 1);
+    const uint64_t *blk = (const uint64_t *)(sa & ~(uintptr_t)7);
+    const uint32_t mis = (uint32_t)(sa & 7), n = len - 1;
+    lds_mag mg{mgs, threadIdx.x};
+    for (uint32_t k = 1; k < 12; ++k) mg.set(k, 2048u);
+    vcg_fsm f;
+    vcg_fsm_init(f, blk, mis, n, T->band_start);
+    while (f.st < VCG_S_DONE) {
+        vcg_fsm_refill(f, blk, mis, n);
+#pragma unroll
+        for (int s = 0; s < VCG_WSTEP; ++s) vcg_fsm_step(f, mg, T->band_start, out);
+    }
+    const int bad = f.st == VCG_S_BAD;
+    const uint32_t cnt = f.cnt;
+#else
+    (void)mgs;
     uint32_t cnt = 0;
-    if (vcg_rc_decode_atom(p + 1, len - 1, T->band_start, lists + (size_t)i * VCG_LIST_CAP, &cnt)) {
+    const int bad = vcg_rc_decode_atom(p + 1, len - 1, T->band_start, out, &cnt);
+#endif
+    if (bad) {
         it.status = 2; it.cnt = 0;
         atomicOr(status, 1u);
     } else {
@@ -277,7 +308,8 @@ extern "C" int vcg_decode_payloads(size_t n, const uint8_t *pay, const uint64_t 
     uint64_t total = 0;
     for (size_t i = 0; i < n; ++i) if (offs[i] + lens[i] > total) total = offs[i] + lens[i];
     uint8_t *dpay = nullptr; uint64_t *doffs = nullptr; uint32_t *dlens = nullptr; int32_t *ddq = nullptr;
-    VCG_CHECK(hipMalloc(&dpay, total ? total : 1));
+    VCG_CHECK(hipMalloc(&dpay, total + 64));
+    VCG_CHECK(hipMemsetAsync(dpay + total, 0, 64, c->s));
     VCG_CHECK(hipMalloc(&doffs, n * 8));
     VCG_CHECK(hipMalloc(&dlens, n * 4));
     VCG_CHECK(hipMalloc(&ddq, 2 * VCG_NFREQ * 4));

@@ -143,13 +143,15 @@ extern "C" vcg_arc *vcg_arc_create(const uint8_t *buf, const vcg_geom *g) {
     vcg_ctx *c = vcg_get_ctx();
     vcg_arc *a = new vcg_arc();
     a->host = *g;
-    if (hipMalloc(&a->buf, g->len ? g->len : 1) != hipSuccess ||
+    // padded: the entropy kernel reads whole aligned 8-byte blocks around each payload
+    if (hipMalloc(&a->buf, g->len + 64) != hipSuccess ||
         hipMalloc(&a->geom, sizeof(vcg_geom)) != hipSuccess) {
         (void)hipGetLastError();
         if (a->buf) (void)hipFree(a->buf);
         delete a;
         return nullptr;
     }
+    VCG_CHECK(hipMemsetAsync(a->buf + g->len, 0, 64, c->s));
     VCG_CHECK(hipMemcpyAsync(a->buf, buf, g->len, hipMemcpyHostToDevice, c->s));
     VCG_CHECK(hipMemcpyAsync(a->geom, g, sizeof(vcg_geom), hipMemcpyHostToDevice, c->s));
     VCG_CHECK(hipStreamSynchronize(c->s));
