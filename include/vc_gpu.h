@@ -28,6 +28,10 @@ void     *vcx_dev_alloc(size_t bytes);
 void      vcx_dev_free(void *p);
 vc_status vcx_h2d(void *dst_dev, const void *src_host, size_t bytes);
 vc_status vcx_d2h(void *dst_host, const void *src_dev, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte aligned
+ * pointers, bytes a multiple of 16; else VC_ERR_RANGE): a known-size
+ * reference for hardware byte counters. */
+vc_status vcx_copy_kernel(void *dst_dev, const void *src_dev, size_t bytes);
 
 /* Decode a whole LOD / a voxel box into device memory (raster of the LOD /
  * of the box).  sync=1: wait and return the status like vc_decode_lod.

@@ -123,6 +123,24 @@ extern "C" void vcg_set_stage_events(void *ev[3]) {
 
 __global__ void k_reset_word(uint32_t *w) { *w = 0; }
 
+// plain 16-byte-per-lane copy: the reference point for HBM byte counters
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+extern "C" int vcg_copy_kernel(void *dst, const void *src, size_t bytes) {
+    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16) return -1;
+    vcg_ctx *c = vcg_get_ctx();
+    const size_t n = bytes / 16;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 256 * 64) blocks = 256 * 64;
+    if (!blocks) return 0;
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, c->s, (const uint4 *)src, (uint4 *)dst, n);
+    VCG_CHECK(hipGetLastError());
+    VCG_CHECK(hipStreamSynchronize(c->s));
+    return 0;
+}
+
 extern "C" int vcg_sync(void) {
     vcg_ctx *c = vcg_get_ctx();
     VCG_CHECK(hipMemcpyAsync(c->status_h, c->status_d, 4, hipMemcpyDeviceToHost, c->s));

@@ -11,6 +11,10 @@ void *vcx_dev_alloc(size_t bytes) { vc__require_gpu(); return vcg_alloc(bytes); 
 void vcx_dev_free(void *p) { if (p) vcg_free(p); }
 vc_status vcx_h2d(void *dst, const void *src, size_t n) { vc__require_gpu(); vcg_copy(dst, src, n, 1); return VC_OK; }
 vc_status vcx_d2h(void *dst, const void *src, size_t n) { vc__require_gpu(); vcg_copy(dst, src, n, 2); return VC_OK; }
+vc_status vcx_copy_kernel(void *dst, const void *src, size_t n) {
+    vc__require_gpu();
+    return vcg_copy_kernel(dst, src, n) ? VC_ERR_RANGE : VC_OK;
+}
 vc_status vcx_sync(void) { vc__require_gpu(); return vcg_sync() ? VC_ERR_FORMAT : VC_OK; }
 void vcx_set_stage_events(void *events[3]) { vc__require_gpu(); vcg_set_stage_events(events); }
 

@@ -50,6 +50,7 @@ void *vcg_alloc(size_t bytes);              /* NULL on failure */
 void  vcg_free(void *p);
 int   vcg_copy(void *dst, const void *src, size_t bytes, int kind); /* 1 h2d, 2 d2h, 3 d2d */
 void  vcg_set_stage_events(void *ev[3]);
+int   vcg_copy_kernel(void *dst, const void *src, size_t bytes); /* 16-B aligned, bytes % 16 == 0 */
 int   vcg_sync(void);                       /* wait; returns accumulated vc status */
 
 /* ---- device-resident archive ---- */

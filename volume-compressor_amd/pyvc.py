@@ -69,6 +69,7 @@ def lib():
         "vcx_dev_free": (None, [vp]),
         "vcx_h2d": (i32, [vp, vp, sz]),
         "vcx_d2h": (i32, [vp, vp, sz]),
+        "vcx_copy_kernel": (i32, [vp, vp, sz]),
         "vcx_decode_lod_device": (i32, [vp, i32, vp, i32]),
         "vcx_decode_region_device": (i32, [vp, i32, Box, vp, i32]),
         "vcx_sync": (i32, []),
