@@ -8,7 +8,8 @@
 #include <string.h>
 #include "vcg_xform.hpp"
 
-static const int NT = 256;   // threads per workgroup in the kernels
+static const int NT = 256;   // threads per workgroup in the forward kernel
+static const int NTI = 512;  // threads per workgroup in the inverse kernel
 
 extern "C" {
 uint32_t kc_encode(const int16_t *L, uint32_t eob, const uint32_t *band_start, uint8_t *out, uint32_t cap,
@@ -22,11 +23,11 @@ int kc_decode(const uint8_t *s, uint32_t n, const uint32_t *band_start, uint32_t
 // an 8-aligned, zero-padded buffer (as in the padded device archive)
 int kc_decode_fsm(const uint8_t *s, uint32_t n, uint32_t mis, const uint32_t *band_start, uint32_t *out,
                   uint32_t *cnt) {
-    size_t words = (mis + n + 7) / 8 + 4;
+    size_t words = (mis + n + 7) / 8 + 4;          // 24+ bytes of slack past the stream
     uint64_t *buf = (uint64_t *)calloc(words, 8);
     memset(buf, 0xA5, words * 8);                 // garbage around the stream must be masked
     memcpy((uint8_t *)buf + mis, s, n);
-    int r = vcg_rc_decode_atom_fsm(buf, mis, n, band_start, out, cnt);
+    int r = vcg_rc_decode_atom_fsm((const uint8_t *)buf + mis, n, band_start, out, cnt);
     free(buf);
     return r;
 }
@@ -64,21 +65,23 @@ int kc_fdct(const uint8_t *vol, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t 
 int kc_idct(const uint32_t *L, uint32_t cnt, int dc, const int32_t *st, const int32_t *bi,
             const int32_t cmat[32][32], const uint16_t *scan, int force, uint8_t *out) {
     int *t = (int *)malloc(VCG_TILE_INTS * sizeof(int));
-    for (int tid = 0; tid < NT; ++tid) vcg_tile_zero(t, tid, NT);
+    for (int tid = 0; tid < NTI; ++tid) vcg_tile_zero(t, tid, NTI);
     const int fmax = cnt ? vcg_freq(scan[L[cnt - 1] >> 16]) : -1;
     uint64_t l1 = 0;
-    for (int tid = 0; tid < NT; ++tid) l1 += vcg_scatter(t, L, cnt, scan, st, bi, tid, NT);
+    for (int tid = 0; tid < NTI; ++tid) l1 += vcg_scatter(t, L, cnt, scan, st, bi, tid, NTI);
     const bool fast = force == 1 ? true : force == 2 ? false : l1 <= 8000000ull;
-    auto emit = [&](int z, int y, int x, uint8_t v) { out[(z * 32 + y) * 32 + x] = v; };
+    static uint8_t vox[NTI][64];
     if (fast) {
-        for (int tid = 0; tid < NT; ++tid) vcg_inv_pass_x<true>(t, cmat, fmax, tid, NT);
-        for (int tid = 0; tid < NT; ++tid) vcg_inv_pass_y<true>(t, cmat, fmax, tid, NT);
-        for (int tid = 0; tid < NT; ++tid) vcg_inv_pass_z<true>(t, cmat, fmax, dc, tid, NT, emit);
+        for (int tid = 0; tid < NTI; ++tid) vcg_inv_pass_x<true>(t, cmat, fmax, tid, NTI);
+        for (int tid = 0; tid < NTI; ++tid) vcg_inv_pass_y<true>(t, cmat, fmax, tid, NTI);
+        for (int tid = 0; tid < NTI; ++tid) vcg_inv_pass_z<true>(t, cmat, fmax, dc, tid, vox[tid]);
     } else {
-        for (int tid = 0; tid < NT; ++tid) vcg_inv_pass_x<false>(t, cmat, fmax, tid, NT);
-        for (int tid = 0; tid < NT; ++tid) vcg_inv_pass_y<false>(t, cmat, fmax, tid, NT);
-        for (int tid = 0; tid < NT; ++tid) vcg_inv_pass_z<false>(t, cmat, fmax, dc, tid, NT, emit);
+        for (int tid = 0; tid < NTI; ++tid) vcg_inv_pass_x<false>(t, cmat, fmax, tid, NTI);
+        for (int tid = 0; tid < NTI; ++tid) vcg_inv_pass_y<false>(t, cmat, fmax, tid, NTI);
+        for (int tid = 0; tid < NTI; ++tid) vcg_inv_pass_z<false>(t, cmat, fmax, dc, tid, vox[tid]);
     }
+    // the staged raster is exactly the output atom
+    for (int tid = 0; tid < NTI; ++tid) vcg_inv_stage(out, vox[tid], tid);
     free(t);
     return fast ? 1 : 2;
 }

@@ -108,170 +108,247 @@ VCG_HD int vcg_rc_decode_atom(const uint8_t *s, uint32_t n, const uint32_t *band
 // ---------------------------------------------------------------------------
 // Decoder as a one-bin-per-step state machine (the batched kernels' form).
 //
-// Every lane executes the same instructions each step whatever symbol it is
-// in (significance, magnitude unary, Exp-Golomb prefix/suffix, sign), so a
-// wave of 64 atoms does not serialise on divergent branches.  Input bytes come
-// from a 16-byte little-endian register window; every VCG_WSTEP steps lanes
-// whose window holds <= 8 bytes append the next 8, which were loaded (as an
-// aligned u64) two refills earlier, so global-load latency hides behind the
-// steps in between.  The window never underflows: a step consumes at most one
-// byte (single renormalisation per bin, see vcg_rd_bit).  Magnitude contexts
-// 1..11 live in `mg` (LDS on the GPU); the one a step may need next is read
-// at the start of the step and mag[0] stays in a register.
+// A wavefront decodes 64 atoms in lockstep, so the step is written as
+// selects: every lane runs the same instructions whatever symbol it is in.
+// Bin states S:
+//    0      significance bin of position pos (context psig, in a register)
+//    1..12  magnitude unary bin with context mag[S-1]
+//    13     Exp-Golomb prefix bin (bypass); c counts its 1s
+//    14     tail: while c > 0 an Exp-Golomb suffix bin (bypass), then the
+//           sign bin (bypass), which emits the coefficient
+// mx holds the magnitude: S at a unary exit, 11 + X for an Exp-Golomb value X
+// built as X = 2X + bit from X = 1 (vc.c:491-506).  A lane that has reached
+// EOB (or failed) is parked: code = 0 and an empty, never refilled window make
+// every later bin a zero significance bin, which emits nothing.
 //
-// Reads are whole aligned 8-byte blocks covering [s, s+n): the caller
-// guarantees the buffer extends to the next 8-byte boundary past s+n plus 16
-// bytes (the device archive copy is padded); bytes outside [s, s+n) are masked
-// to zero (past-end bytes read as 0, vc.c:425).
+// The probability of the next bin is resolved inside the step: for a next
+// state 0 it is psig, for a next magnitude state the context read (ahead of
+// the bit) from slot S+1 of the lane's context memory, else VCG_P_BYPASS.
+// A bypass bin splits the range like a context bin with p = 2048, but keeps
+// floor(range/2) for both bit values and adapts nothing (vc.c:434-441); bit
+// 12 of P marks it and vanishes in P << 20.  Context memory slot k+1 holds
+// mag[k]; slots 0, 13, 14 (and 15 for reads) are write sinks.
+//
+// Input: a 64-bit little-endian window of wb valid bits ending at stream byte
+// E.  Every 8 steps (a step consumes at most 8 bits: one renormalisation per
+// bin, see vcg_rd_bit) the window is topped up to 64 bits.  Refills alternate
+// between two 16-byte look-ahead buffers, each loaded at the window end of its
+// previous use, two refills (16 steps) before it is needed: E has moved by at
+// most 8 bytes since, so the bytes to append lie inside it.  Bytes at or past
+// s[n] are masked to zero (past-end bytes read as 0, vc.c:425); loads are
+// clamped to s + n, so the buffer must extend 16 bytes past s[n-1].
+//
+// Output: each step writes its would-be entry into slot cnt % 8 of the lane's
+// ring and advances cnt only on an emission; every 16 steps the caller copies
+// ring slots [flushed, flushed+6) to the list (at most 6 emissions fit in 16
+// steps: a coefficient takes at least 3 bins), so list stores are unmasked and
+// entries past cnt are overwritten by the next flush.
 // ---------------------------------------------------------------------------
-#define VCG_WSTEP 8
 #ifndef VCG_ENTROPY_FSM
-#define VCG_ENTROPY_FSM 0
+#define VCG_ENTROPY_FSM 1
 #endif
-enum { VCG_S_SIG = 0, VCG_S_MAG = 1, VCG_S_EGP = 2, VCG_S_EGS = 3, VCG_S_SIGN = 4, VCG_S_DONE = 5, VCG_S_BAD = 6 };
+#define VCG_REFILL_STEPS 8
+#define VCG_FLUSH_STEPS 16
+#define VCG_FLUSH_N 6u
+#define VCG_RING 8u
+#define VCG_PARK_E 0x40000000u
+#define VCG_P_BYPASS (2048u | 4096u)
 
-// aligned 8-byte block `blk` of the stream whose first byte sits `mis` bytes
-// into block 0 and which holds `n` bytes; bytes outside the stream are zero
-VCG_HD uint64_t vcg_fetch8(const uint64_t *base, uint32_t blk, uint32_t mis, uint32_t n) {
-    const int64_t lo = (int64_t)mis - 8 * (int64_t)blk;          // first valid byte index within the block
-    const int64_t hi = (int64_t)mis + n - 8 * (int64_t)blk;      // one past the last
-    if (hi <= 0 || lo >= 8) return 0;
-    uint64_t v = base[blk];
-    if (lo > 0) v &= ~0ull << (8 * lo);
-    if (hi < 8) v &= (1ull << (8 * hi)) - 1ull;
-    return v;
+// mask keeping the stream bytes of the 8 at byte E (those before s[n])
+VCG_HD uint64_t vcg_tail_mask(uint32_t E, uint32_t n) {
+    const int32_t rem = (int32_t)n - (int32_t)E;
+    const uint32_t keep = rem <= 0 ? 0u : rem >= 8 ? 8u : (uint32_t)rem;
+    return keep >= 8u ? ~0ull : (1ull << (8u * keep)) - 1ull;
+}
+
+// 16 stream bytes starting at byte e (address clamped to s + n)
+struct vcg_look {
+    uint64_t lo, hi;
+    uint32_t e;
+};
+VCG_HD void vcg_look_load(vcg_look &L, const uint8_t *s, uint32_t n, uint32_t e) {
+    uint64_t v[2];
+    __builtin_memcpy(v, s + (e < n ? e : n), 16);
+    L.lo = v[0]; L.hi = v[1]; L.e = e;
 }
 
 struct vcg_fsm {
-    // range decoder
     uint32_t code, range;
-    // byte window (little endian, wn valid bytes) and two pending blocks
-    uint64_t wlo, whi, p1, p2;
-    uint32_t wn, blk;
-    // symbol state
-    uint32_t st, pos, eob, k, nb, x, m;
-    uint32_t psig, pm, pnext, m0;
-    uint32_t band, nextb, cnt;
+    uint64_t w;                  // window, wb valid bits, ending at stream byte E
+    uint32_t wb, E;
+    uint32_t P, S, psig;
+    uint32_t pos16, lim16, eob16;   // scan position, next band edge / EOB, all << 16
+    uint32_t c, mx;
+    uint32_t cnt8, fl8;             // emitted / flushed entry counts, << 8 (ring slot offsets)
+    uint32_t band, bad;
 };
 
-VCG_HD void vcg_win_pop(vcg_fsm &f, uint32_t &byte) {
-    byte = (uint32_t)f.wlo & 0xFFu;
-    f.wlo = (f.wlo >> 8) | (f.whi << 56);
-    f.whi >>= 8;
-    f.wn--;
+VCG_HD void vcg_fsm_park(vcg_fsm &f) {
+    f.code = 0; f.w = 0; f.E = VCG_PARK_E; f.S = 0; f.lim16 = 0xFFFFFFFFu; f.c = 0;
 }
-// append 8 bytes at byte offset wn (wn <= 8)
-VCG_HD void vcg_win_push8(vcg_fsm &f, uint64_t v) {
-    const uint32_t sh = 8 * f.wn;
-    if (sh == 0) { f.wlo |= v; }
-    else if (sh == 64) { f.whi |= v; }
-    else { f.wlo |= v << sh; f.whi |= v >> (64 - sh); }
-    f.wn += 8;
+VCG_HD bool vcg_fsm_parked(const vcg_fsm &f) { return f.E >= VCG_PARK_E; }
+
+// top the window up to 64 bits from L (loaded at L.e, E - L.e <= 8), then
+// reload L at the new window end
+VCG_HD void vcg_fsm_refill(vcg_fsm &f, vcg_look &L, const uint8_t *s, uint32_t n) {
+    const uint32_t o = f.E - L.e < 8u ? f.E - L.e : 8u;   // >= 8 only for a parked lane
+    const uint32_t a = 4u * o, b = 32u - a;                 // two shifts each, all < 64
+    const uint64_t data = ((L.lo >> a) >> a) | ((L.hi << b) << b);
+    const uint64_t app = f.wb < 64u ? (data & vcg_tail_mask(f.E, n)) << f.wb : 0ull;
+    f.w |= app;
+    f.E += (64u - f.wb) >> 3;
+    f.wb = 64u;
+    vcg_look_load(L, s, n, f.E);
 }
 
-VCG_HD void vcg_fsm_init(vcg_fsm &f, const uint64_t *base, uint32_t mis, uint32_t n, const uint32_t *band_start) {
-    f.wlo = vcg_fetch8(base, 0, mis, n) >> (8 * mis);
-    f.whi = 0;
-    f.wn = 8 - mis;
-    vcg_win_push8(f, vcg_fetch8(base, 1, mis, n));
-    f.p1 = vcg_fetch8(base, 2, mis, n);
-    f.p2 = vcg_fetch8(base, 3, mis, n);
-    f.blk = 4;
-    f.code = 0; f.range = 0xFFFFFFFFu;
-    uint32_t b;
-    for (int k = 0; k < 5; ++k) { vcg_win_pop(f, b); f.code = (f.code << 8) | b; }   // vc.c:406-409
-    // EOB: 16 bypass bins (vc.c:520-525).  The window holds >= 4 bytes here and
-    // 16 halvings of a full range renormalise at most 3 times.
+// low `width` bits of x (width 0..31)
+VCG_HD uint32_t vcg_low_bits(uint32_t x, uint32_t width) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ubfe(x, 0u, width);
+#else
+    return x & ((1u << width) - 1u);
+#endif
+}
+
+VCG_HD void vcg_fsm_renorm(vcg_fsm &f) {
+    const uint32_t sh = (uint32_t)__builtin_clz(f.range) & 8u;   // range >= 2^17 here
+    f.code = (f.code << sh) | vcg_low_bits((uint32_t)f.w, sh);
+    f.range <<= sh;
+    f.w >>= sh;
+    f.wb -= sh;
+}
+
+// s: first range-coder byte; n: byte count.  Context memory slots 1..12 must
+// hold 2048 on entry.
+VCG_HD void vcg_fsm_init(vcg_fsm &f, vcg_look &A, vcg_look &B, const uint8_t *s, uint32_t n,
+                         const uint32_t *band_start) {
+    f.w = 0; f.wb = 0; f.E = 0;
+    vcg_look_load(A, s, n, 0);
+    vcg_fsm_refill(f, A, s, n);                       // bytes [0, 8): wb = 64
+    f.code = 0;
+    for (int k = 0; k < 5; ++k) { f.code = (f.code << 8) | ((uint32_t)f.w & 0xFFu); f.w >>= 8; f.wb -= 8; }   // vc.c:406-409
+    f.range = 0xFFFFFFFFu;
+    vcg_fsm_refill(f, A, s, n);                       // A was reloaded at 8: bytes [8, 13), wb = 64;
+                                                      // the EOB renormalises at most twice
     uint32_t eob = 0;
-    for (int k = 0; k < 16; ++k) {
+    for (int k = 0; k < 16; ++k) {                    // vc.c:520-525
         f.range >>= 1;
-        uint32_t bit = f.code >= f.range;
+        const uint32_t bit = f.code >= f.range;
         if (bit) f.code -= f.range;
-        if (f.range < (1u << 24)) { vcg_win_pop(f, b); f.code = (f.code << 8) | b; f.range <<= 8; }
+        vcg_fsm_renorm(f);
         eob = (eob << 1) | bit;
     }
-    f.eob = eob > 32768u ? 32768u : eob;                                   // vc.c:549
-    f.st = f.eob ? VCG_S_SIG : VCG_S_DONE;
-    f.pos = 0; f.k = 0; f.nb = 0; f.x = 0; f.m = 0;
-    f.psig = 2048u; f.pm = 2048u; f.pnext = 2048u; f.m0 = 2048u;
-    f.band = 0; f.nextb = band_start[1]; f.cnt = 0;
+    B = A;                                            // both at the window end (13)
+    eob = eob > 32768u ? 32768u : eob;                // vc.c:549
+    f.eob16 = eob << 16;
+    f.P = 2048u; f.S = 0; f.psig = 2048u; f.pos16 = 0; f.c = 0; f.mx = 0; f.cnt8 = 0; f.fl8 = 0;
+    f.band = 0; f.bad = 0;
+    f.lim16 = (band_start[1] < eob ? band_start[1] : eob) << 16;
+    if (eob == 0) vcg_fsm_park(f);
 }
 
-// refill (every VCG_WSTEP steps): lanes holding <= 8 bytes append the oldest pending block
-VCG_HD void vcg_fsm_refill(vcg_fsm &f, const uint64_t *base, uint32_t mis, uint32_t n) {
-    if (f.wn <= 8) {
-        vcg_win_push8(f, f.p1);
-        f.p1 = f.p2;
-        f.p2 = vcg_fetch8(base, f.blk, mis, n);
-        f.blk++;
-    }
+// one bin.  Ctx: ld(slot), st(slot, v) on the lane's context memory; Ring:
+// st(slot << 8, v) on the lane's output ring.  Returns nonzero when pos reached a
+// band boundary or EOB, or the Exp-Golomb prefix passed its cap (the caller
+// then runs vcg_fsm_edge).
+template <class Ctx, class Ring>
+VCG_HD uint32_t vcg_fsm_step(vcg_fsm &f, Ctx &ctx, Ring &ring) {
+    const uint32_t S = f.S, P = f.P, c = f.c;
+    const uint32_t R = ctx.ld(S + 1u);
+    // binary decision (vc.c:414-441)
+    const uint32_t r0 = (uint32_t)(((uint64_t)f.range * (P << 20)) >> 32);
+    const bool bit = f.code >= r0;
+    f.code = bit ? f.code - r0 : f.code;
+    f.range = (bit && P < 4096u) ? f.range - r0 : r0;
+    // adaptation (vc.c:420-431): bit 1: P - (P>>5); bit 0: P + ((4096-P)>>5)
+    // = P + 128 - ((P+31)>>5), written without a branch
+    const uint32_t nb = bit ? 0u : 1u;
+    const uint32_t pn = P + (nb << 7) - ((P + nb * 31u) >> 5);
+    ctx.st(S, pn);
+    const bool isSig = S == 0u;
+    f.psig = isSig ? pn : f.psig;
+    vcg_fsm_renorm(f);
+    // symbol bookkeeping
+    const bool isTail = S == 14u;
+    const bool cz = c == 0u;
+    const bool sgn = isTail && cz;
+    const bool sfx = isTail && !cz;
+    const uint32_t mx = (S - 1u < 12u) ? S : sfx ? 2u * f.mx - 11u + (uint32_t)bit : f.mx;
+    f.mx = mx;
+    ring.st(f.cnt8 & ((VCG_RING - 1u) << 8), f.pos16 | ((bit ? 0u - mx : mx) & 0xFFFFu));
+    f.cnt8 += sgn ? 256u : 0u;
+    f.c = c + ((S == 13u && bit) ? 1u : 0u) - (sfx ? 1u : 0u);
+    f.pos16 += ((isSig && !bit) || sgn) ? 0x10000u : 0u;
+    const uint32_t up = S + 1u < 13u ? S + 1u : 13u;
+    const uint32_t t = bit ? up : (isSig ? 0u : 14u);
+    const uint32_t Sn = isTail ? (cz ? 0u : 14u) : t;
+    f.S = Sn;
+    f.P = Sn == 0u ? f.psig : (Sn <= 12u ? R : VCG_P_BYPASS);
+    return (f.pos16 == f.lim16) | (f.c > VCG_EG_PREFIX_CAP);
 }
 
-// one bin of the symbol stream; mg = magnitude contexts (index 1..11 used)
-template <class MagMem>
-VCG_HD void vcg_fsm_step(vcg_fsm &f, MagMem &mg, const uint32_t *band_start, uint32_t *out) {
-    const uint32_t st = f.st;
-    if (st >= VCG_S_DONE) return;
-    // context read for the magnitude bin after this one (used only if this bin is 1)
-    if (st == VCG_S_MAG && f.k + 1 < 12) f.pnext = mg.get(f.k + 1);
-    const bool byp = st >= VCG_S_EGP;
-    const uint32_t p = st == VCG_S_SIG ? f.psig : f.pm;
-    const uint32_t r0 = byp ? (f.range >> 1) : (uint32_t)(((uint64_t)f.range * p) >> 12);
-    const uint32_t bit = f.code >= r0;
-    if (bit) f.code -= r0;
-    f.range = (byp || !bit) ? r0 : f.range - r0;
-    const uint32_t pn = bit ? p - (p >> 5) : p + ((4096u - p) >> 5);
-    if (f.range < (1u << 24)) { uint32_t b; vcg_win_pop(f, b); f.code = (f.code << 8) | b; f.range <<= 8; }
-    if (st == VCG_S_SIG) {
-        f.psig = pn;
-        if (bit) { f.st = VCG_S_MAG; f.k = 0; f.pm = f.m0; return; }
-        f.pos++;
-    } else if (st == VCG_S_MAG) {
-        if (f.k == 0) f.m0 = pn; else mg.set(f.k, pn);
-        if (bit) {
-            f.k++;
-            if (f.k == 12) { f.st = VCG_S_EGP; f.nb = 0; }
-            else f.pm = f.pnext;
-        } else { f.m = f.k + 1; f.st = VCG_S_SIGN; }
-        return;
-    } else if (st == VCG_S_EGP) {
-        if (bit) { if (++f.nb > VCG_EG_PREFIX_CAP) f.st = VCG_S_BAD; }
-        else { f.x = 1; if (f.nb == 0) { f.m = 12; f.st = VCG_S_SIGN; } else f.st = VCG_S_EGS; }
-        return;
-    } else if (st == VCG_S_EGS) {
-        f.x = (f.x << 1) | bit;
-        if (--f.nb == 0) { f.m = 11u + f.x; f.st = VCG_S_SIGN; }     // v + (x-1) + 1 with v = 11
-        return;
-    } else {                                                          // sign bypass: emit
-        const uint32_t lv = bit ? 0u - f.m : f.m;
-        out[f.cnt++] = (f.pos << 16) | (lv & 0xFFFFu);
-        f.pos++;
-        f.st = VCG_S_SIG;
-    }
-    // advanced past a position (zero significance or emitted coefficient)
-    if (f.pos == f.eob) { f.st = VCG_S_DONE; return; }
-    if (f.pos == f.nextb) { f.band++; f.nextb = band_start[f.band + 1]; f.psig = 2048u; }
+// band boundary / EOB / Exp-Golomb cap for a lane whose step returned nonzero
+VCG_HD void vcg_fsm_edge(vcg_fsm &f, const uint32_t *band_start) {
+    if (f.c > VCG_EG_PREFIX_CAP) { f.bad = 1; vcg_fsm_park(f); return; }
+    if (f.pos16 == f.eob16) { vcg_fsm_park(f); return; }
+    // a new band starts from the reset context (vc.c:552-555); the next bin
+    // is a significance bin, so its probability is the reset value too
+    f.band++;
+    const uint32_t nb = band_start[f.band + 1] << 16;
+    f.lim16 = nb < f.eob16 ? nb : f.eob16;
+    f.psig = 2048u;
+    f.P = 2048u;
 }
 
-// Whole-atom decode with the state machine (CPU-side reference of the kernel's
-// per-lane logic).  Returns 0, or 1 for an over-long Exp-Golomb prefix.
-struct vcg_mag_array {
-    uint32_t v[12];
-    VCG_HDM uint32_t get(uint32_t k) const { return v[k]; }
-    VCG_HDM void set(uint32_t k, uint32_t p) { v[k] = p; }
+// copy ring slots [flushed, flushed + VCG_FLUSH_N) to out (entries past cnt
+// are placeholders that a later flush overwrites; out needs VCG_FLUSH_N - 1
+// slack entries past the list capacity).  Returns the number of real entries.
+template <class Ring>
+VCG_HD uint32_t vcg_fsm_flush(vcg_fsm &f, Ring &ring, uint32_t *out) {
+    const uint32_t pending = (f.cnt8 - f.fl8) >> 8;
+    uint32_t *o = out + (f.fl8 >> 8);
+#pragma unroll
+    for (uint32_t k = 0; k < VCG_FLUSH_N; ++k) o[k] = ring.ld((f.fl8 + (k << 8)) & ((VCG_RING - 1u) << 8));
+    f.fl8 = f.cnt8;
+    return pending;
+}
+
+// Whole-atom decode with the state machine, driven as the kernel drives it
+// (CPU-side reference of the kernel's per-lane logic).  Returns 0, 1 for an
+// over-long Exp-Golomb prefix, or 2 if a flush ever held more than
+// VCG_FLUSH_N entries (never, by the 3-bins-per-coefficient bound).
+struct vcg_ctx_array {
+    uint32_t v[16];
+    VCG_HDM uint32_t ld(uint32_t k) const { return v[k]; }
+    VCG_HDM void st(uint32_t k, uint32_t p) { v[k] = p; }
 };
-VCG_HD int vcg_rc_decode_atom_fsm(const uint64_t *base, uint32_t mis, uint32_t n, const uint32_t *band_start,
-                                  uint32_t *out, uint32_t *cnt_out) {
+struct vcg_ring_array {
+    uint32_t v[VCG_RING];
+    VCG_HDM uint32_t ld(uint32_t off) const { return v[off >> 8]; }
+    VCG_HDM void st(uint32_t off, uint32_t p) { v[off >> 8] = p; }
+};
+VCG_HD int vcg_rc_decode_atom_fsm(const uint8_t *s, uint32_t n, const uint32_t *band_start, uint32_t *out,
+                                  uint32_t *cnt_out) {
+    vcg_ctx_array ctx;
+    vcg_ring_array ring;
+    for (int k = 0; k < 16; ++k) ctx.v[k] = 2048u;
+    for (uint32_t k = 0; k < VCG_RING; ++k) ring.v[k] = 0xDEADBEEFu;
     vcg_fsm f;
-    vcg_fsm_init(f, base, mis, n, band_start);
-    vcg_mag_array mg;
-    for (int k = 0; k < 12; ++k) mg.v[k] = 2048u;
-    while (f.st < VCG_S_DONE) {
-        vcg_fsm_refill(f, base, mis, n);
-        for (int s = 0; s < VCG_WSTEP; ++s) vcg_fsm_step(f, mg, band_start, out);
+    vcg_look A, B;
+    vcg_fsm_init(f, A, B, s, n, band_start);
+    int over = 0;
+    while (!vcg_fsm_parked(f)) {
+        vcg_fsm_refill(f, A, s, n);
+        if (vcg_fsm_flush(f, ring, out) > VCG_FLUSH_N) over = 1;
+        for (int k = 0; k < VCG_REFILL_STEPS; ++k)
+            if (vcg_fsm_step(f, ctx, ring)) vcg_fsm_edge(f, band_start);
+        vcg_fsm_refill(f, B, s, n);
+        for (int k = 0; k < VCG_REFILL_STEPS; ++k)
+            if (vcg_fsm_step(f, ctx, ring)) vcg_fsm_edge(f, band_start);
     }
-    *cnt_out = f.st == VCG_S_BAD ? 0 : f.cnt;
-    return f.st == VCG_S_BAD;
+    if (vcg_fsm_flush(f, ring, out) > VCG_FLUSH_N) over = 1;
+    *cnt_out = f.bad ? 0 : f.cnt8 >> 8;
+    return over ? 2 : (int)f.bad;
 }
 
 // ---------------------------------------------------------------------------

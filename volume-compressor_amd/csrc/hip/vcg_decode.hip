@@ -56,19 +56,33 @@ __device__ __forceinline__ void item_atom(const dec_job &j, uint32_t gi, int &lo
     }
 }
 
-// magnitude contexts 1..11 of the 64 lanes of a wave, k-major (lane-contiguous)
-struct lds_mag {
-    uint32_t *base;
-    uint32_t lane;
-    __device__ __forceinline__ uint32_t get(uint32_t k) const { return base[k * 64 + lane]; }
-    __device__ __forceinline__ void set(uint32_t k, uint32_t p) { base[k * 64 + lane] = p; }
+// per-lane LDS array (context memory: 16 slots; output ring: 8 slots),
+// slot-major over the 64 lanes of the wave so a wave's access is conflict-free
+struct lds_ctx {
+    uint32_t *b;
+    __device__ __forceinline__ uint32_t ld(uint32_t k) const { return b[k * 64u]; }
+    __device__ __forceinline__ void st(uint32_t k, uint32_t v) { b[k * 64u] = v; }
+};
+// output ring: slot offsets come pre-scaled (slot << 8 = slot * 64 lanes * 4 B)
+struct lds_ring {
+    uint32_t *b;
+    __device__ __forceinline__ uint32_t ld(uint32_t off) const {
+        return *(const uint32_t *)((const char *)b + off);
+    }
+    __device__ __forceinline__ void st(uint32_t off, uint32_t v) { *(uint32_t *)((char *)b + off) = v; }
 };
 
 __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__restrict__ T,
                                                 uint32_t *__restrict__ lists,
                                                 vcg_item *__restrict__ items,
                                                 uint32_t *__restrict__ status) {
-    __shared__ uint32_t mgs[12 * 64];
+    __shared__ uint32_t ctxm[16 * 64];
+    __shared__ uint32_t ringm[VCG_RING * 64];
+    __shared__ uint32_t bands[9];
+    // band table in LDS: a band edge then waits on LDS only, not on the
+    // look-ahead loads in flight
+    if (threadIdx.x < 9) bands[threadIdx.x] = T->band_start[threadIdx.x];
+    __syncthreads();
     const uint32_t i = blockIdx.x * 64u + threadIdx.x;
     if (i >= j.n) return;
     const uint32_t gi = j.base + i;
@@ -94,26 +108,34 @@ __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__res
     }
     const uint8_t *p = j.arc + off;
     it.dc = p[0];
-    uint32_t *out = lists + (size_t)i * VCG_LIST_CAP;
+    uint32_t *out = lists + (size_t)i * VCG_LIST_STRIDE;
 #if VCG_ENTROPY_FSM
-    // range-coder bytes [p+1, p+len): aligned 8-byte blocks (the device buffer is
-    // padded past its end), misalignment mis
-    const uintptr_t sa = (uintptr_t)(p + 1);
-    const uint64_t *blk = (const uint64_t *)(sa & ~(uintptr_t)7);
-    const uint32_t mis = (uint32_t)(sa & 7), n = len - 1;
-    lds_mag mg{mgs, threadIdx.x};
-    for (uint32_t k = 1; k < 12; ++k) mg.set(k, 2048u);
+    // range-coder bytes [p+1, p+len); the device buffers are padded past
+    // their end for the 16-byte look-ahead loads
+    lds_ctx ctx{ctxm + threadIdx.x};
+    lds_ring ring{ringm + threadIdx.x};
+    for (uint32_t k = 1; k <= 12; ++k) ctx.st(k, 2048u);
+    const uint8_t *s = p + 1;
+    const uint32_t n = len - 1;
     vcg_fsm f;
-    vcg_fsm_init(f, blk, mis, n, T->band_start);
-    while (f.st < VCG_S_DONE) {
-        vcg_fsm_refill(f, blk, mis, n);
+    vcg_look A, B;
+    vcg_fsm_init(f, A, B, s, n, bands);
+    while (__any(!vcg_fsm_parked(f))) {
+        vcg_fsm_refill(f, A, s, n);
+        vcg_fsm_flush(f, ring, out);
 #pragma unroll
-        for (int s = 0; s < VCG_WSTEP; ++s) vcg_fsm_step(f, mg, T->band_start, out);
+        for (int k = 0; k < VCG_REFILL_STEPS; ++k)
+            if (vcg_fsm_step(f, ctx, ring)) vcg_fsm_edge(f, bands);
+        vcg_fsm_refill(f, B, s, n);
+#pragma unroll
+        for (int k = 0; k < VCG_REFILL_STEPS; ++k)
+            if (vcg_fsm_step(f, ctx, ring)) vcg_fsm_edge(f, bands);
     }
-    const int bad = f.st == VCG_S_BAD;
-    const uint32_t cnt = f.cnt;
+    vcg_fsm_flush(f, ring, out);
+    const int bad = (int)f.bad;
+    const uint32_t cnt = f.cnt8 >> 8;
 #else
-    (void)mgs;
+    (void)ctxm; (void)ringm; (void)bands;
     uint32_t cnt = 0;
     const int bad = vcg_rc_decode_atom(p + 1, len - 1, T->band_start, out, &cnt);
 #endif
@@ -129,7 +151,7 @@ __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__res
 // ---------------------------------------------------------------------------
 // inverse transform: phases from vcg_xform.hpp separated by barriers
 // ---------------------------------------------------------------------------
-#define IDCT_THREADS 256
+#define IDCT_THREADS 512
 
 __device__ __forceinline__ void store_voxel(const dec_job &j, uint32_t gi, uint32_t ax, uint32_t ay,
                                             uint32_t az, int z, int y, int x, uint8_t v) {
@@ -140,6 +162,38 @@ __device__ __forceinline__ void store_voxel(const dec_job &j, uint32_t gi, uint3
         j.out[((uint64_t)(vz - j.bz0) * oy + (vy - j.by0)) * ox + (vx - j.bx0)] = v;
     } else {
         j.out[(uint64_t)gi * VCG_A3 + (uint32_t)((z * 32 + y) * 32 + x)] = v;
+    }
+}
+
+// Store the staged raster atom: thread tid owns bytes [64*tid, 64*tid + 64) =
+// rows (z, y) and (z, y+1).  Rows that lie wholly inside the box at a 16-byte
+// aligned destination go out as 16-byte stores, others byte by byte.
+__device__ __forceinline__ void store_atom(const dec_job &j, uint32_t gi, uint32_t ax, uint32_t ay, uint32_t az,
+                                           const uint8_t *stage, int tid) {
+    const int z = tid >> 4, y0 = (tid & 15) * 2;
+    const uint4 *src = (const uint4 *)(stage + 64 * tid);
+    if (j.mode != MODE_BOX) {
+        uint4 *dst = (uint4 *)(j.out + (uint64_t)gi * VCG_A3 + 64 * tid);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dst[q] = src[q];
+        return;
+    }
+    const uint32_t vz = az * 32 + z, vx = ax * 32;
+    const uint64_t ox = j.bx1 - j.bx0, oy = j.by1 - j.by0;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const uint32_t vy = ay * 32 + y0 + r;
+        if (vz < j.bz0 || vz >= j.bz1 || vy < j.by0 || vy >= j.by1) continue;
+        uint8_t *row = j.out + ((uint64_t)(vz - j.bz0) * oy + (vy - j.by0)) * ox;
+        if (vx >= j.bx0 && vx + 32 <= j.bx1 && ((uintptr_t)(row + (vx - j.bx0)) & 15) == 0) {
+            uint4 *dst = (uint4 *)(row + (vx - j.bx0));
+            dst[0] = src[2 * r];
+            dst[1] = src[2 * r + 1];
+        } else {
+            const uint8_t *b = stage + 64 * tid + 32 * r;
+            for (int x = 0; x < 32; ++x)
+                if (vx + x >= j.bx0 && vx + x < j.bx1) row[vx + x - j.bx0] = b[x];
+        }
     }
 }
 
@@ -155,16 +209,18 @@ __global__ __launch_bounds__(IDCT_THREADS) void k_idct(dec_job j, const vcg_dtab
     uint32_t ax, ay, az;
     item_atom(j, gi, lod, ax, ay, az);
     const vcg_item it = items[i];
+    uint8_t *stage = (uint8_t *)buf;
     if (it.status != 1) {
-        for (int v = tid; v < (int)VCG_A3; v += IDCT_THREADS)
-            store_voxel(j, gi, ax, ay, az, v >> 10, (v >> 5) & 31, v & 31, 0);
+        for (int v = tid; v < (int)VCG_A3 / 4; v += IDCT_THREADS) ((uint32_t *)stage)[v] = 0;
+        __syncthreads();
+        store_atom(j, gi, ax, ay, az, stage, tid);
         return;
     }
     vcg_tile_zero(buf, tid, IDCT_THREADS);
     if (tid == 0) l1sum = 0;
     __syncthreads();
     const uint32_t cnt = it.cnt;
-    const uint32_t *L = lists + (size_t)i * VCG_LIST_CAP;
+    const uint32_t *L = lists + (size_t)i * VCG_LIST_STRIDE;
     // entries are in increasing scan position, so the last has the highest frequency
     const int fmax = cnt ? vcg_freq(T->scan[L[cnt - 1] >> 16]) : -1;
     const int32_t *st = j.dqs + (j.mode == MODE_PAYLOAD ? 0 : (int)it.lod * VCG_NFREQ);
@@ -178,21 +234,25 @@ __global__ __launch_bounds__(IDCT_THREADS) void k_idct(dec_job j, const vcg_dtab
     // stay below 8.02e6 < 2^23, so the 24-bit multiply path is exact.
     const bool fast = l1sum <= 8000000ull;
     const int32_t (*C)[32] = T->cmat;
-    auto emit = [&](int z, int y, int x, uint8_t v) { store_voxel(j, gi, ax, ay, az, z, y, x, v); };
     const int dc = (int)it.dc;
+    uint8_t vox[64];
     if (fast) {
         vcg_inv_pass_x<true>(buf, C, fmax, tid, IDCT_THREADS);
         __syncthreads();
         vcg_inv_pass_y<true>(buf, C, fmax, tid, IDCT_THREADS);
         __syncthreads();
-        vcg_inv_pass_z<true>(buf, C, fmax, dc, tid, IDCT_THREADS, emit);
+        vcg_inv_pass_z<true>(buf, C, fmax, dc, tid, vox);
     } else {
         vcg_inv_pass_x<false>(buf, C, fmax, tid, IDCT_THREADS);
         __syncthreads();
         vcg_inv_pass_y<false>(buf, C, fmax, tid, IDCT_THREADS);
         __syncthreads();
-        vcg_inv_pass_z<false>(buf, C, fmax, dc, tid, IDCT_THREADS, emit);
+        vcg_inv_pass_z<false>(buf, C, fmax, dc, tid, vox);
     }
+    __syncthreads();
+    vcg_inv_stage(stage, vox, tid);
+    __syncthreads();
+    store_atom(j, gi, ax, ay, az, stage, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -213,7 +273,7 @@ static void ensure_attr() {
 static void run_batch(vcg_ctx *c, dec_job j, uint32_t total) {
     ensure_attr();
     uint32_t maxb = total < MAX_BATCH ? total : MAX_BATCH;
-    vcg_grow((void **)&c->lists, &c->lists_items, maxb, (size_t)VCG_LIST_CAP * 4);
+    vcg_grow((void **)&c->lists, &c->lists_items, maxb, (size_t)VCG_LIST_STRIDE * 4);
     vcg_grow((void **)&c->items, &c->items_cap, maxb, sizeof(vcg_item));
     for (uint32_t b = 0; b < total; b += maxb) {
         j.base = b;

@@ -32,6 +32,9 @@ struct vcg_item {
 // Coefficient entry: (scan position << 16) | (uint16_t)level.  Entries of one
 // item are written in increasing scan position.
 #define VCG_LIST_CAP 32768u
+// per-atom list stride in entries: padded past a power of two so the lists of
+// the 64 lanes of a wave (written concurrently) do not share cache sets
+#define VCG_LIST_STRIDE (VCG_LIST_CAP + 64u)
 
 // Per-thread execution context (one HIP stream, grow-only scratch).
 struct vcg_ctx {

@@ -161,7 +161,7 @@ extern "C" vcg_arc *vcg_arc_create(const uint8_t *buf, const vcg_geom *g) {
     vcg_ctx *c = vcg_get_ctx();
     vcg_arc *a = new vcg_arc();
     a->host = *g;
-    // padded: the entropy kernel reads whole aligned 8-byte blocks around each payload
+    // padded: the entropy kernel reads 16-byte look-ahead blocks up to 15 bytes past a payload
     if (hipMalloc(&a->buf, g->len + 64) != hipSuccess ||
         hipMalloc(&a->geom, sizeof(vcg_geom)) != hipSuccess) {
         (void)hipGetLastError();

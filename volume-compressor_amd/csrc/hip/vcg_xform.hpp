@@ -45,38 +45,69 @@ VCG_HD int vcg_mac(int a, int c) {
     else return (int)((uint32_t)a * (uint32_t)c);
 }
 
+// acc + c*a (i32 wrap): FAST = 24-bit signed operands, one v_mad_i32_i24 with
+// the basis constant c read from an SGPR (c is workgroup-uniform)
+template <bool FAST>
+VCG_HD int vcg_acc(int acc, int c, int a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (FAST) {
+        int r;
+        asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "s"(c), "v"(a), "v"(acc));
+        return r;
+    }
+#endif
+    return (int)((uint32_t)acc + (uint32_t)vcg_mac<FAST>(c, a));
+}
+
 // ===========================================================================
 // inverse
 // ===========================================================================
-// One inverse line read from the tile (element k at src[k*stride]) with only
-// the first nk coefficients possibly nonzero (nk even):
+// Two inverse lines read from the tile (element k of line i at s_i[k*stride])
+// with only the first nk coefficients possibly nonzero (nk a multiple of 4):
 //   evn[n] = sum_{k even} C[k][n] v[k],  odd[n] = sum_{k odd} C[k][n] v[k]
 //   o[n] = (evn+odd+8192)>>14,  o[31-n] = (evn-odd+8192)>>14     (i32 wrap)
-// src/vc/vc.c:178-184.  The k loop stays rolled so each step loads one basis
-// row pair instead of keeping all 512 constants live.
+// src/vc/vc.c:178-184.  The lines share every basis row load, and each loop
+// step covers two row pairs, so a workgroup-uniform step loads 64 constants.
 template <bool FAST>
-VCG_HD void vcg_inv_line(const int32_t (*C)[32], const int *src, int stride, int nk, int *o) {
-    int evn[16], odd[16];
+VCG_HD void vcg_inv_line2(const int32_t (*C)[32], const int *s0, const int *s1, int stride, int nk, int *o0,
+                          int *o1) {
+    int e0[16], d0[16], e1[16], d1[16];
 #pragma unroll
-    for (int n = 0; n < 16; ++n) { evn[n] = 0; odd[n] = 0; }
+    for (int n = 0; n < 16; ++n) { e0[n] = 0; d0[n] = 0; e1[n] = 0; d1[n] = 0; }
+    struct row16 { int32_t v[16]; };                  // C[k][0..15], one 64-byte load
 #pragma unroll 1
-    for (int k = 0; k < nk; k += 2) {
-        const int ve = src[k * stride], vo = src[(k + 1) * stride];
+    for (int k = 0; k < nk; k += 4) {
 #pragma unroll
-        for (int n = 0; n < 16; ++n) {
-            evn[n] += vcg_mac<FAST>(C[k][n], ve);
-            odd[n] += vcg_mac<FAST>(C[k + 1][n], vo);
+        for (int kk = 0; kk < 4; kk += 2) {
+            const int a0 = s0[(k + kk) * stride], b0 = s0[(k + kk + 1) * stride];
+            const int a1 = s1[(k + kk) * stride], b1 = s1[(k + kk + 1) * stride];
+            const row16 re = *(const row16 *)C[k + kk], ro = *(const row16 *)C[k + kk + 1];
+#pragma unroll
+            for (int n = 0; n < 16; ++n) {
+                const int ce = re.v[n], co = ro.v[n];
+                e0[n] = vcg_acc<FAST>(e0[n], ce, a0);
+                d0[n] = vcg_acc<FAST>(d0[n], co, b0);
+                e1[n] = vcg_acc<FAST>(e1[n], ce, a1);
+                d1[n] = vcg_acc<FAST>(d1[n], co, b1);
+            }
         }
     }
 #pragma unroll
     for (int n = 0; n < 16; ++n) {
-        o[n] = (int)((uint32_t)evn[n] + (uint32_t)odd[n] + 8192u) >> 14;
-        o[31 - n] = (int)((uint32_t)evn[n] - (uint32_t)odd[n] + 8192u) >> 14;
+        o0[n] = (int)((uint32_t)e0[n] + (uint32_t)d0[n] + 8192u) >> 14;
+        o0[31 - n] = (int)((uint32_t)e0[n] - (uint32_t)d0[n] + 8192u) >> 14;
+        o1[n] = (int)((uint32_t)e1[n] + (uint32_t)d1[n] + 8192u) >> 14;
+        o1[31 - n] = (int)((uint32_t)e1[n] - (uint32_t)d1[n] + 8192u) >> 14;
     }
 }
 
+// term count covering coefficients 0..kmax, rounded up to a multiple of 4
+VCG_HD int vcg_nk4(int kmax) { return kmax < 0 ? 0 : (kmax > 27 ? 32 : ((kmax + 4) & ~3)); }
+
 VCG_HD void vcg_tile_zero(int *t, int tid, int nthr) {
-    for (int k = tid; k < VCG_TILE_INTS; k += nthr) t[k] = 0;
+    struct i4 { int a, b, c, d; };
+    i4 *q = (i4 *)t;
+    for (int k = tid; k < VCG_TILE_INTS / 4; k += nthr) q[k] = i4{0, 0, 0, 0};
 }
 
 // Dequantize entries [tid, cnt) step nthr into the tile (src/vc/vc.c:327-338);
@@ -98,53 +129,77 @@ VCG_HD uint64_t vcg_scatter(int *t, const uint32_t *L, uint32_t cnt, const uint1
     return mine;
 }
 
+// The three passes give each thread the line pairs (line, line + nthr) for
+// line = tid, tid + 2*nthr, ... (nthr divides 512).  The pair's term count is
+// the larger line's, made wave-uniform on the GPU (vcg_nk) and rounded to 4;
+// the extra terms multiply zeros.
+
 // pass 1 (along x): lines (kz, ky); coefficients beyond kx = fmax-kz-ky are zero
 template <bool FAST>
 VCG_HD void vcg_inv_pass_x(int *t, const int32_t (*C)[32], int fmax, int tid, int nthr) {
-    for (int line = tid; line < 1024; line += nthr) {
-        const int cz = line >> 5, cy = line & 31;
-        const int kmax = fmax - cz - cy;
-        const int nk = vcg_nk(kmax);
+    for (int l0 = tid; l0 < 1024; l0 += 2 * nthr) {
+        const int l1 = l0 + nthr;
+        const int km0 = fmax - (l0 >> 5) - (l0 & 31), km1 = fmax - (l1 >> 5) - (l1 & 31);
+        const int nk = vcg_nk4(vcg_nk(km0 > km1 ? km0 : km1) - 1);
         if (nk == 0) continue;
-        int *row = t + vcg_tidx(cz, cy, 0);
-        int o[32];
-        vcg_inv_line<FAST>(C, row, VCG_SX, nk, o);
-        if (kmax >= 0) {
+        int *r0 = t + vcg_tidx(l0 >> 5, l0 & 31, 0), *r1 = t + vcg_tidx(l1 >> 5, l1 & 31, 0);
+        int o0[32], o1[32];
+        vcg_inv_line2<FAST>(C, r0, r1, VCG_SX, nk, o0, o1);
+        if (km0 >= 0) {
 #pragma unroll
-            for (int k = 0; k < 32; ++k) row[k * VCG_SX] = o[k];
+            for (int k = 0; k < 32; ++k) r0[k * VCG_SX] = o0[k];
+        }
+        if (km1 >= 0) {
+#pragma unroll
+            for (int k = 0; k < 32; ++k) r1[k * VCG_SX] = o1[k];
         }
     }
 }
 // pass 2 (along y): lines (kz, x); ky beyond fmax-kz are zero
 template <bool FAST>
 VCG_HD void vcg_inv_pass_y(int *t, const int32_t (*C)[32], int fmax, int tid, int nthr) {
-    for (int line = tid; line < 1024; line += nthr) {
-        const int cz = line >> 5, x = line & 31;
-        const int kmax = fmax - cz;
-        const int nk = vcg_nk(kmax);
+    for (int l0 = tid; l0 < 1024; l0 += 2 * nthr) {
+        const int l1 = l0 + nthr;
+        const int km0 = fmax - (l0 >> 5), km1 = fmax - (l1 >> 5);
+        const int nk = vcg_nk4(vcg_nk(km0 > km1 ? km0 : km1) - 1);
         if (nk == 0) continue;
-        int *col = t + vcg_tidx(cz, 0, x);
-        int o[32];
-        vcg_inv_line<FAST>(C, col, VCG_SY, nk, o);
-        if (kmax >= 0) {
+        int *c0 = t + vcg_tidx(l0 >> 5, 0, l0 & 31), *c1 = t + vcg_tidx(l1 >> 5, 0, l1 & 31);
+        int o0[32], o1[32];
+        vcg_inv_line2<FAST>(C, c0, c1, VCG_SY, nk, o0, o1);
+        if (km0 >= 0) {
 #pragma unroll
-            for (int k = 0; k < 32; ++k) col[k * VCG_SY] = o[k];
+            for (int k = 0; k < 32; ++k) c0[k * VCG_SY] = o0[k];
+        }
+        if (km1 >= 0) {
+#pragma unroll
+            for (int k = 0; k < 32; ++k) c1[k * VCG_SY] = o1[k];
         }
     }
 }
-// pass 3 (along z): line (y, x) -> 32 output voxels (dc added, clamped) via emit(z, y, x, v)
-template <bool FAST, class Emit>
-VCG_HD void vcg_inv_pass_z(const int *t, const int32_t (*C)[32], int fmax, int dc, int tid, int nthr, Emit emit) {
-    const int nk = fmax < 0 ? 0 : (fmax > 31 ? 32 : ((fmax + 2) & ~1));
-    for (int line = tid; line < 1024; line += nthr) {
-        const int y = line >> 5, x = line & 31;
-        int o[32];
-        vcg_inv_line<FAST>(C, t + vcg_tidx(0, y, x), VCG_SZ, nk, o);
+// pass 3 (along z): lines (y, x) -> 32 output voxels each (dc added, clamped).
+// For nthr = 512 each thread has one pair; vox[0..31] = line tid (z = 0..31),
+// vox[32..63] = line tid + 512.
+template <bool FAST>
+VCG_HD void vcg_inv_pass_z(const int *t, const int32_t (*C)[32], int fmax, int dc, int tid, uint8_t *vox) {
+    const int nk = vcg_nk4(fmax);
+    const int l0 = tid, l1 = tid + 512;
+    int o0[32], o1[32];
+    vcg_inv_line2<FAST>(C, t + vcg_tidx(0, l0 >> 5, l0 & 31), t + vcg_tidx(0, l1 >> 5, l1 & 31), VCG_SZ, nk, o0,
+                        o1);
 #pragma unroll
-        for (int z = 0; z < 32; ++z) {
-            const int v = o[z] + dc;
-            emit(z, y, x, (uint8_t)(v < 0 ? 0 : v > 255 ? 255 : v));
-        }
+    for (int z = 0; z < 32; ++z) {
+        const int v0 = o0[z] + dc, v1 = o1[z] + dc;
+        vox[z] = (uint8_t)(v0 < 0 ? 0 : v0 > 255 ? 255 : v0);
+        vox[32 + z] = (uint8_t)(v1 < 0 ? 0 : v1 > 255 ? 255 : v1);
+    }
+}
+// after a barrier: the pass-3 voxels of thread tid into a raster u8 atom
+// (z, y, x) at stage (the tile's memory reused)
+VCG_HD void vcg_inv_stage(uint8_t *stage, const uint8_t *vox, int tid) {
+#pragma unroll
+    for (int z = 0; z < 32; ++z) {
+        stage[z * 1024 + tid] = vox[z];
+        stage[z * 1024 + 512 + tid] = vox[32 + z];
     }
 }
 
