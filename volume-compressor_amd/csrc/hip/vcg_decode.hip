@@ -99,7 +99,7 @@ __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__res
         r = vcg_resolve(j.arc, j.geom, lod, az, ay, ax, &off, &len);
     }
     vcg_item it;
-    it.cnt = 0; it.lod = (uint32_t)lod; it.dc = 0;
+    it.cnt = 0; it.lod = (uint16_t)lod; it.dc = 0; it.fmax = -1;
     if (r != 1) {
         it.status = r == 2 ? 2u : 0u;
         items[i] = it;
@@ -134,16 +134,19 @@ __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__res
     vcg_fsm_flush(f, ring, out);
     const int bad = (int)f.bad;
     const uint32_t cnt = f.cnt8 >> 8;
+    // the last entry (highest scan position) is still in the ring
+    const int fmax = cnt ? vcg_freq(T->scan[ring.ld((f.cnt8 - 256u) & ((VCG_RING - 1u) << 8)) >> 16]) : -1;
 #else
     (void)ctxm; (void)ringm; (void)bands;
     uint32_t cnt = 0;
     const int bad = vcg_rc_decode_atom(p + 1, len - 1, T->band_start, out, &cnt);
+    const int fmax = cnt && !bad ? vcg_freq(T->scan[out[cnt - 1] >> 16]) : -1;
 #endif
     if (bad) {
         it.status = 2; it.cnt = 0;
         atomicOr(status, 1u);
     } else {
-        it.status = 1; it.cnt = cnt;
+        it.status = 1; it.cnt = cnt; it.fmax = (int16_t)fmax;
     }
     items[i] = it;
 }
@@ -216,16 +219,19 @@ __global__ __launch_bounds__(IDCT_THREADS) void k_idct(dec_job j, const vcg_dtab
         store_atom(j, gi, ax, ay, az, stage, tid);
         return;
     }
+    __shared__ int32_t dqs[2 * VCG_NFREQ];
+    {
+        const int32_t *st = j.dqs + (j.mode == MODE_PAYLOAD ? 0 : (int)it.lod * VCG_NFREQ);
+        const int32_t *bi = j.dqb + (j.mode == MODE_PAYLOAD ? 0 : (int)it.lod * VCG_NFREQ);
+        if (tid < VCG_NFREQ) { dqs[tid] = st[tid]; dqs[VCG_NFREQ + tid] = bi[tid]; }
+    }
     vcg_tile_zero(buf, tid, IDCT_THREADS);
     if (tid == 0) l1sum = 0;
     __syncthreads();
     const uint32_t cnt = it.cnt;
     const uint32_t *L = lists + (size_t)i * VCG_LIST_STRIDE;
-    // entries are in increasing scan position, so the last has the highest frequency
-    const int fmax = cnt ? vcg_freq(T->scan[L[cnt - 1] >> 16]) : -1;
-    const int32_t *st = j.dqs + (j.mode == MODE_PAYLOAD ? 0 : (int)it.lod * VCG_NFREQ);
-    const int32_t *bi = j.dqb + (j.mode == MODE_PAYLOAD ? 0 : (int)it.lod * VCG_NFREQ);
-    const unsigned long long mine = vcg_scatter(buf, L, cnt, T->scan, st, bi, tid, IDCT_THREADS);
+    const int fmax = it.fmax;
+    const unsigned long long mine = vcg_scatter(buf, L, cnt, T->scan, dqs, dqs + VCG_NFREQ, tid, IDCT_THREADS);
     if (mine) atomicAdd(&l1sum, mine);
     __syncthreads();
     // Every intermediate element is bounded by the atom's L2 norm (each 1-D

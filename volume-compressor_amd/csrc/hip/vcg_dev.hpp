@@ -26,7 +26,8 @@ struct vcg_item {
     uint32_t cnt;     // coefficient entries written
     uint32_t status;  // 0: all-zero output, 1: coefficients, 2: corrupt (zeros + FORMAT)
     uint32_t dc;      // payload dc byte
-    uint32_t lod;     // dequant table
+    uint16_t lod;     // dequant table
+    int16_t fmax;     // highest coded frequency cz+cy+cx (-1: none)
 };
 
 // Coefficient entry: (scan position << 16) | (uint16_t)level.  Entries of one

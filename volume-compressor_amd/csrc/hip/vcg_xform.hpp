@@ -111,20 +111,29 @@ VCG_HD void vcg_tile_zero(int *t, int tid, int nthr) {
 }
 
 // Dequantize entries [tid, cnt) step nthr into the tile (src/vc/vc.c:327-338);
-// returns this thread's sum of |coef| (for the 24-bit bound).
+// returns this thread's sum of |coef| (for the 24-bit bound).  Entries go in
+// groups of 8 per thread so their list and scan-table loads overlap.
 VCG_HD uint64_t vcg_scatter(int *t, const uint32_t *L, uint32_t cnt, const uint16_t *scan, const int32_t *st,
                             const int32_t *bi, int tid, int nthr) {
     uint64_t mine = 0;
-    for (uint32_t e = (uint32_t)tid; e < cnt; e += (uint32_t)nthr) {
-        const uint32_t v = L[e];
-        const int lv = (int)(int16_t)(v & 0xFFFFu);
-        const uint32_t idx = scan[v >> 16];
-        const int f = vcg_freq(idx);
-        const uint32_t mag = (uint32_t)(lv < 0 ? -lv : lv);
-        int r = (int)(mag * (uint32_t)st[f] + (uint32_t)bi[f] + 128u) >> 8;
-        r = lv < 0 ? (int)(0u - (uint32_t)r) : r;
-        t[vcg_tidx((int)(idx >> 10), (int)((idx >> 5) & 31), (int)(idx & 31))] = r;
-        mine += (uint64_t)(r < 0 ? -(int64_t)r : (int64_t)r);
+    const uint32_t un = (uint32_t)nthr;
+    for (uint32_t e0 = (uint32_t)tid; e0 < cnt; e0 += 8u * un) {
+        uint32_t v[8], idx[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = e0 + q * un < cnt ? L[e0 + q * un] : 0u;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) idx[q] = e0 + q * un < cnt ? scan[v[q] >> 16] : 0u;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (e0 + q * un >= cnt) break;
+            const int lv = (int)(int16_t)(v[q] & 0xFFFFu);
+            const int f = vcg_freq(idx[q]);
+            const uint32_t mag = (uint32_t)(lv < 0 ? -lv : lv);
+            int r = (int)(mag * (uint32_t)st[f] + (uint32_t)bi[f] + 128u) >> 8;
+            r = lv < 0 ? (int)(0u - (uint32_t)r) : r;
+            t[vcg_tidx((int)(idx[q] >> 10), (int)((idx[q] >> 5) & 31), (int)(idx[q] & 31))] = r;
+            mine += (uint64_t)(r < 0 ? -(int64_t)r : (int64_t)r);
+        }
     }
     return mine;
 }
