@@ -40,6 +40,17 @@ extern "C" int vcg_init(const vcg_tables *t) {
                      lds_block, lds_optin, need);
             vc__panic(msg);
         }
+        // the inverse transform regroups its even-row sums by these exact
+        // symmetries of the basis (they hold for the Q14 table by construction)
+        for (int k = 0; k < 32; ++k)
+            for (int n = 0; n < 16; ++n) {
+                const int sgn = (k & 1) ? -1 : 1;
+                bool ok = t->cmat[k][31 - n] == sgn * t->cmat[k][n];
+                if ((k & 1) == 0 && n < 8) ok = ok && t->cmat[k][15 - n] == ((k & 2) ? -1 : 1) * t->cmat[k][n];
+                if ((k & 3) == 0 && n < 4) ok = ok && t->cmat[k][7 - n] == ((k & 4) ? -1 : 1) * t->cmat[k][n];
+                if ((k & 7) == 0 && n < 2) ok = ok && t->cmat[k][3 - n] == ((k & 8) ? -1 : 1) * t->cmat[k][n];
+                if (!ok) vc__panic("DCT basis lacks the symmetries the inverse transform relies on");
+            }
         vcg_dtab h;
         memcpy(h.cmat, t->cmat, sizeof h.cmat);
         memcpy(h.gstep, t->gstep, sizeof h.gstep);

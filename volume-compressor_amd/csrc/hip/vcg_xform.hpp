@@ -63,46 +63,106 @@ VCG_HD int vcg_acc(int acc, int c, int a) {
 // inverse
 // ===========================================================================
 // Two inverse lines read from the tile (element k of line i at s_i[k*stride])
-// with only the first nk coefficients possibly nonzero (nk a multiple of 4):
+// with only the first nk coefficients possibly nonzero (nk a multiple of 8):
 //   evn[n] = sum_{k even} C[k][n] v[k],  odd[n] = sum_{k odd} C[k][n] v[k]
 //   o[n] = (evn+odd+8192)>>14,  o[31-n] = (evn-odd+8192)>>14     (i32 wrap)
-// src/vc/vc.c:178-184.  The lines share every basis row load, and each loop
-// step covers two row pairs, so a workgroup-uniform step loads 64 constants.
+// src/vc/vc.c:178-184.  evn is regrouped exactly (integer sums, mod 2^32) by
+// the even-row symmetries of the basis, C[2j][15-n] = (-1)^j C[2j][n],
+// C[4m][7-n] = (-1)^m C[4m][n], C[8p][3-n] = (-1)^p C[8p][n] (checked on the
+// table at device init): evn from 8x8 + 4x4 + 2x2 + 2x2 products instead of
+// 16x16, 344 multiply-adds per line instead of 512.  The two lines share
+// every basis load; basis rows are workgroup-uniform (scalar registers).
 template <bool FAST>
 VCG_HD void vcg_inv_line2(const int32_t (*C)[32], const int *s0, const int *s1, int stride, int nk, int *o0,
                           int *o1) {
-    int e0[16], d0[16], e1[16], d1[16];
+    struct row16 { int32_t v[16]; };
+    struct row8 { int32_t v[8]; };
+    struct row4 { int32_t v[4]; };
+    struct row2 { int32_t v[2]; };
+    int d0[16], d1[16], q0[8], q1[8];
 #pragma unroll
-    for (int n = 0; n < 16; ++n) { e0[n] = 0; d0[n] = 0; e1[n] = 0; d1[n] = 0; }
-    struct row16 { int32_t v[16]; };                  // C[k][0..15], one 64-byte load
+    for (int n = 0; n < 16; ++n) { d0[n] = 0; d1[n] = 0; }
+#pragma unroll
+    for (int n = 0; n < 8; ++n) { q0[n] = 0; q1[n] = 0; }
+    // odd rows k = 1,3,...  and rows k = 2 mod 4, eight coefficients per step
 #pragma unroll 1
-    for (int k = 0; k < nk; k += 4) {
+    for (int kb = 0; kb < nk; kb += 8) {
 #pragma unroll
-        for (int kk = 0; kk < 4; kk += 2) {
-            const int a0 = s0[(k + kk) * stride], b0 = s0[(k + kk + 1) * stride];
-            const int a1 = s1[(k + kk) * stride], b1 = s1[(k + kk + 1) * stride];
-            const row16 re = *(const row16 *)C[k + kk], ro = *(const row16 *)C[k + kk + 1];
+        for (int j = 1; j < 8; j += 2) {
+            const int a0 = s0[(kb + j) * stride], a1 = s1[(kb + j) * stride];
+            const row16 r = *(const row16 *)C[kb + j];
 #pragma unroll
             for (int n = 0; n < 16; ++n) {
-                const int ce = re.v[n], co = ro.v[n];
-                e0[n] = vcg_acc<FAST>(e0[n], ce, a0);
-                d0[n] = vcg_acc<FAST>(d0[n], co, b0);
-                e1[n] = vcg_acc<FAST>(e1[n], ce, a1);
-                d1[n] = vcg_acc<FAST>(d1[n], co, b1);
+                d0[n] = vcg_acc<FAST>(d0[n], r.v[n], a0);
+                d1[n] = vcg_acc<FAST>(d1[n], r.v[n], a1);
             }
+        }
+#pragma unroll
+        for (int j = 2; j < 8; j += 4) {
+            const int a0 = s0[(kb + j) * stride], a1 = s1[(kb + j) * stride];
+            const row8 r = *(const row8 *)C[kb + j];
+#pragma unroll
+            for (int n = 0; n < 8; ++n) {
+                q0[n] = vcg_acc<FAST>(q0[n], r.v[n], a0);
+                q1[n] = vcg_acc<FAST>(q1[n], r.v[n], a1);
+            }
+        }
+    }
+    // rows k = 4 mod 8 (n < 4), k = 8 mod 16 and k = 0 mod 16 (n < 2)
+    int eeo0[4] = {0, 0, 0, 0}, eeo1[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 4; k < 32; k += 8) {
+        const int a0 = s0[k * stride], a1 = s1[k * stride];
+        const row4 r = *(const row4 *)C[k];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            eeo0[n] = vcg_acc<FAST>(eeo0[n], r.v[n], a0);
+            eeo1[n] = vcg_acc<FAST>(eeo1[n], r.v[n], a1);
+        }
+    }
+    int x0[2] = {0, 0}, x1[2] = {0, 0}, y0[2] = {0, 0}, y1[2] = {0, 0};   // x: k = 8, 24; y: k = 0, 16
+#pragma unroll
+    for (int k = 0; k < 32; k += 8) {
+        const int a0 = s0[k * stride], a1 = s1[k * stride];
+        const row2 r = *(const row2 *)C[k];
+#pragma unroll
+        for (int n = 0; n < 2; ++n) {
+            if (k & 8) { x0[n] = vcg_acc<FAST>(x0[n], r.v[n], a0); x1[n] = vcg_acc<FAST>(x1[n], r.v[n], a1); }
+            else { y0[n] = vcg_acc<FAST>(y0[n], r.v[n], a0); y1[n] = vcg_acc<FAST>(y1[n], r.v[n], a1); }
+        }
+    }
+    // recombine: eee (n<4) -> ee (n<8) -> evn (n<16) -> outputs
+    uint32_t e0[16], e1[16];
+    {
+        uint32_t a[4], b[4], c[8], d[8];
+#pragma unroll
+        for (int n = 0; n < 2; ++n) {
+            a[n] = (uint32_t)y0[n] + (uint32_t)x0[n]; a[3 - n] = (uint32_t)y0[n] - (uint32_t)x0[n];
+            b[n] = (uint32_t)y1[n] + (uint32_t)x1[n]; b[3 - n] = (uint32_t)y1[n] - (uint32_t)x1[n];
+        }
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            c[n] = a[n] + (uint32_t)eeo0[n]; c[7 - n] = a[n] - (uint32_t)eeo0[n];
+            d[n] = b[n] + (uint32_t)eeo1[n]; d[7 - n] = b[n] - (uint32_t)eeo1[n];
+        }
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            e0[n] = c[n] + (uint32_t)q0[n]; e0[15 - n] = c[n] - (uint32_t)q0[n];
+            e1[n] = d[n] + (uint32_t)q1[n]; e1[15 - n] = d[n] - (uint32_t)q1[n];
         }
     }
 #pragma unroll
     for (int n = 0; n < 16; ++n) {
-        o0[n] = (int)((uint32_t)e0[n] + (uint32_t)d0[n] + 8192u) >> 14;
-        o0[31 - n] = (int)((uint32_t)e0[n] - (uint32_t)d0[n] + 8192u) >> 14;
-        o1[n] = (int)((uint32_t)e1[n] + (uint32_t)d1[n] + 8192u) >> 14;
-        o1[31 - n] = (int)((uint32_t)e1[n] - (uint32_t)d1[n] + 8192u) >> 14;
+        o0[n] = (int)(e0[n] + (uint32_t)d0[n] + 8192u) >> 14;
+        o0[31 - n] = (int)(e0[n] - (uint32_t)d0[n] + 8192u) >> 14;
+        o1[n] = (int)(e1[n] + (uint32_t)d1[n] + 8192u) >> 14;
+        o1[31 - n] = (int)(e1[n] - (uint32_t)d1[n] + 8192u) >> 14;
     }
 }
 
-// term count covering coefficients 0..kmax, rounded up to a multiple of 4
-VCG_HD int vcg_nk4(int kmax) { return kmax < 0 ? 0 : (kmax > 27 ? 32 : ((kmax + 4) & ~3)); }
+// term count covering coefficients 0..kmax, rounded up to a multiple of 8
+VCG_HD int vcg_nk8(int kmax) { return kmax < 0 ? 0 : (kmax > 23 ? 32 : ((kmax + 8) & ~7)); }
+
 
 VCG_HD void vcg_tile_zero(int *t, int tid, int nthr) {
     struct i4 { int a, b, c, d; };
@@ -140,7 +200,7 @@ VCG_HD uint64_t vcg_scatter(int *t, const uint32_t *L, uint32_t cnt, const uint1
 
 // The three passes give each thread the line pairs (line, line + nthr) for
 // line = tid, tid + 2*nthr, ... (nthr divides 512).  The pair's term count is
-// the larger line's, made wave-uniform on the GPU (vcg_nk) and rounded to 4;
+// the larger line's, made wave-uniform on the GPU (vcg_nk) and rounded to 8;
 // the extra terms multiply zeros.
 
 // pass 1 (along x): lines (kz, ky); coefficients beyond kx = fmax-kz-ky are zero
@@ -149,7 +209,7 @@ VCG_HD void vcg_inv_pass_x(int *t, const int32_t (*C)[32], int fmax, int tid, in
     for (int l0 = tid; l0 < 1024; l0 += 2 * nthr) {
         const int l1 = l0 + nthr;
         const int km0 = fmax - (l0 >> 5) - (l0 & 31), km1 = fmax - (l1 >> 5) - (l1 & 31);
-        const int nk = vcg_nk4(vcg_nk(km0 > km1 ? km0 : km1) - 1);
+        const int nk = vcg_nk8(vcg_nk(km0 > km1 ? km0 : km1) - 1);
         if (nk == 0) continue;
         int *r0 = t + vcg_tidx(l0 >> 5, l0 & 31, 0), *r1 = t + vcg_tidx(l1 >> 5, l1 & 31, 0);
         int o0[32], o1[32];
@@ -170,7 +230,7 @@ VCG_HD void vcg_inv_pass_y(int *t, const int32_t (*C)[32], int fmax, int tid, in
     for (int l0 = tid; l0 < 1024; l0 += 2 * nthr) {
         const int l1 = l0 + nthr;
         const int km0 = fmax - (l0 >> 5), km1 = fmax - (l1 >> 5);
-        const int nk = vcg_nk4(vcg_nk(km0 > km1 ? km0 : km1) - 1);
+        const int nk = vcg_nk8(vcg_nk(km0 > km1 ? km0 : km1) - 1);
         if (nk == 0) continue;
         int *c0 = t + vcg_tidx(l0 >> 5, 0, l0 & 31), *c1 = t + vcg_tidx(l1 >> 5, 0, l1 & 31);
         int o0[32], o1[32];
@@ -190,7 +250,7 @@ VCG_HD void vcg_inv_pass_y(int *t, const int32_t (*C)[32], int fmax, int tid, in
 // vox[32..63] = line tid + 512.
 template <bool FAST>
 VCG_HD void vcg_inv_pass_z(const int *t, const int32_t (*C)[32], int fmax, int dc, int tid, uint8_t *vox) {
-    const int nk = vcg_nk4(fmax);
+    const int nk = vcg_nk8(fmax);
     const int l0 = tid, l1 = tid + 512;
     int o0[32], o1[32];
     vcg_inv_line2<FAST>(C, t + vcg_tidx(0, l0 >> 5, l0 & 31), t + vcg_tidx(0, l1 >> 5, l1 & 31), VCG_SZ, nk, o0,
