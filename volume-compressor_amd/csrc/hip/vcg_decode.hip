@@ -76,6 +76,9 @@ __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__res
                                                 uint32_t *__restrict__ lists,
                                                 vcg_item *__restrict__ items,
                                                 uint32_t *__restrict__ status) {
+    // issue priority over transform waves sharing the SIMD (the serial
+    // decode is the critical path; the transform fills its idle issue cycles)
+    __builtin_amdgcn_s_setprio(3);
     __shared__ uint32_t ctxm[16 * 64];
     __shared__ uint32_t ringm[VCG_RING * 64];
     __shared__ uint32_t bands[9];
@@ -276,24 +279,45 @@ static void ensure_attr() {
     }
 }
 
-static void run_batch(vcg_ctx *c, dec_job j, uint32_t total) {
+// Launch the entropy / transform pairs for `total` items on the internal
+// streams (see vcg_ctx) and join the result into c->s.  inputs_on_s: the job
+// reads buffers this call staged on c->s (list quads, payloads), so the entropy
+// pass waits for them too; the transform always waits for earlier work on c->s
+// (it writes the caller's output).  Returns the scratch slot of the last batch.
+static int run_batch(vcg_ctx *c, dec_job j, uint32_t total, bool inputs_on_s) {
     ensure_attr();
     uint32_t maxb = total < MAX_BATCH ? total : MAX_BATCH;
-    vcg_grow((void **)&c->lists, &c->lists_items, maxb, (size_t)VCG_LIST_STRIDE * 4);
-    vcg_grow((void **)&c->items, &c->items_cap, maxb, sizeof(vcg_item));
+    for (int k = 0; k < 2; ++k) {
+        vcg_grow((void **)&c->lists[k], &c->lists_items[k], maxb, (size_t)VCG_LIST_STRIDE * 4);
+        vcg_grow((void **)&c->items[k], &c->items_cap[k], maxb, sizeof(vcg_item));
+    }
+    VCG_CHECK(hipEventRecord(c->evF, c->s));
     for (uint32_t b = 0; b < total; b += maxb) {
+        const int sl = c->slot;
+        c->slot ^= 1;
         j.base = b;
         j.n = total - b < maxb ? total - b : maxb;
-        if (c->ev[0]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[0], c->s));
-        hipLaunchKernelGGL(k_entropy, dim3((j.n + 63) / 64), dim3(64), 0, c->s, j, g_dtab, c->lists,
-                           c->items, c->status_d);
+        // entropy: after the transform that last read this slot
+        if (inputs_on_s) VCG_CHECK(hipStreamWaitEvent(c->sE, c->evF, 0));
+        VCG_CHECK(hipStreamWaitEvent(c->sE, c->evI[sl], 0));
+        if (c->ev[0]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[0], c->sE));
+        hipLaunchKernelGGL(k_entropy, dim3((j.n + 63) / 64), dim3(64), 0, c->sE, j, g_dtab, c->lists[sl],
+                           c->items[sl], c->status_d);
         VCG_CHECK(hipGetLastError());
-        if (c->ev[1]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[1], c->s));
-        hipLaunchKernelGGL(k_idct, dim3(j.n), dim3(IDCT_THREADS), LDS_BYTES, c->s, j, g_dtab,
-                           c->lists, c->items);
+        if (c->ev[1]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[1], c->sE));
+        VCG_CHECK(hipEventRecord(c->evE[sl], c->sE));
+        // transform: after its entropy pass and after earlier work on c->s
+        VCG_CHECK(hipStreamWaitEvent(c->sI, c->evE[sl], 0));
+        VCG_CHECK(hipStreamWaitEvent(c->sI, c->evF, 0));
+        hipLaunchKernelGGL(k_idct, dim3(j.n), dim3(IDCT_THREADS), LDS_BYTES, c->sI, j, g_dtab,
+                           c->lists[sl], c->items[sl]);
         VCG_CHECK(hipGetLastError());
-        if (c->ev[2]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[2], c->s));
+        if (c->ev[2]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[2], c->sI));
+        VCG_CHECK(hipEventRecord(c->evI[sl], c->sI));
+        c->last_slot = sl;
     }
+    VCG_CHECK(hipStreamWaitEvent(c->s, c->evI[c->last_slot], 0));
+    return c->last_slot;
 }
 
 extern "C" int vcg_decode_box(vcg_arc *a, int lod, const vcg_box *box, uint8_t *out, int out_host,
@@ -323,7 +347,7 @@ extern "C" int vcg_decode_box(vcg_arc *a, int lod, const vcg_box *box, uint8_t *
     } else {
         j.out = out;
     }
-    run_batch(c, j, j.nax * j.nay * naz);
+    run_batch(c, j, j.nax * j.nay * naz, false);
     if (out_host) VCG_CHECK(hipMemcpyAsync(out, c->stage, bytes, hipMemcpyDeviceToHost, c->s));
     if (sync || out_host) return vcg_sync();
     return 0;
@@ -352,10 +376,10 @@ extern "C" int vcg_decode_list(vcg_arc *a, size_t n, const int32_t *quads, uint8
         dec_job jj = j;
         jj.quads = c->quads + 4 * b;
         jj.out = c->stage + b * (size_t)VCG_A3;
-        run_batch(c, jj, cnt);
+        const int sl = run_batch(c, jj, cnt, true);
         if (per_item) {
             vcg_item *hi = (vcg_item *)malloc(cnt * sizeof(vcg_item));
-            VCG_CHECK(hipMemcpyAsync(hi, c->items, cnt * sizeof(vcg_item), hipMemcpyDeviceToHost, c->s));
+            VCG_CHECK(hipMemcpyAsync(hi, c->items[sl], cnt * sizeof(vcg_item), hipMemcpyDeviceToHost, c->s));
             VCG_CHECK(hipStreamSynchronize(c->s));
             for (uint32_t k = 0; k < cnt; ++k) per_item[b + k] = hi[k].status == 2 ? 1 : 0;
             free(hi);
@@ -396,10 +420,10 @@ extern "C" int vcg_decode_payloads(size_t n, const uint8_t *pay, const uint64_t 
         jj.base = 0;
         jj.offs = doffs + b; jj.lens = dlens + b;
         jj.out = c->stage + b * (size_t)VCG_A3;
-        run_batch(c, jj, cnt);
+        const int sl = run_batch(c, jj, cnt, true);
         if (per_item) {
             vcg_item *hi = (vcg_item *)malloc(cnt * sizeof(vcg_item));
-            VCG_CHECK(hipMemcpyAsync(hi, c->items, cnt * sizeof(vcg_item), hipMemcpyDeviceToHost, c->s));
+            VCG_CHECK(hipMemcpyAsync(hi, c->items[sl], cnt * sizeof(vcg_item), hipMemcpyDeviceToHost, c->s));
             VCG_CHECK(hipStreamSynchronize(c->s));
             for (uint32_t k = 0; k < cnt; ++k) per_item[b + k] = hi[k].status == 2 ? 1 : 0;
             free(hi);

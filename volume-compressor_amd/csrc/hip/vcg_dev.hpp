@@ -38,10 +38,17 @@ struct vcg_item {
 #define VCG_LIST_STRIDE (VCG_LIST_CAP + 64u)
 
 // Per-thread execution context (one HIP stream, grow-only scratch).
+// Per host thread.  s is the thread's public stream (vcx_stream): calls are
+// ordered on it.  A decode runs its entropy and transform launches on two
+// internal streams sE / sI with two scratch slots, so the entropy pass of one
+// batch overlaps the transform of the previous one; slot reuse, inputs staged
+// on s, and the output (joined back into s) are ordered by events.
 struct vcg_ctx {
-    hipStream_t s = nullptr;
-    uint32_t *lists = nullptr;  size_t lists_items = 0;
-    vcg_item *items = nullptr;  size_t items_cap = 0;
+    hipStream_t s = nullptr, sE = nullptr, sI = nullptr;
+    hipEvent_t evF = nullptr, evE[2] = {nullptr, nullptr}, evI[2] = {nullptr, nullptr};
+    int slot = 0, last_slot = 0;
+    uint32_t *lists[2] = {nullptr, nullptr};  size_t lists_items[2] = {0, 0};
+    vcg_item *items[2] = {nullptr, nullptr};  size_t items_cap[2] = {0, 0};
     uint8_t  *stage = nullptr;  size_t stage_cap = 0;
     int32_t  *quads = nullptr;  size_t quads_cap = 0;
     uint32_t *status_d = nullptr;

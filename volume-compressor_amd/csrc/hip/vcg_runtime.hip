@@ -84,6 +84,13 @@ vcg_ctx *vcg_get_ctx() {
     vcg_ctx *c = new vcg_ctx();
     VCG_CHECK(hipSetDevice(0));
     VCG_CHECK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    VCG_CHECK(hipStreamCreateWithFlags(&c->sE, hipStreamNonBlocking));
+    VCG_CHECK(hipStreamCreateWithFlags(&c->sI, hipStreamNonBlocking));
+    VCG_CHECK(hipEventCreateWithFlags(&c->evF, hipEventDisableTiming));
+    for (int k = 0; k < 2; ++k) {
+        VCG_CHECK(hipEventCreateWithFlags(&c->evE[k], hipEventDisableTiming));
+        VCG_CHECK(hipEventCreateWithFlags(&c->evI[k], hipEventDisableTiming));
+    }
     VCG_CHECK(hipMalloc(&c->status_d, 4));
     VCG_CHECK(hipHostMalloc((void **)&c->status_h, 4, hipHostMallocDefault));
     VCG_CHECK(hipMemsetAsync(c->status_d, 0, 4, c->s));
