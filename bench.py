@@ -114,6 +114,69 @@ def cpu_baseline(arc, atoms_zyx, budget_s=12.0):
                       f"({reps * n * 32768 / 2**20:.0f} MiB out), {t:.1f} s, {label}, OpenMP schedule(dynamic)"}
 
 
+def serve_bench(pyvc, a, nreq=300, qps=400.0, seed=11):
+    """Open-loop Poisson trace through the continuous-batching server:
+    requests of round(exp(U(ln 1, ln 4096))) atoms, LOD 0 with p = 0.7 else
+    uniform over the coarser LODs, coordinates uniform over that LOD's atom
+    grid.  Latency = completion (server clock) - scheduled arrival."""
+    rng = np.random.default_rng(seed)
+    nlod = 0
+    while a.lod_dims(nlod)[0] == 0:
+        nlod += 1
+    reqs = []
+    for _ in range(nreq):
+        n = int(round(np.exp(rng.uniform(0.0, np.log(4096.0)))))
+        lod = 0 if rng.random() < 0.7 or nlod == 1 else int(rng.integers(1, nlod))
+        _, (nx, ny, nz) = a.lod_dims(lod)
+        q = np.stack([np.full(n, lod), rng.integers(0, (nx + 31) // 32, n), rng.integers(0, (ny + 31) // 32, n),
+                      rng.integers(0, (nz + 31) // 32, n)], 1).astype(np.int32)
+        reqs.append(q)
+    gaps = rng.exponential(1.0 / qps, nreq)
+    total = sum(len(q) for q in reqs) * 32768
+    arena = pyvc.PinnedBuffer(total)
+    srv = pyvc.Server(a, 6 << 30)
+    offs = np.cumsum([0] + [len(q) * 32768 for q in reqs])
+    ids, arrive = {}, np.zeros(nreq)
+    done_t, bad = {}, 0
+    try:
+        t0 = pyvc.now_ns() + 2_000_000
+        sched = t0 + np.cumsum(gaps) * 1e9
+        for k, q in enumerate(reqs):
+            while pyvc.now_ns() < sched[k]:
+                pass
+            i = srv.submit_atoms(q, arena.p + int(offs[k]))
+            assert i >= 0
+            ids[int(i)] = k
+            arrive[k] = sched[k]
+            got, st, t = srv.poll(wait_ms=0)
+            for g, s_, tt in zip(got, st, t):
+                done_t[ids[int(g)]] = int(tt)
+                bad += int(s_ != 0)
+        while len(done_t) < nreq:
+            got, st, t = srv.poll(wait_ms=100)
+            for g, s_, tt in zip(got, st, t):
+                done_t[ids[int(g)]] = int(tt)
+                bad += int(s_ != 0)
+    finally:
+        srv.close()
+    lat = np.array([(done_t[k] - arrive[k]) / 1e6 for k in range(nreq)])
+    span = (max(done_t.values()) - arrive[0]) / 1e9
+    atoms = sum(len(q) for q in reqs)
+    # spot check: three served requests against the plain per-atom path
+    exact = True
+    for k in (0, nreq // 2, nreq - 1):
+        for m in range(min(4, len(reqs[k]))):
+            lod, ax, ay, az = (int(x) for x in reqs[k][m])
+            st, ref = a.decode_atom(lod, ax, ay, az)
+            base = int(offs[k]) + m * 32768
+            exact &= st == 0 and np.array_equal(arena.a[base:base + 32768], ref.ravel())
+    arena.free()
+    return {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+            "requests": nreq, "offered_qps": qps, "achieved_rps": round(nreq / span, 1),
+            "atoms_per_s": round(atoms / span, 1), "mean_atoms_per_request": round(atoms / nreq, 1),
+            "errors": bad, "spot_check_exact": bool(exact)}
+
+
 def pmc_traffic(kernel, size, ratio):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (calibrated against a plain copy), or None."""
@@ -136,6 +199,8 @@ def main():
     ap.add_argument("--ratio", type=float, default=10.0)
     ap.add_argument("--sheets", type=int, default=160)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--qps", type=float, default=400.0, help="offered request rate of the serving trace")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="profiler run: only the LOD0 decode launches (no per-atom calls, no CPU leg), "
                          "then one 1 GiB calibration copy kernel")
@@ -245,6 +310,16 @@ def main():
     lat = np.array(lat[10:])
     p50, p99 = (float(np.percentile(lat, 50)), float(np.percentile(lat, 99))) if lat.size else (None, None)
 
+    # ---- batched serving: continuous-batching request loop under a Poisson trace ----
+    served = None
+    if not (args.no_serve or args.profile):
+        served = serve_bench(pyvc, a, qps=args.qps)
+        log(f"serving: {served}")
+        if p50:
+            # the plain path serves a request with one vc_decode_atom call per atom
+            served["plain_rps_estimate"] = round(1e3 / (p50 * served["mean_atoms_per_request"]), 3)
+            served["rps_vs_plain"] = round(served["achieved_rps"] / served["plain_rps_estimate"], 1)
+
     # ---- CPU baseline ----
     cpu = None
     if not (args.no_cpu or args.profile):
@@ -284,7 +359,7 @@ def main():
         "single_call_ms": round(float(np.mean(k12)), 4),
         "plain_atom_latency_ms": ({"p50": round(p50, 4), "p99": round(p99, 4), "n": int(lat.size)}
                                   if lat.size else None),
-        "batched_atom_latency_ms": None,
+        "batched_atom_latency_ms": served,
         "encode_s": round(t_enc, 3),
         "cpu_baseline": cpu,
     }

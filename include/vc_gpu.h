@@ -84,6 +84,36 @@ vc_status vcx_decode_payloads(size_t n, const uint8_t *pay, const uint64_t *offs
  * memory. */
 vc_status vcx_synth_device(uint64_t seed, vc_dims dims, int nsheets, void *dev_out);
 
+/* ---- serving: device-resident request loop with continuous batching ----
+ * No counterpart in the reference (its vc_decode_atom / vc_decode_region are
+ * one call per request); callers are request servers that want many
+ * concurrent atom and region requests decoded together.
+ *
+ * vcx_server_create keeps the archive on the device and starts a worker
+ * thread.  device_bytes bounds the output memory of the batches in flight
+ * (three batch slots share it; a request larger than one slot is rejected).
+ * Requests are admitted in arrival order as slots free up; batches overlap on
+ * the device.  Submit returns a request id (>= 0) or -1 (invalid request or
+ * too large).  Atom requests: quads = (lod, ax, ay, az) x n, out receives n
+ * atoms of VC_ATOM3 bytes; region requests: out receives the box raster as
+ * vc_decode_region writes it.  out must stay valid until the request
+ * completes; pinned memory (vcx_host_alloc) keeps the result copy
+ * asynchronous.  A request decodes to the same bytes alone or in any mix.
+ * vcx_server_poll returns up to max completions (id, status as vc_status,
+ * completion time on CLOCK_MONOTONIC in ns, see vcx_now_ns), waiting up to
+ * wait_ms for the first one.  Destroy waits for every submitted request. */
+typedef struct vcx_server vcx_server;
+vcx_server *vcx_server_create(vc_archive *a, size_t device_bytes);
+void        vcx_server_destroy(vcx_server *s);
+int64_t     vcx_server_submit_atoms(vcx_server *s, size_t n, const int32_t *quads, uint8_t *out);
+int64_t     vcx_server_submit_region(vcx_server *s, int lod, vc_box box, uint8_t *out);
+size_t      vcx_server_poll(vcx_server *s, int64_t *ids, int32_t *status, uint64_t *t_done_ns, size_t max,
+                            int wait_ms);
+size_t      vcx_server_capacity(const vcx_server *s);   /* atoms per batch slot */
+uint64_t    vcx_now_ns(void);
+void       *vcx_host_alloc(size_t bytes);               /* pinned host memory */
+void        vcx_host_free(void *p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,6 +38,7 @@ struct dec_job {
     const uint32_t *lens;
     // output placement
     uint8_t *out;
+    int32_t *status_out;                // optional: per item 1 = corrupt payload, else 0
     uint32_t bx0, by0, bz0, bx1, by1, bz1;   // box mode: voxel box
 };
 
@@ -216,6 +217,7 @@ __global__ __launch_bounds__(IDCT_THREADS) void k_idct(dec_job j, const vcg_dtab
     item_atom(j, gi, lod, ax, ay, az);
     const vcg_item it = items[i];
     uint8_t *stage = (uint8_t *)buf;
+    if (tid == 0 && j.status_out) j.status_out[gi] = it.status == 2 ? 1 : 0;
     if (it.status != 1) {
         for (int v = tid; v < (int)VCG_A3 / 4; v += IDCT_THREADS) ((uint32_t *)stage)[v] = 0;
         __syncthreads();
@@ -284,28 +286,34 @@ static void ensure_attr() {
 // reads buffers this call staged on c->s (list quads, payloads), so the entropy
 // pass waits for them too; the transform always waits for earlier work on c->s
 // (it writes the caller's output).  Returns the scratch slot of the last batch.
-static int run_batch(vcg_ctx *c, dec_job j, uint32_t total, bool inputs_on_s) {
+static int run_batch(vcg_ctx *c, dec_job j, uint32_t total, bool inputs_on_s, const void *up_src = nullptr,
+                     void *up_dst = nullptr, size_t up_bytes = 0) {
     ensure_attr();
     uint32_t maxb = total < MAX_BATCH ? total : MAX_BATCH;
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < VCG_SLOTS; ++k) {
         vcg_grow((void **)&c->lists[k], &c->lists_items[k], maxb, (size_t)VCG_LIST_STRIDE * 4);
         vcg_grow((void **)&c->items[k], &c->items_cap[k], maxb, sizeof(vcg_item));
     }
     VCG_CHECK(hipEventRecord(c->evF, c->s));
+    // inputs staged on the first batch's entropy stream (pinned source)
+    if (up_bytes) VCG_CHECK(hipMemcpyAsync(up_dst, up_src, up_bytes, hipMemcpyHostToDevice, c->sE[c->slot]));
+    const int first = c->slot;
     for (uint32_t b = 0; b < total; b += maxb) {
         const int sl = c->slot;
-        c->slot ^= 1;
+        c->slot = (c->slot + 1) % VCG_SLOTS;
+        hipStream_t sE = c->sE[sl];
         j.base = b;
         j.n = total - b < maxb ? total - b : maxb;
         // entropy: after the transform that last read this slot
-        if (inputs_on_s) VCG_CHECK(hipStreamWaitEvent(c->sE, c->evF, 0));
-        VCG_CHECK(hipStreamWaitEvent(c->sE, c->evI[sl], 0));
-        if (c->ev[0]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[0], c->sE));
-        hipLaunchKernelGGL(k_entropy, dim3((j.n + 63) / 64), dim3(64), 0, c->sE, j, g_dtab, c->lists[sl],
+        if (inputs_on_s) VCG_CHECK(hipStreamWaitEvent(sE, c->evF, 0));
+        if (up_bytes && sl != first) VCG_CHECK(hipStreamWaitEvent(sE, c->evE[first], 0));
+        VCG_CHECK(hipStreamWaitEvent(sE, c->evI[sl], 0));
+        if (c->ev[0]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[0], sE));
+        hipLaunchKernelGGL(k_entropy, dim3((j.n + 63) / 64), dim3(64), 0, sE, j, g_dtab, c->lists[sl],
                            c->items[sl], c->status_d);
         VCG_CHECK(hipGetLastError());
-        if (c->ev[1]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[1], c->sE));
-        VCG_CHECK(hipEventRecord(c->evE[sl], c->sE));
+        if (c->ev[1]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[1], sE));
+        VCG_CHECK(hipEventRecord(c->evE[sl], sE));
         // transform: after its entropy pass and after earlier work on c->s
         VCG_CHECK(hipStreamWaitEvent(c->sI, c->evE[sl], 0));
         VCG_CHECK(hipStreamWaitEvent(c->sI, c->evF, 0));
@@ -369,25 +377,22 @@ extern "C" int vcg_decode_list(vcg_arc *a, size_t n, const int32_t *quads, uint8
     j.mode = MODE_LIST;
     j.quads = c->quads;
     j.out = c->stage;
-    int rc = 0;
-    // per-item statuses need the item records of each launch batch
+    int32_t *dst = nullptr;
+    if (per_item) {
+        vcg_grow((void **)&c->stat, &c->stat_cap, n, 4);
+        dst = c->stat;
+    }
     for (size_t b = 0; b < n; b += MAX_BATCH) {
         uint32_t cnt = (uint32_t)(n - b < MAX_BATCH ? n - b : MAX_BATCH);
         dec_job jj = j;
         jj.quads = c->quads + 4 * b;
         jj.out = c->stage + b * (size_t)VCG_A3;
-        const int sl = run_batch(c, jj, cnt, true);
-        if (per_item) {
-            vcg_item *hi = (vcg_item *)malloc(cnt * sizeof(vcg_item));
-            VCG_CHECK(hipMemcpyAsync(hi, c->items[sl], cnt * sizeof(vcg_item), hipMemcpyDeviceToHost, c->s));
-            VCG_CHECK(hipStreamSynchronize(c->s));
-            for (uint32_t k = 0; k < cnt; ++k) per_item[b + k] = hi[k].status == 2 ? 1 : 0;
-            free(hi);
-        }
+        jj.status_out = dst ? dst + b : nullptr;
+        run_batch(c, jj, cnt, true);
     }
+    if (per_item) VCG_CHECK(hipMemcpyAsync(per_item, dst, n * 4, hipMemcpyDeviceToHost, c->s));
     VCG_CHECK(hipMemcpyAsync(out, c->stage, n * (size_t)VCG_A3, hipMemcpyDeviceToHost, c->s));
-    rc = vcg_sync();
-    return rc;
+    return vcg_sync();
 }
 
 extern "C" int vcg_decode_payloads(size_t n, const uint8_t *pay, const uint64_t *offs, const uint32_t *lens,
@@ -414,23 +419,69 @@ extern "C" int vcg_decode_payloads(size_t n, const uint8_t *pay, const uint64_t 
     j.arc = dpay; j.mode = MODE_PAYLOAD; j.offs = doffs; j.lens = dlens;
     j.dqs = ddq; j.dqb = ddq + VCG_NFREQ;
     j.out = c->stage;
+    int32_t *dst = nullptr;
+    if (per_item) {
+        vcg_grow((void **)&c->stat, &c->stat_cap, n, 4);
+        dst = c->stat;
+    }
     for (size_t b = 0; b < n; b += MAX_BATCH) {
         uint32_t cnt = (uint32_t)(n - b < MAX_BATCH ? n - b : MAX_BATCH);
         dec_job jj = j;
         jj.base = 0;
         jj.offs = doffs + b; jj.lens = dlens + b;
         jj.out = c->stage + b * (size_t)VCG_A3;
-        const int sl = run_batch(c, jj, cnt, true);
-        if (per_item) {
-            vcg_item *hi = (vcg_item *)malloc(cnt * sizeof(vcg_item));
-            VCG_CHECK(hipMemcpyAsync(hi, c->items[sl], cnt * sizeof(vcg_item), hipMemcpyDeviceToHost, c->s));
-            VCG_CHECK(hipStreamSynchronize(c->s));
-            for (uint32_t k = 0; k < cnt; ++k) per_item[b + k] = hi[k].status == 2 ? 1 : 0;
-            free(hi);
-        }
+        jj.status_out = dst ? dst + b : nullptr;
+        run_batch(c, jj, cnt, true);
     }
+    if (per_item) VCG_CHECK(hipMemcpyAsync(per_item, dst, n * 4, hipMemcpyDeviceToHost, c->s));
     VCG_CHECK(hipMemcpyAsync(out, c->stage, n * (size_t)VCG_A3, hipMemcpyDeviceToHost, c->s));
     int rc = vcg_sync();
     VCG_CHECK(hipFree(dpay)); VCG_CHECK(hipFree(doffs)); VCG_CHECK(hipFree(dlens)); VCG_CHECK(hipFree(ddq));
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// asynchronous launchers (serving loop): enqueue and return; completion is
+// observed through an event recorded on the calling thread's stream
+// ---------------------------------------------------------------------------
+extern "C" int vcg_list_launch(vcg_arc *a, size_t n, const int32_t *quads_pinned, int32_t *quads_dev,
+                               uint8_t *out_dev, int32_t *status_dev) {
+    vcg_ctx *c = vcg_get_ctx();
+    if (n == 0 || n > MAX_BATCH) return -1;
+    const vcg_geom *gd = vcg_arc_geom_dev(a);
+    dec_job j;
+    memset(&j, 0, sizeof j);
+    j.arc = vcg_arc_buf(a);
+    j.geom = gd;
+    j.dqs = &gd->dq_step[0][0];
+    j.dqb = &gd->dq_bias[0][0];
+    j.mode = MODE_LIST;
+    j.quads = quads_dev;
+    j.out = out_dev;
+    j.status_out = status_dev;
+    run_batch(c, j, (uint32_t)n, false, quads_pinned, quads_dev, n * 16);
+    return 0;
+}
+
+extern "C" int vcg_box_launch(vcg_arc *a, int lod, const vcg_box *box, uint8_t *out_dev, int32_t *status_dev) {
+    vcg_ctx *c = vcg_get_ctx();
+    const vcg_geom *gd = vcg_arc_geom_dev(a);
+    dec_job j;
+    memset(&j, 0, sizeof j);
+    j.arc = vcg_arc_buf(a);
+    j.geom = gd;
+    j.dqs = &gd->dq_step[0][0];
+    j.dqb = &gd->dq_bias[0][0];
+    j.mode = MODE_BOX;
+    j.lod = lod;
+    j.ax0 = box->x0 / 32; j.ay0 = box->y0 / 32; j.az0 = box->z0 / 32;
+    j.nax = (box->x1 - 1) / 32 - j.ax0 + 1;
+    j.nay = (box->y1 - 1) / 32 - j.ay0 + 1;
+    const uint32_t naz = (box->z1 - 1) / 32 - j.az0 + 1;
+    j.bx0 = box->x0; j.by0 = box->y0; j.bz0 = box->z0;
+    j.bx1 = box->x1; j.by1 = box->y1; j.bz1 = box->z1;
+    j.out = out_dev;
+    j.status_out = status_dev;
+    run_batch(c, j, j.nax * j.nay * naz, false);
+    return 0;
 }

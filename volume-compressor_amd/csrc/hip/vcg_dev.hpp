@@ -39,18 +39,22 @@ struct vcg_item {
 
 // Per-thread execution context (one HIP stream, grow-only scratch).
 // Per host thread.  s is the thread's public stream (vcx_stream): calls are
-// ordered on it.  A decode runs its entropy and transform launches on two
-// internal streams sE / sI with two scratch slots, so the entropy pass of one
-// batch overlaps the transform of the previous one; slot reuse, inputs staged
-// on s, and the output (joined back into s) are ordered by events.
+// ordered on it.  A decode batch runs its entropy launch on the stream of one
+// of VCG_SLOTS scratch slots (sE[slot]) and its transform on sI; batches use
+// the slots in turn, so the entropy passes of consecutive batches or calls run
+// side by side (one pass keeps only half the SIMDs busy) and overlap the
+// transforms.  Slot reuse, inputs staged on s and the output (joined back into
+// s) are ordered by events.
+#define VCG_SLOTS 2
 struct vcg_ctx {
-    hipStream_t s = nullptr, sE = nullptr, sI = nullptr;
-    hipEvent_t evF = nullptr, evE[2] = {nullptr, nullptr}, evI[2] = {nullptr, nullptr};
+    hipStream_t s = nullptr, sE[VCG_SLOTS] = {}, sI = nullptr;
+    hipEvent_t evF = nullptr, evE[VCG_SLOTS] = {}, evI[VCG_SLOTS] = {};
     int slot = 0, last_slot = 0;
-    uint32_t *lists[2] = {nullptr, nullptr};  size_t lists_items[2] = {0, 0};
-    vcg_item *items[2] = {nullptr, nullptr};  size_t items_cap[2] = {0, 0};
+    uint32_t *lists[VCG_SLOTS] = {};  size_t lists_items[VCG_SLOTS] = {};
+    vcg_item *items[VCG_SLOTS] = {};  size_t items_cap[VCG_SLOTS] = {};
     uint8_t  *stage = nullptr;  size_t stage_cap = 0;
     int32_t  *quads = nullptr;  size_t quads_cap = 0;
+    int32_t  *stat = nullptr;   size_t stat_cap = 0;
     uint32_t *status_d = nullptr;
     uint32_t *status_h = nullptr;  // pinned
     void     *ev[3] = {nullptr, nullptr, nullptr};

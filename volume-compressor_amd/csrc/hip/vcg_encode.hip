@@ -221,12 +221,12 @@ static void enc_setup() {
     static bool done = false;
     if (done) return;
     vcg_dtab h;
-    VCG_CHECK(hipMemcpy(&h, g_dtab, sizeof h, hipMemcpyDeviceToHost));
+    vcg_copy(&h, g_dtab, sizeof h, 2);
     vcg_fwd_tab f;
     for (int n = 0; n < 16; ++n)
         for (int j = 0; j < 16; ++j) { f.e[n][j] = h.cmat[2 * j][n]; f.o[n][j] = h.cmat[2 * j + 1][n]; }
     VCG_CHECK(hipMalloc(&g_ftab, sizeof f));
-    VCG_CHECK(hipMemcpy(g_ftab, &f, sizeof f, hipMemcpyHostToDevice));
+    vcg_copy(g_ftab, &f, sizeof f, 1);
     VCG_CHECK(hipFuncSetAttribute((const void *)k_fdct, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_BYTES));
     done = true;
 }

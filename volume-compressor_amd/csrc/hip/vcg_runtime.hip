@@ -19,6 +19,8 @@ void vcg_fail(const char *what, hipError_t e, const char *file, int line) {
     __builtin_unreachable();
 }
 
+extern "C" int vcg_copy(void *dst, const void *src, size_t bytes, int kind);
+
 extern "C" int vcg_init(const vcg_tables *t) {
     std::call_once(g_init_once, [t] {
         int n = 0;
@@ -57,7 +59,9 @@ extern "C" int vcg_init(const vcg_tables *t) {
         memcpy(h.band_start, t->band_start, sizeof h.band_start);
         memcpy(h.scan, t->scan, sizeof h.scan);
         VCG_CHECK(hipMalloc(&g_dtab, sizeof(vcg_dtab)));
-        VCG_CHECK(hipMemcpy(g_dtab, &h, sizeof h, hipMemcpyHostToDevice));
+        // through this thread's stream: the library never touches the null
+        // stream, which would take one of the device's few hardware queues
+        vcg_copy(g_dtab, &h, sizeof h, 1);
         g_init_rc = 0;
     });
     return g_init_rc;
@@ -84,10 +88,10 @@ vcg_ctx *vcg_get_ctx() {
     vcg_ctx *c = new vcg_ctx();
     VCG_CHECK(hipSetDevice(0));
     VCG_CHECK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
-    VCG_CHECK(hipStreamCreateWithFlags(&c->sE, hipStreamNonBlocking));
+    for (int k = 0; k < VCG_SLOTS; ++k) VCG_CHECK(hipStreamCreateWithFlags(&c->sE[k], hipStreamNonBlocking));
     VCG_CHECK(hipStreamCreateWithFlags(&c->sI, hipStreamNonBlocking));
     VCG_CHECK(hipEventCreateWithFlags(&c->evF, hipEventDisableTiming));
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < VCG_SLOTS; ++k) {
         VCG_CHECK(hipEventCreateWithFlags(&c->evE[k], hipEventDisableTiming));
         VCG_CHECK(hipEventCreateWithFlags(&c->evI[k], hipEventDisableTiming));
     }
@@ -133,6 +137,34 @@ extern "C" int vcg_copy(void *dst, const void *src, size_t bytes, int kind) {
     VCG_CHECK(hipStreamSynchronize(c->s));
     return 0;
 }
+
+extern "C" int vcg_copy_async(void *dst, const void *src, size_t bytes, int kind) {
+    vcg_ctx *c = vcg_get_ctx();
+    hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    VCG_CHECK(hipMemcpyAsync(dst, src, bytes, k, c->s));
+    return 0;
+}
+
+extern "C" void *vcg_event_create(void) {
+    hipEvent_t e = nullptr;
+    VCG_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return (void *)e;
+}
+extern "C" void vcg_event_destroy(void *e) { if (e) VCG_CHECK(hipEventDestroy((hipEvent_t)e)); }
+extern "C" void vcg_event_record(void *e) { VCG_CHECK(hipEventRecord((hipEvent_t)e, vcg_get_ctx()->s)); }
+extern "C" int vcg_event_done(void *e) {
+    const hipError_t r = hipEventQuery((hipEvent_t)e);
+    if (r == hipSuccess) return 1;
+    if (r == hipErrorNotReady) { (void)hipGetLastError(); return 0; }
+    vcg_fail("hipEventQuery", r, __FILE__, __LINE__);
+    return 0;
+}
+extern "C" void *vcg_host_alloc(size_t bytes) {
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+    return p;
+}
+extern "C" void vcg_host_free(void *p) { if (p) VCG_CHECK(hipHostFree(p)); }
 
 extern "C" void vcg_set_stage_events(void *ev[3]) {
     vcg_ctx *c = vcg_get_ctx();

@@ -283,6 +283,10 @@ static vc_status region_check(const vc_archive *a, int lod, vc_box b) {
     if (b.x0 >= b.x1 || b.y0 >= b.y1 || b.z0 >= b.z1) return VC_ERR_RANGE;
     return VC_OK;
 }
+vc_status vc__region_check(const vc_archive *a, int lod, vc_box b) { return region_check(a, lod, b); }
+int vc__atom_ok(const vc_archive *a, int lod, int ax, int ay, int az) {
+    return lod >= 0 && lod < a->nlod && atom_in_range(a, lod, ax, ay, az);
+}
 
 /* Same atom set and clipped copy as src/vc/vc.c:1535-1555.  When an atom's
  * slot is invalid the call still writes every other atom and reads that one

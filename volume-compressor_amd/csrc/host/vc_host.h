@@ -80,6 +80,8 @@ uint8_t vc__resolve(const vc_archive *a, int lod, uint32_t az, uint32_t ay, uint
                     uint64_t *off, uint32_t *len);
 int     vc__rd(const vc_archive *a, uint64_t off, uint32_t len, uint8_t *dst);
 vcg_arc *vc__arc_dev(vc_archive *a);   /* NULL on allocation failure */
+vc_status vc__region_check(const vc_archive *a, int lod, vc_box b);
+int     vc__atom_ok(const vc_archive *a, int lod, int ax, int ay, int az);
 /* host-resolved batched decode of whole atoms (streaming archives, writer views) */
 vc_status vc__decode_atoms_host(vc_archive *a, size_t n, const int32_t *quads, uint8_t *out,
                                 vc_status *per_atom);

@@ -52,6 +52,14 @@ int   vcg_copy(void *dst, const void *src, size_t bytes, int kind); /* 1 h2d, 2 
 void  vcg_set_stage_events(void *ev[3]);
 int   vcg_copy_kernel(void *dst, const void *src, size_t bytes); /* 16-B aligned, bytes % 16 == 0 */
 int   vcg_sync(void);                       /* wait; returns accumulated vc status */
+/* asynchronous helpers on the calling thread's stream (serving loop) */
+int   vcg_copy_async(void *dst, const void *src, size_t bytes, int kind);
+void *vcg_event_create(void);
+void  vcg_event_destroy(void *e);
+void  vcg_event_record(void *e);
+int   vcg_event_done(void *e);              /* 1 when everything before the record finished */
+void *vcg_host_alloc(size_t bytes);         /* pinned host memory */
+void  vcg_host_free(void *p);
 
 /* ---- device-resident archive ---- */
 typedef struct vcg_arc vcg_arc;
@@ -65,6 +73,13 @@ int vcg_decode_box(vcg_arc *a, int lod, const vcg_box *box, uint8_t *out, int ou
 /* Decode n whole atoms given as (lod,ax,ay,az) quads into host out (n*32768).
  * Coordinates must already be range-checked. per_item: status per atom. */
 int vcg_decode_list(vcg_arc *a, size_t n, const int32_t *quads, uint8_t *out, int32_t *per_item);
+/* enqueue only: n <= 32768 quads from pinned memory (staged into quads_dev),
+ * atoms to out_dev (n * 32768 B), per-atom 1 = corrupt / 0 to status_dev */
+int vcg_list_launch(vcg_arc *a, size_t n, const int32_t *quads_pinned, int32_t *quads_dev, uint8_t *out_dev,
+                    int32_t *status_dev);
+/* enqueue only: box raster to out_dev, per-atom status (atoms of the box in
+ * x-fastest order) to status_dev */
+int vcg_box_launch(vcg_arc *a, int lod, const vcg_box *box, uint8_t *out_dev, int32_t *status_dev);
 /* Decode n raw payloads (no archive) at one base_q. */
 int vcg_decode_payloads(size_t n, const uint8_t *pay, const uint64_t *offs, const uint32_t *lens,
                         const int32_t dq_step[VCG_NFREQ], const int32_t dq_bias[VCG_NFREQ],

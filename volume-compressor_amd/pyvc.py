@@ -70,6 +70,15 @@ def lib():
         "vcx_h2d": (i32, [vp, vp, sz]),
         "vcx_d2h": (i32, [vp, vp, sz]),
         "vcx_copy_kernel": (i32, [vp, vp, sz]),
+        "vcx_server_create": (vp, [vp, sz]),
+        "vcx_server_destroy": (None, [vp]),
+        "vcx_server_submit_atoms": (C.c_int64, [vp, sz, vp, vp]),
+        "vcx_server_submit_region": (C.c_int64, [vp, i32, Box, vp]),
+        "vcx_server_poll": (sz, [vp, vp, vp, vp, sz, i32]),
+        "vcx_server_capacity": (sz, [vp]),
+        "vcx_now_ns": (C.c_uint64, []),
+        "vcx_host_alloc": (vp, [sz]),
+        "vcx_host_free": (None, [vp]),
         "vcx_decode_lod_device": (i32, [vp, i32, vp, i32]),
         "vcx_decode_region_device": (i32, [vp, i32, Box, vp, i32]),
         "vcx_sync": (i32, []),
@@ -304,6 +313,67 @@ class Writer:
 
 
 # ---- device helpers -------------------------------------------------------------
+class PinnedBuffer:
+    """Pinned host memory (vcx_host_alloc) viewed as a numpy uint8 array."""
+
+    def __init__(self, n):
+        self.L = lib()
+        self.n = n
+        self.p = self.L.vcx_host_alloc(n)
+        if not self.p:
+            raise MemoryError("vcx_host_alloc failed")
+        self.a = np.ctypeslib.as_array((C.c_uint8 * n).from_address(self.p))
+
+    def free(self):
+        if getattr(self, "p", None):
+            self.a = None
+            self.L.vcx_host_free(self.p)
+            self.p = None
+
+    def __del__(self):
+        self.free()
+
+
+class Server:
+    """vcx_server_*: continuous-batching request loop over an opened archive."""
+
+    def __init__(self, archive, device_bytes=4 << 30):
+        self.L = lib()
+        self.archive = archive
+        self.h = self.L.vcx_server_create(archive.h, device_bytes)
+        if not self.h:
+            raise RuntimeError("vcx_server_create failed")
+
+    def capacity(self):
+        return self.L.vcx_server_capacity(self.h)
+
+    def submit_atoms(self, quads, out_ptr):
+        q = np.ascontiguousarray(quads, np.int32).reshape(-1, 4)
+        return self.L.vcx_server_submit_atoms(self.h, len(q), q.ctypes.data, out_ptr)
+
+    def submit_region(self, lod, box, out_ptr):
+        return self.L.vcx_server_submit_region(self.h, lod, Box(*box), out_ptr)
+
+    def poll(self, max_n=1024, wait_ms=10):
+        ids = np.zeros(max_n, np.int64)
+        st = np.zeros(max_n, np.int32)
+        t = np.zeros(max_n, np.uint64)
+        k = self.L.vcx_server_poll(self.h, ids.ctypes.data, st.ctypes.data, t.ctypes.data, max_n, wait_ms)
+        return ids[:k], st[:k], t[:k]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.vcx_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def now_ns():
+    return lib().vcx_now_ns()
+
+
 def device_init():
     return lib().vcx_device_init()
 
