@@ -202,8 +202,8 @@ def main():
     ap.add_argument("--qps", type=float, default=400.0, help="offered request rate of the serving trace")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--profile", action="store_true",
-                    help="profiler run: only the LOD0 decode launches (no per-atom calls, no CPU leg), "
-                         "then one 1 GiB calibration copy kernel")
+                    help="profiler run: only the LOD0 decode launches, each call synchronous (no per-atom "
+                         "calls, no serving trace, no CPU leg), then one 1 GiB calibration copy kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the decode shards by atom "
@@ -268,7 +268,9 @@ def main():
     hip.L.hipStreamSynchronize(stream)
     hip.record(e0, stream)
     for _ in range(args.steps):
-        a.decode_lod_device(0, dout, 0)
+        # profile runs keep every call synchronous so per-kernel durations are
+        # those of isolated launches (the throughput loop overlaps them)
+        a.decode_lod_device(0, dout, 1 if args.profile else 0)
     hip.record(e1, stream)
     ms_total = hip.ms(e0, e1)
     assert L.vcx_sync() == 0
