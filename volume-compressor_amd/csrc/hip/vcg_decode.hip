@@ -281,11 +281,12 @@ static void ensure_attr() {
     }
 }
 
-// Launch the entropy / transform pairs for `total` items on the internal
-// streams (see vcg_ctx) and join the result into c->s.  inputs_on_s: the job
-// reads buffers this call staged on c->s (list quads, payloads), so the entropy
-// pass waits for them too; the transform always waits for earlier work on c->s
-// (it writes the caller's output).  Returns the scratch slot of the last batch.
+// Launch the entropy / transform pairs for `total` items on the slot streams
+// (see vcg_ctx) and join the result into c->s.  inputs_on_s: the job reads
+// buffers this call staged on c->s (list quads, payloads), so the entropy pass
+// waits for them too; the transform always waits for earlier work on c->s (it
+// writes the caller's output).  up_*: an input upload done on the first slot
+// stream instead (pinned source).  Returns the scratch slot of the last batch.
 static int run_batch(vcg_ctx *c, dec_job j, uint32_t total, bool inputs_on_s, const void *up_src = nullptr,
                      void *up_dst = nullptr, size_t up_bytes = 0) {
     ensure_attr();
@@ -295,36 +296,35 @@ static int run_batch(vcg_ctx *c, dec_job j, uint32_t total, bool inputs_on_s, co
         vcg_grow((void **)&c->items[k], &c->items_cap[k], maxb, sizeof(vcg_item));
     }
     VCG_CHECK(hipEventRecord(c->evF, c->s));
-    // inputs staged on the first batch's entropy stream (pinned source)
-    if (up_bytes) VCG_CHECK(hipMemcpyAsync(up_dst, up_src, up_bytes, hipMemcpyHostToDevice, c->sE[c->slot]));
     const int first = c->slot;
+    if (up_bytes) VCG_CHECK(hipMemcpyAsync(up_dst, up_src, up_bytes, hipMemcpyHostToDevice, c->sE[first]));
     for (uint32_t b = 0; b < total; b += maxb) {
         const int sl = c->slot;
         c->slot = (c->slot + 1) % VCG_SLOTS;
         hipStream_t sE = c->sE[sl];
         j.base = b;
         j.n = total - b < maxb ? total - b : maxb;
-        // entropy: after the transform that last read this slot
+        // entropy: after the transform that last used this slot (same stream)
         if (inputs_on_s) VCG_CHECK(hipStreamWaitEvent(sE, c->evF, 0));
-        if (up_bytes && sl != first) VCG_CHECK(hipStreamWaitEvent(sE, c->evE[first], 0));
-        VCG_CHECK(hipStreamWaitEvent(sE, c->evI[sl], 0));
+        if (up_bytes && sl != first) VCG_CHECK(hipStreamWaitEvent(sE, c->evI[first], 0));
         if (c->ev[0]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[0], sE));
         hipLaunchKernelGGL(k_entropy, dim3((j.n + 63) / 64), dim3(64), 0, sE, j, g_dtab, c->lists[sl],
                            c->items[sl], c->status_d);
         VCG_CHECK(hipGetLastError());
         if (c->ev[1]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[1], sE));
-        VCG_CHECK(hipEventRecord(c->evE[sl], sE));
-        // transform: after its entropy pass and after earlier work on c->s
-        VCG_CHECK(hipStreamWaitEvent(c->sI, c->evE[sl], 0));
-        VCG_CHECK(hipStreamWaitEvent(c->sI, c->evF, 0));
-        hipLaunchKernelGGL(k_idct, dim3(j.n), dim3(IDCT_THREADS), LDS_BYTES, c->sI, j, g_dtab,
-                           c->lists[sl], c->items[sl]);
+        // transform: after its entropy pass (same stream) and earlier work on c->s
+        if (!inputs_on_s) VCG_CHECK(hipStreamWaitEvent(sE, c->evF, 0));
+        hipLaunchKernelGGL(k_idct, dim3(j.n), dim3(IDCT_THREADS), LDS_BYTES, sE, j, g_dtab, c->lists[sl],
+                           c->items[sl]);
         VCG_CHECK(hipGetLastError());
-        if (c->ev[2]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[2], c->sI));
-        VCG_CHECK(hipEventRecord(c->evI[sl], c->sI));
+        if (c->ev[2]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[2], sE));
+        VCG_CHECK(hipEventRecord(c->evI[sl], sE));
         c->last_slot = sl;
     }
-    VCG_CHECK(hipStreamWaitEvent(c->s, c->evI[c->last_slot], 0));
+    // join every slot this call used into c->s
+    const uint32_t nb = (total + maxb - 1) / maxb;
+    for (uint32_t k = 0; k < nb && k < VCG_SLOTS; ++k)
+        VCG_CHECK(hipStreamWaitEvent(c->s, c->evI[(first + k) % VCG_SLOTS], 0));
     return c->last_slot;
 }
 

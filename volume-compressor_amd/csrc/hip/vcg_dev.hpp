@@ -39,16 +39,18 @@ struct vcg_item {
 
 // Per-thread execution context (one HIP stream, grow-only scratch).
 // Per host thread.  s is the thread's public stream (vcx_stream): calls are
-// ordered on it.  A decode batch runs its entropy launch on the stream of one
-// of VCG_SLOTS scratch slots (sE[slot]) and its transform on sI; batches use
-// the slots in turn, so the entropy passes of consecutive batches or calls run
-// side by side (one pass keeps only half the SIMDs busy) and overlap the
-// transforms.  Slot reuse, inputs staged on s and the output (joined back into
-// s) are ordered by events.
-#define VCG_SLOTS 2
+// ordered on it.  A decode batch runs its entropy and transform launches on the
+// stream of one of VCG_SLOTS scratch slots (sE[slot]); batches use the slots
+// in turn, so the entropy passes of consecutive batches or calls run side by
+// side (one pass keeps only half the SIMDs busy) and overlap the transforms.
+// A transform also waits for earlier work on s (it writes the caller's
+// output), and the output is joined back into s by an event.  With the public
+// stream that makes four streams: the device's default number of hardware
+// queues.
+#define VCG_SLOTS 3
 struct vcg_ctx {
-    hipStream_t s = nullptr, sE[VCG_SLOTS] = {}, sI = nullptr;
-    hipEvent_t evF = nullptr, evE[VCG_SLOTS] = {}, evI[VCG_SLOTS] = {};
+    hipStream_t s = nullptr, sE[VCG_SLOTS] = {};
+    hipEvent_t evF = nullptr, evI[VCG_SLOTS] = {};
     int slot = 0, last_slot = 0;
     uint32_t *lists[VCG_SLOTS] = {};  size_t lists_items[VCG_SLOTS] = {};
     vcg_item *items[VCG_SLOTS] = {};  size_t items_cap[VCG_SLOTS] = {};

@@ -89,12 +89,8 @@ vcg_ctx *vcg_get_ctx() {
     VCG_CHECK(hipSetDevice(0));
     VCG_CHECK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (int k = 0; k < VCG_SLOTS; ++k) VCG_CHECK(hipStreamCreateWithFlags(&c->sE[k], hipStreamNonBlocking));
-    VCG_CHECK(hipStreamCreateWithFlags(&c->sI, hipStreamNonBlocking));
     VCG_CHECK(hipEventCreateWithFlags(&c->evF, hipEventDisableTiming));
-    for (int k = 0; k < VCG_SLOTS; ++k) {
-        VCG_CHECK(hipEventCreateWithFlags(&c->evE[k], hipEventDisableTiming));
-        VCG_CHECK(hipEventCreateWithFlags(&c->evI[k], hipEventDisableTiming));
-    }
+    for (int k = 0; k < VCG_SLOTS; ++k) VCG_CHECK(hipEventCreateWithFlags(&c->evI[k], hipEventDisableTiming));
     VCG_CHECK(hipMalloc(&c->status_d, 4));
     VCG_CHECK(hipHostMalloc((void **)&c->status_h, 4, hipHostMallocDefault));
     VCG_CHECK(hipMemsetAsync(c->status_d, 0, 4, c->s));
