@@ -355,17 +355,31 @@ VCG_HD int vcg_rc_decode_atom_fsm(const uint8_t *s, uint32_t n, const uint32_t *
 // encoder
 // ---------------------------------------------------------------------------
 struct vcg_renc {
-    uint8_t *out;
+    uint8_t *out;               // NULL: count bytes only
     uint32_t pos, cap;
     uint64_t low;
     uint32_t range;
     uint32_t cache;
     uint32_t csize;
     int ovf;
+    // optional shared byte budget: every 1024 bytes a lane adds 1024 to *kb;
+    // once the total passes `budget` the encode stops (stop = 1)
+    uint32_t *kb;
+    uint32_t budget;
+    int stop;
 };
+VCG_HD uint32_t vcg_atomic_add(uint32_t *p, uint32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return atomicAdd(p, v);
+#else
+    const uint32_t o = *p; *p = o + v; return o;
+#endif
+}
 VCG_HD void vcg_re_put(vcg_renc &e, uint32_t b) {
-    if (e.pos < e.cap) e.out[e.pos] = (uint8_t)b; else e.ovf = 1;
+    if (e.pos < e.cap) { if (e.out) e.out[e.pos] = (uint8_t)b; }
+    else e.ovf = 1;
     e.pos++;
+    if (e.kb && (e.pos & 1023u) == 0u && vcg_atomic_add(e.kb, 1024u) + 1024u > e.budget) e.stop = 1;
 }
 VCG_HD void vcg_re_shift_low(vcg_renc &e) {                  // vc.c:369-380
     if ((uint32_t)(e.low >> 32) != 0u || e.low < 0xFF000000ull) {
@@ -393,12 +407,43 @@ VCG_HD int vcg_nbits(uint32_t t) {                          // floor(log2 t), t 
     return nb;
 }
 
-// Encode levels given in scan order (L[p], p < eob) into out (capacity cap).
-// Returns the byte count; *ovf set if cap was exceeded.  vc.c:513-544,399-401.
+// Sequential reader of the 32768 scan-order levels of one atom (16-byte
+// aligned): 8 levels per 16-byte load, the next block requested one block
+// ahead, so a lane's load latency hides behind the 8 positions before it.
+struct vcg_lvread {
+    struct b16 { uint32_t w[4]; };
+    const b16 *blk;
+    b16 cur, nxt;
+    VCG_HDM void init(const int16_t *L) {
+        blk = (const b16 *)L;
+        cur = blk[0];
+        nxt = blk[1];
+    }
+    // level at position p; positions must be requested in increasing order, one by one
+    VCG_HDM int get(uint32_t p) {
+        if ((p & 7u) == 0u && p) {
+            cur = nxt;
+            const uint32_t b = (p >> 3) + 1u;
+            if (b < 4096u) nxt = blk[b];
+        }
+        const uint32_t q = (p >> 1) & 3u;
+        const uint32_t w = q == 0 ? cur.w[0] : q == 1 ? cur.w[1] : q == 2 ? cur.w[2] : cur.w[3];
+        return (int)(int16_t)(uint16_t)((p & 1u) ? w >> 16 : w);
+    }
+};
+
+// Encode levels given in scan order (L[p], p < eob) into out (capacity cap;
+// out NULL: count only).  Returns the byte count; *ovf set if cap was
+// exceeded.  vc.c:513-544,399-401.  kb/budget: optional shared byte budget
+// (see vcg_renc); *stopped set when the encode gave up past it.
 VCG_HD uint32_t vcg_rc_encode_atom(const int16_t *L, uint32_t eob, const uint32_t *band_start, uint8_t *out,
-                                   uint32_t cap, int *ovf) {
+                                   uint32_t cap, int *ovf, uint32_t *kb = nullptr, uint32_t budget = 0,
+                                   int *stopped = nullptr) {
+    vcg_lvread rd;
+    rd.init(L);
     vcg_renc e;
     e.out = out; e.pos = 0; e.cap = cap; e.low = 0; e.range = 0xFFFFFFFFu; e.cache = 0; e.csize = 1; e.ovf = 0;
+    e.kb = kb; e.budget = budget; e.stop = 0;
     for (int b = 15; b >= 0; --b) vcg_re_bypass(e, (eob >> b) & 1);
     uint32_t m0 = 2048u, m1 = 2048u, m2 = 2048u, m3 = 2048u, m4 = 2048u, m5 = 2048u,
              m6 = 2048u, m7 = 2048u, m8 = 2048u, m9 = 2048u, m10 = 2048u, m11 = 2048u;
@@ -406,8 +451,9 @@ VCG_HD uint32_t vcg_rc_encode_atom(const int16_t *L, uint32_t eob, const uint32_
     int band = 0;
     uint32_t nextb = band_start[1];
     for (uint32_t p = 0; p < eob; ++p) {
+        if (e.stop) break;
         while (p >= nextb) { ++band; nextb = band_start[band + 1]; psig = 2048u; }
-        const int v = L[p];
+        const int v = rd.get(p);
         if (v == 0) { vcg_re_bit(e, psig, 0); continue; }
         vcg_re_bit(e, psig, 1);
         const uint32_t mm = (uint32_t)(v < 0 ? -v : v) - 1u;   // ones before the stop bin (vc.c:471-490)
@@ -449,6 +495,7 @@ VCG_HD uint32_t vcg_rc_encode_atom(const int16_t *L, uint32_t eob, const uint32_
     }
     for (int k = 0; k < 5; ++k) vcg_re_shift_low(e);
     *ovf = e.ovf;
+    if (stopped) *stopped = e.stop;
     return e.pos;
 }
 

@@ -154,6 +154,54 @@ __global__ __launch_bounds__(FDCT_THREADS) void k_quant(const int32_t *__restric
     if (tid == 0) items[i].eob = (uint32_t)seob;
 }
 
+// quantize stored sample coefficients at nq base_q values at once:
+// block (atom i, candidate c) -> lv_out[(c*n + i) * 32768 ...], eobs[c*n + i]
+__global__ __launch_bounds__(FDCT_THREADS) void k_quant_multi(const int32_t *__restrict__ coef,
+                                                              const float *__restrict__ qs,
+                                                              const vcg_dtab *__restrict__ T,
+                                                              int16_t *__restrict__ lv_out,
+                                                              const enc_item *__restrict__ items,
+                                                              uint32_t *__restrict__ eobs, uint32_t n) {
+    __shared__ int seob;
+    const int tid = threadIdx.x;
+    const uint32_t i = blockIdx.x, c = blockIdx.y;
+    const size_t g = (size_t)c * n + i;
+    if (items[i].zero) { if (tid == 0) eobs[g] = 0; return; }
+    if (tid == 0) seob = 0;
+    __syncthreads();
+    const float q = qs[c];
+    const int32_t *co = coef + (size_t)i * VCG_A3;
+    int16_t *lo = lv_out + g * VCG_A3;
+    int last = 0;
+    for (int p = tid; p < 32768; p += FDCT_THREADS) {
+        const uint32_t idx = T->scan[p];
+        const int l = vcg_quant1(co[idx], q, T->gstep[vcg_freq(idx)]);
+        lo[p] = (int16_t)l;
+        if (l) last = p + 1;
+    }
+    if (last) atomicMax(&seob, last);
+    __syncthreads();
+    if (tid == 0) eobs[g] = (uint32_t)seob;
+}
+
+// byte counts of the sample at nq base_q values (lane per (candidate, atom)),
+// payload bytes not stored.  A candidate whose running total passes `budget`
+// is abandoned (its lanes report 0xFFFFFFFE); 0xFFFFFFFF = coder bound exceeded.
+__global__ __launch_bounds__(64) void k_rc_count(uint32_t n, uint32_t nq, const int16_t *__restrict__ lv,
+                                                 const enc_item *__restrict__ items,
+                                                 const uint32_t *__restrict__ eobs,
+                                                 const vcg_dtab *__restrict__ T, uint32_t budget,
+                                                 uint32_t *__restrict__ kb, uint32_t *__restrict__ lens) {
+    const uint32_t g = blockIdx.x * 64u + threadIdx.x;
+    if (g >= n * nq) return;
+    const uint32_t c = g / n, i = g % n;
+    if (items[i].zero) { lens[g] = 0; return; }
+    int ovf = 0, stopped = 0;
+    const uint32_t len = vcg_rc_encode_atom(lv + (size_t)g * VCG_A3, eobs[g], T->band_start, nullptr, VCG_RC_CAP,
+                                            &ovf, kb + c, budget, &stopped);
+    lens[g] = ovf ? 0xFFFFFFFFu : stopped ? 0xFFFFFFFEu : len + 1;
+}
+
 // ---------------------------------------------------------------------------
 // one lane per atom: payload = [dc][range-coder bytes]
 __global__ __launch_bounds__(64) void k_rc_encode(uint32_t n, const int16_t *__restrict__ lv,
@@ -304,8 +352,14 @@ struct enc_scratch {
     enc_item *items = nullptr; size_t items_cap = 0;
     uint8_t *pay = nullptr; size_t pay_cap = 0;
     uint32_t *lens = nullptr; size_t lens_cap = 0;
+    uint32_t *aux = nullptr; size_t aux_cap = 0;      // rate-control candidates: eobs, counters, q values
 };
 static thread_local enc_scratch tl_es;
+
+// clear n words with a kernel (no memset node in code that may be captured)
+__global__ void k_reset_words(uint32_t *w, uint32_t n) {
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) w[k] = 0;
+}
 
 static void enc_run(vcg_ctx *c, enc_src s, uint32_t n, const int32_t *coef_in, float q_coef,
                     enc_item *items, uint32_t pay_cap) {
@@ -326,11 +380,7 @@ static void enc_run(vcg_ctx *c, enc_src s, uint32_t n, const int32_t *coef_in, f
     VCG_CHECK(hipGetLastError());
 }
 
-extern "C" int vcg_enc_sample(vcg_enc *e, const uint32_t *atoms, int n, float q, uint64_t *bytes,
-                              uint8_t *zero_mask) {
-    vcg_ctx *c = vcg_get_ctx();
-    *bytes = 0;
-    if (n <= 0) return 0;
+static void sample_prepare(vcg_enc *e, vcg_ctx *c, const uint32_t *atoms, int n) {
     if (!e->sample_coef) {
         e->sample_n = n;
         e->sample_list = (uint32_t *)dmalloc((size_t)n * 12);
@@ -343,6 +393,14 @@ extern "C" int vcg_enc_sample(vcg_enc *e, const uint32_t *atoms, int n, float q,
                            g_ftab, e->sample_coef, (int16_t *)nullptr, e->sample_items);
         VCG_CHECK(hipGetLastError());
     }
+}
+
+extern "C" int vcg_enc_sample(vcg_enc *e, const uint32_t *atoms, int n, float q, uint64_t *bytes,
+                              uint8_t *zero_mask) {
+    vcg_ctx *c = vcg_get_ctx();
+    *bytes = 0;
+    if (n <= 0) return 0;
+    sample_prepare(e, c, atoms, n);
     enc_run(c, e->src, (uint32_t)n, e->sample_coef, q, e->sample_items, VCG_PAY_CAP);
     std::vector<uint32_t> lens(n);
     std::vector<enc_item> its(n);
@@ -357,6 +415,57 @@ extern "C" int vcg_enc_sample(vcg_enc *e, const uint32_t *atoms, int n, float q,
     }
     *bytes = b;
     return ovf;
+}
+
+extern "C" int vcg_enc_sample_zero(vcg_enc *e, const uint32_t *atoms, int n, uint8_t *zero_mask) {
+    vcg_ctx *c = vcg_get_ctx();
+    if (n <= 0) return 0;
+    sample_prepare(e, c, atoms, n);
+    std::vector<enc_item> its(n);
+    VCG_CHECK(hipMemcpyAsync(its.data(), e->sample_items, (size_t)n * sizeof(enc_item), hipMemcpyDeviceToHost, c->s));
+    VCG_CHECK(hipStreamSynchronize(c->s));
+    for (int i = 0; i < n; ++i) zero_mask[i] = (uint8_t)its[i].zero;
+    return 0;
+}
+
+// Sample byte totals at nq base_q values in one launch pair.  bytes[k]: the
+// total (dc bytes included); flags[k]: 1 = coder bound exceeded, 2 = total
+// passed `budget` (the encode was abandoned; bytes[k] is then a lower bound).
+extern "C" int vcg_enc_sample_multi(vcg_enc *e, const uint32_t *atoms, int n, const float *qs, int nq,
+                                    uint64_t budget, uint64_t *bytes, int *flags) {
+    vcg_ctx *c = vcg_get_ctx();
+    for (int k = 0; k < nq; ++k) { bytes[k] = 0; flags[k] = 0; }
+    if (n <= 0 || nq <= 0) return 0;
+    sample_prepare(e, c, atoms, n);
+    enc_scratch &es = tl_es;
+    const size_t m = (size_t)n * nq;
+    vcg_grow((void **)&es.lv, &es.lv_cap, m, (size_t)VCG_A3 * 2);
+    vcg_grow((void **)&es.lens, &es.lens_cap, m, 4);
+    vcg_grow((void **)&es.aux, &es.aux_cap, m + 2 * (size_t)nq, 4);
+    uint32_t *eobs = es.aux, *kb = es.aux + m;
+    float *dq = (float *)(es.aux + m + nq);
+    VCG_CHECK(hipMemcpyAsync(dq, qs, (size_t)nq * 4, hipMemcpyHostToDevice, c->s));
+    hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(64), 0, c->s, kb, (uint32_t)nq);
+    hipLaunchKernelGGL(k_quant_multi, dim3(n, nq), dim3(FDCT_THREADS), 0, c->s, e->sample_coef, (const float *)dq,
+                       g_dtab, es.lv, e->sample_items, eobs, (uint32_t)n);
+    VCG_CHECK(hipGetLastError());
+    // a candidate's counted total stays below 2^32 (at most 512 atoms x the
+    // coder bound), so a budget of 2^32-1 or more never stops an encode
+    const uint32_t b32 = budget >= 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)budget;
+    hipLaunchKernelGGL(k_rc_count, dim3((unsigned)((m + 63) / 64)), dim3(64), 0, c->s, (uint32_t)n, (uint32_t)nq,
+                       es.lv, e->sample_items, eobs, g_dtab, b32, kb, es.lens);
+    VCG_CHECK(hipGetLastError());
+    std::vector<uint32_t> lens(m);
+    VCG_CHECK(hipMemcpyAsync(lens.data(), es.lens, m * 4, hipMemcpyDeviceToHost, c->s));
+    VCG_CHECK(hipStreamSynchronize(c->s));
+    for (int k = 0; k < nq; ++k)
+        for (int i = 0; i < n; ++i) {
+            const uint32_t L = lens[(size_t)k * n + i];
+            if (L == 0xFFFFFFFFu) flags[k] |= 1;
+            else if (L == 0xFFFFFFFEu) flags[k] |= 2;
+            else bytes[k] += L;
+        }
+    return 0;
 }
 
 extern "C" int vcg_enc_all(vcg_enc *e, const float qs[VCG_NLOD], uint32_t *lens, uint64_t total) {

@@ -36,6 +36,17 @@ static int lattice(uint32_t n, uint32_t *idx) {
     return 8;
 }
 
+/* The search below is exactly the sequential one (finest q, coarsest q, then
+ * 16 bisection steps mid = sqrtf(lo*hi); meets -> hi = mid, else lo = mid),
+ * evaluated RC_DEPTH steps per launch: each round encodes the sample at every
+ * midpoint of the next RC_DEPTH levels of the bisection tree (plus both bounds
+ * in the first round) and then follows the path the sequential search takes.
+ * A candidate's encode stops once its byte total can no longer meet the
+ * target (the largest meeting total is fixed by the voxel count), which
+ * decides it the same way. */
+#define RC_DEPTH 4
+#define RC_NODES ((1 << RC_DEPTH) - 1)
+
 static float rate_control(vcg_enc *e, vc_dims d, float target) {
     uint32_t ix[8], iy[8], iz[8];
     int nx = lattice((d.nx + 31) / 32, ix), ny = lattice((d.ny + 31) / 32, iy), nz = lattice((d.nz + 31) / 32, iz);
@@ -50,33 +61,54 @@ static float rate_control(vcg_enc *e, vc_dims d, float target) {
         uint32_t ex = d.nx - ix[c] * 32, ey = d.ny - iy[b] * 32, ez = d.nz - iz[a] * 32;
         vox[k] = (ex < 32 ? ex : 32) * (ey < 32 ? ey : 32) * (ez < 32 ? ez : 32);
     }
-    uint64_t bytes = 0;
-#define MEETS(q, res)                                                                 \
-    do {                                                                              \
-        int ovf_ = vcg_enc_sample(e, list, ns, (q), &bytes, zero);                    \
-        uint64_t v_ = 0;                                                              \
-        for (int i_ = 0; i_ < ns; ++i_) if (!zero[i_]) v_ += vox[i_];                \
-        (res) = !ovf_ && (double)v_ >= (double)target * (double)bytes;                \
-        nonzero = v_ > 0;                                                             \
-    } while (0)
-    int nonzero = 0, m = 0;
-    float q0, lo = 0.05f, hi = 4096.0f;
-    MEETS(lo, m);
-    if (!nonzero) q0 = 1.0f;              /* all-zero sample: nothing to calibrate */
-    else if (m) q0 = lo;
-    else {
-        MEETS(hi, m);
-        if (!m) q0 = hi;
-        else {
-            for (int it = 0; it < 16; ++it) {
-                float mid = sqrtf(lo * hi);
-                MEETS(mid, m);
-                if (m) hi = mid; else lo = mid;
+    vcg_enc_sample_zero(e, list, ns, zero);
+    uint64_t v = 0;
+    for (int i = 0; i < ns; ++i) if (!zero[i]) v += vox[i];
+    float q0;
+    if (v == 0) {
+        q0 = 1.0f;                        /* all-zero sample: nothing to calibrate */
+    } else {
+        /* meets(q): no overflow and voxels >= target * bytes (double); budget =
+         * the largest byte total that meets */
+        uint64_t budget = (uint64_t)((double)v / (double)target);
+        while ((double)v >= (double)target * (double)(budget + 1)) budget++;
+        while (budget > 0 && !((double)v >= (double)target * (double)budget)) budget--;
+        float lo = 0.05f, hi = 4096.0f;
+        float qs[RC_NODES + 2], mid[RC_NODES + 1], L[RC_NODES + 1], H[RC_NODES + 1];
+        uint64_t bytes[RC_NODES + 2];
+        int flags[RC_NODES + 2], meets[RC_NODES + 2];
+        int first = 1, steps = 0, done = 0;
+        q0 = hi;
+        while (!done) {
+            /* level-order midpoints of the next RC_DEPTH levels from (lo, hi) */
+            L[1] = lo; H[1] = hi;
+            for (int t = 1; t <= RC_NODES; ++t) {
+                mid[t] = sqrtf(L[t] * H[t]);
+                if (2 * t + 1 <= RC_NODES) {
+                    L[2 * t] = L[t]; H[2 * t] = mid[t];          /* meets: hi = mid */
+                    L[2 * t + 1] = mid[t]; H[2 * t + 1] = H[t];  /* else:  lo = mid */
+                }
             }
-            q0 = hi;
+            int nq = 0;
+            if (first) { qs[nq++] = lo; qs[nq++] = hi; }
+            const int base = nq;
+            for (int t = 1; t <= RC_NODES; ++t) qs[nq++] = mid[t];
+            vcg_enc_sample_multi(e, list, ns, qs, nq, budget, bytes, flags);
+            for (int j = 0; j < nq; ++j)
+                meets[j] = flags[j] == 0 && (double)v >= (double)target * (double)bytes[j];
+            if (first) {
+                first = 0;
+                if (meets[0]) { q0 = lo; break; }
+                if (!meets[1]) { q0 = hi; break; }
+            }
+            int t = 1;
+            for (int l = 0; l < RC_DEPTH && steps < 16; ++l, ++steps) {
+                if (meets[base + t - 1]) { hi = mid[t]; t = 2 * t; }
+                else { lo = mid[t]; t = 2 * t + 1; }
+            }
+            if (steps >= 16) { q0 = hi; done = 1; }
         }
     }
-#undef MEETS
     free(list); free(vox); free(zero);
     return q0;
 }

@@ -94,6 +94,13 @@ vcg_enc *vcg_enc_begin(const uint8_t *vol, int vol_on_device, const uint32_t dim
  * if any atom overflowed the coder bound.  zero_mask[i] = 1 for all-zero atoms. */
 int  vcg_enc_sample(vcg_enc *e, const uint32_t *atoms_zyx, int n, float base_q,
                     uint64_t *bytes, uint8_t *zero_mask);
+/* zero_mask[i] = 1 for the listed atoms that are all zero (caches coefficients) */
+int  vcg_enc_sample_zero(vcg_enc *e, const uint32_t *atoms_zyx, int n, uint8_t *zero_mask);
+/* The sample's byte total at nq base_q values in one launch pair: bytes[k]
+ * (dc bytes included), flags[k] bit 0 = coder bound exceeded, bit 1 = the
+ * total passed `budget` and the encode stopped early (bytes[k] not final). */
+int  vcg_enc_sample_multi(vcg_enc *e, const uint32_t *atoms_zyx, int n, const float *qs, int nq,
+                          uint64_t budget, uint64_t *bytes, int *flags);
 /* Encode every atom of every LOD (LOD-major, raster) at qs[lod]; lens gets
  * each payload length (0 = all-zero atom).  Returns 0, or 1 on coder overflow. */
 int  vcg_enc_all(vcg_enc *e, const float qs[VCG_NLOD], uint32_t *lens, uint64_t total_atoms);
