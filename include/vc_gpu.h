@@ -63,6 +63,17 @@ vc_status vcx_encode_q(const uint8_t *vol, vc_dims dims, float target_ratio,
 /* vc_encode from a volume already resident in device memory. */
 vc_status vcx_encode_device(const void *dev_vol, vc_dims dims, float target_ratio,
                             uint8_t **out_archive, size_t *out_len);
+/* Batched encode of n independent volumes (all host or all device memory,
+ * any dims): archive i is byte-identical to vc_encode(vols[i], dims[i],
+ * target_ratios[i]).  Volumes are encoded in groups that share launches
+ * (rate-control rounds, transform, quantization and range coding across
+ * volumes) while their device footprint fits a fixed budget.  outs[i] gets a
+ * malloc buffer (caller frees) and out_lens[i] its length, statuses[i] the
+ * status vc_encode would return (outs[i] NULL unless VC_OK).  Returns VC_OK,
+ * or the first failing status. */
+vc_status vcx_encode_batch(size_t n, const uint8_t *const *vols, int vols_on_device, const vc_dims *dims,
+                           const float *target_ratios, uint8_t **outs, size_t *out_lens,
+                           vc_status *statuses);
 /* The LOD0 base_q that vc_encode's rate control picks for this volume. */
 float     vcx_rate_control(const uint8_t *vol, vc_dims dims, float target_ratio);
 

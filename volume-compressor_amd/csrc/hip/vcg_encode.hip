@@ -2,15 +2,19 @@
 //
 //   k_downsample : LOD l -> l+1, integer rounded mean of the in-bounds voxels
 //                  of each 2x2x2 cell (src/vc/vc.c:1562-1577).
-//   k_fdct       : one 256-thread workgroup per atom.  Gathers the 32^3 atom
-//                  (zero padded), sums it (zero atom -> no payload, dc =
-//                  (sum+16384)/32768, src/vc/vc.c:599-604), runs the three
-//                  forward partial-butterfly passes along x, y, z in LDS
+//   k_fdct       : one 256-thread workgroup per atom, atoms of one or many
+//                  volumes per launch.  Gathers the 32^3 atom (zero padded),
+//                  sums it (zero atom -> no payload, dc = (sum+16384)/32768,
+//                  src/vc/vc.c:599-604), runs the three forward
+//                  partial-butterfly passes along x, y, z in LDS
 //                  (src/vc/vc.c:142-171, 212-218) and either stores raster
 //                  coefficients (rate-control sample) or quantizes them
 //                  (src/vc/vc.c:293-306, strict IEEE in source order) and
 //                  stores the levels in scan order plus the EOB.
-//   k_quant      : re-quantize stored coefficients at a new base_q.
+//   k_quant_groups / k_rc_count_groups : rate-control candidates (a volume's
+//                  sample at one base_q), many per launch: re-quantize the
+//                  stored coefficients, then count payload bytes per atom,
+//                  abandoning a candidate once its total passes its budget.
 //   k_rc_encode  : one lane per atom.  Adaptive binary range coder with LZMA
 //                  carry and the coefficient context coder
 //                  (src/vc/vc.c:360-401, 471-544), payload = [dc][bytes].
@@ -23,19 +27,6 @@
 #include "vcg_xform.hpp"
 
 #define FDCT_THREADS 256
-#define ENC_BATCH 32768u
-
-struct enc_src {
-    const uint8_t *lod[VCG_NLOD];
-    uint32_t nx[VCG_NLOD], ny[VCG_NLOD], nz[VCG_NLOD];
-    uint32_t acx[VCG_NLOD], acy[VCG_NLOD];
-    uint64_t prefix[VCG_NLOD + 1];    // atom index base per LOD
-    float q[VCG_NLOD];
-    int nlod;
-    const uint32_t *list;             // optional explicit LOD0 atoms (z,y,x)
-    uint64_t base;                    // first global atom of this launch
-    uint32_t n;
-};
 
 struct enc_item {
     uint32_t eob;
@@ -44,11 +35,26 @@ struct enc_item {
     uint32_t lod;
 };
 
-__device__ __forceinline__ void enc_atom_of(const enc_src &s, uint64_t gi, int &lod, uint32_t &ax,
-                                            uint32_t &ay, uint32_t &az) {
-    if (s.list) {
+// One volume's LOD pyramid on device.  Its atoms are numbered LOD-major,
+// raster (x fastest) within a LOD; the rate-control sample is an explicit
+// list of LOD0 atoms whose raster coefficients and items go to coef/sitems.
+struct enc_src {
+    const uint8_t *lod[VCG_NLOD];
+    uint32_t nx[VCG_NLOD], ny[VCG_NLOD], nz[VCG_NLOD];
+    uint32_t acx[VCG_NLOD], acy[VCG_NLOD];
+    uint64_t prefix[VCG_NLOD + 1];    // atom index base per LOD
+    float q[VCG_NLOD];
+    int nlod;
+    const uint32_t *list;             // sample atoms (z,y,x)
+    int32_t *coef;                    // sample coefficients, 32768 per atom
+    enc_item *sitems;                 // sample items
+};
+
+__device__ __forceinline__ void enc_atom_of(const enc_src &s, const uint32_t *list, uint64_t gi, int &lod,
+                                            uint32_t &ax, uint32_t &ay, uint32_t &az) {
+    if (list) {
         lod = 0;
-        az = s.list[3 * gi]; ay = s.list[3 * gi + 1]; ax = s.list[3 * gi + 2];
+        az = list[3 * gi]; ay = list[3 * gi + 1]; ax = list[3 * gi + 2];
         return;
     }
     int l = 0;
@@ -58,6 +64,16 @@ __device__ __forceinline__ void enc_atom_of(const enc_src &s, uint64_t gi, int &
     ax = (uint32_t)(r % s.acx[l]);
     ay = (uint32_t)((r / s.acx[l]) % s.acy[l]);
     az = (uint32_t)(r / ((uint64_t)s.acx[l] * s.acy[l]));
+}
+
+// index of the last base <= x in an increasing array (n >= 1, base[0] <= x)
+__device__ __forceinline__ uint32_t upper_index(const uint64_t *base, uint32_t n, uint64_t x) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) / 2;
+        if (base[m] <= x) lo = m; else hi = m;
+    }
+    return lo;
 }
 
 // ---------------------------------------------------------------------------
@@ -86,19 +102,31 @@ __global__ void k_downsample(const uint8_t *__restrict__ in, uint32_t nx, uint32
 
 enum { FDCT_STORE_COEF = 0, FDCT_QUANT = 1 };
 
-__global__ __launch_bounds__(FDCT_THREADS) void k_fdct(enc_src s, int mode, const vcg_dtab *__restrict__ T,
+// Block i handles atom base + i of the concatenation of nv volumes (volume
+// v's atoms start at vbase[v]; in FDCT_STORE_COEF mode a volume's atoms are
+// its sample list).  FDCT_QUANT: levels to lv_out and items to items, both
+// indexed by i; FDCT_STORE_COEF: coefficients and items to the volume's
+// sample arrays, indexed by the atom's place in its list.
+__global__ __launch_bounds__(FDCT_THREADS) void k_fdct(const enc_src *__restrict__ srcs,
+                                                       const uint64_t *__restrict__ vbase, uint32_t nv,
+                                                       uint64_t base, int mode,
+                                                       const vcg_dtab *__restrict__ T,
                                                        const vcg_fwd_tab *__restrict__ F,
-                                                       int32_t *__restrict__ coef_out,
                                                        int16_t *__restrict__ lv_out,
-                                                       enc_item *__restrict__ items) {
+                                                       enc_item *__restrict__ items_out) {
     extern __shared__ int buf[];
     __shared__ uint32_t ssum;
     __shared__ int seob;
     const int tid = threadIdx.x;
     const uint32_t i = blockIdx.x;
-    const uint64_t gi = s.base + i;
+    const uint64_t flat = base + i;
+    const uint32_t v = nv > 1 ? upper_index(vbase, nv, flat) : 0;
+    const enc_src &s = srcs[v];
+    const uint64_t gi = flat - vbase[v];
     int lod; uint32_t ax, ay, az;
-    enc_atom_of(s, gi, lod, ax, ay, az);
+    const bool store = mode == FDCT_STORE_COEF;
+    enc_atom_of(s, store ? s.list : nullptr, gi, lod, ax, ay, az);
+    enc_item *items = store ? s.sitems + gi : items_out + i;
     if (tid == 0) { ssum = 0; seob = 0; }
     __syncthreads();
     const uint32_t part = vcg_gather(buf, s.lod[lod], s.nx[lod], s.ny[lod], s.nz[lod], ax, ay, az, tid, FDCT_THREADS);
@@ -106,7 +134,7 @@ __global__ __launch_bounds__(FDCT_THREADS) void k_fdct(enc_src s, int mode, cons
     __syncthreads();
     const uint32_t sum = ssum;
     if (sum == 0) {                          // all-zero atom: KNOWN_ZERO, no payload
-        if (tid == 0) { enc_item it = {0, 0, 1, (uint32_t)lod}; items[i] = it; }
+        if (tid == 0) { enc_item it = {0, 0, 1, (uint32_t)lod}; *items = it; }
         return;
     }
     const int dc = (int)((sum + 16384u) / 32768u);
@@ -118,60 +146,54 @@ __global__ __launch_bounds__(FDCT_THREADS) void k_fdct(enc_src s, int mode, cons
     __syncthreads();
     vcg_fwd_pass_z(buf, F, tid, FDCT_THREADS);
     __syncthreads();
-    if (mode == FDCT_STORE_COEF) {
-        int32_t *co = coef_out + (size_t)i * VCG_A3;
+    if (store) {
+        int32_t *co = s.coef + gi * VCG_A3;
         for (int k = tid; k < 32768; k += FDCT_THREADS) co[k] = buf[vcg_tidx(k >> 10, (k >> 5) & 31, k & 31)];
-        if (tid == 0) { enc_item it = {0, (uint32_t)dc, 0, (uint32_t)lod}; items[i] = it; }
+        if (tid == 0) { enc_item it = {0, (uint32_t)dc, 0, (uint32_t)lod}; *items = it; }
         return;
     }
     const int last = vcg_quant_scan(buf, T->scan, T->gstep, s.q[lod], lv_out + (size_t)i * VCG_A3, tid, FDCT_THREADS);
     if (last) atomicMax(&seob, last);
     __syncthreads();
-    if (tid == 0) { enc_item it = {(uint32_t)seob, (uint32_t)dc, 0, (uint32_t)lod}; items[i] = it; }
+    if (tid == 0) { enc_item it = {(uint32_t)seob, (uint32_t)dc, 0, (uint32_t)lod}; *items = it; }
 }
 
-__global__ __launch_bounds__(FDCT_THREADS) void k_quant(const int32_t *__restrict__ coef, float q,
-                                                        const vcg_dtab *__restrict__ T,
-                                                        int16_t *__restrict__ lv_out,
-                                                        enc_item *__restrict__ items) {
-    __shared__ int seob;
-    const int tid = threadIdx.x;
-    const uint32_t i = blockIdx.x;
-    if (items[i].zero) return;
-    if (tid == 0) seob = 0;
-    __syncthreads();
-    const int32_t *co = coef + (size_t)i * VCG_A3;
-    int16_t *lo = lv_out + (size_t)i * VCG_A3;
-    int last = 0;
-    for (int p = tid; p < 32768; p += FDCT_THREADS) {
-        uint32_t idx = T->scan[p];
-        int l = vcg_quant1(co[idx], q, T->gstep[vcg_freq(idx)]);
-        lo[p] = (int16_t)l;
-        if (l) last = p + 1;
+// Rate-control candidates: group g = (volume, base_q); its n sample atoms
+// occupy rows [off, off + n) of the candidate scratch.
+struct samp_group {
+    const int32_t *coef;         // the volume's cached sample coefficients
+    const enc_item *items;
+    uint64_t off;
+    uint32_t n;
+    float q;
+    uint32_t budget;             // byte total past which the group cannot meet its target
+    uint32_t pad;
+};
+__device__ __forceinline__ uint32_t group_of(const samp_group *G, uint32_t ng, uint64_t row) {
+    uint32_t lo = 0, hi = ng;
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) / 2;
+        if (G[m].off <= row) lo = m; else hi = m;
     }
-    if (last) atomicMax(&seob, last);
-    __syncthreads();
-    if (tid == 0) items[i].eob = (uint32_t)seob;
+    return lo;
 }
 
-// quantize stored sample coefficients at nq base_q values at once:
-// block (atom i, candidate c) -> lv_out[(c*n + i) * 32768 ...], eobs[c*n + i]
-__global__ __launch_bounds__(FDCT_THREADS) void k_quant_multi(const int32_t *__restrict__ coef,
-                                                              const float *__restrict__ qs,
-                                                              const vcg_dtab *__restrict__ T,
-                                                              int16_t *__restrict__ lv_out,
-                                                              const enc_item *__restrict__ items,
-                                                              uint32_t *__restrict__ eobs, uint32_t n) {
+// quantize every candidate's sample atoms: block = one row
+__global__ __launch_bounds__(FDCT_THREADS) void k_quant_groups(const samp_group *__restrict__ G, uint32_t ng,
+                                                               const vcg_dtab *__restrict__ T,
+                                                               int16_t *__restrict__ lv_out,
+                                                               uint32_t *__restrict__ eobs) {
     __shared__ int seob;
     const int tid = threadIdx.x;
-    const uint32_t i = blockIdx.x, c = blockIdx.y;
-    const size_t g = (size_t)c * n + i;
-    if (items[i].zero) { if (tid == 0) eobs[g] = 0; return; }
+    const uint64_t row = blockIdx.x;
+    const samp_group &g = G[group_of(G, ng, row)];
+    const uint32_t i = (uint32_t)(row - g.off);
+    if (g.items[i].zero) { if (tid == 0) eobs[row] = 0; return; }
     if (tid == 0) seob = 0;
     __syncthreads();
-    const float q = qs[c];
-    const int32_t *co = coef + (size_t)i * VCG_A3;
-    int16_t *lo = lv_out + g * VCG_A3;
+    const float q = g.q;
+    const int32_t *co = g.coef + (size_t)i * VCG_A3;
+    int16_t *lo = lv_out + row * VCG_A3;
     int last = 0;
     for (int p = tid; p < 32768; p += FDCT_THREADS) {
         const uint32_t idx = T->scan[p];
@@ -181,25 +203,26 @@ __global__ __launch_bounds__(FDCT_THREADS) void k_quant_multi(const int32_t *__r
     }
     if (last) atomicMax(&seob, last);
     __syncthreads();
-    if (tid == 0) eobs[g] = (uint32_t)seob;
+    if (tid == 0) eobs[row] = (uint32_t)seob;
 }
 
-// byte counts of the sample at nq base_q values (lane per (candidate, atom)),
-// payload bytes not stored.  A candidate whose running total passes `budget`
-// is abandoned (its lanes report 0xFFFFFFFE); 0xFFFFFFFF = coder bound exceeded.
-__global__ __launch_bounds__(64) void k_rc_count(uint32_t n, uint32_t nq, const int16_t *__restrict__ lv,
-                                                 const enc_item *__restrict__ items,
-                                                 const uint32_t *__restrict__ eobs,
-                                                 const vcg_dtab *__restrict__ T, uint32_t budget,
-                                                 uint32_t *__restrict__ kb, uint32_t *__restrict__ lens) {
-    const uint32_t g = blockIdx.x * 64u + threadIdx.x;
-    if (g >= n * nq) return;
-    const uint32_t c = g / n, i = g % n;
-    if (items[i].zero) { lens[g] = 0; return; }
+// byte count of every row (lane per row), payload bytes not stored.  A group
+// whose running total passes its budget is abandoned (lanes report
+// 0xFFFFFFFE); 0xFFFFFFFF = coder bound exceeded.
+__global__ __launch_bounds__(64) void k_rc_count_groups(const samp_group *__restrict__ G, uint32_t ng,
+                                                        uint64_t rows, const int16_t *__restrict__ lv,
+                                                        const uint32_t *__restrict__ eobs,
+                                                        const vcg_dtab *__restrict__ T, uint32_t *__restrict__ kb,
+                                                        uint32_t *__restrict__ lens) {
+    const uint64_t row = (uint64_t)blockIdx.x * 64u + threadIdx.x;
+    if (row >= rows) return;
+    const uint32_t gi = group_of(G, ng, row);
+    const samp_group &g = G[gi];
+    if (g.items[row - g.off].zero) { lens[row] = 0; return; }
     int ovf = 0, stopped = 0;
-    const uint32_t len = vcg_rc_encode_atom(lv + (size_t)g * VCG_A3, eobs[g], T->band_start, nullptr, VCG_RC_CAP,
-                                            &ovf, kb + c, budget, &stopped);
-    lens[g] = ovf ? 0xFFFFFFFFu : stopped ? 0xFFFFFFFEu : len + 1;
+    const uint32_t len = vcg_rc_encode_atom(lv + row * VCG_A3, eobs[row], T->band_start, nullptr, VCG_RC_CAP,
+                                            &ovf, kb + gi, g.budget, &stopped);
+    lens[row] = ovf ? 0xFFFFFFFFu : stopped ? 0xFFFFFFFEu : len + 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -221,15 +244,15 @@ __global__ __launch_bounds__(64) void k_rc_encode(uint32_t n, const int16_t *__r
     lens[i] = ovf ? 0xFFFFFFFFu : len + 1;
 }
 
-// copy payload i (len at lens[i]) from batch scratch into the pool at pool_off[i]
+// copy payload i (len at lens[i]) from batch scratch to the device address dst[i]
 __global__ void k_compact(const uint8_t *__restrict__ pay, uint32_t pay_cap, const uint32_t *__restrict__ lens,
-                          const uint64_t *__restrict__ pool_off, uint8_t *__restrict__ pool) {
+                          const uint64_t *__restrict__ dst) {
     const uint32_t i = blockIdx.x;
     const uint32_t len = lens[i];
     if (len == 0 || len == 0xFFFFFFFFu) return;
     const uint8_t *src = pay + (size_t)i * pay_cap;
-    uint8_t *dst = pool + pool_off[i];
-    for (uint32_t b = threadIdx.x; b < len; b += blockDim.x) dst[b] = src[b];
+    uint8_t *d = (uint8_t *)(uintptr_t)dst[i];
+    for (uint32_t b = threadIdx.x; b < len; b += blockDim.x) d[b] = src[b];
 }
 
 // slot + payload for atom i of the archive image (layout computed on the host)
@@ -281,17 +304,16 @@ static void enc_setup() {
 
 struct vcg_enc {
     enc_src src;
+    uint64_t total = 0;                // atoms over every LOD
     uint8_t *lod_buf[VCG_NLOD] = {nullptr};
-    bool own0 = false;
     // rate-control sample
     uint32_t *sample_list = nullptr;
     int32_t *sample_coef = nullptr;
     enc_item *sample_items = nullptr;
     int sample_n = 0;
-    // payload pool
+    // payload pool: atom i's payload at pool + pool_off[i]
     uint8_t *pool = nullptr; uint64_t pool_cap = 0, pool_len = 0;
     std::vector<uint64_t> pool_off;
-    uint64_t *pool_off_d = nullptr;
 };
 
 static void *dmalloc(size_t n) {
@@ -327,6 +349,7 @@ extern "C" vcg_enc *vcg_enc_begin(const uint8_t *vol, int on_dev, const uint32_t
     }
     e->src.prefix[nlod] = acc;
     e->src.nlod = nlod;
+    e->total = acc;
     size_t n0 = (size_t)dims[0] * dims[1] * dims[2];
     if (on_dev) {
         e->src.lod[0] = vol;
@@ -334,7 +357,6 @@ extern "C" vcg_enc *vcg_enc_begin(const uint8_t *vol, int on_dev, const uint32_t
         e->lod_buf[0] = (uint8_t *)dmalloc(n0);
         VCG_CHECK(hipMemcpyAsync(e->lod_buf[0], vol, n0, hipMemcpyHostToDevice, c->s));
         e->src.lod[0] = e->lod_buf[0];
-        e->own0 = true;
     }
     for (int l = 1; l < nlod; ++l) {
         size_t nl = (size_t)e->src.nx[l] * e->src.ny[l] * e->src.nz[l];
@@ -352,176 +374,236 @@ struct enc_scratch {
     enc_item *items = nullptr; size_t items_cap = 0;
     uint8_t *pay = nullptr; size_t pay_cap = 0;
     uint32_t *lens = nullptr; size_t lens_cap = 0;
-    uint32_t *aux = nullptr; size_t aux_cap = 0;      // rate-control candidates: eobs, counters, q values
+    uint32_t *aux = nullptr; size_t aux_cap = 0;      // rate-control candidates: eobs, counters
+    uint8_t *desc = nullptr; size_t desc_cap = 0;     // per-call descriptors (sources, groups, addresses)
+    uint8_t *img = nullptr; size_t img_cap = 0;       // archive image
 };
 static thread_local enc_scratch tl_es;
 
+// rows of 32768 levels per launch (rate-control candidates or atoms)
+#define ENC_ROWS 65536u
+
 // clear n words with a kernel (no memset node in code that may be captured)
-__global__ void k_reset_words(uint32_t *w, uint32_t n) {
-    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) w[k] = 0;
+__global__ void k_reset_words(uint32_t *w, uint64_t n) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x)
+        w[k] = 0;
 }
 
-static void enc_run(vcg_ctx *c, enc_src s, uint32_t n, const int32_t *coef_in, float q_coef,
-                    enc_item *items, uint32_t pay_cap) {
-    enc_scratch &es = tl_es;
-    vcg_grow((void **)&es.lv, &es.lv_cap, n, (size_t)VCG_A3 * 2);
-    vcg_grow((void **)&es.pay, &es.pay_cap, (size_t)n * pay_cap, 1);
-    vcg_grow((void **)&es.lens, &es.lens_cap, n, 4);
-    if (coef_in) {
-        hipLaunchKernelGGL(k_quant, dim3(n), dim3(FDCT_THREADS), 0, c->s, coef_in, q_coef, g_dtab, es.lv, items);
-    } else {
-        s.n = n;
-        hipLaunchKernelGGL(k_fdct, dim3(n), dim3(FDCT_THREADS), LDS_BYTES, c->s, s, (int)FDCT_QUANT, g_dtab,
-                           g_ftab, (int32_t *)nullptr, es.lv, items);
+// descriptor scratch: bytes [0, n) of this call's descriptor area
+static uint8_t *desc_area(size_t n) {
+    vcg_grow((void **)&tl_es.desc, &tl_es.desc_cap, n, 1);
+    return tl_es.desc;
+}
+
+// sources of volumes es[0..nv) and their atom bases (device copies at *dsrc/*dbase)
+static uint64_t upload_sources(vcg_ctx *c, int nv, vcg_enc *const *es, const float (*qs)[VCG_NLOD],
+                               bool sample, std::vector<uint8_t> &host, enc_src **dsrc, uint64_t **dbase) {
+    const size_t so = 0, bo = ((size_t)nv * sizeof(enc_src) + 255) & ~(size_t)255;
+    host.assign(bo + (size_t)nv * 8, 0);
+    enc_src *hs = (enc_src *)host.data();
+    uint64_t *hb = (uint64_t *)(host.data() + bo);
+    uint64_t tot = 0;
+    for (int v = 0; v < nv; ++v) {
+        enc_src s = es[v]->src;
+        if (qs) for (int l = 0; l < VCG_NLOD; ++l) s.q[l] = qs[v][l];
+        if (sample) { s.list = es[v]->sample_list; s.coef = es[v]->sample_coef; s.sitems = es[v]->sample_items; }
+        hs[v] = s;
+        hb[v] = tot;
+        tot += sample ? (uint64_t)es[v]->sample_n : es[v]->total;
     }
-    VCG_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_rc_encode, dim3((n + 63) / 64), dim3(64), 0, c->s, n, es.lv, items, g_dtab, es.pay,
-                       pay_cap, es.lens);
-    VCG_CHECK(hipGetLastError());
+    uint8_t *d = desc_area(host.size());
+    VCG_CHECK(hipMemcpyAsync(d, host.data(), host.size(), hipMemcpyHostToDevice, c->s));
+    *dsrc = (enc_src *)(d + so);
+    *dbase = (uint64_t *)(d + bo);
+    return tot;
 }
 
-static void sample_prepare(vcg_enc *e, vcg_ctx *c, const uint32_t *atoms, int n) {
-    if (!e->sample_coef) {
+// Rate-control samples of nv encoders in one launch: encoder v's ns[v] listed
+// LOD0 atoms are transformed and their coefficients kept on device;
+// zero_masks[v][i] = 1 for its all-zero atoms.
+extern "C" int vcg_enc_samples(int nv, vcg_enc *const *es, const uint32_t *const *lists, const int *ns,
+                               uint8_t *const *zero_masks) {
+    vcg_ctx *c = vcg_get_ctx();
+    if (nv <= 0) return 0;
+    for (int v = 0; v < nv; ++v) {
+        vcg_enc *e = es[v];
+        if (e->sample_coef) vc__panic("encode sample prepared twice");
+        const int n = ns[v] > 0 ? ns[v] : 0;
         e->sample_n = n;
         e->sample_list = (uint32_t *)dmalloc((size_t)n * 12);
         e->sample_coef = (int32_t *)dmalloc((size_t)n * VCG_A3 * 4);
         e->sample_items = (enc_item *)dmalloc((size_t)n * sizeof(enc_item));
-        VCG_CHECK(hipMemcpyAsync(e->sample_list, atoms, (size_t)n * 12, hipMemcpyHostToDevice, c->s));
-        enc_src s = e->src;
-        s.list = e->sample_list; s.base = 0; s.n = (uint32_t)n;
-        hipLaunchKernelGGL(k_fdct, dim3(n), dim3(FDCT_THREADS), LDS_BYTES, c->s, s, (int)FDCT_STORE_COEF, g_dtab,
-                           g_ftab, e->sample_coef, (int16_t *)nullptr, e->sample_items);
+        if (n) VCG_CHECK(hipMemcpyAsync(e->sample_list, lists[v], (size_t)n * 12, hipMemcpyHostToDevice, c->s));
+    }
+    std::vector<uint8_t> host;
+    enc_src *dsrc; uint64_t *dbase;
+    const uint64_t tot = upload_sources(c, nv, es, nullptr, true, host, &dsrc, &dbase);
+    if (tot) {
+        hipLaunchKernelGGL(k_fdct, dim3((uint32_t)tot), dim3(FDCT_THREADS), LDS_BYTES, c->s, dsrc, dbase,
+                           (uint32_t)nv, (uint64_t)0, (int)FDCT_STORE_COEF, g_dtab, g_ftab, (int16_t *)nullptr,
+                           (enc_item *)nullptr);
         VCG_CHECK(hipGetLastError());
     }
-}
-
-extern "C" int vcg_enc_sample(vcg_enc *e, const uint32_t *atoms, int n, float q, uint64_t *bytes,
-                              uint8_t *zero_mask) {
-    vcg_ctx *c = vcg_get_ctx();
-    *bytes = 0;
-    if (n <= 0) return 0;
-    sample_prepare(e, c, atoms, n);
-    enc_run(c, e->src, (uint32_t)n, e->sample_coef, q, e->sample_items, VCG_PAY_CAP);
-    std::vector<uint32_t> lens(n);
-    std::vector<enc_item> its(n);
-    VCG_CHECK(hipMemcpyAsync(lens.data(), tl_es.lens, (size_t)n * 4, hipMemcpyDeviceToHost, c->s));
-    VCG_CHECK(hipMemcpyAsync(its.data(), e->sample_items, (size_t)n * sizeof(enc_item), hipMemcpyDeviceToHost, c->s));
+    std::vector<enc_item> its(tot);
+    for (int v = 0, k = 0; v < nv; k += es[v]->sample_n, ++v)
+        if (es[v]->sample_n)
+            VCG_CHECK(hipMemcpyAsync(its.data() + k, es[v]->sample_items, (size_t)es[v]->sample_n * sizeof(enc_item),
+                                     hipMemcpyDeviceToHost, c->s));
     VCG_CHECK(hipStreamSynchronize(c->s));
-    int ovf = 0;
-    uint64_t b = 0;
-    for (int i = 0; i < n; ++i) {
-        if (zero_mask) zero_mask[i] = (uint8_t)its[i].zero;
-        if (lens[i] == 0xFFFFFFFFu) ovf = 1; else b += lens[i];
-    }
-    *bytes = b;
-    return ovf;
-}
-
-extern "C" int vcg_enc_sample_zero(vcg_enc *e, const uint32_t *atoms, int n, uint8_t *zero_mask) {
-    vcg_ctx *c = vcg_get_ctx();
-    if (n <= 0) return 0;
-    sample_prepare(e, c, atoms, n);
-    std::vector<enc_item> its(n);
-    VCG_CHECK(hipMemcpyAsync(its.data(), e->sample_items, (size_t)n * sizeof(enc_item), hipMemcpyDeviceToHost, c->s));
-    VCG_CHECK(hipStreamSynchronize(c->s));
-    for (int i = 0; i < n; ++i) zero_mask[i] = (uint8_t)its[i].zero;
+    for (int v = 0, k = 0; v < nv; k += es[v]->sample_n, ++v)
+        for (int i = 0; i < es[v]->sample_n; ++i) zero_masks[v][i] = (uint8_t)its[k + i].zero;
     return 0;
 }
 
-// Sample byte totals at nq base_q values in one launch pair.  bytes[k]: the
-// total (dc bytes included); flags[k]: 1 = coder bound exceeded, 2 = total
-// passed `budget` (the encode was abandoned; bytes[k] is then a lower bound).
-extern "C" int vcg_enc_sample_multi(vcg_enc *e, const uint32_t *atoms, int n, const float *qs, int nq,
-                                    uint64_t budget, uint64_t *bytes, int *flags) {
+// Sample byte totals of ng rate-control candidates: candidate g is encoder
+// es[g]'s sample quantized at base_q qs[g].  bytes[g] = the total (dc bytes
+// included); flags[g] bit 0 = coder bound exceeded, bit 1 = the total passed
+// budgets[g] and the candidate was abandoned (bytes[g] is then not final).
+extern "C" int vcg_enc_candidates(int ng, vcg_enc *const *es, const float *qs, const uint64_t *budgets,
+                                  uint64_t *bytes, int *flags) {
     vcg_ctx *c = vcg_get_ctx();
-    for (int k = 0; k < nq; ++k) { bytes[k] = 0; flags[k] = 0; }
-    if (n <= 0 || nq <= 0) return 0;
-    sample_prepare(e, c, atoms, n);
-    enc_scratch &es = tl_es;
-    const size_t m = (size_t)n * nq;
-    vcg_grow((void **)&es.lv, &es.lv_cap, m, (size_t)VCG_A3 * 2);
-    vcg_grow((void **)&es.lens, &es.lens_cap, m, 4);
-    vcg_grow((void **)&es.aux, &es.aux_cap, m + 2 * (size_t)nq, 4);
-    uint32_t *eobs = es.aux, *kb = es.aux + m;
-    float *dq = (float *)(es.aux + m + nq);
-    VCG_CHECK(hipMemcpyAsync(dq, qs, (size_t)nq * 4, hipMemcpyHostToDevice, c->s));
-    hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(64), 0, c->s, kb, (uint32_t)nq);
-    hipLaunchKernelGGL(k_quant_multi, dim3(n, nq), dim3(FDCT_THREADS), 0, c->s, e->sample_coef, (const float *)dq,
-                       g_dtab, es.lv, e->sample_items, eobs, (uint32_t)n);
-    VCG_CHECK(hipGetLastError());
-    // a candidate's counted total stays below 2^32 (at most 512 atoms x the
-    // coder bound), so a budget of 2^32-1 or more never stops an encode
-    const uint32_t b32 = budget >= 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)budget;
-    hipLaunchKernelGGL(k_rc_count, dim3((unsigned)((m + 63) / 64)), dim3(64), 0, c->s, (uint32_t)n, (uint32_t)nq,
-                       es.lv, e->sample_items, eobs, g_dtab, b32, kb, es.lens);
-    VCG_CHECK(hipGetLastError());
-    std::vector<uint32_t> lens(m);
-    VCG_CHECK(hipMemcpyAsync(lens.data(), es.lens, m * 4, hipMemcpyDeviceToHost, c->s));
+    for (int g = 0; g < ng; ++g) { bytes[g] = 0; flags[g] = 0; }
+    if (ng <= 0) return 0;
+    enc_scratch &es_ = tl_es;
+    // launches of whole groups, at most ENC_ROWS rows each
+    std::vector<int> cut{0};
+    std::vector<samp_group> G(ng);
+    uint64_t rows = 0, tot = 0;
+    for (int g = 0; g < ng; ++g) {
+        const uint32_t n = (uint32_t)es[g]->sample_n;
+        if (n > ENC_ROWS) vc__panic("rate-control sample larger than a launch");
+        if (rows + n > ENC_ROWS) { cut.push_back(g); rows = 0; }
+        // a candidate's counted total stays below 2^32 (at most 512 atoms x the
+        // coder bound), so a budget of 2^32-1 or more never stops an encode
+        const uint32_t b32 = budgets[g] >= 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)budgets[g];
+        G[g] = samp_group{es[g]->sample_coef, es[g]->sample_items, rows, n, qs[g], b32, 0};
+        rows += n; tot += n;
+    }
+    cut.push_back(ng);
+    const size_t maxrows = tot < ENC_ROWS ? (size_t)tot : (size_t)ENC_ROWS;
+    vcg_grow((void **)&es_.lv, &es_.lv_cap, maxrows, (size_t)VCG_A3 * 2);
+    vcg_grow((void **)&es_.lens, &es_.lens_cap, tot, 4);
+    vcg_grow((void **)&es_.aux, &es_.aux_cap, maxrows + (size_t)ng, 4);
+    samp_group *dG = (samp_group *)desc_area((size_t)ng * sizeof(samp_group));
+    VCG_CHECK(hipMemcpyAsync(dG, G.data(), (size_t)ng * sizeof(samp_group), hipMemcpyHostToDevice, c->s));
+    uint32_t *eobs = es_.aux, *kb = es_.aux + maxrows;
+    hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(256), 0, c->s, kb, (uint64_t)ng);
+    uint64_t done = 0;
+    for (size_t k = 0; k + 1 < cut.size(); ++k) {
+        const int g0 = cut[k], g1 = cut[k + 1];
+        const uint64_t r = (uint64_t)G[g1 - 1].off + G[g1 - 1].n;
+        if (r == 0) continue;
+        hipLaunchKernelGGL(k_quant_groups, dim3((uint32_t)r), dim3(FDCT_THREADS), 0, c->s, dG + g0,
+                           (uint32_t)(g1 - g0), g_dtab, es_.lv, eobs);
+        VCG_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_rc_count_groups, dim3((uint32_t)((r + 63) / 64)), dim3(64), 0, c->s, dG + g0,
+                           (uint32_t)(g1 - g0), r, es_.lv, eobs, g_dtab, kb + g0, es_.lens + done);
+        VCG_CHECK(hipGetLastError());
+        done += r;
+    }
+    std::vector<uint32_t> lens(tot);
+    if (tot) VCG_CHECK(hipMemcpyAsync(lens.data(), es_.lens, tot * 4, hipMemcpyDeviceToHost, c->s));
     VCG_CHECK(hipStreamSynchronize(c->s));
-    for (int k = 0; k < nq; ++k)
-        for (int i = 0; i < n; ++i) {
-            const uint32_t L = lens[(size_t)k * n + i];
-            if (L == 0xFFFFFFFFu) flags[k] |= 1;
-            else if (L == 0xFFFFFFFEu) flags[k] |= 2;
-            else bytes[k] += L;
+    for (int g = 0, k = 0; g < ng; ++g)
+        for (uint32_t i = 0; i < G[g].n; ++i, ++k) {
+            const uint32_t L = lens[k];
+            if (L == 0xFFFFFFFFu) flags[g] |= 1;
+            else if (L == 0xFFFFFFFEu) flags[g] |= 2;
+            else bytes[g] += L;
         }
     return 0;
 }
 
-extern "C" int vcg_enc_all(vcg_enc *e, const float qs[VCG_NLOD], uint32_t *lens, uint64_t total) {
+// make room for `need` bytes in e's payload pool (keeps its contents)
+static void pool_reserve(vcg_ctx *c, vcg_enc *e, uint64_t need) {
+    if (need <= e->pool_cap) return;
+    const uint64_t nc = need + need / 2 + 4096;
+    uint8_t *np = (uint8_t *)dmalloc(nc);
+    if (e->pool_len) VCG_CHECK(hipMemcpyAsync(np, e->pool, e->pool_len, hipMemcpyDeviceToDevice, c->s));
+    VCG_CHECK(hipStreamSynchronize(c->s));
+    if (e->pool) VCG_CHECK(hipFree(e->pool));
+    e->pool = np; e->pool_cap = nc;
+}
+
+// Encode every atom of every LOD of nv encoders (encoder v at qs[v][lod]) in
+// launches of up to ENC_ROWS atoms taken across volumes.  lens[v][i]: payload
+// length of encoder v's atom i (0 = all-zero atom); ovf[v] = 1 if one of its
+// payloads exceeded the coder bound.
+extern "C" int vcg_enc_all(int nv, vcg_enc *const *es, const float (*qs)[VCG_NLOD], uint32_t *const *lens,
+                           int *ovf) {
     vcg_ctx *c = vcg_get_ctx();
-    enc_scratch &es = tl_es;
-    enc_src s = e->src;
-    for (int l = 0; l < VCG_NLOD; ++l) s.q[l] = qs[l];
-    s.list = nullptr;
-    e->pool_off.assign(total, 0);
-    e->pool_len = 0;
-    uint32_t maxb = total < ENC_BATCH ? (uint32_t)total : ENC_BATCH;
-    vcg_grow((void **)&es.items, &es.items_cap, maxb, sizeof(enc_item));
-    uint64_t *poff_d = nullptr; size_t poff_cap = 0;
-    int ovf = 0;
-    for (uint64_t b = 0; b < total; b += maxb) {
-        uint32_t n = (uint32_t)(total - b < maxb ? total - b : maxb);
-        s.base = b;
-        enc_run(c, s, n, nullptr, 0.f, es.items, VCG_PAY_CAP);
-        VCG_CHECK(hipMemcpyAsync(lens + b, es.lens, (size_t)n * 4, hipMemcpyDeviceToHost, c->s));
-        VCG_CHECK(hipStreamSynchronize(c->s));
-        uint64_t add = 0;
-        for (uint32_t k = 0; k < n; ++k) {
-            if (lens[b + k] == 0xFFFFFFFFu) { ovf = 1; lens[b + k] = 0; continue; }
-            e->pool_off[b + k] = e->pool_len + add;
-            add += lens[b + k];
-        }
-        if (e->pool_len + add > e->pool_cap) {
-            uint64_t nc = (e->pool_len + add) + (e->pool_len + add) / 2 + 4096;
-            uint8_t *np = (uint8_t *)dmalloc(nc);
-            if (e->pool_len) VCG_CHECK(hipMemcpyAsync(np, e->pool, e->pool_len, hipMemcpyDeviceToDevice, c->s));
-            VCG_CHECK(hipStreamSynchronize(c->s));
-            if (e->pool) VCG_CHECK(hipFree(e->pool));
-            e->pool = np; e->pool_cap = nc;
-        }
-        vcg_grow((void **)&poff_d, &poff_cap, n, 8);
-        VCG_CHECK(hipMemcpyAsync(poff_d, e->pool_off.data() + b, (size_t)n * 8, hipMemcpyHostToDevice, c->s));
-        hipLaunchKernelGGL(k_compact, dim3(n), dim3(256), 0, c->s, es.pay, (uint32_t)VCG_PAY_CAP, es.lens, poff_d,
-                           e->pool);
-        VCG_CHECK(hipGetLastError());
-        VCG_CHECK(hipStreamSynchronize(c->s));
-        e->pool_len += add;
+    enc_scratch &S = tl_es;
+    if (nv <= 0) return 0;
+    std::vector<uint8_t> host;
+    enc_src *dsrc; uint64_t *dbase;
+    const uint64_t tot = upload_sources(c, nv, es, qs, false, host, &dsrc, &dbase);
+    std::vector<uint64_t> vb(nv + 1);
+    for (int v = 0; v < nv; ++v) {
+        vb[v + 1] = vb[v] + es[v]->total;
+        es[v]->pool_off.assign(es[v]->total, 0);
+        es[v]->pool_len = 0;
+        ovf[v] = 0;
     }
-    if (poff_d) VCG_CHECK(hipFree(poff_d));
-    return ovf;
+    const uint32_t maxb = tot < ENC_ROWS ? (uint32_t)tot : ENC_ROWS;
+    vcg_grow((void **)&S.lv, &S.lv_cap, maxb, (size_t)VCG_A3 * 2);
+    vcg_grow((void **)&S.items, &S.items_cap, maxb, sizeof(enc_item));
+    vcg_grow((void **)&S.pay, &S.pay_cap, (size_t)maxb * VCG_PAY_CAP, 1);
+    vcg_grow((void **)&S.lens, &S.lens_cap, maxb, 4);
+    vcg_grow((void **)&S.aux, &S.aux_cap, (size_t)maxb * 2, 4);
+    uint64_t *ddst = (uint64_t *)S.aux;       // destination address of each payload
+    std::vector<uint32_t> hl(maxb);
+    std::vector<uint64_t> dst(maxb), add(nv);
+    for (uint64_t b = 0; b < tot; b += maxb) {
+        const uint32_t n = (uint32_t)(tot - b < maxb ? tot - b : maxb);
+        hipLaunchKernelGGL(k_fdct, dim3(n), dim3(FDCT_THREADS), LDS_BYTES, c->s, dsrc, dbase, (uint32_t)nv, b,
+                           (int)FDCT_QUANT, g_dtab, g_ftab, S.lv, S.items);
+        VCG_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_rc_encode, dim3((n + 63) / 64), dim3(64), 0, c->s, n, S.lv, S.items, g_dtab, S.pay,
+                           (uint32_t)VCG_PAY_CAP, S.lens);
+        VCG_CHECK(hipGetLastError());
+        VCG_CHECK(hipMemcpyAsync(hl.data(), S.lens, (size_t)n * 4, hipMemcpyDeviceToHost, c->s));
+        VCG_CHECK(hipStreamSynchronize(c->s));
+        // atom b + k belongs to the volume v with vb[v] <= b + k < vb[v + 1]
+        int v0 = 0;
+        while (vb[v0 + 1] <= b) ++v0;
+        for (int v = v0; v < nv && vb[v] < b + n; ++v) add[v] = 0;
+        for (uint32_t k = 0, v = v0; k < n; ++k) {
+            while (b + k >= vb[v + 1]) ++v;
+            const uint64_t i = b + k - vb[v];
+            uint32_t L = hl[k];
+            if (L == 0xFFFFFFFFu) { ovf[v] = 1; L = 0; }
+            lens[v][i] = L;
+            es[v]->pool_off[i] = es[v]->pool_len + add[v];
+            add[v] += L;
+        }
+        for (int v = v0; v < nv && vb[v] < b + n; ++v) pool_reserve(c, es[v], es[v]->pool_len + add[v]);
+        for (uint32_t k = 0, v = v0; k < n; ++k) {
+            while (b + k >= vb[v + 1]) ++v;
+            dst[k] = (uint64_t)(uintptr_t)(es[v]->pool + es[v]->pool_off[b + k - vb[v]]);
+        }
+        VCG_CHECK(hipMemcpyAsync(ddst, dst.data(), (size_t)n * 8, hipMemcpyHostToDevice, c->s));
+        hipLaunchKernelGGL(k_compact, dim3(n), dim3(256), 0, c->s, S.pay, (uint32_t)VCG_PAY_CAP, S.lens,
+                           (const uint64_t *)ddst);
+        VCG_CHECK(hipGetLastError());
+        for (int v = v0; v < nv && vb[v] < b + n; ++v) es[v]->pool_len += add[v];
+        VCG_CHECK(hipStreamSynchronize(c->s));
+    }
+    return 0;
 }
 
 extern "C" int vcg_enc_assemble(vcg_enc *e, const uint8_t *head, size_t head_len, const uint64_t *pay_off,
                                 const uint64_t *slot_off, const uint32_t *lens, uint64_t total,
                                 uint8_t *out, uint64_t total_len) {
     vcg_ctx *c = vcg_get_ctx();
-    uint8_t *img = (uint8_t *)dmalloc(total_len);
-    uint64_t *d_pool_off = (uint64_t *)dmalloc(total * 8);
-    uint64_t *d_pay_off = (uint64_t *)dmalloc(total * 8);
-    uint64_t *d_slot_off = (uint64_t *)dmalloc(total * 8);
-    uint32_t *d_lens = (uint32_t *)dmalloc(total * 4);
-    VCG_CHECK(hipMemsetAsync(img, 0, total_len, c->s));
+    enc_scratch &S = tl_es;
+    // per-atom arrays in the descriptor area, the image in its own scratch
+    const size_t o1 = total * 8, o2 = 2 * o1, o3 = 3 * o1;
+    uint8_t *d = desc_area(3 * o1 + total * 4);
+    uint64_t *d_pool_off = (uint64_t *)d, *d_pay_off = (uint64_t *)(d + o1), *d_slot_off = (uint64_t *)(d + o2);
+    uint32_t *d_lens = (uint32_t *)(d + o3);
+    vcg_grow((void **)&S.img, &S.img_cap, total_len, 1);
+    uint8_t *img = S.img;
+    hipLaunchKernelGGL(k_reset_words, dim3(1024), dim3(256), 0, c->s, (uint32_t *)img, (total_len + 3) / 4);
     VCG_CHECK(hipMemcpyAsync(img, head, head_len, hipMemcpyHostToDevice, c->s));
     VCG_CHECK(hipMemcpyAsync(d_pool_off, e->pool_off.data(), total * 8, hipMemcpyHostToDevice, c->s));
     VCG_CHECK(hipMemcpyAsync(d_pay_off, pay_off, total * 8, hipMemcpyHostToDevice, c->s));
@@ -535,8 +617,6 @@ extern "C" int vcg_enc_assemble(vcg_enc *e, const uint8_t *head, size_t head_len
     }
     VCG_CHECK(hipMemcpyAsync(out, img, total_len, hipMemcpyDeviceToHost, c->s));
     VCG_CHECK(hipStreamSynchronize(c->s));
-    VCG_CHECK(hipFree(img)); VCG_CHECK(hipFree(d_pool_off)); VCG_CHECK(hipFree(d_pay_off));
-    VCG_CHECK(hipFree(d_slot_off)); VCG_CHECK(hipFree(d_lens));
     return 0;
 }
 
@@ -556,34 +636,30 @@ extern "C" int vcg_encode_atoms(size_t n, const uint8_t *vox, float q, uint8_t *
     vcg_ctx *c = vcg_get_ctx();
     enc_setup();
     if (n == 0) return 0;
-    // treat the atoms as one LOD0 volume of n atoms along x: 32n x 32 x 32 is not the
-    // same memory order, so gather explicitly through a (32, 32, 32n) z-stacked volume
+    // the atoms as one LOD0 volume of 32 x 32 x 32n voxels (atom k = z slab k)
     uint8_t *d = (uint8_t *)dmalloc(n * (size_t)VCG_A3);
     VCG_CHECK(hipMemcpyAsync(d, vox, n * (size_t)VCG_A3, hipMemcpyHostToDevice, c->s));
-    enc_src s;
-    memset(&s, 0, sizeof s);
-    s.lod[0] = d; s.nx[0] = 32; s.ny[0] = 32; s.nz[0] = (uint32_t)(32 * n);
-    s.acx[0] = 1; s.acy[0] = 1; s.prefix[0] = 0; s.prefix[1] = n; s.nlod = 1; s.q[0] = q;
-    enc_scratch &es = tl_es;
-    int rc = 0;
-    std::vector<uint32_t> hl;
-    for (size_t b = 0; b < n; b += ENC_BATCH) {
-        uint32_t cnt = (uint32_t)(n - b < ENC_BATCH ? n - b : ENC_BATCH);
-        vcg_grow((void **)&es.items, &es.items_cap, cnt, sizeof(enc_item));
-        s.base = b;
-        enc_run(c, s, cnt, nullptr, 0.f, es.items, VCG_PAY_CAP);
-        hl.resize(cnt);
-        VCG_CHECK(hipMemcpyAsync(hl.data(), es.lens, (size_t)cnt * 4, hipMemcpyDeviceToHost, c->s));
-        VCG_CHECK(hipStreamSynchronize(c->s));
-        for (uint32_t k = 0; k < cnt; ++k) {
-            uint32_t L = hl[k];
-            if (L == 0xFFFFFFFFu || L > cap) { rc = 1; lens[b + k] = 0; continue; }
-            lens[b + k] = L;
-            if (L) VCG_CHECK(hipMemcpyAsync(out + (b + k) * cap, es.pay + (size_t)k * VCG_PAY_CAP, L,
-                                            hipMemcpyDeviceToHost, c->s));
-        }
-        VCG_CHECK(hipStreamSynchronize(c->s));
+    vcg_enc e;
+    memset(&e.src, 0, sizeof e.src);
+    e.src.lod[0] = d; e.src.nx[0] = 32; e.src.ny[0] = 32; e.src.nz[0] = (uint32_t)(32 * n);
+    e.src.acx[0] = 1; e.src.acy[0] = 1; e.src.prefix[0] = 0; e.src.prefix[1] = n; e.src.nlod = 1;
+    e.total = n;
+    vcg_enc *ep = &e;
+    float qs[1][VCG_NLOD] = {{q}};
+    std::vector<uint32_t> L(n);
+    uint32_t *lp = L.data();
+    int ovf = 0;
+    vcg_enc_all(1, &ep, qs, &lp, &ovf);
+    int rc = ovf;
+    std::vector<uint8_t> pool(e.pool_len);
+    if (e.pool_len) VCG_CHECK(hipMemcpyAsync(pool.data(), e.pool, e.pool_len, hipMemcpyDeviceToHost, c->s));
+    VCG_CHECK(hipStreamSynchronize(c->s));
+    for (size_t k = 0; k < n; ++k) {
+        if (L[k] > cap) { rc = 1; lens[k] = 0; continue; }
+        lens[k] = L[k];
+        if (L[k]) memcpy(out + k * cap, pool.data() + e.pool_off[k], L[k]);
     }
+    if (e.pool) VCG_CHECK(hipFree(e.pool));
     VCG_CHECK(hipFree(d));
     return rc;
 }

@@ -38,79 +38,139 @@ static int lattice(uint32_t n, uint32_t *idx) {
 
 /* The search below is exactly the sequential one (finest q, coarsest q, then
  * 16 bisection steps mid = sqrtf(lo*hi); meets -> hi = mid, else lo = mid),
- * evaluated RC_DEPTH steps per launch: each round encodes the sample at every
- * midpoint of the next RC_DEPTH levels of the bisection tree (plus both bounds
- * in the first round) and then follows the path the sequential search takes.
- * A candidate's encode stops once its byte total can no longer meet the
- * target (the largest meeting total is fixed by the voxel count), which
- * decides it the same way. */
-#define RC_DEPTH 4
-#define RC_NODES ((1 << RC_DEPTH) - 1)
+ * evaluated `depth` steps per launch: each round encodes the sample at every
+ * midpoint of the next `depth` levels of the bisection tree (plus both bounds
+ * in the first round) and then follows the path the sequential search takes,
+ * so every depth gives the same q.  A candidate's encode stops once its byte
+ * total can no longer meet the target (the largest meeting total is fixed by
+ * the voxel count), which decides it the same way.  The rounds of many
+ * volumes share launches. */
+#define RC_MAX_DEPTH 4
+#define RC_NODES ((1 << RC_MAX_DEPTH) - 1)
+#define RC_STEPS 16
 
-static float rate_control(vcg_enc *e, vc_dims d, float target) {
+typedef struct {
+    vcg_enc *e;
+    vc_dims d;
+    float target;
+    uint32_t *list, *vox;
+    uint8_t *zero;
+    int ns;
+    uint64_t v, budget;
+    float lo, hi, q0;
+    int first, steps, done;
+    int nq, base, depth;                  /* the current round's candidates */
+    float qs[RC_NODES + 2], mid[RC_NODES + 1];
+} rc_vol;
+
+/* the calibration lattice and each sample atom's in-bounds voxel count */
+static void rc_lattice(rc_vol *r) {
     uint32_t ix[8], iy[8], iz[8];
+    const vc_dims d = r->d;
     int nx = lattice((d.nx + 31) / 32, ix), ny = lattice((d.ny + 31) / 32, iy), nz = lattice((d.nz + 31) / 32, iz);
-    int ns = nx * ny * nz;
-    uint32_t *list = (uint32_t *)malloc((size_t)ns * 12);
-    uint32_t *vox = (uint32_t *)malloc((size_t)ns * 4);
-    uint8_t *zero = (uint8_t *)malloc((size_t)ns);
-    if (!list || !vox || !zero) vc__panic("out of memory");
+    r->ns = nx * ny * nz;
+    r->list = (uint32_t *)malloc((size_t)r->ns * 12);
+    r->vox = (uint32_t *)malloc((size_t)r->ns * 4);
+    r->zero = (uint8_t *)malloc((size_t)r->ns);
+    if (!r->list || !r->vox || !r->zero) vc__panic("out of memory");
     int k = 0;
     for (int a = 0; a < nz; ++a) for (int b = 0; b < ny; ++b) for (int c = 0; c < nx; ++c, ++k) {
-        list[3 * k] = iz[a]; list[3 * k + 1] = iy[b]; list[3 * k + 2] = ix[c];
+        r->list[3 * k] = iz[a]; r->list[3 * k + 1] = iy[b]; r->list[3 * k + 2] = ix[c];
         uint32_t ex = d.nx - ix[c] * 32, ey = d.ny - iy[b] * 32, ez = d.nz - iz[a] * 32;
-        vox[k] = (ex < 32 ? ex : 32) * (ey < 32 ? ey : 32) * (ez < 32 ? ez : 32);
+        r->vox[k] = (ex < 32 ? ex : 32) * (ey < 32 ? ey : 32) * (ez < 32 ? ez : 32);
     }
-    vcg_enc_sample_zero(e, list, ns, zero);
-    uint64_t v = 0;
-    for (int i = 0; i < ns; ++i) if (!zero[i]) v += vox[i];
-    float q0;
-    if (v == 0) {
-        q0 = 1.0f;                        /* all-zero sample: nothing to calibrate */
-    } else {
-        /* meets(q): no overflow and voxels >= target * bytes (double); budget =
-         * the largest byte total that meets */
-        uint64_t budget = (uint64_t)((double)v / (double)target);
-        while ((double)v >= (double)target * (double)(budget + 1)) budget++;
-        while (budget > 0 && !((double)v >= (double)target * (double)budget)) budget--;
-        float lo = 0.05f, hi = 4096.0f;
-        float qs[RC_NODES + 2], mid[RC_NODES + 1], L[RC_NODES + 1], H[RC_NODES + 1];
-        uint64_t bytes[RC_NODES + 2];
-        int flags[RC_NODES + 2], meets[RC_NODES + 2];
-        int first = 1, steps = 0, done = 0;
-        q0 = hi;
-        while (!done) {
-            /* level-order midpoints of the next RC_DEPTH levels from (lo, hi) */
-            L[1] = lo; H[1] = hi;
-            for (int t = 1; t <= RC_NODES; ++t) {
-                mid[t] = sqrtf(L[t] * H[t]);
-                if (2 * t + 1 <= RC_NODES) {
-                    L[2 * t] = L[t]; H[2 * t] = mid[t];          /* meets: hi = mid */
-                    L[2 * t + 1] = mid[t]; H[2 * t + 1] = H[t];  /* else:  lo = mid */
-                }
-            }
-            int nq = 0;
-            if (first) { qs[nq++] = lo; qs[nq++] = hi; }
-            const int base = nq;
-            for (int t = 1; t <= RC_NODES; ++t) qs[nq++] = mid[t];
-            vcg_enc_sample_multi(e, list, ns, qs, nq, budget, bytes, flags);
-            for (int j = 0; j < nq; ++j)
-                meets[j] = flags[j] == 0 && (double)v >= (double)target * (double)bytes[j];
-            if (first) {
-                first = 0;
-                if (meets[0]) { q0 = lo; break; }
-                if (!meets[1]) { q0 = hi; break; }
-            }
-            int t = 1;
-            for (int l = 0; l < RC_DEPTH && steps < 16; ++l, ++steps) {
-                if (meets[base + t - 1]) { hi = mid[t]; t = 2 * t; }
-                else { lo = mid[t]; t = 2 * t + 1; }
-            }
-            if (steps >= 16) { q0 = hi; done = 1; }
+}
+
+/* after the zero masks: the voxel count, the byte budget, the search bounds */
+static void rc_start(rc_vol *r) {
+    r->v = 0;
+    for (int i = 0; i < r->ns; ++i) if (!r->zero[i]) r->v += r->vox[i];
+    r->lo = 0.05f; r->hi = 4096.0f;
+    r->first = 1; r->steps = 0; r->done = 0;
+    if (r->v == 0) { r->q0 = 1.0f; r->done = 1; return; }   /* all-zero sample: nothing to calibrate */
+    /* meets(q): no overflow and voxels >= target * bytes (double); budget =
+     * the largest byte total that meets */
+    const double v = (double)r->v, t = (double)r->target;
+    uint64_t budget = (uint64_t)(v / t);
+    while (v >= t * (double)(budget + 1)) budget++;
+    while (budget > 0 && !(v >= t * (double)budget)) budget--;
+    r->budget = budget;
+    r->q0 = r->hi;
+}
+
+/* level-order midpoints of the next r->depth levels from (lo, hi) */
+static void rc_propose(rc_vol *r, int depth) {
+    if (depth > RC_STEPS - r->steps) depth = RC_STEPS - r->steps;
+    const int nodes = (1 << depth) - 1;
+    float L[RC_NODES + 1], H[RC_NODES + 1];
+    L[1] = r->lo; H[1] = r->hi;
+    for (int t = 1; t <= nodes; ++t) {
+        r->mid[t] = sqrtf(L[t] * H[t]);
+        if (2 * t + 1 <= nodes) {
+            L[2 * t] = L[t]; H[2 * t] = r->mid[t];          /* meets: hi = mid */
+            L[2 * t + 1] = r->mid[t]; H[2 * t + 1] = H[t];  /* else:  lo = mid */
         }
     }
-    free(list); free(vox); free(zero);
-    return q0;
+    r->depth = depth;
+    r->nq = 0;
+    if (r->first) { r->qs[r->nq++] = r->lo; r->qs[r->nq++] = r->hi; }
+    r->base = r->nq;
+    for (int t = 1; t <= nodes; ++t) r->qs[r->nq++] = r->mid[t];
+}
+
+static void rc_accept(rc_vol *r, const uint64_t *bytes, const int *flags) {
+    int meets[RC_NODES + 2];
+    for (int j = 0; j < r->nq; ++j)
+        meets[j] = flags[j] == 0 && (double)r->v >= (double)r->target * (double)bytes[j];
+    if (r->first) {
+        r->first = 0;
+        if (meets[0]) { r->q0 = r->lo; r->done = 1; return; }
+        if (!meets[1]) { r->q0 = r->hi; r->done = 1; return; }
+    }
+    int t = 1;
+    for (int l = 0; l < r->depth; ++l, ++r->steps) {
+        if (meets[r->base + t - 1]) { r->hi = r->mid[t]; t = 2 * t; }
+        else { r->lo = r->mid[t]; t = 2 * t + 1; }
+    }
+    if (r->steps >= RC_STEPS) { r->q0 = r->hi; r->done = 1; }
+}
+
+/* base_q of nv volumes (r[v].e, r[v].d, r[v].target set); depth = bisection
+ * levels per launch (1..RC_MAX_DEPTH) */
+static void rate_control(int nv, rc_vol *r, int depth) {
+    vcg_enc **es = (vcg_enc **)malloc((size_t)nv * sizeof *es);
+    const uint32_t **lists = (const uint32_t **)malloc((size_t)nv * sizeof *lists);
+    uint8_t **zeros = (uint8_t **)malloc((size_t)nv * sizeof *zeros);
+    int *ns = (int *)malloc((size_t)nv * sizeof *ns);
+    const size_t cap = (size_t)nv * (RC_NODES + 2);
+    vcg_enc **ge = (vcg_enc **)malloc(cap * sizeof *ge);
+    float *gq = (float *)malloc(cap * sizeof *gq);
+    uint64_t *gb = (uint64_t *)malloc(cap * 8), *bytes = (uint64_t *)malloc(cap * 8);
+    int *flags = (int *)malloc(cap * sizeof *flags);
+    if (!es || !lists || !zeros || !ns || !ge || !gq || !gb || !bytes || !flags) vc__panic("out of memory");
+    for (int v = 0; v < nv; ++v) {
+        rc_lattice(&r[v]);
+        es[v] = r[v].e; lists[v] = r[v].list; zeros[v] = r[v].zero; ns[v] = r[v].ns;
+    }
+    vcg_enc_samples(nv, es, lists, ns, zeros);
+    for (int v = 0; v < nv; ++v) rc_start(&r[v]);
+    for (;;) {
+        int ng = 0;
+        for (int v = 0; v < nv; ++v) {
+            if (r[v].done) continue;
+            rc_propose(&r[v], depth);
+            for (int j = 0; j < r[v].nq; ++j, ++ng) { ge[ng] = r[v].e; gq[ng] = r[v].qs[j]; gb[ng] = r[v].budget; }
+        }
+        if (!ng) break;
+        vcg_enc_candidates(ng, ge, gq, gb, bytes, flags);
+        for (int v = 0, g = 0; v < nv; ++v) {
+            if (r[v].done) continue;
+            rc_accept(&r[v], bytes + g, flags + g);
+            g += r[v].nq;
+        }
+    }
+    for (int v = 0; v < nv; ++v) { free(r[v].list); free(r[v].vox); free(r[v].zero); }
+    free(es); free(lists); free(zeros); free(ns); free(ge); free(gq); free(gb); free(bytes); free(flags);
 }
 
 static void lod_schedule(float q0, int nlod, float qs[VC_NLOD]) {
@@ -196,35 +256,119 @@ done:
     return rc;
 }
 
-static vc_status encode_impl(const uint8_t *vol, int on_dev, vc_dims dims, float target, const float *qs_in,
-                             uint8_t **out, size_t *out_len, float *q0_only) {
-    if (!vol || (!q0_only && (!out || !out_len))) return VC_ERR_RANGE;
-    if (!vc__dims_valid(dims) || !(target >= 1.0f)) return VC_ERR_RANGE;
-    vc__require_gpu();
-    int nlod = vc__nlod(dims);
-    uint32_t d[3] = {dims.nx, dims.ny, dims.nz};
-    vcg_enc *e = vcg_enc_begin(vol, on_dev, d, nlod);
-    float qs[VC_NLOD];
-    if (qs_in) {
-        for (int l = 0; l < VC_NLOD; ++l) qs[l] = l < nlod ? vc__clamp_q(qs_in[l]) : 0.0f;
-    } else {
-        float q0 = rate_control(e, dims, target);
-        if (q0_only) { *q0_only = q0; vcg_enc_end(e); return VC_OK; }
-        lod_schedule(q0, nlod, qs);
-    }
+static uint64_t atom_total(vc_dims dims, int nlod) {
     uint64_t total = 0;
     for (int l = 0; l < nlod; ++l) {
         vc_dims dl = vc__lod_dims(dims, l);
         total += (uint64_t)((dl.nx + 31) / 32) * ((dl.ny + 31) / 32) * ((dl.nz + 31) / 32);
     }
-    uint32_t *lens = (uint32_t *)malloc(total * 4);
-    if (!lens) { vcg_enc_end(e); return VC_ERR_OOM; }
-    vc_status rc;
-    if (vcg_enc_all(e, qs, lens, total)) rc = VC_ERR_RANGE;    /* a payload exceeded the coder bound */
-    else rc = assemble(e, dims, nlod, target, qs, lens, total, out, out_len);
-    free(lens);
-    vcg_enc_end(e);
-    return rc;
+    return total;
+}
+
+/* Encode nv volumes together: pyramids, rate control rounds shared across
+ * volumes, one set of encode launches over every atom, then each archive.
+ * qs_in (one volume only): explicit base_q per LOD instead of rate control. */
+static void encode_group(int nv, const uint8_t *const *vols, int on_dev, const vc_dims *dims, const float *targets,
+                         const float *qs_in, uint8_t **outs, size_t *out_lens, vc_status *st, float *q0_only) {
+    rc_vol *r = (rc_vol *)calloc((size_t)nv, sizeof *r);
+    vcg_enc **es = (vcg_enc **)malloc((size_t)nv * sizeof *es);
+    float (*qs)[VC_NLOD] = (float (*)[VC_NLOD])malloc((size_t)nv * sizeof *qs);
+    uint32_t **lens = (uint32_t **)calloc((size_t)nv, sizeof *lens);
+    int *nlod = (int *)malloc((size_t)nv * sizeof *nlod), *ovf = (int *)malloc((size_t)nv * sizeof *ovf);
+    uint64_t *total = (uint64_t *)malloc((size_t)nv * 8);
+    if (!r || !es || !qs || !lens || !nlod || !ovf || !total) vc__panic("out of memory");
+    for (int v = 0; v < nv; ++v) {
+        nlod[v] = vc__nlod(dims[v]);
+        uint32_t d[3] = {dims[v].nx, dims[v].ny, dims[v].nz};
+        es[v] = vcg_enc_begin(vols[v], on_dev, d, nlod[v]);
+        r[v].e = es[v]; r[v].d = dims[v]; r[v].target = targets[v];
+    }
+    if (qs_in) {
+        for (int l = 0; l < VC_NLOD; ++l) qs[0][l] = l < nlod[0] ? vc__clamp_q(qs_in[l]) : 0.0f;
+    } else {
+        rate_control(nv, r, nv >= 16 ? 2 : RC_MAX_DEPTH);
+        for (int v = 0; v < nv; ++v) lod_schedule(r[v].q0, nlod[v], qs[v]);
+    }
+    if (q0_only) {
+        *q0_only = r[0].q0;
+    } else {
+        for (int v = 0; v < nv; ++v) {
+            total[v] = atom_total(dims[v], nlod[v]);
+            lens[v] = (uint32_t *)malloc(total[v] * 4);
+            if (!lens[v]) vc__panic("out of memory");
+        }
+        vcg_enc_all(nv, es, (const float (*)[VC_NLOD])qs, lens, ovf);
+        for (int v = 0; v < nv; ++v) {
+            outs[v] = NULL; out_lens[v] = 0;
+            if (ovf[v]) st[v] = VC_ERR_RANGE;        /* a payload exceeded the coder bound */
+            else st[v] = assemble(es[v], dims[v], nlod[v], targets[v], qs[v], lens[v], total[v], &outs[v],
+                                  &out_lens[v]);
+        }
+    }
+    for (int v = 0; v < nv; ++v) { vcg_enc_end(es[v]); free(lens[v]); }
+    free(r); free(es); free(qs); free(lens); free(nlod); free(ovf); free(total);
+}
+
+static vc_status encode_impl(const uint8_t *vol, int on_dev, vc_dims dims, float target, const float *qs_in,
+                             uint8_t **out, size_t *out_len, float *q0_only) {
+    if (!vol || (!q0_only && (!out || !out_len))) return VC_ERR_RANGE;
+    if (!vc__dims_valid(dims) || !(target >= 1.0f)) return VC_ERR_RANGE;
+    vc__require_gpu();
+    uint8_t *o = NULL; size_t ol = 0;
+    vc_status st = VC_OK;
+    encode_group(1, &vol, on_dev, &dims, &target, qs_in, &o, &ol, &st, q0_only);
+    if (!q0_only) { *out = o; *out_len = ol; }
+    return st;
+}
+
+/* device bytes a volume holds while its group encodes: the pyramid (at most
+ * 8/7 of LOD0) and the sample coefficients */
+static uint64_t group_bytes(vc_dims d) {
+    uint32_t ix[8];
+    uint64_t s = (uint64_t)lattice((d.nx + 31) / 32, ix) * lattice((d.ny + 31) / 32, ix) *
+                 lattice((d.nz + 31) / 32, ix);
+    return (uint64_t)d.nx * d.ny * d.nz / 7 * 8 + s * (32768u * 4u + 16u);
+}
+#define VC_GROUP_BYTES (24ull << 30)
+
+vc_status vcx_encode_batch(size_t n, const uint8_t *const *vols, int vols_on_device, const vc_dims *dims,
+                           const float *target_ratios, uint8_t **outs, size_t *out_lens, vc_status *statuses) {
+    if (n == 0) return VC_OK;
+    if (!vols || !dims || !target_ratios || !outs || !out_lens || !statuses) return VC_ERR_RANGE;
+    vc__require_gpu();
+    vc_status first = VC_OK;
+    size_t *idx = (size_t *)malloc(n * sizeof *idx);
+    const uint8_t **gv = (const uint8_t **)malloc(n * sizeof *gv);
+    vc_dims *gd = (vc_dims *)malloc(n * sizeof *gd);
+    float *gt = (float *)malloc(n * sizeof *gt);
+    uint8_t **go = (uint8_t **)malloc(n * sizeof *go);
+    size_t *gl = (size_t *)malloc(n * sizeof *gl);
+    vc_status *gs = (vc_status *)malloc(n * sizeof *gs);
+    if (!idx || !gv || !gd || !gt || !go || !gl || !gs) vc__panic("out of memory");
+    size_t i = 0;
+    while (i < n) {
+        /* the next group: valid volumes in order while their device bytes fit */
+        int ng = 0;
+        uint64_t bytes = 0;
+        for (; i < n; ++i) {
+            outs[i] = NULL; out_lens[i] = 0;
+            if (!vols[i] || !vc__dims_valid(dims[i]) || !(target_ratios[i] >= 1.0f)) {
+                statuses[i] = VC_ERR_RANGE;
+                continue;
+            }
+            const uint64_t b = group_bytes(dims[i]);
+            if (ng && bytes + b > VC_GROUP_BYTES) break;
+            bytes += b;
+            idx[ng] = i; gv[ng] = vols[i]; gd[ng] = dims[i]; gt[ng] = target_ratios[i];
+            ++ng;
+        }
+        if (!ng) break;
+        encode_group(ng, gv, vols_on_device, gd, gt, NULL, go, gl, gs, NULL);
+        for (int k = 0; k < ng; ++k) { outs[idx[k]] = go[k]; out_lens[idx[k]] = gl[k]; statuses[idx[k]] = gs[k]; }
+    }
+    for (size_t k = 0; k < n; ++k) if (statuses[k] != VC_OK && first == VC_OK) first = statuses[k];
+    free(idx); free(gv); free(gd); free(gt); free(go); free(gl); free(gs);
+    return first;
 }
 
 vc_status vc_encode(const uint8_t *vol, vc_dims dims, float target_ratio, uint8_t **out_archive, size_t *out_len) {

@@ -89,21 +89,22 @@ int vcg_decode_payloads(size_t n, const uint8_t *pay, const uint64_t *offs, cons
 typedef struct vcg_enc vcg_enc;
 /* Upload (or adopt) a LOD0 volume and build the nlod-level pyramid on device. */
 vcg_enc *vcg_enc_begin(const uint8_t *vol, int vol_on_device, const uint32_t dims[3], int nlod);
-/* Rate-control probe: total payload bytes of the listed LOD0 atoms (zyx
- * triples) at base_q.  First call caches their coefficients.  Returns 0, or 1
- * if any atom overflowed the coder bound.  zero_mask[i] = 1 for all-zero atoms. */
-int  vcg_enc_sample(vcg_enc *e, const uint32_t *atoms_zyx, int n, float base_q,
-                    uint64_t *bytes, uint8_t *zero_mask);
-/* zero_mask[i] = 1 for the listed atoms that are all zero (caches coefficients) */
-int  vcg_enc_sample_zero(vcg_enc *e, const uint32_t *atoms_zyx, int n, uint8_t *zero_mask);
-/* The sample's byte total at nq base_q values in one launch pair: bytes[k]
- * (dc bytes included), flags[k] bit 0 = coder bound exceeded, bit 1 = the
- * total passed `budget` and the encode stopped early (bytes[k] not final). */
-int  vcg_enc_sample_multi(vcg_enc *e, const uint32_t *atoms_zyx, int n, const float *qs, int nq,
-                          uint64_t budget, uint64_t *bytes, int *flags);
-/* Encode every atom of every LOD (LOD-major, raster) at qs[lod]; lens gets
- * each payload length (0 = all-zero atom).  Returns 0, or 1 on coder overflow. */
-int  vcg_enc_all(vcg_enc *e, const float qs[VCG_NLOD], uint32_t *lens, uint64_t total_atoms);
+/* Rate-control samples of nv encoders in one launch: encoder v's ns[v] listed
+ * LOD0 atoms (zyx triples) are transformed and their coefficients kept on
+ * device (once per encoder); zero_masks[v][i] = 1 for its all-zero atoms. */
+int  vcg_enc_samples(int nv, vcg_enc *const *es, const uint32_t *const *lists_zyx, const int *ns,
+                     uint8_t *const *zero_masks);
+/* Byte totals of ng rate-control candidates in as few launches as fit:
+ * candidate g = encoder es[g]'s sample at base_q qs[g].  bytes[g] (dc bytes
+ * included), flags[g] bit 0 = coder bound exceeded, bit 1 = the total passed
+ * budgets[g] and the candidate was abandoned (bytes[g] not final). */
+int  vcg_enc_candidates(int ng, vcg_enc *const *es, const float *qs, const uint64_t *budgets,
+                        uint64_t *bytes, int *flags);
+/* Encode every atom of every LOD (LOD-major, raster) of nv encoders, encoder
+ * v at qs[v][lod], in launches taken across volumes; lens[v][i] = payload
+ * length of encoder v's atom i (0 = all-zero atom); ovf[v] = 1 if one of its
+ * payloads exceeded the coder bound. */
+int  vcg_enc_all(int nv, vcg_enc *const *es, const float (*qs)[VCG_NLOD], uint32_t *const *lens, int *ovf);
 /* Assemble the archive image: head (header + directory, head_len bytes) at 0,
  * then for atom i with lens[i] > 0 its payload at pay_off[i] and, for every
  * atom with slot_off[i] != 0, its 16-byte slot at slot_off[i] (flags 1 with

@@ -87,6 +87,7 @@ def lib():
         "vcx_encode_q": (i32, [vp, Dims, f32, vp, P(vp), P(sz)]),
         "vcx_encode_device": (i32, [vp, Dims, f32, P(vp), P(sz)]),
         "vcx_rate_control": (f32, [vp, Dims, f32]),
+        "vcx_encode_batch": (i32, [sz, vp, i32, vp, vp, vp, vp, vp]),
         "vcx_encode_atoms": (i32, [sz, vp, f32, vp, sz, vp]),
         "vcx_decode_payloads": (i32, [sz, vp, vp, vp, f32, vp, vp]),
         "vcx_synth_device": (i32, [u64, Dims, i32, vp]),
@@ -140,6 +141,36 @@ def encode_device(dev_ptr, dims, target_ratio):
     p, n = C.c_void_p(), C.c_size_t()
     st = lib().vcx_encode_device(dev_ptr, Dims(*dims), C.c_float(target_ratio), C.byref(p), C.byref(n))
     return st, (_take(p, n.value) if st == VC_OK else None)
+
+
+def encode_batch(vols, target_ratios, on_device=False, dims=None):
+    """vcx_encode_batch: vols are numpy u8 volumes (z, y, x), or device
+    pointers with dims [(nx, ny, nz)] when on_device.  Returns (rc, [(status,
+    archive bytes or None)])."""
+    n = len(vols)
+    vp = C.c_void_p
+    if on_device:
+        ptrs = (vp * n)(*[int(p) for p in vols])
+        dd = (Dims * n)(*[Dims(*d) for d in dims])
+        keep = None
+    else:
+        keep = [_u8(v) for v in vols]
+        ptrs = (vp * n)(*[k.ctypes.data for k in keep])
+        dd = (Dims * n)(*[dims_of(k) for k in keep])
+    t = np.ascontiguousarray(target_ratios, np.float32)
+    outs = (vp * n)()
+    lens = np.zeros(n, np.uint64)
+    st = np.zeros(n, np.int32)
+    rc = lib().vcx_encode_batch(n, C.cast(ptrs, vp), 1 if on_device else 0, C.cast(dd, vp), t.ctypes.data,
+                                C.cast(outs, vp), lens.ctypes.data, st.ctypes.data)
+    res = []
+    for i in range(n):
+        if st[i] == VC_OK:
+            res.append((0, _take(C.c_void_p(outs[i]), int(lens[i]))))
+        else:
+            res.append((int(st[i]), None))
+    del keep
+    return rc, res
 
 
 def rate_control(vol, target_ratio):
