@@ -74,6 +74,11 @@ vc_status vcx_encode_device(const void *dev_vol, vc_dims dims, float target_rati
 vc_status vcx_encode_batch(size_t n, const uint8_t *const *vols, int vols_on_device, const vc_dims *dims,
                            const float *target_ratios, uint8_t **outs, size_t *out_lens,
                            vc_status *statuses);
+/* Test hook: cap the rate-control rows (candidate x sample atom) per launch
+ * (0 = default, 262144; at least 512).  Smaller caps make vcx_encode_batch
+ * evaluate fewer bisection levels per launch and split rounds over more
+ * launches; archives do not change. */
+void      vcx_test_rc_rows(uint32_t rows);
 /* The LOD0 base_q that vc_encode's rate control picks for this volume. */
 float     vcx_rate_control(const uint8_t *vol, vc_dims dims, float target_ratio);
 

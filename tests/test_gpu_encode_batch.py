@@ -26,8 +26,8 @@ def _volumes(oracle, seed, n, lo, hi):
 
 
 def test_encode_batch_matches_oracle(pyvc, oracle):
-    """18 volumes (at least 16, the shared-round width used for large
-    groups), one all-zero and one of a single voxel, dims not multiples of 32."""
+    """18 volumes, one all-zero and one of a single voxel, dims not multiples
+    of 32."""
     vols, targets = _volumes(oracle, 5, 16, 40, 110)
     vols += [np.zeros((45, 38, 70), np.uint8), oracle.synth(77, 1, 1, 1, 0)]
     targets += [10.0, 50.0]
@@ -57,6 +57,31 @@ def test_encode_batch_small_group_and_errors(pyvc, oracle):
         assert st == 0 and res[k] == (0, arc), k
     ost, oarc = oracle.encode(vols[0], targets[0])
     assert res[0][1] == oarc
+
+
+def test_encode_batch_launch_capacities(pyvc, oracle):
+    """The number of bisection levels evaluated per launch (and the split of
+    a round over launches) follows the launch capacity; the archives do not.
+    Capacities from 512 rows (one level per launch, rounds over several
+    launches) up to the default."""
+    vols, targets = _volumes(oracle, 21, 7, 60, 230)
+    ref = []
+    for v, t in zip(vols, targets):
+        st, arc = pyvc.encode(v, t)
+        assert st == 0
+        ref.append(arc)
+    try:
+        for rows in (512, 1500, 4000, 9000, 0):
+            pyvc.test_rc_rows(rows)
+            say("rc rows", rows)
+            rc, res = pyvc.encode_batch(vols, targets)
+            assert rc == 0
+            for k in range(len(vols)):
+                assert res[k] == (0, ref[k]), (rows, k)
+    finally:
+        pyvc.test_rc_rows(0)
+    ost, oarc = oracle.encode(vols[0], targets[0])
+    assert oarc == ref[0]
 
 
 def test_encode_batch_device_resident(pyvc):

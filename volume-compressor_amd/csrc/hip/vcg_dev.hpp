@@ -18,6 +18,7 @@ struct vcg_dtab {
     float    gstep[VCG_NFREQ];
     uint32_t band_start[9];
     uint16_t scan[VCG_A3];
+    uint8_t  fscan[VCG_A3];   // frequency cz+cy+cx of scan position p (non-decreasing in p)
 };
 extern vcg_dtab *g_dtab;  // device pointer
 

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
 #include <string.h>
+#include <thread>
 #include <vector>
 #include "vcg_dev.hpp"
 #include "vcg_xform.hpp"
@@ -66,12 +67,21 @@ __device__ __forceinline__ void enc_atom_of(const enc_src &s, const uint32_t *li
     az = (uint32_t)(r / ((uint64_t)s.acx[l] * s.acy[l]));
 }
 
-// index of the last base <= x in an increasing array (n >= 1, base[0] <= x)
-__device__ __forceinline__ uint32_t upper_index(const uint64_t *base, uint32_t n, uint64_t x) {
+// A launch's atoms are a concatenation of segments: flat atoms [base, next
+// segment's base) are atoms gi0, gi0 + 1, ... of volume v.
+struct enc_seg {
+    uint64_t base;
+    uint64_t gi0;
+    uint32_t v;
+    uint32_t pad;
+};
+
+// the segment holding flat atom x (segments sorted by base, seg[0].base <= x)
+__device__ __forceinline__ uint32_t seg_of(const enc_seg *seg, uint32_t n, uint64_t x) {
     uint32_t lo = 0, hi = n;
     while (hi - lo > 1) {
         const uint32_t m = (lo + hi) / 2;
-        if (base[m] <= x) lo = m; else hi = m;
+        if (seg[m].base <= x) lo = m; else hi = m;
     }
     return lo;
 }
@@ -102,13 +112,13 @@ __global__ void k_downsample(const uint8_t *__restrict__ in, uint32_t nx, uint32
 
 enum { FDCT_STORE_COEF = 0, FDCT_QUANT = 1 };
 
-// Block i handles atom base + i of the concatenation of nv volumes (volume
-// v's atoms start at vbase[v]; in FDCT_STORE_COEF mode a volume's atoms are
-// its sample list).  FDCT_QUANT: levels to lv_out and items to items, both
-// indexed by i; FDCT_STORE_COEF: coefficients and items to the volume's
-// sample arrays, indexed by the atom's place in its list.
+// Block i handles flat atom base + i of the segments (in FDCT_STORE_COEF
+// mode a volume's atoms are its sample list).  FDCT_QUANT: levels to lv_out
+// and items to items_out, both indexed by i; FDCT_STORE_COEF: coefficients
+// (scan order) and items to the volume's sample arrays, indexed by the
+// atom's place in its list.
 __global__ __launch_bounds__(FDCT_THREADS) void k_fdct(const enc_src *__restrict__ srcs,
-                                                       const uint64_t *__restrict__ vbase, uint32_t nv,
+                                                       const enc_seg *__restrict__ segs, uint32_t nseg,
                                                        uint64_t base, int mode,
                                                        const vcg_dtab *__restrict__ T,
                                                        const vcg_fwd_tab *__restrict__ F,
@@ -120,9 +130,9 @@ __global__ __launch_bounds__(FDCT_THREADS) void k_fdct(const enc_src *__restrict
     const int tid = threadIdx.x;
     const uint32_t i = blockIdx.x;
     const uint64_t flat = base + i;
-    const uint32_t v = nv > 1 ? upper_index(vbase, nv, flat) : 0;
-    const enc_src &s = srcs[v];
-    const uint64_t gi = flat - vbase[v];
+    const enc_seg &sg = segs[nseg > 1 ? seg_of(segs, nseg, flat) : 0];
+    const enc_src &s = srcs[sg.v];
+    const uint64_t gi = sg.gi0 + (flat - sg.base);
     int lod; uint32_t ax, ay, az;
     const bool store = mode == FDCT_STORE_COEF;
     enc_atom_of(s, store ? s.list : nullptr, gi, lod, ax, ay, az);
@@ -148,7 +158,10 @@ __global__ __launch_bounds__(FDCT_THREADS) void k_fdct(const enc_src *__restrict
     __syncthreads();
     if (store) {
         int32_t *co = s.coef + gi * VCG_A3;
-        for (int k = tid; k < 32768; k += FDCT_THREADS) co[k] = buf[vcg_tidx(k >> 10, (k >> 5) & 31, k & 31)];
+        for (int p = tid; p < 32768; p += FDCT_THREADS) {
+            const uint32_t idx = T->scan[p];
+            co[p] = buf[vcg_tidx((int)(idx >> 10), (int)((idx >> 5) & 31), (int)(idx & 31))];
+        }
         if (tid == 0) { enc_item it = {0, (uint32_t)dc, 0, (uint32_t)lod}; *items = it; }
         return;
     }
@@ -178,28 +191,38 @@ __device__ __forceinline__ uint32_t group_of(const samp_group *G, uint32_t ng, u
     return lo;
 }
 
-// quantize every candidate's sample atoms: block = one row
+// quantize every candidate's sample atoms (coefficients in scan order):
+// block = one row, four consecutive positions per thread and step
 __global__ __launch_bounds__(FDCT_THREADS) void k_quant_groups(const samp_group *__restrict__ G, uint32_t ng,
                                                                const vcg_dtab *__restrict__ T,
                                                                int16_t *__restrict__ lv_out,
                                                                uint32_t *__restrict__ eobs) {
     __shared__ int seob;
+    __shared__ float gs[VCG_NFREQ];
     const int tid = threadIdx.x;
     const uint64_t row = blockIdx.x;
     const samp_group &g = G[group_of(G, ng, row)];
     const uint32_t i = (uint32_t)(row - g.off);
     if (g.items[i].zero) { if (tid == 0) eobs[row] = 0; return; }
     if (tid == 0) seob = 0;
+    if (tid < VCG_NFREQ) gs[tid] = T->gstep[tid];
     __syncthreads();
     const float q = g.q;
-    const int32_t *co = g.coef + (size_t)i * VCG_A3;
-    int16_t *lo = lv_out + row * VCG_A3;
+    const int4 *co = (const int4 *)(g.coef + (size_t)i * VCG_A3);
+    const uint32_t *fq = (const uint32_t *)T->fscan;
+    uint2 *lo = (uint2 *)(lv_out + row * VCG_A3);
     int last = 0;
-    for (int p = tid; p < 32768; p += FDCT_THREADS) {
-        const uint32_t idx = T->scan[p];
-        const int l = vcg_quant1(co[idx], q, T->gstep[vcg_freq(idx)]);
-        lo[p] = (int16_t)l;
-        if (l) last = p + 1;
+    for (int k = tid; k < 32768 / 4; k += FDCT_THREADS) {
+        const int4 c = co[k];
+        const uint32_t f = fq[k];
+        const int l0 = vcg_quant1(c.x, q, gs[f & 0xFF]);
+        const int l1 = vcg_quant1(c.y, q, gs[(f >> 8) & 0xFF]);
+        const int l2 = vcg_quant1(c.z, q, gs[(f >> 16) & 0xFF]);
+        const int l3 = vcg_quant1(c.w, q, gs[f >> 24]);
+        lo[k] = make_uint2((uint32_t)(uint16_t)l0 | ((uint32_t)(uint16_t)l1 << 16),
+                           (uint32_t)(uint16_t)l2 | ((uint32_t)(uint16_t)l3 << 16));
+        const int p = 4 * k;
+        if (l3) last = p + 4; else if (l2) last = p + 3; else if (l1) last = p + 2; else if (l0) last = p + 1;
     }
     if (last) atomicMax(&seob, last);
     __syncthreads();
@@ -304,6 +327,7 @@ static void enc_setup() {
 
 struct vcg_enc {
     enc_src src;
+    bool arena = false;                // device memory from the thread's encode arena
     uint64_t total = 0;                // atoms over every LOD
     uint8_t *lod_buf[VCG_NLOD] = {nullptr};
     // rate-control sample
@@ -322,6 +346,49 @@ static void *dmalloc(size_t n) {
     return p;
 }
 
+// Bump arena for the encoders of one group: a few large chunks instead of a
+// dozen allocations and frees per volume (hipFree waits for the device).
+struct enc_arena {
+    bool open = false;
+    std::vector<std::pair<uint8_t *, size_t>> chunks;
+    size_t used = 0;                  // bytes used in the last chunk
+};
+static thread_local enc_arena tl_arena;
+#define ARENA_CHUNK (1ull << 30)
+
+extern "C" void vcg_enc_arena_open(void) { tl_arena.open = true; }
+
+static void scratch_release();
+
+// closing the arena also releases the thread's encode scratch, which a
+// group of many volumes grows to several GiB
+extern "C" void vcg_enc_arena_close(void) {
+    enc_arena &A = tl_arena;
+    VCG_CHECK(hipStreamSynchronize(vcg_get_ctx()->s));
+    for (auto &ch : A.chunks) VCG_CHECK(hipFree(ch.first));
+    A.chunks.clear();
+    A.used = 0;
+    A.open = false;
+    scratch_release();
+}
+
+static void *enc_alloc(bool arena, size_t n) {
+    if (!arena) return dmalloc(n);
+    enc_arena &A = tl_arena;
+    n = (n + 255) & ~(size_t)255;
+    if (A.chunks.empty() || A.used + n > A.chunks.back().second) {
+        const size_t cs = n > ARENA_CHUNK ? n : ARENA_CHUNK;
+        A.chunks.push_back({(uint8_t *)dmalloc(cs), cs});
+        A.used = 0;
+    }
+    void *p = A.chunks.back().first + A.used;
+    A.used += n;
+    return p;
+}
+static void enc_free(bool arena, void *p) {
+    if (!arena && p) VCG_CHECK(hipFree(p));
+}
+
 extern "C" int vcg_downsample(const void *in, const uint32_t d[3], void *out) {
     vcg_ctx *c = vcg_get_ctx();
     uint64_t total = (uint64_t)((d[0] + 1) / 2) * ((d[1] + 1) / 2) * ((d[2] + 1) / 2);
@@ -338,6 +405,7 @@ extern "C" vcg_enc *vcg_enc_begin(const uint8_t *vol, int on_dev, const uint32_t
     enc_setup();
     vcg_enc *e = new vcg_enc();
     memset(&e->src, 0, sizeof e->src);
+    e->arena = tl_arena.open;
     uint32_t x = dims[0], y = dims[1], z = dims[2];
     uint64_t acc = 0;
     for (int l = 0; l < nlod; ++l) {
@@ -354,13 +422,13 @@ extern "C" vcg_enc *vcg_enc_begin(const uint8_t *vol, int on_dev, const uint32_t
     if (on_dev) {
         e->src.lod[0] = vol;
     } else {
-        e->lod_buf[0] = (uint8_t *)dmalloc(n0);
+        e->lod_buf[0] = (uint8_t *)enc_alloc(e->arena, n0);
         VCG_CHECK(hipMemcpyAsync(e->lod_buf[0], vol, n0, hipMemcpyHostToDevice, c->s));
         e->src.lod[0] = e->lod_buf[0];
     }
     for (int l = 1; l < nlod; ++l) {
         size_t nl = (size_t)e->src.nx[l] * e->src.ny[l] * e->src.nz[l];
-        e->lod_buf[l] = (uint8_t *)dmalloc(nl);
+        e->lod_buf[l] = (uint8_t *)enc_alloc(e->arena, nl);
         uint32_t d[3] = {e->src.nx[l - 1], e->src.ny[l - 1], e->src.nz[l - 1]};
         vcg_downsample(e->src.lod[l - 1], d, e->lod_buf[l]);
         e->src.lod[l] = e->lod_buf[l];
@@ -377,11 +445,48 @@ struct enc_scratch {
     uint32_t *aux = nullptr; size_t aux_cap = 0;      // rate-control candidates: eobs, counters
     uint8_t *desc = nullptr; size_t desc_cap = 0;     // per-call descriptors (sources, groups, addresses)
     uint8_t *img = nullptr; size_t img_cap = 0;       // archive image
+    uint8_t *hstage[2] = {nullptr, nullptr};          // pinned halves of one allocation
+    hipEvent_t hev[2] = {nullptr, nullptr};
 };
+#define HSTAGE_BYTES (32u << 20)
+
+// memcpy into a freshly allocated (pageable) buffer: the first touch of its
+// pages dominates, so large copies are split over a few host threads
+static void par_copy(uint8_t *dst, const uint8_t *src, size_t n) {
+    const size_t part = 1u << 20;
+    size_t nt = n / part;
+    if (nt < 2) { memcpy(dst, src, n); return; }
+    if (nt > 8) nt = 8;
+    const size_t each = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; ++t) {
+        const size_t o = t * each, m = n - o < each ? n - o : each;
+        th.emplace_back([=] { memcpy(dst + o, src + o, m); });
+    }
+    memcpy(dst, src, each);
+    for (auto &t : th) t.join();
+}
 static thread_local enc_scratch tl_es;
 
-// rows of 32768 levels per launch (rate-control candidates or atoms)
+static void scratch_release() {
+    enc_scratch &S = tl_es;
+    void *bufs[] = {S.lv, S.items, S.pay, S.lens, S.aux, S.desc, S.img};
+    for (void *b : bufs) if (b) VCG_CHECK(hipFree(b));
+    if (S.hstage[0]) VCG_CHECK(hipHostFree(S.hstage[0]));
+    for (int k = 0; k < 2; ++k) if (S.hev[k]) VCG_CHECK(hipEventDestroy(S.hev[k]));
+    S = enc_scratch();
+}
+
+// atoms per encode launch (levels + payload scratch: 192 KiB each)
 #define ENC_ROWS 65536u
+// rate-control rows (candidate x sample atom) per launch: VCG_RC_ROWS
+// (levels: 64 KiB each).  A round's rows share one launch up to this size:
+// the count lanes are latency-bound, so more waves per SIMD cost little.
+static uint32_t g_rc_rows = VCG_RC_ROWS;
+extern "C" uint32_t vcg_rc_rows(void) { return g_rc_rows; }
+extern "C" void vcg_set_rc_rows(uint32_t rows) {
+    g_rc_rows = rows == 0 || rows > VCG_RC_ROWS ? VCG_RC_ROWS : rows < 512 ? 512 : rows;
+}
 
 // clear n words with a kernel (no memset node in code that may be captured)
 __global__ void k_reset_words(uint32_t *w, uint64_t n) {
@@ -395,26 +500,44 @@ static uint8_t *desc_area(size_t n) {
     return tl_es.desc;
 }
 
-// sources of volumes es[0..nv) and their atom bases (device copies at *dsrc/*dbase)
+// sources of volumes es[0..nv) and the segments of the flat atom order
+// (device copies at *dsrc / *dseg); returns the flat atom count.  Samples:
+// volume-major.  Full encode: coarsest LOD first across every volume, so the
+// heaviest atoms (coarse LODs carry the most coefficients) share launches.
 static uint64_t upload_sources(vcg_ctx *c, int nv, vcg_enc *const *es, const float (*qs)[VCG_NLOD],
-                               bool sample, std::vector<uint8_t> &host, enc_src **dsrc, uint64_t **dbase) {
-    const size_t so = 0, bo = ((size_t)nv * sizeof(enc_src) + 255) & ~(size_t)255;
-    host.assign(bo + (size_t)nv * 8, 0);
-    enc_src *hs = (enc_src *)host.data();
-    uint64_t *hb = (uint64_t *)(host.data() + bo);
+                               bool sample, std::vector<uint8_t> &host, std::vector<enc_seg> &seg,
+                               enc_src **dsrc, enc_seg **dseg) {
+    seg.clear();
     uint64_t tot = 0;
+    if (sample) {
+        for (int v = 0; v < nv; ++v) {
+            if (!es[v]->sample_n) continue;
+            seg.push_back(enc_seg{tot, 0, (uint32_t)v, 0});
+            tot += (uint64_t)es[v]->sample_n;
+        }
+    } else {
+        for (int l = VCG_NLOD - 1; l >= 0; --l)
+            for (int v = 0; v < nv; ++v) {
+                const enc_src &s = es[v]->src;
+                if (l >= s.nlod || s.prefix[l + 1] == s.prefix[l]) continue;
+                seg.push_back(enc_seg{tot, s.prefix[l], (uint32_t)v, 0});
+                tot += s.prefix[l + 1] - s.prefix[l];
+            }
+    }
+    const size_t bo = ((size_t)nv * sizeof(enc_src) + 255) & ~(size_t)255;
+    host.assign(bo + seg.size() * sizeof(enc_seg), 0);
+    enc_src *hs = (enc_src *)host.data();
     for (int v = 0; v < nv; ++v) {
         enc_src s = es[v]->src;
         if (qs) for (int l = 0; l < VCG_NLOD; ++l) s.q[l] = qs[v][l];
         if (sample) { s.list = es[v]->sample_list; s.coef = es[v]->sample_coef; s.sitems = es[v]->sample_items; }
         hs[v] = s;
-        hb[v] = tot;
-        tot += sample ? (uint64_t)es[v]->sample_n : es[v]->total;
     }
+    if (!seg.empty()) memcpy(host.data() + bo, seg.data(), seg.size() * sizeof(enc_seg));
     uint8_t *d = desc_area(host.size());
     VCG_CHECK(hipMemcpyAsync(d, host.data(), host.size(), hipMemcpyHostToDevice, c->s));
-    *dsrc = (enc_src *)(d + so);
-    *dbase = (uint64_t *)(d + bo);
+    *dsrc = (enc_src *)d;
+    *dseg = (enc_seg *)(d + bo);
     return tot;
 }
 
@@ -430,18 +553,19 @@ extern "C" int vcg_enc_samples(int nv, vcg_enc *const *es, const uint32_t *const
         if (e->sample_coef) vc__panic("encode sample prepared twice");
         const int n = ns[v] > 0 ? ns[v] : 0;
         e->sample_n = n;
-        e->sample_list = (uint32_t *)dmalloc((size_t)n * 12);
-        e->sample_coef = (int32_t *)dmalloc((size_t)n * VCG_A3 * 4);
-        e->sample_items = (enc_item *)dmalloc((size_t)n * sizeof(enc_item));
+        e->sample_list = (uint32_t *)enc_alloc(e->arena, (size_t)n * 12);
+        e->sample_coef = (int32_t *)enc_alloc(e->arena, (size_t)n * VCG_A3 * 4);
+        e->sample_items = (enc_item *)enc_alloc(e->arena, (size_t)n * sizeof(enc_item));
         if (n) VCG_CHECK(hipMemcpyAsync(e->sample_list, lists[v], (size_t)n * 12, hipMemcpyHostToDevice, c->s));
     }
     std::vector<uint8_t> host;
-    enc_src *dsrc; uint64_t *dbase;
-    const uint64_t tot = upload_sources(c, nv, es, nullptr, true, host, &dsrc, &dbase);
+    std::vector<enc_seg> seg;
+    enc_src *dsrc; enc_seg *dseg;
+    const uint64_t tot = upload_sources(c, nv, es, nullptr, true, host, seg, &dsrc, &dseg);
     if (tot) {
-        hipLaunchKernelGGL(k_fdct, dim3((uint32_t)tot), dim3(FDCT_THREADS), LDS_BYTES, c->s, dsrc, dbase,
-                           (uint32_t)nv, (uint64_t)0, (int)FDCT_STORE_COEF, g_dtab, g_ftab, (int16_t *)nullptr,
-                           (enc_item *)nullptr);
+        hipLaunchKernelGGL(k_fdct, dim3((uint32_t)tot), dim3(FDCT_THREADS), LDS_BYTES, c->s, dsrc, dseg,
+                           (uint32_t)seg.size(), (uint64_t)0, (int)FDCT_STORE_COEF, g_dtab, g_ftab,
+                           (int16_t *)nullptr, (enc_item *)nullptr);
         VCG_CHECK(hipGetLastError());
     }
     std::vector<enc_item> its(tot);
@@ -465,14 +589,15 @@ extern "C" int vcg_enc_candidates(int ng, vcg_enc *const *es, const float *qs, c
     for (int g = 0; g < ng; ++g) { bytes[g] = 0; flags[g] = 0; }
     if (ng <= 0) return 0;
     enc_scratch &es_ = tl_es;
-    // launches of whole groups, at most ENC_ROWS rows each
+    // launches of whole groups, at most vcg_rc_rows() rows each
+    const uint32_t cap = g_rc_rows;
     std::vector<int> cut{0};
     std::vector<samp_group> G(ng);
     uint64_t rows = 0, tot = 0;
     for (int g = 0; g < ng; ++g) {
         const uint32_t n = (uint32_t)es[g]->sample_n;
-        if (n > ENC_ROWS) vc__panic("rate-control sample larger than a launch");
-        if (rows + n > ENC_ROWS) { cut.push_back(g); rows = 0; }
+        if (n > cap) vc__panic("rate-control sample larger than a launch");
+        if (rows + n > cap) { cut.push_back(g); rows = 0; }
         // a candidate's counted total stays below 2^32 (at most 512 atoms x the
         // coder bound), so a budget of 2^32-1 or more never stops an encode
         const uint32_t b32 = budgets[g] >= 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)budgets[g];
@@ -480,7 +605,7 @@ extern "C" int vcg_enc_candidates(int ng, vcg_enc *const *es, const float *qs, c
         rows += n; tot += n;
     }
     cut.push_back(ng);
-    const size_t maxrows = tot < ENC_ROWS ? (size_t)tot : (size_t)ENC_ROWS;
+    const size_t maxrows = tot < cap ? (size_t)tot : (size_t)cap;
     vcg_grow((void **)&es_.lv, &es_.lv_cap, maxrows, (size_t)VCG_A3 * 2);
     vcg_grow((void **)&es_.lens, &es_.lens_cap, tot, 4);
     vcg_grow((void **)&es_.aux, &es_.aux_cap, maxrows + (size_t)ng, 4);
@@ -518,32 +643,34 @@ extern "C" int vcg_enc_candidates(int ng, vcg_enc *const *es, const float *qs, c
 static void pool_reserve(vcg_ctx *c, vcg_enc *e, uint64_t need) {
     if (need <= e->pool_cap) return;
     const uint64_t nc = need + need / 2 + 4096;
-    uint8_t *np = (uint8_t *)dmalloc(nc);
+    uint8_t *np = (uint8_t *)enc_alloc(e->arena, nc);
     if (e->pool_len) VCG_CHECK(hipMemcpyAsync(np, e->pool, e->pool_len, hipMemcpyDeviceToDevice, c->s));
-    VCG_CHECK(hipStreamSynchronize(c->s));
-    if (e->pool) VCG_CHECK(hipFree(e->pool));
+    if (!e->arena) {
+        VCG_CHECK(hipStreamSynchronize(c->s));
+        enc_free(false, e->pool);
+    }
     e->pool = np; e->pool_cap = nc;
 }
 
 // Encode every atom of every LOD of nv encoders (encoder v at qs[v][lod]) in
-// launches of up to ENC_ROWS atoms taken across volumes.  lens[v][i]: payload
-// length of encoder v's atom i (0 = all-zero atom); ovf[v] = 1 if one of its
-// payloads exceeded the coder bound.
+// launches of up to ENC_ROWS atoms taken across volumes, coarsest LOD first.
+// lens[v][i]: payload length of encoder v's atom i (0 = all-zero atom);
+// ovf[v] = 1 if one of its payloads exceeded the coder bound.
 extern "C" int vcg_enc_all(int nv, vcg_enc *const *es, const float (*qs)[VCG_NLOD], uint32_t *const *lens,
                            int *ovf) {
     vcg_ctx *c = vcg_get_ctx();
     enc_scratch &S = tl_es;
     if (nv <= 0) return 0;
     std::vector<uint8_t> host;
-    enc_src *dsrc; uint64_t *dbase;
-    const uint64_t tot = upload_sources(c, nv, es, qs, false, host, &dsrc, &dbase);
-    std::vector<uint64_t> vb(nv + 1);
+    std::vector<enc_seg> seg;
+    enc_src *dsrc; enc_seg *dseg;
+    const uint64_t tot = upload_sources(c, nv, es, qs, false, host, seg, &dsrc, &dseg);
     for (int v = 0; v < nv; ++v) {
-        vb[v + 1] = vb[v] + es[v]->total;
         es[v]->pool_off.assign(es[v]->total, 0);
         es[v]->pool_len = 0;
         ovf[v] = 0;
     }
+    seg.push_back(enc_seg{tot, 0, 0, 0});           // sentinel
     const uint32_t maxb = tot < ENC_ROWS ? (uint32_t)tot : ENC_ROWS;
     vcg_grow((void **)&S.lv, &S.lv_cap, maxb, (size_t)VCG_A3 * 2);
     vcg_grow((void **)&S.items, &S.items_cap, maxb, sizeof(enc_item));
@@ -553,39 +680,39 @@ extern "C" int vcg_enc_all(int nv, vcg_enc *const *es, const float (*qs)[VCG_NLO
     uint64_t *ddst = (uint64_t *)S.aux;       // destination address of each payload
     std::vector<uint32_t> hl(maxb);
     std::vector<uint64_t> dst(maxb), add(nv);
+    std::vector<uint32_t> vk(maxb);           // volume of each atom of the launch
+    std::vector<uint64_t> ik(maxb);           // its index in that volume
+    size_t sg = 0;
     for (uint64_t b = 0; b < tot; b += maxb) {
         const uint32_t n = (uint32_t)(tot - b < maxb ? tot - b : maxb);
-        hipLaunchKernelGGL(k_fdct, dim3(n), dim3(FDCT_THREADS), LDS_BYTES, c->s, dsrc, dbase, (uint32_t)nv, b,
-                           (int)FDCT_QUANT, g_dtab, g_ftab, S.lv, S.items);
+        hipLaunchKernelGGL(k_fdct, dim3(n), dim3(FDCT_THREADS), LDS_BYTES, c->s, dsrc, dseg,
+                           (uint32_t)(seg.size() - 1), b, (int)FDCT_QUANT, g_dtab, g_ftab, S.lv, S.items);
         VCG_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_rc_encode, dim3((n + 63) / 64), dim3(64), 0, c->s, n, S.lv, S.items, g_dtab, S.pay,
                            (uint32_t)VCG_PAY_CAP, S.lens);
         VCG_CHECK(hipGetLastError());
         VCG_CHECK(hipMemcpyAsync(hl.data(), S.lens, (size_t)n * 4, hipMemcpyDeviceToHost, c->s));
         VCG_CHECK(hipStreamSynchronize(c->s));
-        // atom b + k belongs to the volume v with vb[v] <= b + k < vb[v + 1]
-        int v0 = 0;
-        while (vb[v0 + 1] <= b) ++v0;
-        for (int v = v0; v < nv && vb[v] < b + n; ++v) add[v] = 0;
-        for (uint32_t k = 0, v = v0; k < n; ++k) {
-            while (b + k >= vb[v + 1]) ++v;
-            const uint64_t i = b + k - vb[v];
+        for (uint32_t k = 0; k < n; ++k) {
+            while (b + k >= seg[sg + 1].base) ++sg;
+            const uint32_t v = seg[sg].v;
+            const uint64_t i = seg[sg].gi0 + (b + k - seg[sg].base);
+            vk[k] = v; ik[k] = i;
             uint32_t L = hl[k];
             if (L == 0xFFFFFFFFu) { ovf[v] = 1; L = 0; }
             lens[v][i] = L;
             es[v]->pool_off[i] = es[v]->pool_len + add[v];
             add[v] += L;
         }
-        for (int v = v0; v < nv && vb[v] < b + n; ++v) pool_reserve(c, es[v], es[v]->pool_len + add[v]);
-        for (uint32_t k = 0, v = v0; k < n; ++k) {
-            while (b + k >= vb[v + 1]) ++v;
-            dst[k] = (uint64_t)(uintptr_t)(es[v]->pool + es[v]->pool_off[b + k - vb[v]]);
-        }
+        for (int v = 0; v < nv; ++v)
+            if (add[v]) { pool_reserve(c, es[v], es[v]->pool_len + add[v]); }
+        for (uint32_t k = 0; k < n; ++k)
+            dst[k] = (uint64_t)(uintptr_t)(es[vk[k]]->pool + es[vk[k]]->pool_off[ik[k]]);
         VCG_CHECK(hipMemcpyAsync(ddst, dst.data(), (size_t)n * 8, hipMemcpyHostToDevice, c->s));
         hipLaunchKernelGGL(k_compact, dim3(n), dim3(256), 0, c->s, S.pay, (uint32_t)VCG_PAY_CAP, S.lens,
                            (const uint64_t *)ddst);
         VCG_CHECK(hipGetLastError());
-        for (int v = v0; v < nv && vb[v] < b + n; ++v) es[v]->pool_len += add[v];
+        for (int v = 0; v < nv; ++v) { es[v]->pool_len += add[v]; add[v] = 0; }
         VCG_CHECK(hipStreamSynchronize(c->s));
     }
     return 0;
@@ -615,8 +742,30 @@ extern "C" int vcg_enc_assemble(vcg_enc *e, const uint8_t *head, size_t head_len
                            d_pay_off + b, d_slot_off + b, img);
         VCG_CHECK(hipGetLastError());
     }
-    VCG_CHECK(hipMemcpyAsync(out, img, total_len, hipMemcpyDeviceToHost, c->s));
-    VCG_CHECK(hipStreamSynchronize(c->s));
+    // to the caller's (pageable) buffer through two pinned halves: the copy
+    // of one half out overlaps the transfer into the other
+    if (!S.hstage[0]) {
+        if (hipHostMalloc((void **)&S.hstage[0], 2 * (size_t)HSTAGE_BYTES, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            vc__panic("out of pinned host memory (encode)");
+        }
+        S.hstage[1] = S.hstage[0] + HSTAGE_BYTES;
+        for (int k = 0; k < 2; ++k)
+            if (!S.hev[k]) VCG_CHECK(hipEventCreateWithFlags(&S.hev[k], hipEventDisableTiming));
+    }
+    const uint64_t nch = (total_len + HSTAGE_BYTES - 1) / HSTAGE_BYTES;
+    auto chunk = [&](uint64_t k) {
+        const uint64_t off = k * HSTAGE_BYTES, n = total_len - off < HSTAGE_BYTES ? total_len - off : HSTAGE_BYTES;
+        VCG_CHECK(hipMemcpyAsync(S.hstage[k & 1], img + off, n, hipMemcpyDeviceToHost, c->s));
+        VCG_CHECK(hipEventRecord(S.hev[k & 1], c->s));
+    };
+    chunk(0);
+    for (uint64_t k = 0; k < nch; ++k) {
+        if (k + 1 < nch) chunk(k + 1);
+        VCG_CHECK(hipEventSynchronize(S.hev[k & 1]));
+        const uint64_t off = k * HSTAGE_BYTES, n = total_len - off < HSTAGE_BYTES ? total_len - off : HSTAGE_BYTES;
+        par_copy(out + off, S.hstage[k & 1], n);
+    }
     return 0;
 }
 
@@ -624,11 +773,11 @@ extern "C" void vcg_enc_end(vcg_enc *e) {
     if (!e) return;
     vcg_ctx *c = vcg_get_ctx();
     VCG_CHECK(hipStreamSynchronize(c->s));
-    for (int l = 0; l < VCG_NLOD; ++l) if (e->lod_buf[l]) VCG_CHECK(hipFree(e->lod_buf[l]));
-    if (e->sample_list) VCG_CHECK(hipFree(e->sample_list));
-    if (e->sample_coef) VCG_CHECK(hipFree(e->sample_coef));
-    if (e->sample_items) VCG_CHECK(hipFree(e->sample_items));
-    if (e->pool) VCG_CHECK(hipFree(e->pool));
+    for (int l = 0; l < VCG_NLOD; ++l) enc_free(e->arena, e->lod_buf[l]);
+    enc_free(e->arena, e->sample_list);
+    enc_free(e->arena, e->sample_coef);
+    enc_free(e->arena, e->sample_items);
+    enc_free(e->arena, e->pool);
     delete e;
 }
 

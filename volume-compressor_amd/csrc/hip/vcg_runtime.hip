@@ -58,6 +58,8 @@ extern "C" int vcg_init(const vcg_tables *t) {
         memcpy(h.gstep, t->gstep, sizeof h.gstep);
         memcpy(h.band_start, t->band_start, sizeof h.band_start);
         memcpy(h.scan, t->scan, sizeof h.scan);
+        for (int q = 0; q < VCG_A3; ++q)
+            h.fscan[q] = (uint8_t)((h.scan[q] >> 10) + ((h.scan[q] >> 5) & 31) + (h.scan[q] & 31));
         VCG_CHECK(hipMalloc(&g_dtab, sizeof(vcg_dtab)));
         // through this thread's stream: the library never touches the null
         // stream, which would take one of the device's few hardware queues

@@ -256,6 +256,20 @@ done:
     return rc;
 }
 
+/* Bisection levels per launch for a group: the most whose rounds (2^depth - 1
+ * midpoints per volume) fit one launch of vcg_rc_rows() sample rows; the
+ * first round, which adds both bounds, may take two. */
+static int rc_depth(int nv, const vc_dims *dims) {
+    uint64_t ns = 0;
+    uint32_t ix[8];
+    for (int v = 0; v < nv; ++v)
+        ns += (uint64_t)lattice((dims[v].nx + 31) / 32, ix) * lattice((dims[v].ny + 31) / 32, ix) *
+              lattice((dims[v].nz + 31) / 32, ix);
+    int d = RC_MAX_DEPTH;
+    while (d > 1 && ns * (uint64_t)((1 << d) - 1) > vcg_rc_rows()) --d;
+    return d;
+}
+
 static uint64_t atom_total(vc_dims dims, int nlod) {
     uint64_t total = 0;
     for (int l = 0; l < nlod; ++l) {
@@ -286,7 +300,7 @@ static void encode_group(int nv, const uint8_t *const *vols, int on_dev, const v
     if (qs_in) {
         for (int l = 0; l < VC_NLOD; ++l) qs[0][l] = l < nlod[0] ? vc__clamp_q(qs_in[l]) : 0.0f;
     } else {
-        rate_control(nv, r, nv >= 16 ? 2 : RC_MAX_DEPTH);
+        rate_control(nv, r, rc_depth(nv, dims));
         for (int v = 0; v < nv; ++v) lod_schedule(r[v].q0, nlod[v], qs[v]);
     }
     if (q0_only) {
@@ -320,6 +334,8 @@ static vc_status encode_impl(const uint8_t *vol, int on_dev, vc_dims dims, float
     if (!q0_only) { *out = o; *out_len = ol; }
     return st;
 }
+
+void vcx_test_rc_rows(uint32_t rows) { vcg_set_rc_rows(rows); }
 
 /* device bytes a volume holds while its group encodes: the pyramid (at most
  * 8/7 of LOD0) and the sample coefficients */
@@ -363,7 +379,9 @@ vc_status vcx_encode_batch(size_t n, const uint8_t *const *vols, int vols_on_dev
             ++ng;
         }
         if (!ng) break;
+        vcg_enc_arena_open();
         encode_group(ng, gv, vols_on_device, gd, gt, NULL, go, gl, gs, NULL);
+        vcg_enc_arena_close();
         for (int k = 0; k < ng; ++k) { outs[idx[k]] = go[k]; out_lens[idx[k]] = gl[k]; statuses[idx[k]] = gs[k]; }
     }
     for (size_t k = 0; k < n; ++k) if (statuses[k] != VC_OK && first == VC_OK) first = statuses[k];

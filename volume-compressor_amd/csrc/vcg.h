@@ -87,6 +87,11 @@ int vcg_decode_payloads(size_t n, const uint8_t *pay, const uint64_t *offs, cons
 
 /* ---- encode ---- */
 typedef struct vcg_enc vcg_enc;
+/* Encoders begun between these calls (on this thread) take their device
+ * memory from one arena, released by vcg_enc_arena_close after their
+ * vcg_enc_end. */
+void vcg_enc_arena_open(void);
+void vcg_enc_arena_close(void);
 /* Upload (or adopt) a LOD0 volume and build the nlod-level pyramid on device. */
 vcg_enc *vcg_enc_begin(const uint8_t *vol, int vol_on_device, const uint32_t dims[3], int nlod);
 /* Rate-control samples of nv encoders in one launch: encoder v's ns[v] listed
@@ -94,6 +99,11 @@ vcg_enc *vcg_enc_begin(const uint8_t *vol, int vol_on_device, const uint32_t dim
  * device (once per encoder); zero_masks[v][i] = 1 for its all-zero atoms. */
 int  vcg_enc_samples(int nv, vcg_enc *const *es, const uint32_t *const *lists_zyx, const int *ns,
                      uint8_t *const *zero_masks);
+/* Rate-control sample rows (candidate x atom) one launch takes: VCG_RC_ROWS,
+ * or a smaller capacity set for tests (vcx_test_rc_rows). */
+#define VCG_RC_ROWS 262144u
+uint32_t vcg_rc_rows(void);
+void vcg_set_rc_rows(uint32_t rows);
 /* Byte totals of ng rate-control candidates in as few launches as fit:
  * candidate g = encoder es[g]'s sample at base_q qs[g].  bytes[g] (dc bytes
  * included), flags[g] bit 0 = coder bound exceeded, bit 1 = the total passed

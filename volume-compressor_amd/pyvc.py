@@ -88,6 +88,7 @@ def lib():
         "vcx_encode_device": (i32, [vp, Dims, f32, P(vp), P(sz)]),
         "vcx_rate_control": (f32, [vp, Dims, f32]),
         "vcx_encode_batch": (i32, [sz, vp, i32, vp, vp, vp, vp, vp]),
+        "vcx_test_rc_rows": (None, [u32]),
         "vcx_encode_atoms": (i32, [sz, vp, f32, vp, sz, vp]),
         "vcx_decode_payloads": (i32, [sz, vp, vp, vp, f32, vp, vp]),
         "vcx_synth_device": (i32, [u64, Dims, i32, vp]),
@@ -171,6 +172,10 @@ def encode_batch(vols, target_ratios, on_device=False, dims=None):
             res.append((int(st[i]), None))
     del keep
     return rc, res
+
+
+def test_rc_rows(rows):
+    lib().vcx_test_rc_rows(rows)
 
 
 def rate_control(vol, target_ratio):
