@@ -6,12 +6,26 @@ HBM into a device output buffer.  Steps are issued back to back through the
 asynchronous device API, which overlaps the transform of one call with the
 entropy pass of the next; the timed region brackets all K steps with HIP
 events on the library's own stream.  The latency of one synchronous call is
-reported beside it (single_call_ms).  Also reported: per-kernel durations (stage events), the dominant
-kernel's roofline, plain per-call vc_decode_atom latency p50/p99, a
-bit-exactness check against the CPU oracle on sampled atoms, and the CPU
-baseline (the reference's vc_decode_atom under OpenMP on the host cores).
+reported beside it (single_call_ms).  Also reported: per-kernel durations
+(stage events), the dominant kernel's roofline, plain per-call vc_decode_atom
+latency p50/p99, a bit-exactness check against the CPU oracle on sampled
+atoms, and the CPU baseline (the reference's vc_decode_atom under OpenMP on
+the host cores).
+
+Further sections, each beside its plain per-call counterpart:
+  other_ratios   : the same volume at target ratios 50 and 100 (decode GB/s,
+                   encode time, sampled atoms against the oracle);
+  serving        : a 2048^3 ratio-50 archive (320 sheets) served through the
+                   continuous-batching request loop under open-loop Poisson
+                   traces of atom requests (1-4096 atoms, log-uniform, LOD 0
+                   with p = 0.7) and of region requests (log-uniform edges
+                   17-600 voxels, LOD 0-7), p50/p99 latency;
+  batched_encode : 256 device-resident volumes of uneven dims (axes uniform
+                   in [40, 512]) at ratios {10, 20, 50, 100} in one
+                   vcx_encode_batch call, against per-volume encodes.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 1024] [--ratio 10]
+                       [--no-serve] [--no-ratios] [--no-encode-batch] [--no-cpu]
 """
 import argparse
 import ctypes as C
@@ -114,15 +128,17 @@ def cpu_baseline(arc, atoms_zyx, budget_s=12.0):
                       f"({reps * n * 32768 / 2**20:.0f} MiB out), {t:.1f} s, {label}, OpenMP schedule(dynamic)"}
 
 
-def serve_bench(pyvc, a, nreq=300, qps=400.0, seed=11):
-    """Open-loop Poisson trace through the continuous-batching server:
-    requests of round(exp(U(ln 1, ln 4096))) atoms, LOD 0 with p = 0.7 else
-    uniform over the coarser LODs, coordinates uniform over that LOD's atom
-    grid.  Latency = completion (server clock) - scheduled arrival."""
-    rng = np.random.default_rng(seed)
+def lod_count(a):
     nlod = 0
-    while a.lod_dims(nlod)[0] == 0:
+    while nlod < 8 and a.lod_dims(nlod)[0] == 0:
         nlod += 1
+    return nlod
+
+
+def atom_requests(a, nreq, rng):
+    """Requests of round(exp(U(ln 1, ln 4096))) atoms, LOD 0 with p = 0.7 else
+    uniform over the coarser LODs, coordinates uniform over that LOD's atom grid."""
+    nlod = lod_count(a)
     reqs = []
     for _ in range(nreq):
         n = int(round(np.exp(rng.uniform(0.0, np.log(4096.0)))))
@@ -130,22 +146,61 @@ def serve_bench(pyvc, a, nreq=300, qps=400.0, seed=11):
         _, (nx, ny, nz) = a.lod_dims(lod)
         q = np.stack([np.full(n, lod), rng.integers(0, (nx + 31) // 32, n), rng.integers(0, (ny + 31) // 32, n),
                       rng.integers(0, (nz + 31) // 32, n)], 1).astype(np.int32)
-        reqs.append(q)
+        reqs.append(("atoms", q))
+    return reqs
+
+
+def region_requests(a, nreq, rng):
+    """Boxes with log-uniform edges in [17, 600] voxels per axis (independent,
+    clipped to the LOD), offsets uniform (not 32-aligned), LOD uniform over
+    every LOD of the archive."""
+    nlod = lod_count(a)
+    reqs = []
+    for _ in range(nreq):
+        lod = int(rng.integers(0, nlod))
+        _, dims = a.lod_dims(lod)
+        lo, hi = [], []
+        for d in dims:
+            e = min(d, int(round(np.exp(rng.uniform(np.log(17.0), np.log(600.0))))))
+            o = int(rng.integers(0, d - e + 1))
+            lo.append(o)
+            hi.append(o + e)
+        reqs.append(("region", (lod, (lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]))))
+    return reqs
+
+
+def req_bytes(r):
+    kind, x = r
+    if kind == "atoms":
+        return len(x) * 32768
+    _, (x0, y0, z0, x1, y1, z1) = x
+    return (x1 - x0) * (y1 - y0) * (z1 - z0)
+
+
+def serve_bench(pyvc, a, reqs, qps, seed=11, device_bytes=6 << 30):
+    """Open-loop Poisson trace of `reqs` at `qps` through the continuous-batching
+    server.  Latency = completion (server clock) - scheduled arrival.  Three
+    served requests are checked against the plain per-call path."""
+    rng = np.random.default_rng(seed)
+    nreq = len(reqs)
     gaps = rng.exponential(1.0 / qps, nreq)
-    total = sum(len(q) for q in reqs) * 32768
-    arena = pyvc.PinnedBuffer(total)
-    srv = pyvc.Server(a, 6 << 30)
-    offs = np.cumsum([0] + [len(q) * 32768 for q in reqs])
+    sizes = [req_bytes(r) for r in reqs]
+    offs = np.cumsum([0] + sizes)
+    arena = pyvc.PinnedBuffer(int(offs[-1]))
+    srv = pyvc.Server(a, device_bytes)
     ids, arrive = {}, np.zeros(nreq)
     done_t, bad = {}, 0
     try:
         t0 = pyvc.now_ns() + 2_000_000
         sched = t0 + np.cumsum(gaps) * 1e9
-        for k, q in enumerate(reqs):
+        for k, (kind, x) in enumerate(reqs):
             while pyvc.now_ns() < sched[k]:
                 pass
-            i = srv.submit_atoms(q, arena.p + int(offs[k]))
-            assert i >= 0
+            if kind == "atoms":
+                i = srv.submit_atoms(x, arena.p + int(offs[k]))
+            else:
+                i = srv.submit_region(x[0], x[1], arena.p + int(offs[k]))
+            assert i >= 0, (kind, x if kind == "region" else len(x))
             ids[int(i)] = k
             arrive[k] = sched[k]
             got, st, t = srv.poll(wait_ms=0)
@@ -161,20 +216,176 @@ def serve_bench(pyvc, a, nreq=300, qps=400.0, seed=11):
         srv.close()
     lat = np.array([(done_t[k] - arrive[k]) / 1e6 for k in range(nreq)])
     span = (max(done_t.values()) - arrive[0]) / 1e9
-    atoms = sum(len(q) for q in reqs)
-    # spot check: three served requests against the plain per-atom path
     exact = True
     for k in (0, nreq // 2, nreq - 1):
-        for m in range(min(4, len(reqs[k]))):
-            lod, ax, ay, az = (int(x) for x in reqs[k][m])
-            st, ref = a.decode_atom(lod, ax, ay, az)
-            base = int(offs[k]) + m * 32768
-            exact &= st == 0 and np.array_equal(arena.a[base:base + 32768], ref.ravel())
+        kind, x = reqs[k]
+        base = int(offs[k])
+        if kind == "atoms":
+            for m in range(min(4, len(x))):
+                lod, ax, ay, az = (int(v) for v in x[m])
+                st, ref = a.decode_atom(lod, ax, ay, az)
+                b = base + m * 32768
+                exact &= st == 0 and np.array_equal(arena.a[b:b + 32768], ref.ravel())
+        else:
+            st, ref = a.decode_region(x[0], x[1])
+            exact &= st == 0 and np.array_equal(arena.a[base:base + sizes[k]], ref.ravel())
     arena.free()
+    out = {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+           "requests": nreq, "offered_qps": qps, "achieved_rps": round(nreq / span, 1),
+           "out_GBps": round(sum(sizes) / span / 1e9, 3), "errors": bad, "spot_check_exact": bool(exact)}
+    if reqs[0][0] == "atoms":
+        atoms = sum(len(x) for _, x in reqs)
+        out["atoms_per_s"] = round(atoms / span, 1)
+        out["mean_atoms_per_request"] = round(atoms / nreq, 1)
+    else:
+        out["mean_MB_per_request"] = round(sum(sizes) / nreq / 1e6, 3)
+    return out
+
+
+def plain_latency(pyvc, a, reqs, limit_s=4.0):
+    """The plain per-call path: vc_decode_atom per atom / vc_decode_region per
+    box, host wall clock per request, on the first requests of the trace until
+    `limit_s` is spent."""
+    L = pyvc.lib()
+    out = np.zeros(32768, np.uint8)
+    lat, t_all = [], time.perf_counter()
+    for kind, x in reqs:
+        t0 = time.perf_counter()
+        if kind == "atoms":
+            for lod, ax, ay, az in x:
+                L.vc_decode_atom(a.h, int(lod), int(ax), int(ay), int(az), out.ctypes.data)
+        else:
+            a.decode_region(x[0], x[1])
+        lat.append((time.perf_counter() - t0) * 1e3)
+        if time.perf_counter() - t_all > limit_s and len(lat) >= 5:
+            break
+    lat = np.array(lat)
     return {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
-            "requests": nreq, "offered_qps": qps, "achieved_rps": round(nreq / span, 1),
-            "atoms_per_s": round(atoms / span, 1), "mean_atoms_per_request": round(atoms / nreq, 1),
-            "errors": bad, "spot_check_exact": bool(exact)}
+            "mean": round(float(lat.mean()), 3), "n": int(lat.size),
+            "rps": round(1e3 / float(lat.mean()), 3)}
+
+
+def serving_section(pyvc, args):
+    """Serving configurations: the 2048^3 ratio-50 archive (320 sheets), an
+    atom-request trace and a region-request trace through the server, each
+    beside the plain per-call path on the same requests."""
+    n = args.serve_size
+    dims = (n, n, n)
+    dvol = pyvc.dev_alloc(n ** 3)
+    try:
+        assert pyvc.synth_device(3, dims, args.serve_sheets, dvol) == 0
+        t0 = time.time()
+        st, arc = pyvc.encode_device(dvol, dims, 50.0)
+        t_enc = time.time() - t0
+        assert st == 0, st
+    finally:
+        pyvc.dev_free(dvol)
+    log(f"serving archive {n}^3 ratio 50: {len(arc) / 1e6:.1f} MB, encode {t_enc:.2f} s")
+    a = pyvc.Archive(arc)
+    rng = np.random.default_rng(23)
+    areqs = atom_requests(a, 300, rng)
+    rreqs = region_requests(a, 200, rng)
+    # warm: one request of each kind through the per-call path uploads the archive
+    plain_latency(pyvc, a, areqs[:1], 0.0)
+    res = {"workload": f"{n}^3 u8 synthetic, {args.serve_sheets} sheets, seed 3, target_ratio 50",
+           "archive_bytes": len(arc), "encode_s": round(t_enc, 3)}
+    sa = serve_bench(pyvc, a, areqs, args.qps)
+    pa = plain_latency(pyvc, a, areqs)
+    sa["rps_vs_plain"] = round(sa["achieved_rps"] / pa["rps"], 1)
+    res["atom_requests"] = {"batched": sa, "plain": pa}
+    log(f"atom serving: {sa}; plain {pa}")
+    sr = serve_bench(pyvc, a, rreqs, args.region_qps)
+    pr = plain_latency(pyvc, a, rreqs)
+    sr["rps_vs_plain"] = round(sr["achieved_rps"] / pr["rps"], 1)
+    res["region_requests"] = {"batched": sr, "plain": pr}
+    log(f"region serving: {sr}; plain {pr}")
+    a.close()
+    return res
+
+
+def ratio_section(pyvc, hip, o, dvol, dout, n, ratios, steps=10):
+    """The same volume encoded at other target ratios: encode time, then
+    back-to-back device LOD0 decodes (GB/s) and 16 sampled atoms against the
+    oracle."""
+    L = pyvc.lib()
+    stream = L.vcx_stream()
+    nax = (n + 31) // 32
+    rng = np.random.default_rng(9)
+    out = {}
+    for r in ratios:
+        t0 = time.time()
+        st, arc = pyvc.encode_device(dvol, (n, n, n), r)
+        t_enc = time.time() - t0
+        assert st == 0, st
+        a = pyvc.Archive(arc)
+        for _ in range(2):
+            assert a.decode_lod_device(0, dout, 0) == 0
+        assert L.vcx_sync() == 0
+        e0, e1 = hip.event(), hip.event()
+        hip.L.hipStreamSynchronize(stream)
+        hip.record(e0, stream)
+        for _ in range(steps):
+            a.decode_lod_device(0, dout, 0)
+        hip.record(e1, stream)
+        ms = hip.ms(e0, e1) / steps
+        assert L.vcx_sync() == 0
+        dec = pyvc.d2h(dout, n ** 3).reshape(n, n, n)
+        oa = o.open(arc)
+        exact = True
+        for _ in range(16):
+            az, ay, ax = (int(x) for x in rng.integers(0, nax, 3))
+            ost, oat = oa.decode_atom(0, ax, ay, az)
+            exact &= ost == 0 and np.array_equal(
+                dec[az * 32:az * 32 + 32, ay * 32:ay * 32 + 32, ax * 32:ax * 32 + 32], oat)
+        del dec
+        a.close()
+        out[f"ratio_{r:g}"] = {"decode_GBps": round(n ** 3 / (ms / 1e3) / 1e9, 3), "ms_per_step": round(ms, 4),
+                               "encode_s": round(t_enc, 3), "encode_GBps": round(n ** 3 / t_enc / 1e9, 3),
+                               "archive_bytes": len(arc), "bit_exact_sampled": bool(exact)}
+        log(f"ratio {r:g}: {out[f'ratio_{r:g}']}")
+    return out
+
+
+def encode_batch_section(pyvc, nvol=256, nseq=16, seed=2024):
+    """Batched encode service: nvol device-resident volumes with each axis
+    uniform in [40, 512], sheets scaled with the volume's edge (20 per 128
+    voxels), target ratios uniform over {10, 20, 50, 100}; one
+    vcx_encode_batch call, beside vcx_encode_device called per volume on the
+    first nseq volumes (their archives must match byte for byte)."""
+    rng = np.random.default_rng(seed)
+    dims = [tuple(int(v) for v in rng.integers(40, 513, 3)) for _ in range(nvol)]
+    targets = [float((10, 20, 50, 100)[int(rng.integers(0, 4))]) for _ in range(nvol)]
+    ptrs, tot = [], 0
+    try:
+        for k, d in enumerate(dims):
+            nb = d[0] * d[1] * d[2]
+            tot += nb
+            p = pyvc.dev_alloc(nb)
+            ptrs.append(p)
+            assert pyvc.synth_device(5000 + k, d, max(1, round(20 * nb ** (1 / 3) / 128)), p) == 0
+        pyvc.encode_batch(ptrs[:2], targets[:2], on_device=True, dims=dims[:2])   # warm
+        t0 = time.perf_counter()
+        rc, res = pyvc.encode_batch(ptrs, targets, on_device=True, dims=dims)
+        t_batch = time.perf_counter() - t0
+        assert rc == 0, rc
+        sub = sum(d[0] * d[1] * d[2] for d in dims[:nseq])
+        t0 = time.perf_counter()
+        same = True
+        for k in range(nseq):
+            st, arc = pyvc.encode_device(ptrs[k], dims[k], targets[k])
+            same &= st == 0 and arc == res[k][1]
+        t_seq = time.perf_counter() - t0
+    finally:
+        for p in ptrs:
+            pyvc.dev_free(p)
+    out = {"workload": f"{nvol} volumes, axes uniform in [40, 512], ratios in {{10, 20, 50, 100}}, device-resident",
+           "input_GB": round(tot / 1e9, 3), "archive_GB": round(sum(len(x) for _, x in res) / 1e9, 3),
+           "batched_s": round(t_batch, 3), "batched_GBps": round(tot / t_batch / 1e9, 3),
+           "per_call_GBps": round(sub / t_seq / 1e9, 3), "per_call_volumes": nseq,
+           "batched_vs_per_call": round((tot / t_batch) / (sub / t_seq), 2),
+           "identical_to_per_call": bool(same)}
+    log(f"batched encode: {out}")
+    return out
 
 
 def pmc_traffic(kernel, size, ratio):
@@ -199,8 +410,13 @@ def main():
     ap.add_argument("--ratio", type=float, default=10.0)
     ap.add_argument("--sheets", type=int, default=160)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--qps", type=float, default=400.0, help="offered request rate of the serving trace")
+    ap.add_argument("--qps", type=float, default=400.0, help="offered rate of the atom-request trace")
+    ap.add_argument("--region-qps", type=float, default=100.0, help="offered rate of the region-request trace")
+    ap.add_argument("--serve-size", type=int, default=2048)
+    ap.add_argument("--serve-sheets", type=int, default=320)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-ratios", action="store_true")
+    ap.add_argument("--no-encode-batch", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="profiler run: only the LOD0 decode launches, each call synchronous (no per-atom "
                          "calls, no serving trace, no CPU leg), then one 1 GiB calibration copy kernel")
@@ -312,15 +528,23 @@ def main():
     lat = np.array(lat[10:])
     p50, p99 = (float(np.percentile(lat, 50)), float(np.percentile(lat, 99))) if lat.size else (None, None)
 
-    # ---- batched serving: continuous-batching request loop under a Poisson trace ----
+    # ---- the same volume at the other target ratios ----
+    ratios = None
+    if not (args.no_ratios or args.profile):
+        ratios = ratio_section(pyvc, hip, o, dvol, dout, n, [r for r in (10.0, 50.0, 100.0) if r != args.ratio])
+    pyvc.dev_free(dout)
+    pyvc.dev_free(dvol)
+    dout = dvol = None
+
+    # ---- serving: continuous-batching request loop under Poisson traces ----
     served = None
     if not (args.no_serve or args.profile):
-        served = serve_bench(pyvc, a, qps=args.qps)
-        log(f"serving: {served}")
-        if p50:
-            # the plain path serves a request with one vc_decode_atom call per atom
-            served["plain_rps_estimate"] = round(1e3 / (p50 * served["mean_atoms_per_request"]), 3)
-            served["rps_vs_plain"] = round(served["achieved_rps"] / served["plain_rps_estimate"], 1)
+        served = serving_section(pyvc, args)
+
+    # ---- batched encode service ----
+    enc_batch = None
+    if not (args.no_encode_batch or args.profile):
+        enc_batch = encode_batch_section(pyvc)
 
     # ---- CPU baseline ----
     cpu = None
@@ -361,8 +585,10 @@ def main():
         "single_call_ms": round(float(np.mean(k12)), 4),
         "plain_atom_latency_ms": ({"p50": round(p50, 4), "p99": round(p99, 4), "n": int(lat.size)}
                                   if lat.size else None),
-        "batched_atom_latency_ms": served,
+        "serving": served,
         "encode_s": round(t_enc, 3),
+        "other_ratios": ratios,
+        "batched_encode": enc_batch,
         "cpu_baseline": cpu,
     }
     if args.profile:
@@ -372,8 +598,6 @@ def main():
         assert L.vcx_copy_kernel(dst, src, cb) == 0
         pyvc.dev_free(src)
         pyvc.dev_free(dst)
-    pyvc.dev_free(dout)
-    pyvc.dev_free(dvol)
     print(json.dumps(res), flush=True)
 
 
