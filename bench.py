@@ -411,7 +411,7 @@ def main():
     ap.add_argument("--sheets", type=int, default=160)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--qps", type=float, default=400.0, help="offered rate of the atom-request trace")
-    ap.add_argument("--region-qps", type=float, default=100.0, help="offered rate of the region-request trace")
+    ap.add_argument("--region-qps", type=float, default=400.0, help="offered rate of the region-request trace")
     ap.add_argument("--serve-size", type=int, default=2048)
     ap.add_argument("--serve-sheets", type=int, default=320)
     ap.add_argument("--no-serve", action="store_true")

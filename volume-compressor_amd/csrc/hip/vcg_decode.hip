@@ -33,7 +33,8 @@ struct dec_job {
     int mode, lod;
     uint32_t ax0, ay0, az0, nax, nay;   // box mode atom range
     uint32_t base, n;                   // item range of this launch
-    const int32_t *quads;               // list mode: (lod, ax, ay, az)
+    const int32_t *quads;               // list mode: (lod | placement << 8, ax, ay, az)
+    const vcg_place *places;            // list mode: box placements (or null)
     const uint64_t *offs;               // payload mode
     const uint32_t *lens;
     // output placement
@@ -50,7 +51,7 @@ __device__ __forceinline__ void item_atom(const dec_job &j, uint32_t gi, int &lo
         ay = j.ay0 + (gi / j.nax) % j.nay;
         az = j.az0 + gi / (j.nax * j.nay);
     } else if (j.mode == MODE_LIST) {
-        lod = j.quads[4 * gi]; ax = (uint32_t)j.quads[4 * gi + 1];
+        lod = j.quads[4 * gi] & 0xFF; ax = (uint32_t)j.quads[4 * gi + 1];
         ay = (uint32_t)j.quads[4 * gi + 2]; az = (uint32_t)j.quads[4 * gi + 3];
     } else {
         lod = 0; ax = ay = az = 0;
@@ -160,46 +161,60 @@ __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__res
 // ---------------------------------------------------------------------------
 #define IDCT_THREADS 512
 
-__device__ __forceinline__ void store_voxel(const dec_job &j, uint32_t gi, uint32_t ax, uint32_t ay,
-                                            uint32_t az, int z, int y, int x, uint8_t v) {
+// where an item's voxels go: whole atom at out (box == false), or clipped to
+// a voxel box whose raster starts at out
+struct dst_view {
+    uint8_t *out;
+    bool box;
+    uint32_t bx0, by0, bz0, bx1, by1, bz1;
+};
+__device__ __forceinline__ dst_view item_dst(const dec_job &j, uint32_t gi) {
+    dst_view d;
     if (j.mode == MODE_BOX) {
-        uint32_t vz = az * 32 + z, vy = ay * 32 + y, vx = ax * 32 + x;
-        if (vz < j.bz0 || vz >= j.bz1 || vy < j.by0 || vy >= j.by1 || vx < j.bx0 || vx >= j.bx1) return;
-        uint64_t ox = j.bx1 - j.bx0, oy = j.by1 - j.by0;
-        j.out[((uint64_t)(vz - j.bz0) * oy + (vy - j.by0)) * ox + (vx - j.bx0)] = v;
-    } else {
-        j.out[(uint64_t)gi * VCG_A3 + (uint32_t)((z * 32 + y) * 32 + x)] = v;
+        d.out = j.out; d.box = true;
+        d.bx0 = j.bx0; d.by0 = j.by0; d.bz0 = j.bz0; d.bx1 = j.bx1; d.by1 = j.by1; d.bz1 = j.bz1;
+        return d;
     }
+    const int p = j.places && j.mode == MODE_LIST ? (j.quads[4 * gi] >> 8) - 1 : -1;
+    if (p >= 0) {
+        const vcg_place &q = j.places[p];
+        d.out = j.out + q.out_off; d.box = true;
+        d.bx0 = q.x0; d.by0 = q.y0; d.bz0 = q.z0; d.bx1 = q.x1; d.by1 = q.y1; d.bz1 = q.z1;
+        return d;
+    }
+    d.out = j.out + (uint64_t)gi * VCG_A3; d.box = false;
+    d.bx0 = d.by0 = d.bz0 = d.bx1 = d.by1 = d.bz1 = 0;
+    return d;
 }
 
 // Store the staged raster atom: thread tid owns bytes [64*tid, 64*tid + 64) =
 // rows (z, y) and (z, y+1).  Rows that lie wholly inside the box at a 16-byte
 // aligned destination go out as 16-byte stores, others byte by byte.
-__device__ __forceinline__ void store_atom(const dec_job &j, uint32_t gi, uint32_t ax, uint32_t ay, uint32_t az,
+__device__ __forceinline__ void store_atom(const dst_view &d, uint32_t ax, uint32_t ay, uint32_t az,
                                            const uint8_t *stage, int tid) {
     const int z = tid >> 4, y0 = (tid & 15) * 2;
     const uint4 *src = (const uint4 *)(stage + 64 * tid);
-    if (j.mode != MODE_BOX) {
-        uint4 *dst = (uint4 *)(j.out + (uint64_t)gi * VCG_A3 + 64 * tid);
+    if (!d.box) {
+        uint4 *dst = (uint4 *)(d.out + 64 * tid);
 #pragma unroll
         for (int q = 0; q < 4; ++q) dst[q] = src[q];
         return;
     }
     const uint32_t vz = az * 32 + z, vx = ax * 32;
-    const uint64_t ox = j.bx1 - j.bx0, oy = j.by1 - j.by0;
+    const uint64_t ox = d.bx1 - d.bx0, oy = d.by1 - d.by0;
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
         const uint32_t vy = ay * 32 + y0 + r;
-        if (vz < j.bz0 || vz >= j.bz1 || vy < j.by0 || vy >= j.by1) continue;
-        uint8_t *row = j.out + ((uint64_t)(vz - j.bz0) * oy + (vy - j.by0)) * ox;
-        if (vx >= j.bx0 && vx + 32 <= j.bx1 && ((uintptr_t)(row + (vx - j.bx0)) & 15) == 0) {
-            uint4 *dst = (uint4 *)(row + (vx - j.bx0));
+        if (vz < d.bz0 || vz >= d.bz1 || vy < d.by0 || vy >= d.by1) continue;
+        uint8_t *row = d.out + ((uint64_t)(vz - d.bz0) * oy + (vy - d.by0)) * ox;
+        if (vx >= d.bx0 && vx + 32 <= d.bx1 && ((uintptr_t)(row + (vx - d.bx0)) & 15) == 0) {
+            uint4 *dst = (uint4 *)(row + (vx - d.bx0));
             dst[0] = src[2 * r];
             dst[1] = src[2 * r + 1];
         } else {
             const uint8_t *b = stage + 64 * tid + 32 * r;
             for (int x = 0; x < 32; ++x)
-                if (vx + x >= j.bx0 && vx + x < j.bx1) row[vx + x - j.bx0] = b[x];
+                if (vx + x >= d.bx0 && vx + x < d.bx1) row[vx + x - d.bx0] = b[x];
         }
     }
 }
@@ -221,7 +236,7 @@ __global__ __launch_bounds__(IDCT_THREADS) void k_idct(dec_job j, const vcg_dtab
     if (it.status != 1) {
         for (int v = tid; v < (int)VCG_A3 / 4; v += IDCT_THREADS) ((uint32_t *)stage)[v] = 0;
         __syncthreads();
-        store_atom(j, gi, ax, ay, az, stage, tid);
+        store_atom(item_dst(j, gi), ax, ay, az, stage, tid);
         return;
     }
     __shared__ int32_t dqs[2 * VCG_NFREQ];
@@ -263,7 +278,7 @@ __global__ __launch_bounds__(IDCT_THREADS) void k_idct(dec_job j, const vcg_dtab
     __syncthreads();
     vcg_inv_stage(stage, vox, tid);
     __syncthreads();
-    store_atom(j, gi, ax, ay, az, stage, tid);
+    store_atom(item_dst(j, gi), ax, ay, az, stage, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -445,7 +460,7 @@ extern "C" int vcg_decode_payloads(size_t n, const uint8_t *pay, const uint64_t 
 // observed through an event recorded on the calling thread's stream
 // ---------------------------------------------------------------------------
 extern "C" int vcg_list_launch(vcg_arc *a, size_t n, const int32_t *quads_pinned, int32_t *quads_dev,
-                               uint8_t *out_dev, int32_t *status_dev) {
+                               uint8_t *out_dev, int32_t *status_dev, size_t nplace) {
     vcg_ctx *c = vcg_get_ctx();
     if (n == 0 || n > MAX_BATCH) return -1;
     const vcg_geom *gd = vcg_arc_geom_dev(a);
@@ -457,31 +472,10 @@ extern "C" int vcg_list_launch(vcg_arc *a, size_t n, const int32_t *quads_pinned
     j.dqb = &gd->dq_bias[0][0];
     j.mode = MODE_LIST;
     j.quads = quads_dev;
+    j.places = nplace ? (const vcg_place *)(quads_dev + 4 * n) : nullptr;
     j.out = out_dev;
     j.status_out = status_dev;
-    run_batch(c, j, (uint32_t)n, false, quads_pinned, quads_dev, n * 16);
+    run_batch(c, j, (uint32_t)n, false, quads_pinned, quads_dev, n * 16 + nplace * sizeof(vcg_place));
     return 0;
 }
 
-extern "C" int vcg_box_launch(vcg_arc *a, int lod, const vcg_box *box, uint8_t *out_dev, int32_t *status_dev) {
-    vcg_ctx *c = vcg_get_ctx();
-    const vcg_geom *gd = vcg_arc_geom_dev(a);
-    dec_job j;
-    memset(&j, 0, sizeof j);
-    j.arc = vcg_arc_buf(a);
-    j.geom = gd;
-    j.dqs = &gd->dq_step[0][0];
-    j.dqb = &gd->dq_bias[0][0];
-    j.mode = MODE_BOX;
-    j.lod = lod;
-    j.ax0 = box->x0 / 32; j.ay0 = box->y0 / 32; j.az0 = box->z0 / 32;
-    j.nax = (box->x1 - 1) / 32 - j.ax0 + 1;
-    j.nay = (box->y1 - 1) / 32 - j.ay0 + 1;
-    const uint32_t naz = (box->z1 - 1) / 32 - j.az0 + 1;
-    j.bx0 = box->x0; j.by0 = box->y0; j.bz0 = box->z0;
-    j.bx1 = box->x1; j.by1 = box->y1; j.bz1 = box->z1;
-    j.out = out_dev;
-    j.status_out = status_dev;
-    run_batch(c, j, j.nax * j.nay * naz, false);
-    return 0;
-}

@@ -3,13 +3,15 @@
  * Requests (a list of atoms, or a voxel box of one LOD) are queued by any
  * thread and served by one worker thread that owns a GPU stream.  The worker
  * keeps up to SRV_SLOTS batches in flight: whenever a slot is free it admits
- * the queued atom requests that fit the slot's share of the device-memory
- * budget (in arrival order) as one batched decode, or one region request, and
- * launches it without waiting; consecutive batches overlap on the device (the
- * entropy pass of one with the transform of the previous).  Finished batches
- * are retired by polling their events, and each request is reported with its
- * status and completion time.  Results are copied into the caller's buffer
- * (pinned memory from vcx_host_alloc keeps the copies asynchronous).
+ * the queued requests, atoms and boxes alike, that fit the slot's share of
+ * the device-memory budget (in arrival order) as one batched decode, and
+ * launches it without waiting; consecutive batches overlap on the device.  A
+ * box request becomes the list of its atoms, each clipped into the box raster
+ * by the transform kernel (a box of n atoms needs at most n atoms of output
+ * space).  Finished batches are retired by polling their events, and each
+ * request is reported with its status and completion time.  Results are
+ * copied into the caller's buffer (pinned memory from vcx_host_alloc keeps
+ * the copies asynchronous).
  *
  * Decoding a request in a batch gives the same bytes as decoding it alone:
  * every atom is decoded independently (the batch only shares launches).
@@ -42,7 +44,7 @@ typedef struct {
     void *ev;
     srv_req *reqs;              /* admitted requests, in batch order */
     size_t n;                   /* atoms in the batch */
-    int32_t *q_pin, *st_pin;    /* pinned staging */
+    int32_t *q_pin, *st_pin;    /* pinned staging: quads, then box placements */
     int32_t *q_dev, *st_dev;
     uint8_t *out_dev;
 } srv_slot;
@@ -90,17 +92,26 @@ static void push_done(vcx_server *s, int64_t id, int32_t st, uint64_t t) {
 
 /* worker: enqueue one admitted batch (no lock held) */
 static void launch(vcx_server *s, srv_slot *b) {
-    if (b->reqs->kind == 1) {
-        srv_req *r = b->reqs;
-        vcg_box bx = {r->box.x0, r->box.y0, r->box.z0, r->box.x1, r->box.y1, r->box.z1};
-        vcg_box_launch(s->dev, r->lod, &bx, b->out_dev, b->st_dev);
-        vcg_copy_async(r->out, b->out_dev, r->out_bytes, 2);
-    } else {
-        for (srv_req *r = b->reqs; r; r = r->next) memcpy(b->q_pin + 4 * r->first, r->quads, r->n * 16);
-        vcg_list_launch(s->dev, b->n, b->q_pin, b->q_dev, b->out_dev, b->st_dev);
-        for (srv_req *r = b->reqs; r; r = r->next)
-            vcg_copy_async(r->out, b->out_dev + r->first * (size_t)VC_ATOM3, r->out_bytes, 2);
+    vcg_place *pl = (vcg_place *)(b->q_pin + 4 * b->n);
+    size_t np = 0;
+    for (srv_req *r = b->reqs; r; r = r->next) {
+        int32_t *q = b->q_pin + 4 * r->first;
+        if (r->kind == 0) { memcpy(q, r->quads, r->n * 16); continue; }
+        /* a box: its atoms, clipped into the raster at the request's output space */
+        const vc_box bx = r->box;
+        pl[np] = (vcg_place){bx.x0, bx.y0, bx.z0, bx.x1, bx.y1, bx.z1, r->first * (uint64_t)VC_ATOM3};
+        const int32_t tag = r->lod | (int32_t)(np + 1) << 8;
+        for (uint32_t az = bx.z0 / VC_A; az <= (bx.z1 - 1) / VC_A; ++az)
+            for (uint32_t ay = bx.y0 / VC_A; ay <= (bx.y1 - 1) / VC_A; ++ay)
+                for (uint32_t ax = bx.x0 / VC_A; ax <= (bx.x1 - 1) / VC_A; ++ax) {
+                    q[0] = tag; q[1] = (int32_t)ax; q[2] = (int32_t)ay; q[3] = (int32_t)az;
+                    q += 4;
+                }
+        np++;
     }
+    vcg_list_launch(s->dev, b->n, b->q_pin, b->q_dev, b->out_dev, b->st_dev, np);
+    for (srv_req *r = b->reqs; r; r = r->next)
+        vcg_copy_async(r->out, b->out_dev + r->first * (size_t)VC_ATOM3, r->out_bytes, 2);
     vcg_copy_async(b->st_pin, b->st_dev, b->n * 4, 2);
     vcg_event_record(b->ev);
 }
@@ -139,21 +150,14 @@ static void *worker(void *arg) {
         if (fb && s->qhead) {
             srv_req *tail = NULL;
             fb->n = 0;
-            if (s->qhead->kind == 1) {
+            while (s->qhead && fb->n + s->qhead->n <= s->cap_atoms) {
                 srv_req *r = s->qhead;
                 s->qhead = r->next;
-                r->next = NULL; r->first = 0;
-                fb->reqs = r; fb->n = r->n;
-            } else {
-                while (s->qhead && s->qhead->kind == 0 && fb->n + s->qhead->n <= s->cap_atoms) {
-                    srv_req *r = s->qhead;
-                    s->qhead = r->next;
-                    r->next = NULL;
-                    r->first = fb->n;
-                    fb->n += r->n;
-                    if (tail) tail->next = r; else fb->reqs = r;
-                    tail = r;
-                }
+                r->next = NULL;
+                r->first = fb->n;
+                fb->n += r->n;
+                if (tail) tail->next = r; else fb->reqs = r;
+                tail = r;
             }
             if (!s->qhead) s->qtail = NULL;
             fb->busy = 1;
@@ -195,9 +199,10 @@ vcx_server *vcx_server_create(vc_archive *a, size_t device_bytes) {
     for (int k = 0; k < SRV_SLOTS; ++k) {
         srv_slot *b = &s->slot[k];
         b->ev = vcg_event_create();
-        b->q_pin = (int32_t *)vcg_host_alloc(per * 16);
+        /* quads plus at most one box placement per atom */
+        b->q_pin = (int32_t *)vcg_host_alloc(per * (16 + sizeof(vcg_place)));
         b->st_pin = (int32_t *)vcg_host_alloc(per * 4);
-        b->q_dev = (int32_t *)vcg_alloc(per * 16);
+        b->q_dev = (int32_t *)vcg_alloc(per * (16 + sizeof(vcg_place)));
         b->st_dev = (int32_t *)vcg_alloc(per * 4);
         b->out_dev = (uint8_t *)vcg_alloc(per * (size_t)VC_ATOM3);
         if (!b->q_pin || !b->st_pin || !b->q_dev || !b->st_dev || !b->out_dev) {

@@ -73,13 +73,18 @@ int vcg_decode_box(vcg_arc *a, int lod, const vcg_box *box, uint8_t *out, int ou
 /* Decode n whole atoms given as (lod,ax,ay,az) quads into host out (n*32768).
  * Coordinates must already be range-checked. per_item: status per atom. */
 int vcg_decode_list(vcg_arc *a, size_t n, const int32_t *quads, uint8_t *out, int32_t *per_item);
+/* A box placement for list items: the item's atom is clipped to the voxel
+ * box [x0,x1) x [y0,y1) x [z0,z1) of its LOD and written into the box raster
+ * at out_dev + out_off. */
+typedef struct { uint32_t x0, y0, z0, x1, y1, z1; uint64_t out_off; } vcg_place;
 /* enqueue only: n <= 32768 quads from pinned memory (staged into quads_dev),
- * atoms to out_dev (n * 32768 B), per-atom 1 = corrupt / 0 to status_dev */
+ * per-atom 1 = corrupt / 0 to status_dev.  Item k's atom goes whole to
+ * out_dev + k * 32768, unless nplace > 0 and its quad's lod word carries a
+ * placement: lod | (p + 1) << 8 sends it, clipped, to box placement p.  The
+ * nplace placements follow the quads in the same pinned/device buffers
+ * (16-byte aligned, at quads + 4 * n). */
 int vcg_list_launch(vcg_arc *a, size_t n, const int32_t *quads_pinned, int32_t *quads_dev, uint8_t *out_dev,
-                    int32_t *status_dev);
-/* enqueue only: box raster to out_dev, per-atom status (atoms of the box in
- * x-fastest order) to status_dev */
-int vcg_box_launch(vcg_arc *a, int lod, const vcg_box *box, uint8_t *out_dev, int32_t *status_dev);
+                    int32_t *status_dev, size_t nplace);
 /* Decode n raw payloads (no archive) at one base_q. */
 int vcg_decode_payloads(size_t n, const uint8_t *pay, const uint64_t *offs, const uint32_t *lens,
                         const int32_t dq_step[VCG_NFREQ], const int32_t dq_bias[VCG_NFREQ],
