@@ -233,6 +233,12 @@ def serve_bench(pyvc, a, reqs, qps, seed=11, device_bytes=6 << 30):
     out = {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
            "requests": nreq, "offered_qps": qps, "achieved_rps": round(nreq / span, 1),
            "out_GBps": round(sum(sizes) / span / 1e9, 3), "errors": bad, "spot_check_exact": bool(exact)}
+    # latency by class: requests at LOD0 and at the coarser LODs
+    lod = np.array([int(x[0, 0]) if kind == "atoms" else x[0] for kind, x in reqs])
+    for name, m in (("lod0", lod == 0), ("coarse", lod != 0)):
+        if m.any():
+            out[f"{name}_p50"] = round(float(np.percentile(lat[m], 50)), 3)
+            out[f"{name}_p99"] = round(float(np.percentile(lat[m], 99)), 3)
     if reqs[0][0] == "atoms":
         atoms = sum(len(x) for _, x in reqs)
         out["atoms_per_s"] = round(atoms / span, 1)

@@ -13,6 +13,14 @@
  * copied into the caller's buffer (pinned memory from vcx_host_alloc keeps
  * the copies asynchronous).
  *
+ * Batches are LOD-aware.  Coarser LODs are quantized finer, so their atoms
+ * carry several times the bins of an LOD0 atom, and a batch lasts as long as
+ * its slowest atom.  Requests that touch LOD0 only (class 0) and the others
+ * (class 1) therefore go to separate batches: a free slot takes the class of
+ * the oldest queued request and admits the queued requests of that class in
+ * arrival order, and class 1 holds at most SRV_SLOTS - 1 slots, so LOD0
+ * requests always find one.
+ *
  * Decoding a request in a batch gives the same bytes as decoding it alone:
  * every atom is decoded independently (the batch only shares launches).
  */
@@ -29,6 +37,7 @@
 typedef struct srv_req {
     int64_t id;
     int kind;                   /* 0: atoms, 1: region */
+    int cls;                    /* 0: LOD0 only, 1: touches a coarser LOD */
     size_t n;                   /* atoms decoded */
     int32_t *quads;             /* kind 0: owned copy, 4 per atom */
     int lod;
@@ -41,6 +50,7 @@ typedef struct srv_req {
 
 typedef struct {
     int busy;
+    int cls;                    /* class of the admitted requests */
     void *ev;
     srv_req *reqs;              /* admitted requests, in batch order */
     size_t n;                   /* atoms in the batch */
@@ -144,22 +154,40 @@ static void *worker(void *arg) {
             if (b->busy && vcg_event_done(b->ev)) { retire(s, b, vcx_now_ns()); retired = 1; }
         }
         if (retired) pthread_cond_broadcast(&s->cv_done);
-        /* admit queued requests into a free slot */
+        /* admit queued requests into a free slot: the class of the oldest
+         * queued request, unless class 1 already holds SRV_SLOTS - 1 slots */
         srv_slot *fb = NULL;
-        for (int k = 0; k < SRV_SLOTS && !fb; ++k) if (!s->slot[k].busy) fb = &s->slot[k];
+        int busy1 = 0;
+        for (int k = 0; k < SRV_SLOTS; ++k) {
+            if (!s->slot[k].busy) { if (!fb) fb = &s->slot[k]; }
+            else busy1 += s->slot[k].cls;
+        }
+        int cls = -1;
         if (fb && s->qhead) {
-            srv_req *tail = NULL;
+            cls = s->qhead->cls;
+            if (cls == 1 && busy1 >= SRV_SLOTS - 1) {
+                cls = -1;
+                for (srv_req *r = s->qhead; r; r = r->next) if (r->cls == 0) { cls = 0; break; }
+            }
+        }
+        if (cls >= 0) {
+            srv_req *tail = NULL, *prev = NULL, *r = s->qhead;
             fb->n = 0;
-            while (s->qhead && fb->n + s->qhead->n <= s->cap_atoms) {
-                srv_req *r = s->qhead;
-                s->qhead = r->next;
+            while (r) {
+                srv_req *nx = r->next;
+                if (r->cls != cls) { prev = r; r = nx; continue; }
+                if (fb->n + r->n > s->cap_atoms) break;
+                /* unlink r (queue order of the rest is kept) */
+                if (prev) prev->next = nx; else s->qhead = nx;
+                if (s->qtail == r) s->qtail = prev;
                 r->next = NULL;
                 r->first = fb->n;
                 fb->n += r->n;
                 if (tail) tail->next = r; else fb->reqs = r;
                 tail = r;
+                r = nx;
             }
-            if (!s->qhead) s->qtail = NULL;
+            fb->cls = cls;
             fb->busy = 1;
             s->inflight += 1;
             pthread_mutex_unlock(&s->mu);
@@ -168,6 +196,7 @@ static void *worker(void *arg) {
             continue;
         }
         if (s->stop && !s->qhead && s->inflight == 0) break;
+        /* queued requests that no free slot may take wait for a retirement */
         if (s->inflight == 0) {
             pthread_cond_wait(&s->cv_work, &s->mu);
         } else {
@@ -260,6 +289,7 @@ int64_t vcx_server_submit_atoms(vcx_server *s, size_t n, const int32_t *quads, u
     if (!r->quads) { free(r); return -1; }
     memcpy(r->quads, quads, n * 16);
     r->kind = 0; r->n = n; r->out = out; r->out_bytes = n * (size_t)VC_ATOM3;
+    for (size_t i = 0; i < n; ++i) if (quads[4 * i] != 0) r->cls = 1;
     return enqueue(s, r);
 }
 
@@ -273,6 +303,7 @@ int64_t vcx_server_submit_region(vcx_server *s, int lod, vc_box box, uint8_t *ou
     srv_req *r = (srv_req *)calloc(1, sizeof *r);
     if (!r) return -1;
     r->kind = 1; r->n = nax * nay * naz; r->lod = lod; r->box = box; r->out = out; r->out_bytes = bytes;
+    r->cls = lod != 0;
     return enqueue(s, r);
 }
 
