@@ -16,6 +16,16 @@ uint32_t kc_encode(const int16_t *L, uint32_t eob, const uint32_t *band_start, u
                    int *ovf) {
     return vcg_rc_encode_atom(L, eob, band_start, out, cap, ovf);
 }
+// the kernels' state-machine encoder (out NULL: count only); levels are
+// copied to a 16-byte aligned buffer as the kernels hold them
+uint32_t kc_encode_fsm(const int16_t *L, uint32_t eob, const uint32_t *band_start, uint8_t *out, uint32_t cap,
+                       int *ovf, uint32_t *kb, uint32_t budget, int *stopped) {
+    int16_t *a = (int16_t *)aligned_alloc(16, 32768 * 2);
+    memcpy(a, L, 32768 * 2);
+    uint32_t n = vcg_rc_encode_atom_fsm(a, eob, band_start, out, cap, ovf, kb, budget, stopped);
+    free(a);
+    return n;
+}
 int kc_decode(const uint8_t *s, uint32_t n, const uint32_t *band_start, uint32_t *out, uint32_t *cnt) {
     return vcg_rc_decode_atom(s, n, band_start, out, cnt);
 }

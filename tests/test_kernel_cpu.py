@@ -27,6 +27,9 @@ def kc():
     vp = C.c_void_p
     L.kc_encode.restype = C.c_uint32
     L.kc_encode.argtypes = [vp, C.c_uint32, vp, vp, C.c_uint32, C.POINTER(C.c_int)]
+    L.kc_encode_fsm.restype = C.c_uint32
+    L.kc_encode_fsm.argtypes = [vp, C.c_uint32, vp, vp, C.c_uint32, C.POINTER(C.c_int), vp, C.c_uint32,
+                                C.POINTER(C.c_int)]
     L.kc_decode.argtypes = [vp, C.c_uint32, vp, vp, C.POINTER(C.c_uint32)]
     L.kc_decode_fsm.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp, C.POINTER(C.c_uint32)]
     L.kc_fdct.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -197,3 +200,80 @@ def test_fsm_decoder_matches(kc, tabs, golden_atoms, golden_corrupt):
         for mis in range(8):
             got = fsm_decode(kc, tabs, rc, mis)
             assert got[0] == ref[0] and np.array_equal(got[1], ref[1]), (len(rc), mis)
+
+
+def _enc(kc, bs, lv, eob, cap, fsm, count=False, budget=None):
+    out = np.zeros(max(cap, 1) + 64, np.uint8)
+    ovf, stop = C.c_int(), C.c_int()
+    if not fsm:
+        n = kc.kc_encode(lv.ctypes.data, eob, bs.ctypes.data, out.ctypes.data, cap, C.byref(ovf))
+        return n, ovf.value, out[:min(n, cap)].tobytes(), 0
+    kb = np.zeros(1, np.uint32)
+    n = kc.kc_encode_fsm(lv.ctypes.data, eob, bs.ctypes.data, None if count else out.ctypes.data, cap,
+                         C.byref(ovf), kb.ctypes.data if budget is not None else None,
+                         budget or 0, C.byref(stop))
+    return n, ovf.value, (b"" if count else out[:min(n, cap)].tobytes()), stop.value
+
+
+def _level_fields(rng):
+    """Level fields that reach every bin state: sparse and dense fields, large
+    magnitudes (Exp-Golomb up to the int16 limit), EOBs at 0, 1, band edges and
+    32768, and long zero and long equal-level runs."""
+    out = []
+    for eob in (0, 1, 2, 363, 364, 365, 2600, 8376, 17153, 32767, 32768):
+        lv = np.zeros(32768, np.int16)
+        if eob:
+            n = max(1, eob // 7)
+            idx = rng.choice(eob, n, replace=False)
+            lv[idx] = rng.integers(-40, 41, n)
+            lv[eob - 1] = rng.choice([-1, 1, 2, -30000, 32767])
+        out.append((lv, eob))
+    for scale in (1, 3, 14, 300, 32767):
+        lv = rng.integers(-scale, scale + 1, 32768).astype(np.int16)
+        nz = np.nonzero(lv)[0]
+        out.append((lv, int(nz[-1]) + 1 if nz.size else 0))
+    lv = np.zeros(32768, np.int16)
+    lv[::5] = 13                       # smallest Exp-Golomb value, repeated
+    lv[::7] = -12
+    out.append((lv, 32766))
+    return out
+
+
+def test_fsm_encoder_matches(kc, tabs, golden_atoms):
+    """The kernels' state-machine encoder writes the same bytes, length and
+    overflow flag as the straight-line encoder, and counts the same length,
+    on the reference atoms' levels and on adversarial level fields, at full
+    and at too-small capacities."""
+    cm, gs, scan, bs = tabs
+    atoms, qs = golden_atoms["atoms"], golden_atoms["qs"]
+    cases = []
+    for ai, qi, pay, _ in payloads_of(golden_atoms):
+        z, lv, co, eob, dc = k_fdct(kc, tabs, atoms[ai], 0, 0, 0, float(qs[qi]))
+        if not z:
+            assert bytes([dc]) + _enc(kc, bs, lv, eob, 131136, True)[2] == pay, (ai, qi)
+            cases.append((lv, eob))
+    cases += _level_fields(np.random.default_rng(7))
+    for k, (lv, eob) in enumerate(cases):
+        ref = _enc(kc, bs, lv, eob, 131136, False)
+        got = _enc(kc, bs, lv, eob, 131136, True)
+        assert got[:3] == ref[:3], k
+        assert _enc(kc, bs, lv, eob, 131136, True, count=True)[:2] == ref[:2], k
+        small = max(1, ref[0] // 2)
+        assert _enc(kc, bs, lv, eob, small, True)[:3] == _enc(kc, bs, lv, eob, small, False)[:3], k
+        assert _enc(kc, bs, lv, eob, small, True, count=True)[:2] == _enc(kc, bs, lv, eob, small, False)[:2], k
+
+
+def test_fsm_encoder_budget(kc, tabs):
+    """The counting encoder gives up only when the byte total has passed the
+    budget, and otherwise returns the exact count."""
+    cm, gs, scan, bs = tabs
+    for lv, eob in _level_fields(np.random.default_rng(11)):
+        full = _enc(kc, bs, lv, eob, 131136, False)[0]
+        for budget in (0, 1023, 1024, 2047, full - 1, full, full + 5000):
+            if budget < 0:
+                continue
+            n, ovf, _, stop = _enc(kc, bs, lv, eob, 131136, True, count=True, budget=budget)
+            if stop:
+                assert full > budget, (eob, budget, full)
+            else:
+                assert n == full, (eob, budget)

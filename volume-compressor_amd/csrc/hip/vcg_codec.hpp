@@ -10,6 +10,12 @@
 #pragma once
 #include <stdint.h>
 
+#if defined(__clang__)
+#define VCG_UNPRED(x) __builtin_unpredictable(x)
+#else
+#define VCG_UNPRED(x) (x)
+#endif
+
 #if defined(__HIPCC__)
 #define VCG_HD __host__ __device__ __forceinline__
 #define VCG_HDM __host__ __device__ __forceinline__
@@ -497,6 +503,244 @@ VCG_HD uint32_t vcg_rc_encode_atom(const int16_t *L, uint32_t eob, const uint32_
     *ovf = e.ovf;
     if (stopped) *stopped = e.stop;
     return e.pos;
+}
+
+// ---------------------------------------------------------------------------
+// Encoder as a one-bin-per-step state machine (the batched encode kernels'
+// form), the mirror of the decoder's: the same bin states, each bin's bit
+// taken from the level v being coded (a = |v|):
+//    0      significance of position p: v != 0
+//    1..12  magnitude unary bin S (context mag[S-1]): a > S
+//    13     Exp-Golomb prefix (a >= 13, t = a - 11, nb = floor(log2 t)):
+//           c < nb, c counting the 1s already coded
+//    14     tail: while c > 0 the suffix bit (t >> (c-1)) & 1, then the sign
+// (vc.c:471-490, 528-544).  Range coder (vc.c:364-401): low + range never
+// exceeds its value at the last shift, so low stays below 2^33 and a carry is
+// bit 32; after a bin range >= 2^17, so one 8-bit shift renormalises.  A shift
+// with no carry and low >= 0xFF000000 only lengthens the pending run (csize);
+// otherwise the pending byte (cache + carry) and csize - 1 bytes 0xFF + carry
+// go out.
+//
+// Levels reach the lane through a ring of four 8-level blocks (Lv: ld(p) the
+// level at scan position p, st(k, blk) block k into ring slot k & 3).  Every
+// VCG_EFSM_REFILL steps the ring is brought to blocks k0 .. k0 + 2 with
+// k0 = p >> 3 (position p advances at most once per step, so the ring always
+// holds p + 1), from a block loaded one refill earlier; the load for the next
+// refill is issued unconditionally, so no wait falls on a load in flight.  A
+// finished lane records its byte count and keeps stepping with no effect on
+// the result: its output capacity drops to zero and it puts nothing more.
+// ---------------------------------------------------------------------------
+#define VCG_EFSM_REFILL 8
+struct vcg_blk16 { uint32_t w[4]; };
+struct vcg_efsm {
+    uint64_t low;
+    uint32_t range, cache, csize, pos;   // pos: bytes put out
+    uint32_t S, P, psig, c;
+    uint32_t p, lim, eob, band;
+    int v;
+    uint32_t done, len;                  // len: final byte count once done
+    uint32_t cap;                        // output capacity (0 once done)
+    int ovf;
+    uint32_t k0;                         // ring holds blocks k0 .. k0 + 2
+    const vcg_blk16 *blk;
+    vcg_blk16 pref;                      // block k0 + 3, loaded at the last refill
+};
+
+template <bool WRITE>
+VCG_HD void vcg_efsm_put(vcg_efsm &f, uint8_t *out, uint32_t b) {
+    if (f.done) return;
+    if (WRITE) {
+        if (f.pos < f.cap) out[f.pos] = (uint8_t)b;
+        else f.ovf = 1;
+    }
+    f.pos++;
+}
+
+// general shift of low (vc.c:369-380), for the EOB prefix and the final flush
+template <bool WRITE>
+VCG_HD void vcg_efsm_shift(vcg_efsm &f, uint8_t *out) {
+    const uint32_t hi = (uint32_t)(f.low >> 32), lo = (uint32_t)f.low;
+    if (hi != 0u || lo < 0xFF000000u) {
+        vcg_efsm_put<WRITE>(f, out, (f.cache + hi) & 0xFFu);
+        for (uint32_t k = 1; k < f.csize; ++k) vcg_efsm_put<WRITE>(f, out, (0xFFu + hi) & 0xFFu);
+        f.cache = lo >> 24;
+        f.csize = 1;
+    } else {
+        f.csize++;
+    }
+    f.low = (uint64_t)(lo << 8);
+}
+
+// the five-shift flush (vc.c:399-401); the lane is finished afterwards
+template <bool WRITE>
+VCG_HD void vcg_efsm_finish(vcg_efsm &f, uint8_t *out) {
+    for (int k = 0; k < 5; ++k) vcg_efsm_shift<WRITE>(f, out);
+    f.len = f.pos;
+    f.done = 1;
+    f.cap = 0;
+    f.lim = 0xFFFFFFFFu;
+}
+
+VCG_HD vcg_blk16 vcg_efsm_load(const vcg_efsm &f, uint32_t k) { return f.blk[k < 4095u ? k : 4095u]; }
+
+// L: scan-order levels (16-byte aligned, 32768 of them), eob <= 32768.
+// out: the range-coder bytes (WRITE), capacity cap.  Context memory slots
+// 1..12 must hold 2048 on entry.
+template <bool WRITE, class Lv>
+VCG_HD void vcg_efsm_init(vcg_efsm &f, Lv &lv, const int16_t *L, uint32_t eob, const uint32_t *band_start,
+                          uint8_t *out, uint32_t cap) {
+    f.low = 0; f.range = 0xFFFFFFFFu; f.cache = 0; f.csize = 1; f.pos = 0;
+    f.cap = cap; f.ovf = 0; f.done = 0; f.len = 0;
+    for (int b = 15; b >= 0; --b) {                   // EOB, 16 bypass bits (vc.c:513-519)
+        f.range >>= 1;
+        if ((eob >> b) & 1u) f.low += f.range;
+        if (f.range < (1u << 24)) { vcg_efsm_shift<WRITE>(f, out); f.range <<= 8; }
+    }
+    f.S = 0; f.P = 2048u; f.psig = 2048u; f.c = 0;
+    f.p = 0; f.eob = eob; f.band = 0;
+    f.lim = band_start[1] < eob ? band_start[1] : eob;
+    f.blk = (const vcg_blk16 *)L;
+    f.k0 = 0;
+    for (uint32_t k = 0; k < 3; ++k) lv.st(k, vcg_efsm_load(f, k));
+    f.pref = vcg_efsm_load(f, 3);
+    f.v = lv.ld(0);
+    if (eob == 0) vcg_efsm_finish<WRITE>(f, out);
+}
+
+// every VCG_EFSM_REFILL steps: ring to blocks p>>3 .. (p>>3) + 2
+template <class Lv>
+VCG_HD void vcg_efsm_refill(vcg_efsm &f, Lv &lv) {
+    const uint32_t k = f.p >> 3;                      // k0 or k0 + 1
+    if (k != f.k0) lv.st(k + 2u, f.pref);
+    f.k0 = k;
+    f.pref = vcg_efsm_load(f, k + 3u);
+}
+
+// one bin.  Ctx: ld(slot), st(slot, v) on the lane's context memory (slot
+// k+1 = mag[k]).  Returns nonzero when p reached a band boundary or the EOB
+// (the caller then runs vcg_efsm_edge).
+template <bool WRITE, class Ctx, class Lv>
+VCG_HD uint32_t vcg_efsm_step(vcg_efsm &f, Ctx &ctx, Lv &lv, uint8_t *out) {
+    const uint32_t S = f.S, P = f.P, c = f.c;
+    const uint32_t R = ctx.ld(S + 1u);
+    const int vn = lv.ld(f.p + 1u);                   // the next level, in case p advances
+    const int v = f.v;
+    const uint32_t a = (uint32_t)(v < 0 ? -v : v);
+    const uint32_t t = a - 11u;
+    const uint32_t nb = 31u - (uint32_t)__builtin_clz(t | 1u);
+    const bool isSig = S == 0u, isMag = S - 1u < 12u, isEgp = S == 13u, isTail = S == 14u, cz = c == 0u;
+    // every candidate bit computed and combined as lane masks (the lanes of a
+    // wave sit in different states: no divergent branch)
+    const bool bSfx = ((t >> ((c - 1u) & 31u)) & 1u) != 0u;
+    const bool bTail = (cz & (v < 0)) | (!cz & bSfx);
+    const bool bit = (isSig & (v != 0)) | (isMag & (a > S)) | (isEgp & (c < nb)) | (isTail & bTail);
+    // binary decision (vc.c:381-398); a bypass bin is a context bin with
+    // p = 2048 that keeps floor(range/2) for both bits (bit 12 of P)
+    const uint32_t r0 = (uint32_t)(((uint64_t)f.range * (P << 20)) >> 32);
+    f.low += bit ? (uint64_t)r0 : 0ull;
+    f.range = (bit && P < 4096u) ? f.range - r0 : r0;
+    const uint32_t nb0 = bit ? 0u : 1u;
+    const uint32_t pn = P + (nb0 << 7) - ((P + nb0 * 31u) >> 5);
+    ctx.st(S, pn);
+    f.psig = isSig ? pn : f.psig;
+    // renormalisation
+    const bool sh = f.range < (1u << 24);
+    const uint32_t hi = (uint32_t)(f.low >> 32), lo = (uint32_t)f.low;
+    const bool fl = sh && (hi != 0u || lo < 0xFF000000u);
+    if (WRITE) {
+        if (fl) {
+            vcg_efsm_put<WRITE>(f, out, (f.cache + hi) & 0xFFu);
+            for (uint32_t k = 1; k < f.csize; ++k) vcg_efsm_put<WRITE>(f, out, (0xFFu + hi) & 0xFFu);
+        }
+    } else {
+        f.pos += fl ? f.csize : 0u;
+    }
+    f.cache = fl ? lo >> 24 : f.cache;
+    f.csize = fl ? 1u : f.csize + (sh ? 1u : 0u);
+    f.low = sh ? (uint64_t)(lo << 8) : f.low;
+    f.range = sh ? f.range << 8 : f.range;
+    // symbol bookkeeping
+    const bool adv = (isSig & !bit) | (isTail & cz);
+    f.c = c + ((isEgp & bit) ? 1u : 0u) - ((isTail & !cz) ? 1u : 0u);
+    const uint32_t up = S + 1u < 13u ? S + 1u : 13u;
+    const uint32_t tn = bit ? up : (isSig ? 0u : 14u);
+    const uint32_t Sn = isTail ? (cz ? 0u : 14u) : tn;
+    f.S = Sn;
+    f.P = Sn == 0u ? f.psig : (Sn <= 12u ? R : VCG_P_BYPASS);
+    f.p += adv ? 1u : 0u;
+    f.v = adv ? vn : v;
+    return f.p == f.lim;
+}
+
+// band boundary or EOB for a lane whose step returned nonzero
+template <bool WRITE>
+VCG_HD void vcg_efsm_edge(vcg_efsm &f, const uint32_t *band_start, uint8_t *out) {
+    if (f.p >= f.eob) {
+        vcg_efsm_finish<WRITE>(f, out);
+    } else {
+        // a new band starts from the reset context (vc.c:552-555)
+        uint32_t lim = f.lim, band = f.band;
+        while (f.p == lim) {
+            band++;
+            const uint32_t nb = band_start[band + 1];
+            lim = nb < f.eob ? nb : f.eob;
+        }
+        f.lim = lim;
+        f.band = band;
+    }
+    // the next bin is a significance bin (harmless for a finished lane)
+    f.psig = 2048u;
+    f.P = 2048u;
+}
+
+// the lane's level ring as a plain array (CPU reference)
+struct vcg_lv_array {
+    int16_t v[32];
+    VCG_HDM int ld(uint32_t p) const { return v[p & 31u]; }
+    VCG_HDM void st(uint32_t k, const vcg_blk16 &b) {
+        for (int q = 0; q < 4; ++q) {
+            v[((k & 3u) << 3) + 2 * q] = (int16_t)(uint16_t)b.w[q];
+            v[((k & 3u) << 3) + 2 * q + 1] = (int16_t)(uint16_t)(b.w[q] >> 16);
+        }
+    }
+};
+
+// Whole-atom encode with the state machine, driven as the kernels drive it
+// (CPU-side reference of the kernels' per-lane logic): a ring refill, then
+// VCG_EFSM_REFILL steps, the shared byte budget checked between groups.  Same
+// results as vcg_rc_encode_atom (out NULL: count only).
+VCG_HD uint32_t vcg_rc_encode_atom_fsm(const int16_t *L, uint32_t eob, const uint32_t *band_start, uint8_t *out,
+                                       uint32_t cap, int *ovf, uint32_t *kb = nullptr, uint32_t budget = 0,
+                                       int *stopped = nullptr) {
+    vcg_ctx_array ctx;
+    for (int k = 0; k < 16; ++k) ctx.v[k] = 2048u;
+    vcg_lv_array lv;
+    vcg_efsm f;
+    int stop = 0;
+    uint32_t kbm = 1024u;
+    if (out) {
+        vcg_efsm_init<true>(f, lv, L, eob, band_start, out, cap);
+        while (!f.done) {
+            vcg_efsm_refill(f, lv);
+            for (int k = 0; k < VCG_EFSM_REFILL; ++k)
+                if (vcg_efsm_step<true>(f, ctx, lv, out)) vcg_efsm_edge<true>(f, band_start, out);
+        }
+    } else {
+        vcg_efsm_init<false>(f, lv, L, eob, band_start, nullptr, cap);
+        while (!f.done) {
+            vcg_efsm_refill(f, lv);
+            for (int k = 0; k < VCG_EFSM_REFILL; ++k)
+                if (vcg_efsm_step<false>(f, ctx, lv, nullptr)) vcg_efsm_edge<false>(f, band_start, nullptr);
+            while (kb && !f.done && f.pos >= kbm) {
+                kbm += 1024u;
+                if (vcg_atomic_add(kb, 1024u) + 1024u > budget) { stop = 1; f.done = 1; f.lim = 0xFFFFFFFFu; }
+            }
+        }
+        if (f.len > cap) f.ovf = 1;
+    }
+    *ovf = f.ovf;
+    if (stopped) *stopped = stop;
+    return f.len;
 }
 
 // dead-zone quantizer, evaluated in source order with no contraction

@@ -48,7 +48,9 @@ struct vcg_item {
 // output), and the output is joined back into s by an event.  With the public
 // stream that makes four streams: the device's default number of hardware
 // queues.
+#ifndef VCG_SLOTS
 #define VCG_SLOTS 3
+#endif
 struct vcg_ctx {
     hipStream_t s = nullptr, sE[VCG_SLOTS] = {};
     hipEvent_t evF = nullptr, evI[VCG_SLOTS] = {};

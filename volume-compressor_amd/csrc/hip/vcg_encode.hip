@@ -229,42 +229,110 @@ __global__ __launch_bounds__(FDCT_THREADS) void k_quant_groups(const samp_group 
     if (tid == 0) eobs[row] = (uint32_t)seob;
 }
 
-// byte count of every row (lane per row), payload bytes not stored.  A group
-// whose running total passes its budget is abandoned (lanes report
-// 0xFFFFFFFE); 0xFFFFFFFF = coder bound exceeded.
+// per-lane context memory (16 slots), slot-major over the 64 lanes of the
+// wave so a wave's access is conflict-free
+struct enc_lds_ctx {
+    uint32_t *b;
+    __device__ __forceinline__ uint32_t ld(uint32_t k) const { return b[k * 64u]; }
+    __device__ __forceinline__ void st(uint32_t k, uint32_t v) { b[k * 64u] = v; }
+};
+// per-lane ring of four 8-level blocks (64 B), lanes 68 B apart so the lanes'
+// reads spread over the LDS banks
+#define ENC_LV_STRIDE 68u
+struct enc_lds_lv {
+    uint8_t *b;
+    __device__ __forceinline__ int ld(uint32_t p) const { return *(const int16_t *)(b + ((p & 31u) << 1)); }
+    __device__ __forceinline__ void st(uint32_t k, const vcg_blk16 &blk) {
+        uint32_t *d = (uint32_t *)(b + ((k & 3u) << 4));
+        d[0] = blk.w[0]; d[1] = blk.w[1]; d[2] = blk.w[2]; d[3] = blk.w[3];
+    }
+};
+
+// byte count of every row (lane per row, the state-machine encoder in count
+// mode), payload bytes not stored.  A group whose running total passes its
+// budget is abandoned (lanes report 0xFFFFFFFE); 0xFFFFFFFF = coder bound
+// exceeded.
 __global__ __launch_bounds__(64) void k_rc_count_groups(const samp_group *__restrict__ G, uint32_t ng,
                                                         uint64_t rows, const int16_t *__restrict__ lv,
                                                         const uint32_t *__restrict__ eobs,
                                                         const vcg_dtab *__restrict__ T, uint32_t *__restrict__ kb,
                                                         uint32_t *__restrict__ lens) {
+    __shared__ uint32_t ctxm[16 * 64];
+    __shared__ __align__(16) uint8_t lvm[64 * ENC_LV_STRIDE];
+    __shared__ uint32_t bands[9];
+    if (threadIdx.x < 9) bands[threadIdx.x] = T->band_start[threadIdx.x];
+    __syncthreads();
     const uint64_t row = (uint64_t)blockIdx.x * 64u + threadIdx.x;
-    if (row >= rows) return;
-    const uint32_t gi = group_of(G, ng, row);
+    const bool live = row < rows;
+    const uint64_t r = live ? row : 0;
+    const uint32_t gi = group_of(G, ng, r);
     const samp_group &g = G[gi];
-    if (g.items[row - g.off].zero) { lens[row] = 0; return; }
-    int ovf = 0, stopped = 0;
-    const uint32_t len = vcg_rc_encode_atom(lv + row * VCG_A3, eobs[row], T->band_start, nullptr, VCG_RC_CAP,
-                                            &ovf, kb + gi, g.budget, &stopped);
-    lens[row] = ovf ? 0xFFFFFFFFu : stopped ? 0xFFFFFFFEu : len + 1;
+    const bool zero = g.items[r - g.off].zero != 0;
+    enc_lds_ctx ctx{ctxm + threadIdx.x};
+    enc_lds_lv lr{lvm + threadIdx.x * ENC_LV_STRIDE};
+    for (uint32_t k = 1; k <= 12; ++k) ctx.st(k, 2048u);
+    vcg_efsm f;
+    // lanes past the rows and all-zero atoms run as finished empty encodes
+    vcg_efsm_init<false>(f, lr, lv + r * VCG_A3, live && !zero ? eobs[r] : 0u, bands, nullptr, VCG_RC_CAP);
+    if (!live || zero) f.done = 1;
+    int stop = 0;
+    uint32_t kbm = 1024u;
+    uint32_t *kg = kb + gi;
+    while (__any(!f.done)) {
+        vcg_efsm_refill(f, lr);
+#pragma unroll
+        for (int k = 0; k < VCG_EFSM_REFILL; ++k)
+            if (vcg_efsm_step<false>(f, ctx, lr, nullptr)) vcg_efsm_edge<false>(f, bands, nullptr);
+        // every 1024 bytes a lane adds to its group's total; past the budget
+        // the group is abandoned
+        if (!f.done && f.pos >= kbm) {
+            while (!f.done && f.pos >= kbm) {
+                kbm += 1024u;
+                if (atomicAdd(kg, 1024u) + 1024u > g.budget) { stop = 1; f.done = 1; f.lim = 0xFFFFFFFFu; }
+            }
+        }
+    }
+    if (!live) return;
+    lens[row] = zero ? 0u : stop ? 0xFFFFFFFEu : f.len > VCG_RC_CAP ? 0xFFFFFFFFu : f.len + 1;
 }
 
 // ---------------------------------------------------------------------------
-// one lane per atom: payload = [dc][range-coder bytes]
+// one lane per atom (the state-machine encoder): payload = [dc][range-coder bytes]
 __global__ __launch_bounds__(64) void k_rc_encode(uint32_t n, const int16_t *__restrict__ lv,
                                                   const enc_item *__restrict__ items,
                                                   const vcg_dtab *__restrict__ T,
                                                   uint8_t *__restrict__ pay, uint32_t pay_cap,
                                                   uint32_t *__restrict__ lens) {
+    __shared__ uint32_t ctxm[16 * 64];
+    __shared__ __align__(16) uint8_t lvm[64 * ENC_LV_STRIDE];
+    __shared__ uint32_t bands[9];
+    if (threadIdx.x < 9) bands[threadIdx.x] = T->band_start[threadIdx.x];
+    __syncthreads();
     const uint32_t i = blockIdx.x * 64u + threadIdx.x;
-    if (i >= n) return;
-    const enc_item it = items[i];
-    if (it.zero) { lens[i] = 0; return; }
-    uint8_t *out = pay + (size_t)i * pay_cap;
-    out[0] = (uint8_t)it.dc;
+    const bool live = i < n;
+    const uint32_t r = live ? i : 0u;
+    const enc_item it = items[r];
+    const bool zero = it.zero != 0;
+    uint8_t *out = pay + (size_t)r * pay_cap;
+    if (live && !zero) out[0] = (uint8_t)it.dc;
     const uint32_t cap = pay_cap - 1 < VCG_RC_CAP ? pay_cap - 1 : VCG_RC_CAP;
-    int ovf = 0;
-    uint32_t len = vcg_rc_encode_atom(lv + (size_t)i * VCG_A3, it.eob, T->band_start, out + 1, cap, &ovf);
-    lens[i] = ovf ? 0xFFFFFFFFu : len + 1;
+    enc_lds_ctx ctx{ctxm + threadIdx.x};
+    enc_lds_lv lr{lvm + threadIdx.x * ENC_LV_STRIDE};
+    for (uint32_t k = 1; k <= 12; ++k) ctx.st(k, 2048u);
+    vcg_efsm f;
+    // lanes past the items and all-zero atoms run as finished empty encodes
+    // that write nothing
+    vcg_efsm_init<true>(f, lr, lv + (size_t)r * VCG_A3, live && !zero ? it.eob : 0u, bands, out + 1,
+                        live && !zero ? cap : 0u);
+    if (!live || zero) f.done = 1;
+    while (__any(!f.done)) {
+        vcg_efsm_refill(f, lr);
+#pragma unroll
+        for (int k = 0; k < VCG_EFSM_REFILL; ++k)
+            if (vcg_efsm_step<true>(f, ctx, lr, out + 1)) vcg_efsm_edge<true>(f, bands, out + 1);
+    }
+    if (!live) return;
+    lens[i] = zero ? 0u : f.ovf ? 0xFFFFFFFFu : f.len + 1;
 }
 
 // copy payload i (len at lens[i]) from batch scratch to the device address dst[i]
