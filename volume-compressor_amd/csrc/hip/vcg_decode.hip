@@ -302,10 +302,33 @@ static void ensure_attr() {
 // waits for them too; the transform always waits for earlier work on c->s (it
 // writes the caller's output).  up_*: an input upload done on the first slot
 // stream instead (pinned source).  Returns the scratch slot of the last batch.
+// fixed_slot >= 0 (one batch, total <= MAX_BATCH): everything on that slot's
+// stream, no ordering against c->s and no join into it.
 static int run_batch(vcg_ctx *c, dec_job j, uint32_t total, bool inputs_on_s, const void *up_src = nullptr,
-                     void *up_dst = nullptr, size_t up_bytes = 0) {
+                     void *up_dst = nullptr, size_t up_bytes = 0, int fixed_slot = -1) {
     ensure_attr();
     uint32_t maxb = total < MAX_BATCH ? total : MAX_BATCH;
+    if (fixed_slot >= 0) {
+        hipStream_t sE = c->sE[fixed_slot];
+        // only this slot's scratch, and only after this slot's earlier work
+        // (other slots' work is not ordered against c->s here)
+        if (c->lists_items[fixed_slot] < maxb || c->items_cap[fixed_slot] < maxb) {
+            VCG_CHECK(hipStreamSynchronize(sE));
+            vcg_grow((void **)&c->lists[fixed_slot], &c->lists_items[fixed_slot], maxb,
+                     (size_t)VCG_LIST_STRIDE * 4);
+            vcg_grow((void **)&c->items[fixed_slot], &c->items_cap[fixed_slot], maxb, sizeof(vcg_item));
+        }
+        if (up_bytes) VCG_CHECK(hipMemcpyAsync(up_dst, up_src, up_bytes, hipMemcpyHostToDevice, sE));
+        j.base = 0;
+        j.n = total;
+        hipLaunchKernelGGL(k_entropy, dim3((j.n + 63) / 64), dim3(64), 0, sE, j, g_dtab, c->lists[fixed_slot],
+                           c->items[fixed_slot], c->status_d);
+        VCG_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_idct, dim3(j.n), dim3(IDCT_THREADS), LDS_BYTES, sE, j, g_dtab, c->lists[fixed_slot],
+                           c->items[fixed_slot]);
+        VCG_CHECK(hipGetLastError());
+        return fixed_slot;
+    }
     for (int k = 0; k < VCG_SLOTS; ++k) {
         vcg_grow((void **)&c->lists[k], &c->lists_items[k], maxb, (size_t)VCG_LIST_STRIDE * 4);
         vcg_grow((void **)&c->items[k], &c->items_cap[k], maxb, sizeof(vcg_item));
@@ -460,9 +483,9 @@ extern "C" int vcg_decode_payloads(size_t n, const uint8_t *pay, const uint64_t 
 // observed through an event recorded on the calling thread's stream
 // ---------------------------------------------------------------------------
 extern "C" int vcg_list_launch(vcg_arc *a, size_t n, const int32_t *quads_pinned, int32_t *quads_dev,
-                               uint8_t *out_dev, int32_t *status_dev, size_t nplace) {
+                               uint8_t *out_dev, int32_t *status_dev, size_t nplace, int slot) {
     vcg_ctx *c = vcg_get_ctx();
-    if (n == 0 || n > MAX_BATCH) return -1;
+    if (n == 0 || n > MAX_BATCH || slot < 0 || slot >= VCG_SLOTS) return -1;
     const vcg_geom *gd = vcg_arc_geom_dev(a);
     dec_job j;
     memset(&j, 0, sizeof j);
@@ -475,7 +498,7 @@ extern "C" int vcg_list_launch(vcg_arc *a, size_t n, const int32_t *quads_pinned
     j.places = nplace ? (const vcg_place *)(quads_dev + 4 * n) : nullptr;
     j.out = out_dev;
     j.status_out = status_dev;
-    run_batch(c, j, (uint32_t)n, false, quads_pinned, quads_dev, n * 16 + nplace * sizeof(vcg_place));
+    run_batch(c, j, (uint32_t)n, false, quads_pinned, quads_dev, n * 16 + nplace * sizeof(vcg_place), slot);
     return 0;
 }
 

@@ -136,10 +136,14 @@ extern "C" int vcg_copy(void *dst, const void *src, size_t bytes, int kind) {
     return 0;
 }
 
-extern "C" int vcg_copy_async(void *dst, const void *src, size_t bytes, int kind) {
+static hipStream_t slot_stream(vcg_ctx *c, int slot) {
+    return slot >= 0 && slot < VCG_SLOTS ? c->sE[slot] : c->s;
+}
+
+extern "C" int vcg_copy_async(void *dst, const void *src, size_t bytes, int kind, int slot) {
     vcg_ctx *c = vcg_get_ctx();
     hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
-    VCG_CHECK(hipMemcpyAsync(dst, src, bytes, k, c->s));
+    VCG_CHECK(hipMemcpyAsync(dst, src, bytes, k, slot_stream(c, slot)));
     return 0;
 }
 
@@ -149,7 +153,10 @@ extern "C" void *vcg_event_create(void) {
     return (void *)e;
 }
 extern "C" void vcg_event_destroy(void *e) { if (e) VCG_CHECK(hipEventDestroy((hipEvent_t)e)); }
-extern "C" void vcg_event_record(void *e) { VCG_CHECK(hipEventRecord((hipEvent_t)e, vcg_get_ctx()->s)); }
+extern "C" void vcg_event_record(void *e, int slot) {
+    vcg_ctx *c = vcg_get_ctx();
+    VCG_CHECK(hipEventRecord((hipEvent_t)e, slot_stream(c, slot)));
+}
 extern "C" int vcg_event_done(void *e) {
     const hipError_t r = hipEventQuery((hipEvent_t)e);
     if (r == hipSuccess) return 1;

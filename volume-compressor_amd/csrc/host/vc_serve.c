@@ -119,11 +119,16 @@ static void launch(vcx_server *s, srv_slot *b) {
                 }
         np++;
     }
-    vcg_list_launch(s->dev, b->n, b->q_pin, b->q_dev, b->out_dev, b->st_dev, np);
+    /* the batch, its result copies and its event all go on the stream of
+     * this slot's scratch: batches in other slots neither wait for it nor
+     * delay it, and the slot's next batch follows its copies */
+    const int k = (int)(b - s->slot);
+    if (vcg_list_launch(s->dev, b->n, b->q_pin, b->q_dev, b->out_dev, b->st_dev, np, k) != 0)
+        vc__panic("server batch launch rejected");
     for (srv_req *r = b->reqs; r; r = r->next)
-        vcg_copy_async(r->out, b->out_dev + r->first * (size_t)VC_ATOM3, r->out_bytes, 2);
-    vcg_copy_async(b->st_pin, b->st_dev, b->n * 4, 2);
-    vcg_event_record(b->ev);
+        vcg_copy_async(r->out, b->out_dev + r->first * (size_t)VC_ATOM3, r->out_bytes, 2, k);
+    vcg_copy_async(b->st_pin, b->st_dev, b->n * 4, 2, k);
+    vcg_event_record(b->ev, k);
 }
 
 static void retire(vcx_server *s, srv_slot *b, uint64_t now) {

@@ -52,11 +52,12 @@ int   vcg_copy(void *dst, const void *src, size_t bytes, int kind); /* 1 h2d, 2 
 void  vcg_set_stage_events(void *ev[3]);
 int   vcg_copy_kernel(void *dst, const void *src, size_t bytes); /* 16-B aligned, bytes % 16 == 0 */
 int   vcg_sync(void);                       /* wait; returns accumulated vc status */
-/* asynchronous helpers on the calling thread's stream (serving loop) */
-int   vcg_copy_async(void *dst, const void *src, size_t bytes, int kind);
+/* asynchronous helpers (serving loop): on the calling thread's stream
+ * (slot -1) or on the stream of its scratch slot `slot` (0 .. VCG_SLOTS-1) */
+int   vcg_copy_async(void *dst, const void *src, size_t bytes, int kind, int slot);
 void *vcg_event_create(void);
 void  vcg_event_destroy(void *e);
-void  vcg_event_record(void *e);
+void  vcg_event_record(void *e, int slot);
 int   vcg_event_done(void *e);              /* 1 when everything before the record finished */
 void *vcg_host_alloc(size_t bytes);         /* pinned host memory */
 void  vcg_host_free(void *p);
@@ -82,9 +83,12 @@ typedef struct { uint32_t x0, y0, z0, x1, y1, z1; uint64_t out_off; } vcg_place;
  * out_dev + k * 32768, unless nplace > 0 and its quad's lod word carries a
  * placement: lod | (p + 1) << 8 sends it, clipped, to box placement p.  The
  * nplace placements follow the quads in the same pinned/device buffers
- * (16-byte aligned, at quads + 4 * n). */
+ * (16-byte aligned, at quads + 4 * n).  The whole batch (upload, entropy,
+ * transform) runs in order on the stream of scratch slot `slot`, after
+ * whatever was enqueued there before and independent of the thread's other
+ * streams: the caller orders reuse of its buffers through that stream. */
 int vcg_list_launch(vcg_arc *a, size_t n, const int32_t *quads_pinned, int32_t *quads_dev, uint8_t *out_dev,
-                    int32_t *status_dev, size_t nplace);
+                    int32_t *status_dev, size_t nplace, int slot);
 /* Decode n raw payloads (no archive) at one base_q. */
 int vcg_decode_payloads(size_t n, const uint8_t *pay, const uint64_t *offs, const uint32_t *lens,
                         const int32_t dq_step[VCG_NFREQ], const int32_t dq_bias[VCG_NFREQ],
