@@ -18,7 +18,7 @@
  * its slowest atom.  Requests that touch LOD0 only (class 0) and the others
  * (class 1) therefore go to separate batches: a free slot takes the class of
  * the oldest queued request and admits the queued requests of that class in
- * arrival order, and class 1 holds at most SRV_SLOTS - 1 slots, so LOD0
+ * arrival order, and class 1 holds at most SRV_COARSE_SLOTS slots, so LOD0
  * requests always find one.
  *
  * Decoding a request in a batch gives the same bytes as decoding it alone:
@@ -32,6 +32,11 @@
 #include "vc_host.h"
 
 #define SRV_SLOTS 3
+/* most slots class 1 may hold: one, so LOD0 requests (the common case,
+ * with batches several times shorter) keep two */
+#ifndef SRV_COARSE_SLOTS
+#define SRV_COARSE_SLOTS 1
+#endif
 #define SRV_MAX_BATCH 32768u
 
 typedef struct srv_req {
@@ -160,7 +165,7 @@ static void *worker(void *arg) {
         }
         if (retired) pthread_cond_broadcast(&s->cv_done);
         /* admit queued requests into a free slot: the class of the oldest
-         * queued request, unless class 1 already holds SRV_SLOTS - 1 slots */
+         * queued request, unless class 1 already holds SRV_COARSE_SLOTS slots */
         srv_slot *fb = NULL;
         int busy1 = 0;
         for (int k = 0; k < SRV_SLOTS; ++k) {
@@ -170,7 +175,7 @@ static void *worker(void *arg) {
         int cls = -1;
         if (fb && s->qhead) {
             cls = s->qhead->cls;
-            if (cls == 1 && busy1 >= SRV_SLOTS - 1) {
+            if (cls == 1 && busy1 >= SRV_COARSE_SLOTS) {
                 cls = -1;
                 for (srv_req *r = s->qhead; r; r = r->next) if (r->cls == 0) { cls = 0; break; }
             }
