@@ -519,7 +519,10 @@ VCG_HD uint32_t vcg_rc_encode_atom(const int16_t *L, uint32_t eob, const uint32_
 // bit 32; after a bin range >= 2^17, so one 8-bit shift renormalises.  A shift
 // with no carry and low >= 0xFF000000 only lengthens the pending run (csize);
 // otherwise the pending byte (cache + carry) and csize - 1 bytes 0xFF + carry
-// go out.
+// go out.  The count-only form keeps no low at all: every shift eventually
+// puts out exactly one byte (bytes put + csize = 1 + shifts, and the last of
+// the five flush shifts always flushes, leaving csize = 1), so the byte count
+// is the number of shifts, which depends on range alone.
 //
 // Levels reach the lane through a ring of four 8-level blocks (Lv: ld(p) the
 // level at scan position p, st(k, blk) block k into ring slot k & 3).  Every
@@ -574,7 +577,8 @@ VCG_HD void vcg_efsm_shift(vcg_efsm &f, uint8_t *out) {
 // the five-shift flush (vc.c:399-401); the lane is finished afterwards
 template <bool WRITE>
 VCG_HD void vcg_efsm_finish(vcg_efsm &f, uint8_t *out) {
-    for (int k = 0; k < 5; ++k) vcg_efsm_shift<WRITE>(f, out);
+    if (WRITE) for (int k = 0; k < 5; ++k) vcg_efsm_shift<WRITE>(f, out);
+    else f.pos += 5;                                  // count: one byte per shift
     f.len = f.pos;
     f.done = 1;
     f.cap = 0;
@@ -594,7 +598,11 @@ VCG_HD void vcg_efsm_init(vcg_efsm &f, Lv &lv, const int16_t *L, uint32_t eob, c
     for (int b = 15; b >= 0; --b) {                   // EOB, 16 bypass bits (vc.c:513-519)
         f.range >>= 1;
         if ((eob >> b) & 1u) f.low += f.range;
-        if (f.range < (1u << 24)) { vcg_efsm_shift<WRITE>(f, out); f.range <<= 8; }
+        if (f.range < (1u << 24)) {
+            if (WRITE) vcg_efsm_shift<WRITE>(f, out);
+            else f.pos++;
+            f.range <<= 8;
+        }
     }
     f.S = 0; f.P = 2048u; f.psig = 2048u; f.c = 0;
     f.p = 0; f.eob = eob; f.band = 0;
@@ -626,18 +634,19 @@ VCG_HD uint32_t vcg_efsm_step(vcg_efsm &f, Ctx &ctx, Lv &lv, uint8_t *out) {
     const int vn = lv.ld(f.p + 1u);                   // the next level, in case p advances
     const int v = f.v;
     const uint32_t a = (uint32_t)(v < 0 ? -v : v);
-    const uint32_t t = a - 11u;
-    const uint32_t nb = 31u - (uint32_t)__builtin_clz(t | 1u);
+    // u = 2t >> c with t = a - 11: in the prefix (c ones coded so far) the
+    // next bit is 1 while t >> (c + 1) != 0, i.e. u > 3 (nb = floor(log2 t)
+    // ones in all); a suffix bit is (t >> (c - 1)) & 1 = u & 1
+    const uint32_t u = ((a << 1) - 22u) >> (c & 31u);
     const bool isSig = S == 0u, isMag = S - 1u < 12u, isEgp = S == 13u, isTail = S == 14u, cz = c == 0u;
     // every candidate bit computed and combined as lane masks (the lanes of a
     // wave sit in different states: no divergent branch)
-    const bool bSfx = ((t >> ((c - 1u) & 31u)) & 1u) != 0u;
-    const bool bTail = (cz & (v < 0)) | (!cz & bSfx);
-    const bool bit = (isSig & (v != 0)) | (isMag & (a > S)) | (isEgp & (c < nb)) | (isTail & bTail);
+    const bool bTail = (cz & (v < 0)) | (!cz & ((u & 1u) != 0u));
+    const bool bit = (isSig & (v != 0)) | (isMag & (a > S)) | (isEgp & (u > 3u)) | (isTail & bTail);
     // binary decision (vc.c:381-398); a bypass bin is a context bin with
     // p = 2048 that keeps floor(range/2) for both bits (bit 12 of P)
     const uint32_t r0 = (uint32_t)(((uint64_t)f.range * (P << 20)) >> 32);
-    f.low += bit ? (uint64_t)r0 : 0ull;
+    if (WRITE) f.low += bit ? (uint64_t)r0 : 0ull;
     f.range = (bit && P < 4096u) ? f.range - r0 : r0;
     const uint32_t nb0 = bit ? 0u : 1u;
     const uint32_t pn = P + (nb0 << 7) - ((P + nb0 * 31u) >> 5);
@@ -645,19 +654,19 @@ VCG_HD uint32_t vcg_efsm_step(vcg_efsm &f, Ctx &ctx, Lv &lv, uint8_t *out) {
     f.psig = isSig ? pn : f.psig;
     // renormalisation
     const bool sh = f.range < (1u << 24);
-    const uint32_t hi = (uint32_t)(f.low >> 32), lo = (uint32_t)f.low;
-    const bool fl = sh && (hi != 0u || lo < 0xFF000000u);
     if (WRITE) {
+        const uint32_t hi = (uint32_t)(f.low >> 32), lo = (uint32_t)f.low;
+        const bool fl = sh && (hi != 0u || lo < 0xFF000000u);
         if (fl) {
             vcg_efsm_put<WRITE>(f, out, (f.cache + hi) & 0xFFu);
             for (uint32_t k = 1; k < f.csize; ++k) vcg_efsm_put<WRITE>(f, out, (0xFFu + hi) & 0xFFu);
         }
+        f.cache = fl ? lo >> 24 : f.cache;
+        f.csize = fl ? 1u : f.csize + (sh ? 1u : 0u);
+        f.low = sh ? (uint64_t)(lo << 8) : f.low;
     } else {
-        f.pos += fl ? f.csize : 0u;
+        f.pos += sh ? 1u : 0u;                        // count: one byte per shift
     }
-    f.cache = fl ? lo >> 24 : f.cache;
-    f.csize = fl ? 1u : f.csize + (sh ? 1u : 0u);
-    f.low = sh ? (uint64_t)(lo << 8) : f.low;
     f.range = sh ? f.range << 8 : f.range;
     // symbol bookkeeping
     const bool adv = (isSig & !bit) | (isTail & cz);
