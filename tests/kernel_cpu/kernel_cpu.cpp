@@ -30,14 +30,16 @@ int kc_decode(const uint8_t *s, uint32_t n, const uint32_t *band_start, uint32_t
     return vcg_rc_decode_atom(s, n, band_start, out, cnt);
 }
 // the kernels' state-machine decoder; the stream is copied to offset `mis` of
-// an 8-aligned, zero-padded buffer (as in the padded device archive)
+// an 8-aligned, zero-padded buffer (as in the padded device archive).
+// always_check: test for edges in every block, as a lane does when another
+// lane of its wave is near one
 int kc_decode_fsm(const uint8_t *s, uint32_t n, uint32_t mis, const uint32_t *band_start, uint32_t *out,
-                  uint32_t *cnt) {
+                  uint32_t *cnt, int always_check) {
     size_t words = (mis + n + 7) / 8 + 4;          // 24+ bytes of slack past the stream
     uint64_t *buf = (uint64_t *)calloc(words, 8);
     memset(buf, 0xA5, words * 8);                 // garbage around the stream must be masked
     memcpy((uint8_t *)buf + mis, s, n);
-    int r = vcg_rc_decode_atom_fsm((const uint8_t *)buf + mis, n, band_start, out, cnt);
+    int r = vcg_rc_decode_atom_fsm((const uint8_t *)buf + mis, n, band_start, out, cnt, always_check != 0);
     free(buf);
     return r;
 }
@@ -76,7 +78,7 @@ int kc_idct(const uint32_t *L, uint32_t cnt, int dc, const int32_t *st, const in
             const int32_t cmat[32][32], const uint16_t *scan, int force, uint8_t *out) {
     int *t = (int *)malloc(VCG_TILE_INTS * sizeof(int));
     for (int tid = 0; tid < NTI; ++tid) vcg_tile_zero(t, tid, NTI);
-    const int fmax = cnt ? vcg_freq(scan[L[cnt - 1] >> 16]) : -1;
+    const int fmax = cnt ? vcg_freq(scan[vcg_ent_pos(L[cnt - 1])]) : -1;
     uint64_t l1 = 0;
     for (int tid = 0; tid < NTI; ++tid) l1 += vcg_scatter(t, L, cnt, scan, st, bi, tid, NTI);
     const bool fast = force == 1 ? true : force == 2 ? false : l1 <= 8000000ull;

@@ -31,7 +31,7 @@ def kc():
     L.kc_encode_fsm.argtypes = [vp, C.c_uint32, vp, vp, C.c_uint32, C.POINTER(C.c_int), vp, C.c_uint32,
                                 C.POINTER(C.c_int)]
     L.kc_decode.argtypes = [vp, C.c_uint32, vp, vp, C.POINTER(C.c_uint32)]
-    L.kc_decode_fsm.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp, C.POINTER(C.c_uint32)]
+    L.kc_decode_fsm.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp, C.POINTER(C.c_uint32), C.c_int]
     L.kc_fdct.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                           vp, vp, vp, C.c_float, vp, vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     L.kc_idct.argtypes = [vp, C.c_uint32, C.c_int, vp, vp, vp, vp, C.c_int, vp]
@@ -167,13 +167,23 @@ def test_kernel_quantizer_exhaustive(kc, tabs, oracle):
             assert np.array_equal(out, oracle.quant_levels(q, f, c)), (q, f)
 
 
-def fsm_decode(kc, tabs, rc, mis):
+def fsm_decode(kc, tabs, rc, mis, check=0):
     cm, gs, scan, bs = tabs
     b = np.frombuffer(rc, np.uint8) if len(rc) else np.zeros(1, np.uint8)
     ent = np.zeros(32768 + 8, np.uint32)          # flushes write up to 5 placeholder entries past the end
     cnt = C.c_uint32()
-    st = kc.kc_decode_fsm(b.ctypes.data, len(rc), mis, bs.ctypes.data, ent.ctypes.data, C.byref(cnt))
+    st = kc.kc_decode_fsm(b.ctypes.data, len(rc), mis, bs.ctypes.data, ent.ctypes.data, C.byref(cnt), check)
     return st, ent[:cnt.value]
+
+
+def entry_levels(ent):
+    """Scan positions and int16 levels of list entries (position << 17 |
+    (2 m - 11 + sign) mod 2^17 for magnitude m)."""
+    ent = ent.astype(np.int64)
+    t = ((ent & 0x1FFFF) + 11) & 0x1FFFF
+    m = t >> 1
+    lv = np.where(t & 1, -m, m)
+    return ent >> 17, ((lv + 0x8000) & 0xFFFF) - 0x8000
 
 
 def loop_decode(kc, tabs, rc):
@@ -198,8 +208,28 @@ def test_fsm_decoder_matches(kc, tabs, golden_atoms, golden_corrupt):
     for rc in streams:
         ref = loop_decode(kc, tabs, rc)
         for mis in range(8):
-            got = fsm_decode(kc, tabs, rc, mis)
+            got = fsm_decode(kc, tabs, rc, mis, mis & 1)
             assert got[0] == ref[0] and np.array_equal(got[1], ref[1]), (len(rc), mis)
+
+
+def test_fsm_decoder_level_fields(kc, tabs):
+    """Level fields that reach every bin state (Exp-Golomb magnitudes up to the
+    int16 limit, EOB at band edges and at 32768) come back from the state
+    machine, with and without the per-block edge test, as the straight-line
+    decoder gives them and as they were encoded."""
+    cm, gs, scan, bs = tabs
+    rng = np.random.default_rng(11)
+    for lv, eob in _level_fields(rng):
+        n, ovf, rc, _ = _enc(kc, bs, lv, eob, 32768 * 4, False)
+        assert not ovf
+        ref = loop_decode(kc, tabs, rc)
+        assert ref[0] == 0
+        pos, lev = entry_levels(ref[1])
+        nz = np.nonzero(lv[:eob])[0]
+        assert np.array_equal(pos, nz) and np.array_equal(lev, lv[nz].astype(np.int64)), eob
+        for check in (0, 1):
+            got = fsm_decode(kc, tabs, rc, 3, check)
+            assert got[0] == 0 and np.array_equal(got[1], ref[1]), (eob, check)
 
 
 def _enc(kc, bs, lv, eob, cap, fsm, count=False, budget=None):

@@ -56,9 +56,24 @@ VCG_HD int vcg_rd_bypass(vcg_rdec &d) {
     return b;
 }
 
+// Coefficient list entry: scan position << 17 | (2 m - 11 + sign) mod 2^17
+// for a coefficient of magnitude m.  The reference keeps the level as an
+// int16 (vc.c:559), so m mod 2^16 is all that matters, and 17 bits hold it
+// with the sign.
+VCG_HD uint32_t vcg_ent_make(uint32_t pos, uint32_t m, uint32_t neg) {
+    return (pos << 17) | ((2u * m - 11u + neg) & 0x1FFFFu);
+}
+VCG_HD uint32_t vcg_ent_pos(uint32_t e) { return e >> 17; }
+// the int16 level of an entry, as vc.c:559 stores it
+VCG_HD int vcg_ent_level(uint32_t e) {
+    const uint32_t t = (e + 11u) & 0x1FFFFu;              // 2 m + sign, mod 2^17
+    const uint32_t m = t >> 1;
+    return (int)(int16_t)(uint16_t)((t & 1u) ? 0u - m : m);
+}
+
 // Decode one atom's range-coder bytes (after the dc byte).  Writes the nonzero
-// levels as (scan position << 16 | uint16 level) in increasing position to
-// out (capacity 32768) and their count to *cnt.  Returns 0, or 1 when an
+// coefficients as list entries in increasing position to out (capacity 32768)
+// and their count to *cnt.  Returns 0, or 1 when an
 // Exp-Golomb prefix exceeds VCG_EG_PREFIX_CAP bits (reported as a format error;
 // the reference never returns from such a stream).
 VCG_HD int vcg_rc_decode_atom(const uint8_t *s, uint32_t n, const uint32_t *band_start, uint32_t *out,
@@ -103,9 +118,8 @@ VCG_HD int vcg_rc_decode_atom(const uint8_t *s, uint32_t n, const uint32_t *band
             for (uint32_t k = 0; k < nb; ++k) x = (x << 1) | (uint32_t)vcg_rd_bypass(d);
             m = 11u + (x - 1u) + 1u;
         }
-        int neg = vcg_rd_bypass(d);
-        uint32_t lv = neg ? 0u - m : m;
-        out[cnt++] = (pp << 16) | (lv & 0xFFFFu);
+        const uint32_t neg = (uint32_t)vcg_rd_bypass(d);
+        out[cnt++] = vcg_ent_make(pp, m, neg);
     }
     *cnt_out = cnt;
     return 0;
@@ -116,24 +130,42 @@ VCG_HD int vcg_rc_decode_atom(const uint8_t *s, uint32_t n, const uint32_t *band
 //
 // A wavefront decodes 64 atoms in lockstep, so the step is written as
 // selects: every lane runs the same instructions whatever symbol it is in.
-// Bin states S:
-//    0      significance bin of position pos (context psig, in a register)
-//    1..12  magnitude unary bin with context mag[S-1]
-//    13     Exp-Golomb prefix bin (bypass); c counts its 1s
-//    14     tail: while c > 0 an Exp-Golomb suffix bin (bypass), then the
-//           sign bin (bypass), which emits the coefficient
-// mx holds the magnitude: S at a unary exit, 11 + X for an Exp-Golomb value X
-// built as X = 2X + bit from X = 1 (vc.c:491-506).  A lane that has reached
-// EOB (or failed) is parked: code = 0 and an empty, never refilled window make
-// every later bin a zero significance bin, which emits nothing.
+// Bin states S, numbered so that most transitions are S + 1:
+//    0          significance bin of position pos (context psig, in a register)
+//    1..12      magnitude unary bin with context mag[S-1]
+//    13 + c     Exp-Golomb prefix bin (bypass) after c ones; 78..127 means the
+//               prefix passed VCG_EG_PREFIX_CAP ones
+//    255 - c    Exp-Golomb suffix bin (bypass) with c bits still to come, c >= 1
+//    255        sign bin (bypass), which emits the coefficient
+// Transitions: after bit 1, and after any suffix bin, S + 1.  After bit 0 in
+// a magnitude or prefix state, 268 - max(S, 13): the sign bin after a unary
+// magnitude or an empty prefix, else the suffix with c bits to come.  After
+// bit 0 in state 0, and after the sign bin, state 0 at the next position.
+// acc holds the magnitude: S in a unary state (so 12 = 11 + X, X = 1, when
+// the prefix starts), then 2 acc - 11 + bit in every suffix bin
+// (vc.c:491-506: X = 2X + bit, m = 11 + X) and in the sign bin, where
+// 2 m - 11 + sign is the entry's low part.
 //
-// The probability of the next bin is resolved inside the step: for a next
-// state 0 it is psig, for a next magnitude state the context read (ahead of
-// the bit) from slot S+1 of the lane's context memory, else VCG_P_BYPASS.
-// A bypass bin splits the range like a context bin with p = 2048, but keeps
-// floor(range/2) for both bit values and adapts nothing (vc.c:434-441); bit
-// 12 of P marks it and vanishes in P << 20.  Context memory slot k+1 holds
-// mag[k]; slots 0, 13, 14 (and 15 for reads) are write sinks.
+// The probability of the next bin is resolved inside the step.  Context
+// memory slot k+1 holds mag[k] (k < 12), slots 13 and 15 hold VCG_P_BYPASS
+// and are never written.  A context bin writes its adapted probability to
+// slot S, a bypass bin to the sink slot 14 (Ctx::at picks the slot), and the
+// step reads the slot after the one it writes, ahead of the bit: that is the next probability after
+// every bit 1 that does not start a new position (mag[S] after states 0..11,
+// bypass after 12 and the bypass states); after a bit 0 it is psig at a new
+// position, else bypass.  A bypass bin splits the range like a context bin
+// with p = 2048, but keeps floor(range/2) for both bit values and adapts
+// nothing (vc.c:434-441); bit 12 of P marks it and vanishes in P << 20.
+//
+// A lane that has reached EOB (or failed) is parked: code = 0 and an empty,
+// never refilled window make every later bin a zero significance bin, which
+// emits nothing.
+//
+// Band edges, EOB and the prefix cap are tested after each step only in
+// blocks where some lane can meet one: a step advances pos by at most one
+// position and a prefix state by at most one, so a block of k steps needs no
+// test while every lane is more than k positions before its next edge and
+// below state 78 - k (vcg_fsm_near).
 //
 // Input: a 64-bit little-endian window of wb valid bits ending at stream byte
 // E.  Every 8 steps (a step consumes at most 8 bits: one renormalisation per
@@ -159,6 +191,11 @@ VCG_HD int vcg_rc_decode_atom(const uint8_t *s, uint32_t n, const uint32_t *band
 #define VCG_RING 8u
 #define VCG_PARK_E 0x40000000u
 #define VCG_P_BYPASS (2048u | 4096u)
+#define VCG_S_PFX 13u
+#define VCG_S_BAD (VCG_S_PFX + VCG_EG_PREFIX_CAP + 1u)      // 78
+#define VCG_S_TAIL 128u
+#define VCG_S_SIGN 255u
+#define VCG_CTX_SLOTS 16u
 
 // mask keeping the stream bytes of the 8 at byte E (those before s[n])
 VCG_HD uint64_t vcg_tail_mask(uint32_t E, uint32_t n) {
@@ -182,15 +219,16 @@ struct vcg_fsm {
     uint32_t code, range;
     uint64_t w;                  // window, wb valid bits, ending at stream byte E
     uint32_t wb, E;
-    uint32_t P, S, psig;
-    uint32_t pos16, lim16, eob16;   // scan position, next band edge / EOB, all << 16
-    uint32_t c, mx;
-    uint32_t cnt8, fl8;             // emitted / flushed entry counts, << 8 (ring slot offsets)
+    uint32_t P, S, psig, acc;
+    uint32_t pos17, lim17;       // scan position and next band edge / EOB, << 17 (mod 2^32)
+    uint32_t eob;
+    uint32_t cnt8, fl8;          // emitted / flushed entry counts, << 8 (ring slot offsets)
     uint32_t band, bad;
 };
 
+// lim17 is odd, so no position ever meets it
 VCG_HD void vcg_fsm_park(vcg_fsm &f) {
-    f.code = 0; f.w = 0; f.E = VCG_PARK_E; f.S = 0; f.lim16 = 0xFFFFFFFFu; f.c = 0;
+    f.code = 0; f.w = 0; f.E = VCG_PARK_E; f.S = 0; f.lim17 = 0xFFFFFFFFu; f.P = 2048u; f.psig = 2048u;
 }
 VCG_HD bool vcg_fsm_parked(const vcg_fsm &f) { return f.E >= VCG_PARK_E; }
 
@@ -205,6 +243,15 @@ VCG_HD void vcg_fsm_refill(vcg_fsm &f, vcg_look &L, const uint8_t *s, uint32_t n
     f.E += (64u - f.wb) >> 3;
     f.wb = 64u;
     vcg_look_load(L, s, n, f.E);
+}
+
+// a * b for a, b < 2^24
+VCG_HD uint32_t vcg_mul24(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul24(a, b);
+#else
+    return a * b;
+#endif
 }
 
 // low `width` bits of x (width 0..31)
@@ -225,7 +272,7 @@ VCG_HD void vcg_fsm_renorm(vcg_fsm &f) {
 }
 
 // s: first range-coder byte; n: byte count.  Context memory slots 1..12 must
-// hold 2048 on entry.
+// hold 2048 and slots 13 and 15 VCG_P_BYPASS on entry.
 VCG_HD void vcg_fsm_init(vcg_fsm &f, vcg_look &A, vcg_look &B, const uint8_t *s, uint32_t n,
                          const uint32_t *band_start) {
     f.w = 0; f.wb = 0; f.E = 0;
@@ -246,64 +293,87 @@ VCG_HD void vcg_fsm_init(vcg_fsm &f, vcg_look &A, vcg_look &B, const uint8_t *s,
     }
     B = A;                                            // both at the window end (13)
     eob = eob > 32768u ? 32768u : eob;                // vc.c:549
-    f.eob16 = eob << 16;
-    f.P = 2048u; f.S = 0; f.psig = 2048u; f.pos16 = 0; f.c = 0; f.mx = 0; f.cnt8 = 0; f.fl8 = 0;
+    f.eob = eob;
+    f.P = 2048u; f.S = 0; f.psig = 2048u; f.acc = 0; f.pos17 = 0; f.cnt8 = 0; f.fl8 = 0;
     f.band = 0; f.bad = 0;
-    f.lim16 = (band_start[1] < eob ? band_start[1] : eob) << 16;
+    f.lim17 = (band_start[1] < eob ? band_start[1] : eob) << 17;   // EOB 32768 wraps to 0: met only at the end
     if (eob == 0) vcg_fsm_park(f);
 }
 
-// one bin.  Ctx: ld(slot), st(slot, v) on the lane's context memory; Ring:
-// st(slot << 8, v) on the lane's output ring.  Returns nonzero when pos reached a
-// band boundary or EOB, or the Exp-Golomb prefix passed its cap (the caller
-// then runs vcg_fsm_edge).
-template <class Ctx, class Ring>
+// true when the lane may meet a band edge, EOB or the prefix cap within the
+// next k steps
+VCG_HD bool vcg_fsm_near(const vcg_fsm &f, uint32_t k) {
+    return (f.lim17 - f.pos17 <= (k << 17)) | (f.S - (VCG_S_BAD - k) < VCG_S_TAIL - VCG_S_BAD + k);
+}
+
+// one bin.  Ctx: h = at(S, S < 13) the handle of the slot a bin in state S
+// writes (S, or the sink 14), ld_next(h) the next slot's value, st(h, v);
+// Ring: st(slot << 8, v) on the lane's output ring.  With CHECK, returns nonzero
+// when pos reached a band edge or EOB, or the prefix passed its cap (the
+// caller then runs vcg_fsm_edge); without, the caller has ruled these out.
+template <bool CHECK, class Ctx, class Ring>
 VCG_HD uint32_t vcg_fsm_step(vcg_fsm &f, Ctx &ctx, Ring &ring) {
-    const uint32_t S = f.S, P = f.P, c = f.c;
-    const uint32_t R = ctx.ld(S + 1u);
+    const uint32_t S = f.S, P = f.P;
+    const bool isCtx = S < VCG_S_PFX;
+    const uint32_t a = ctx.at(S, isCtx);
+    const uint32_t R = ctx.ld_next(a);
     // binary decision (vc.c:414-441)
     const uint32_t r0 = (uint32_t)(((uint64_t)f.range * (P << 20)) >> 32);
     const bool bit = f.code >= r0;
     f.code = bit ? f.code - r0 : f.code;
-    f.range = (bit && P < 4096u) ? f.range - r0 : r0;
-    // adaptation (vc.c:420-431): bit 1: P - (P>>5); bit 0: P + ((4096-P)>>5)
-    // = P + 128 - ((P+31)>>5), written without a branch
-    const uint32_t nb = bit ? 0u : 1u;
-    const uint32_t pn = P + (nb << 7) - ((P + nb * 31u) >> 5);
-    ctx.st(S, pn);
+    f.range = (bit && isCtx) ? f.range - r0 : r0;
+    // adaptation (vc.c:420-431): bit 1: P - (P>>5), bit 0: P + ((4096-P)>>5);
+    // both equal (t + 31 P) >> 5 with t = 31 for bit 1 and 4096 for bit 0
     const bool isSig = S == 0u;
+    const uint32_t pn = ((bit ? 31u : 4096u) + vcg_mul24(31u, P)) >> 5;
+    ctx.st(a, pn);
     f.psig = isSig ? pn : f.psig;
     vcg_fsm_renorm(f);
     // symbol bookkeeping
-    const bool isTail = S == 14u;
-    const bool cz = c == 0u;
-    const bool sgn = isTail && cz;
-    const bool sfx = isTail && !cz;
-    const uint32_t mx = (S - 1u < 12u) ? S : sfx ? 2u * f.mx - 11u + (uint32_t)bit : f.mx;
-    f.mx = mx;
-    ring.st(f.cnt8 & ((VCG_RING - 1u) << 8), f.pos16 | ((bit ? 0u - mx : mx) & 0xFFFFu));
+    const bool tail = S >= VCG_S_TAIL;
+    const bool sgn = S == VCG_S_SIGN;
+    const uint32_t acc2 = 2u * f.acc + (bit ? 0u - 10u : 0u - 11u);
+    ring.st(f.cnt8 & ((VCG_RING - 1u) << 8), f.pos17 | (acc2 & 0x1FFFFu));
     f.cnt8 += sgn ? 256u : 0u;
-    f.c = c + ((S == 13u && bit) ? 1u : 0u) - (sfx ? 1u : 0u);
-    f.pos16 += ((isSig && !bit) || sgn) ? 0x10000u : 0u;
-    const uint32_t up = S + 1u < 13u ? S + 1u : 13u;
-    const uint32_t t = bit ? up : (isSig ? 0u : 14u);
-    const uint32_t Sn = isTail ? (cz ? 0u : 14u) : t;
-    f.S = Sn;
-    f.P = Sn == 0u ? f.psig : (Sn <= 12u ? R : VCG_P_BYPASS);
-    return (f.pos16 == f.lim16) | (f.c > VCG_EG_PREFIX_CAP);
+    const bool adv = (isSig & !bit) | sgn;
+    f.pos17 += adv ? 0x20000u : 0u;
+    f.acc = tail ? acc2 : (S < 12u ? S : 12u);
+    const uint32_t up = (bit || tail) ? S + 1u : 268u - (S > VCG_S_PFX ? S : VCG_S_PFX);
+    f.S = adv ? 0u : up;
+    f.P = adv ? f.psig : (bit ? R : VCG_P_BYPASS);
+    if (!CHECK) return 0;
+    return (f.pos17 == f.lim17) | (f.S - VCG_S_BAD < VCG_S_TAIL - VCG_S_BAD);
 }
 
 // band boundary / EOB / Exp-Golomb cap for a lane whose step returned nonzero
 VCG_HD void vcg_fsm_edge(vcg_fsm &f, const uint32_t *band_start) {
-    if (f.c > VCG_EG_PREFIX_CAP) { f.bad = 1; vcg_fsm_park(f); return; }
-    if (f.pos16 == f.eob16) { vcg_fsm_park(f); return; }
-    // a new band starts from the reset context (vc.c:552-555); the next bin
-    // is a significance bin, so its probability is the reset value too
-    f.band++;
-    const uint32_t nb = band_start[f.band + 1] << 16;
-    f.lim16 = nb < f.eob16 ? nb : f.eob16;
+    const bool bad = f.S - VCG_S_BAD < VCG_S_TAIL - VCG_S_BAD;
+    const bool end = bad || f.pos17 == f.eob << 17;
+    // a new band starts from the reset context (vc.c:552-555) and its first
+    // bin is a significance bin; a parked lane runs on the same values
     f.psig = 2048u;
     f.P = 2048u;
+    if (end) {
+        f.bad |= bad ? 1u : 0u;
+        f.code = 0; f.w = 0; f.E = VCG_PARK_E; f.S = 0; f.lim17 = 0xFFFFFFFFu;
+    } else {
+        f.band++;
+        const uint32_t nb = band_start[f.band + 1];
+        f.lim17 = (nb < f.eob ? nb : f.eob) << 17;
+    }
+}
+
+// 8 steps, tested for edges only when `check`
+template <class Ctx, class Ring>
+VCG_HD void vcg_fsm_block(vcg_fsm &f, Ctx &ctx, Ring &ring, const uint32_t *band_start, bool check) {
+    if (check) {
+#pragma unroll
+        for (int k = 0; k < VCG_REFILL_STEPS; ++k)
+            if (vcg_fsm_step<true>(f, ctx, ring)) vcg_fsm_edge(f, band_start);
+    } else {
+#pragma unroll
+        for (int k = 0; k < VCG_REFILL_STEPS; ++k) vcg_fsm_step<false>(f, ctx, ring);
+    }
 }
 
 // copy ring slots [flushed, flushed + VCG_FLUSH_N) to out (entries past cnt
@@ -320,13 +390,17 @@ VCG_HD uint32_t vcg_fsm_flush(vcg_fsm &f, Ring &ring, uint32_t *out) {
 }
 
 // Whole-atom decode with the state machine, driven as the kernel drives it
-// (CPU-side reference of the kernel's per-lane logic).  Returns 0, 1 for an
-// over-long Exp-Golomb prefix, or 2 if a flush ever held more than
-// VCG_FLUSH_N entries (never, by the 3-bins-per-coefficient bound).
+// (CPU-side reference of the kernel's per-lane logic).  always_check tests
+// for edges in every block, as a lane does when another lane of its wave is
+// near one.  Returns 0, 1 for an over-long Exp-Golomb prefix, or 2 if a flush
+// ever held more than VCG_FLUSH_N entries (never, by the 3-bins-per-
+// coefficient bound).
 struct vcg_ctx_array {
-    uint32_t v[16];
+    uint32_t v[VCG_CTX_SLOTS];
+    VCG_HDM uint32_t at(uint32_t S, bool isCtx) const { return isCtx ? S : 14u; }
+    VCG_HDM uint32_t ld_next(uint32_t h) const { return v[h + 1u]; }
     VCG_HDM uint32_t ld(uint32_t k) const { return v[k]; }
-    VCG_HDM void st(uint32_t k, uint32_t p) { v[k] = p; }
+    VCG_HDM void st(uint32_t h, uint32_t p) { v[h] = p; }
 };
 struct vcg_ring_array {
     uint32_t v[VCG_RING];
@@ -334,10 +408,10 @@ struct vcg_ring_array {
     VCG_HDM void st(uint32_t off, uint32_t p) { v[off >> 8] = p; }
 };
 VCG_HD int vcg_rc_decode_atom_fsm(const uint8_t *s, uint32_t n, const uint32_t *band_start, uint32_t *out,
-                                  uint32_t *cnt_out) {
+                                  uint32_t *cnt_out, bool always_check) {
     vcg_ctx_array ctx;
     vcg_ring_array ring;
-    for (int k = 0; k < 16; ++k) ctx.v[k] = 2048u;
+    for (uint32_t k = 0; k < VCG_CTX_SLOTS; ++k) ctx.v[k] = k == 13u || k == 15u ? VCG_P_BYPASS : 2048u;
     for (uint32_t k = 0; k < VCG_RING; ++k) ring.v[k] = 0xDEADBEEFu;
     vcg_fsm f;
     vcg_look A, B;
@@ -346,11 +420,9 @@ VCG_HD int vcg_rc_decode_atom_fsm(const uint8_t *s, uint32_t n, const uint32_t *
     while (!vcg_fsm_parked(f)) {
         vcg_fsm_refill(f, A, s, n);
         if (vcg_fsm_flush(f, ring, out) > VCG_FLUSH_N) over = 1;
-        for (int k = 0; k < VCG_REFILL_STEPS; ++k)
-            if (vcg_fsm_step(f, ctx, ring)) vcg_fsm_edge(f, band_start);
+        vcg_fsm_block(f, ctx, ring, band_start, always_check || vcg_fsm_near(f, VCG_REFILL_STEPS));
         vcg_fsm_refill(f, B, s, n);
-        for (int k = 0; k < VCG_REFILL_STEPS; ++k)
-            if (vcg_fsm_step(f, ctx, ring)) vcg_fsm_edge(f, band_start);
+        vcg_fsm_block(f, ctx, ring, band_start, always_check || vcg_fsm_near(f, VCG_REFILL_STEPS));
     }
     if (vcg_fsm_flush(f, ring, out) > VCG_FLUSH_N) over = 1;
     *cnt_out = f.bad ? 0 : f.cnt8 >> 8;

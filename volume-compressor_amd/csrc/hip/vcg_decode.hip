@@ -58,20 +58,30 @@ __device__ __forceinline__ void item_atom(const dec_job &j, uint32_t gi, int &lo
     }
 }
 
-// per-lane LDS array (context memory: 16 slots; output ring: 8 slots),
-// slot-major over the 64 lanes of the wave so a wave's access is conflict-free
+// per-lane LDS array (context memory: VCG_CTX_SLOTS slots; output ring: 8 slots),
+// slot-major over the 64 lanes of the wave so a wave's access is conflict-free.
+// m is the array (a fixed LDS address), l4 the lane's byte offset (< 256), so
+// a slot's address is one shift-or and the array base goes into the
+// instruction's offset field.  Context handles are byte offsets; sink is the
+// sink slot's, kept opaque to the compiler so that picking a handle stays one
+// select after the shift-or.
 struct lds_ctx {
-    uint32_t *b;
-    __device__ __forceinline__ uint32_t ld(uint32_t k) const { return b[k * 64u]; }
-    __device__ __forceinline__ void st(uint32_t k, uint32_t v) { b[k * 64u] = v; }
+    uint32_t *m;
+    uint32_t l4, sink;
+    __device__ __forceinline__ uint32_t at(uint32_t S, bool isCtx) const { return isCtx ? (S << 8) | l4 : sink; }
+    __device__ __forceinline__ uint32_t ld_next(uint32_t h) const {
+        return *(const uint32_t *)((const char *)m + h + 256u);
+    }
+    __device__ __forceinline__ void st(uint32_t h, uint32_t v) { *(uint32_t *)((char *)m + h) = v; }
 };
 // output ring: slot offsets come pre-scaled (slot << 8 = slot * 64 lanes * 4 B)
 struct lds_ring {
-    uint32_t *b;
+    uint32_t *m;
+    uint32_t l4;
     __device__ __forceinline__ uint32_t ld(uint32_t off) const {
-        return *(const uint32_t *)((const char *)b + off);
+        return *(const uint32_t *)((const char *)m + (off | l4));
     }
-    __device__ __forceinline__ void st(uint32_t off, uint32_t v) { *(uint32_t *)((char *)b + off) = v; }
+    __device__ __forceinline__ void st(uint32_t off, uint32_t v) { *(uint32_t *)((char *)m + (off | l4)) = v; }
 };
 
 __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__restrict__ T,
@@ -81,7 +91,7 @@ __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__res
     // issue priority over transform waves sharing the SIMD (the serial
     // decode is the critical path; the transform fills its idle issue cycles)
     __builtin_amdgcn_s_setprio(3);
-    __shared__ uint32_t ctxm[16 * 64];
+    __shared__ uint32_t ctxm[VCG_CTX_SLOTS * 64];
     __shared__ uint32_t ringm[VCG_RING * 64];
     __shared__ uint32_t bands[9];
     // band table in LDS: a band edge then waits on LDS only, not on the
@@ -117,35 +127,36 @@ __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__res
 #if VCG_ENTROPY_FSM
     // range-coder bytes [p+1, p+len); the device buffers are padded past
     // their end for the 16-byte look-ahead loads
-    lds_ctx ctx{ctxm + threadIdx.x};
-    lds_ring ring{ringm + threadIdx.x};
-    for (uint32_t k = 1; k <= 12; ++k) ctx.st(k, 2048u);
+    uint32_t sink = (14u << 8) | (threadIdx.x * 4u);
+    asm volatile("" : "+v"(sink));
+    lds_ctx ctx{ctxm, threadIdx.x * 4u, sink};
+    lds_ring ring{ringm, threadIdx.x * 4u};
+    for (uint32_t k = 1; k < VCG_CTX_SLOTS; ++k)
+        ctxm[k * 64u + threadIdx.x] = k == 13u || k == 15u ? VCG_P_BYPASS : 2048u;
     const uint8_t *s = p + 1;
     const uint32_t n = len - 1;
     vcg_fsm f;
     vcg_look A, B;
     vcg_fsm_init(f, A, B, s, n, bands);
+    // the edge test is needed in a block only if some lane of the wave can
+    // meet an edge in it (a wave-uniform branch)
     while (__any(!vcg_fsm_parked(f))) {
         vcg_fsm_refill(f, A, s, n);
         vcg_fsm_flush(f, ring, out);
-#pragma unroll
-        for (int k = 0; k < VCG_REFILL_STEPS; ++k)
-            if (vcg_fsm_step(f, ctx, ring)) vcg_fsm_edge(f, bands);
+        vcg_fsm_block(f, ctx, ring, bands, __any(vcg_fsm_near(f, VCG_REFILL_STEPS)));
         vcg_fsm_refill(f, B, s, n);
-#pragma unroll
-        for (int k = 0; k < VCG_REFILL_STEPS; ++k)
-            if (vcg_fsm_step(f, ctx, ring)) vcg_fsm_edge(f, bands);
+        vcg_fsm_block(f, ctx, ring, bands, __any(vcg_fsm_near(f, VCG_REFILL_STEPS)));
     }
     vcg_fsm_flush(f, ring, out);
     const int bad = (int)f.bad;
     const uint32_t cnt = f.cnt8 >> 8;
     // the last entry (highest scan position) is still in the ring
-    const int fmax = cnt ? vcg_freq(T->scan[ring.ld((f.cnt8 - 256u) & ((VCG_RING - 1u) << 8)) >> 16]) : -1;
+    const int fmax = cnt ? vcg_freq(T->scan[vcg_ent_pos(ring.ld((f.cnt8 - 256u) & ((VCG_RING - 1u) << 8)))]) : -1;
 #else
     (void)ctxm; (void)ringm; (void)bands;
     uint32_t cnt = 0;
     const int bad = vcg_rc_decode_atom(p + 1, len - 1, T->band_start, out, &cnt);
-    const int fmax = cnt && !bad ? vcg_freq(T->scan[out[cnt - 1] >> 16]) : -1;
+    const int fmax = cnt && !bad ? vcg_freq(T->scan[vcg_ent_pos(out[cnt - 1])]) : -1;
 #endif
     if (bad) {
         it.status = 2; it.cnt = 0;

@@ -182,11 +182,11 @@ VCG_HD uint64_t vcg_scatter(int *t, const uint32_t *L, uint32_t cnt, const uint1
 #pragma unroll
         for (int q = 0; q < 8; ++q) v[q] = e0 + q * un < cnt ? L[e0 + q * un] : 0u;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) idx[q] = e0 + q * un < cnt ? scan[v[q] >> 16] : 0u;
+        for (int q = 0; q < 8; ++q) idx[q] = e0 + q * un < cnt ? scan[vcg_ent_pos(v[q])] : 0u;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             if (e0 + q * un >= cnt) break;
-            const int lv = (int)(int16_t)(v[q] & 0xFFFFu);
+            const int lv = vcg_ent_level(v[q]);
             const int f = vcg_freq(idx[q]);
             const uint32_t mag = (uint32_t)(lv < 0 ? -lv : lv);
             int r = (int)(mag * (uint32_t)st[f] + (uint32_t)bi[f] + 128u) >> 8;
