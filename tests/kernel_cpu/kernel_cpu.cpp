@@ -17,12 +17,13 @@ uint32_t kc_encode(const int16_t *L, uint32_t eob, const uint32_t *band_start, u
     return vcg_rc_encode_atom(L, eob, band_start, out, cap, ovf);
 }
 // the kernels' state-machine encoder (out NULL: count only); levels are
-// copied to a 16-byte aligned buffer as the kernels hold them
+// copied to a 16-byte aligned buffer as the kernels hold them.
+// always_check: test for edges in every block
 uint32_t kc_encode_fsm(const int16_t *L, uint32_t eob, const uint32_t *band_start, uint8_t *out, uint32_t cap,
-                       int *ovf, uint32_t *kb, uint32_t budget, int *stopped) {
+                       int *ovf, uint32_t *kb, uint32_t budget, int *stopped, int always_check) {
     int16_t *a = (int16_t *)aligned_alloc(16, 32768 * 2);
     memcpy(a, L, 32768 * 2);
-    uint32_t n = vcg_rc_encode_atom_fsm(a, eob, band_start, out, cap, ovf, kb, budget, stopped);
+    uint32_t n = vcg_rc_encode_atom_fsm(a, eob, band_start, out, cap, ovf, kb, budget, stopped, always_check != 0);
     free(a);
     return n;
 }

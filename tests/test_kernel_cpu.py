@@ -29,7 +29,7 @@ def kc():
     L.kc_encode.argtypes = [vp, C.c_uint32, vp, vp, C.c_uint32, C.POINTER(C.c_int)]
     L.kc_encode_fsm.restype = C.c_uint32
     L.kc_encode_fsm.argtypes = [vp, C.c_uint32, vp, vp, C.c_uint32, C.POINTER(C.c_int), vp, C.c_uint32,
-                                C.POINTER(C.c_int)]
+                                C.POINTER(C.c_int), C.c_int]
     L.kc_decode.argtypes = [vp, C.c_uint32, vp, vp, C.POINTER(C.c_uint32)]
     L.kc_decode_fsm.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp, C.POINTER(C.c_uint32), C.c_int]
     L.kc_fdct.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -232,7 +232,7 @@ def test_fsm_decoder_level_fields(kc, tabs):
             assert got[0] == 0 and np.array_equal(got[1], ref[1]), (eob, check)
 
 
-def _enc(kc, bs, lv, eob, cap, fsm, count=False, budget=None):
+def _enc(kc, bs, lv, eob, cap, fsm, count=False, budget=None, check=0):
     out = np.zeros(max(cap, 1) + 64, np.uint8)
     ovf, stop = C.c_int(), C.c_int()
     if not fsm:
@@ -241,7 +241,7 @@ def _enc(kc, bs, lv, eob, cap, fsm, count=False, budget=None):
     kb = np.zeros(1, np.uint32)
     n = kc.kc_encode_fsm(lv.ctypes.data, eob, bs.ctypes.data, None if count else out.ctypes.data, cap,
                          C.byref(ovf), kb.ctypes.data if budget is not None else None,
-                         budget or 0, C.byref(stop))
+                         budget or 0, C.byref(stop), check)
     return n, ovf.value, (b"" if count else out[:min(n, cap)].tobytes()), stop.value
 
 
@@ -285,9 +285,10 @@ def test_fsm_encoder_matches(kc, tabs, golden_atoms):
     cases += _level_fields(np.random.default_rng(7))
     for k, (lv, eob) in enumerate(cases):
         ref = _enc(kc, bs, lv, eob, 131136, False)
-        got = _enc(kc, bs, lv, eob, 131136, True)
-        assert got[:3] == ref[:3], k
-        assert _enc(kc, bs, lv, eob, 131136, True, count=True)[:2] == ref[:2], k
+        for check in (0, 1):
+            got = _enc(kc, bs, lv, eob, 131136, True, check=check)
+            assert got[:3] == ref[:3], (k, check)
+            assert _enc(kc, bs, lv, eob, 131136, True, count=True, check=check)[:2] == ref[:2], (k, check)
         small = max(1, ref[0] // 2)
         assert _enc(kc, bs, lv, eob, small, True)[:3] == _enc(kc, bs, lv, eob, small, False)[:3], k
         assert _enc(kc, bs, lv, eob, small, True, count=True)[:2] == _enc(kc, bs, lv, eob, small, False)[:2], k

@@ -579,17 +579,19 @@ VCG_HD uint32_t vcg_rc_encode_atom(const int16_t *L, uint32_t eob, const uint32_
 
 // ---------------------------------------------------------------------------
 // Encoder as a one-bin-per-step state machine (the batched encode kernels'
-// form), the mirror of the decoder's: the same bin states, each bin's bit
-// taken from the level v being coded (a = |v|):
-//    0      significance of position p: v != 0
-//    1..12  magnitude unary bin S (context mag[S-1]): a > S
-//    13     Exp-Golomb prefix (a >= 13, t = a - 11, nb = floor(log2 t)):
-//           c < nb, c counting the 1s already coded
-//    14     tail: while c > 0 the suffix bit (t >> (c-1)) & 1, then the sign
-// (vc.c:471-490, 528-544).  Range coder (vc.c:364-401): low + range never
-// exceeds its value at the last shift, so low stays below 2^33 and a carry is
-// bit 32; after a bin range >= 2^17, so one 8-bit shift renormalises.  A shift
-// with no carry and low >= 0xFF000000 only lengthens the pending run (csize);
+// form), the mirror of the decoder's: the same bin states and transitions,
+// each bin's bit taken from the level v being coded, with a = |v|,
+// t = a - 11 and T = 2t + sign (vc.c:471-490, 528-544):
+//    0..12      significance and unary magnitude bins: a > S
+//    13 + c     Exp-Golomb prefix after c ones: t >= 2^(c+1), i.e.
+//               a > 10 + (2 << c) (nb = floor(log2 t) ones in all)
+//    255 - c    suffix bin with c bits to come: bit c-1 of t = bit c of T
+//    255        sign bin: bit 0 of T
+// so in a tail state (S >= 128) the bit is bit 255 - S of T: the top bit of
+// T << (S mod 32).  Range coder (vc.c:364-401): low + range never exceeds its
+// value at the last shift, so low stays below 2^33 and a carry is bit 32;
+// after a bin range >= 2^17, so one 8-bit shift renormalises.  A shift with no
+// carry and low >= 0xFF000000 only lengthens the pending run (csize);
 // otherwise the pending byte (cache + carry) and csize - 1 bytes 0xFF + carry
 // go out.  The count-only form keeps no low at all: every shift eventually
 // puts out exactly one byte (bytes put + csize = 1 + shifts, and the last of
@@ -604,13 +606,15 @@ VCG_HD uint32_t vcg_rc_encode_atom(const int16_t *L, uint32_t eob, const uint32_
 // refill is issued unconditionally, so no wait falls on a load in flight.  A
 // finished lane records its byte count and keeps stepping with no effect on
 // the result: its output capacity drops to zero and it puts nothing more.
+// Band edges and EOB are tested after a step only in blocks where some lane
+// can meet one (vcg_efsm_near), as in the decoder.
 // ---------------------------------------------------------------------------
 #define VCG_EFSM_REFILL 8
 struct vcg_blk16 { uint32_t w[4]; };
 struct vcg_efsm {
     uint64_t low;
     uint32_t range, cache, csize, pos;   // pos: bytes put out
-    uint32_t S, P, psig, c;
+    uint32_t S, P, psig;
     uint32_t p, lim, eob, band;
     int v;
     uint32_t done, len;                  // len: final byte count once done
@@ -661,7 +665,7 @@ VCG_HD vcg_blk16 vcg_efsm_load(const vcg_efsm &f, uint32_t k) { return f.blk[k <
 
 // L: scan-order levels (16-byte aligned, 32768 of them), eob <= 32768.
 // out: the range-coder bytes (WRITE), capacity cap.  Context memory slots
-// 1..12 must hold 2048 on entry.
+// 1..12 must hold 2048 and slots 13 and 15 VCG_P_BYPASS on entry.
 template <bool WRITE, class Lv>
 VCG_HD void vcg_efsm_init(vcg_efsm &f, Lv &lv, const int16_t *L, uint32_t eob, const uint32_t *band_start,
                           uint8_t *out, uint32_t cap) {
@@ -676,7 +680,7 @@ VCG_HD void vcg_efsm_init(vcg_efsm &f, Lv &lv, const int16_t *L, uint32_t eob, c
             f.range <<= 8;
         }
     }
-    f.S = 0; f.P = 2048u; f.psig = 2048u; f.c = 0;
+    f.S = 0; f.P = 2048u; f.psig = 2048u;
     f.p = 0; f.eob = eob; f.band = 0;
     f.lim = band_start[1] < eob ? band_start[1] : eob;
     f.blk = (const vcg_blk16 *)L;
@@ -696,33 +700,36 @@ VCG_HD void vcg_efsm_refill(vcg_efsm &f, Lv &lv) {
     f.pref = vcg_efsm_load(f, k + 3u);
 }
 
-// one bin.  Ctx: ld(slot), st(slot, v) on the lane's context memory (slot
-// k+1 = mag[k]).  Returns nonzero when p reached a band boundary or the EOB
-// (the caller then runs vcg_efsm_edge).
-template <bool WRITE, class Ctx, class Lv>
+// true when the lane may meet a band edge or EOB within the next k steps
+VCG_HD bool vcg_efsm_near(const vcg_efsm &f, uint32_t k) { return f.lim - f.p <= k; }
+
+// one bin.  Ctx as for the decoder step.  With CHECK, returns nonzero when p
+// reached a band boundary or the EOB (the caller then runs vcg_efsm_edge);
+// without, the caller has ruled that out.
+template <bool CHECK, bool WRITE, class Ctx, class Lv>
 VCG_HD uint32_t vcg_efsm_step(vcg_efsm &f, Ctx &ctx, Lv &lv, uint8_t *out) {
-    const uint32_t S = f.S, P = f.P, c = f.c;
-    const uint32_t R = ctx.ld(S + 1u);
+    const uint32_t S = f.S, P = f.P;
+    const bool isCtx = S < VCG_S_PFX;
+    const uint32_t h = ctx.at(S, isCtx);
+    const uint32_t R = ctx.ld_next(h);
     const int vn = lv.ld(f.p + 1u);                   // the next level, in case p advances
     const int v = f.v;
     const uint32_t a = (uint32_t)(v < 0 ? -v : v);
-    // u = 2t >> c with t = a - 11: in the prefix (c ones coded so far) the
-    // next bit is 1 while t >> (c + 1) != 0, i.e. u > 3 (nb = floor(log2 t)
-    // ones in all); a suffix bit is (t >> (c - 1)) & 1 = u & 1
-    const uint32_t u = ((a << 1) - 22u) >> (c & 31u);
-    const bool isSig = S == 0u, isMag = S - 1u < 12u, isEgp = S == 13u, isTail = S == 14u, cz = c == 0u;
-    // every candidate bit computed and combined as lane masks (the lanes of a
-    // wave sit in different states: no divergent branch)
-    const bool bTail = (cz & (v < 0)) | (!cz & ((u & 1u) != 0u));
-    const bool bit = (isSig & (v != 0)) | (isMag & (a > S)) | (isEgp & (u > 3u)) | (isTail & bTail);
+    // the bit (lanes of a wave sit in different states: selects, no branch)
+    const uint32_t thr = isCtx ? S : (2u << ((S + 19u) & 31u)) + 10u;   // prefix: 10 + (2 << (S - 13))
+    const uint32_t T = 2u * a - 22u + (v < 0 ? 1u : 0u);
+    const bool tail = S >= VCG_S_TAIL;
+    const bool bT = (int32_t)(T << (S & 31u)) < 0, bA = a > thr;
+    const bool bit = (tail & bT) | (!tail & bA);
     // binary decision (vc.c:381-398); a bypass bin is a context bin with
     // p = 2048 that keeps floor(range/2) for both bits (bit 12 of P)
     const uint32_t r0 = (uint32_t)(((uint64_t)f.range * (P << 20)) >> 32);
     if (WRITE) f.low += bit ? (uint64_t)r0 : 0ull;
-    f.range = (bit && P < 4096u) ? f.range - r0 : r0;
-    const uint32_t nb0 = bit ? 0u : 1u;
-    const uint32_t pn = P + (nb0 << 7) - ((P + nb0 * 31u) >> 5);
-    ctx.st(S, pn);
+    f.range = (bit && isCtx) ? f.range - r0 : r0;
+    // adaptation as in the decoder step: (t + 31 P) >> 5, t = 31 or 4096
+    const bool isSig = S == 0u;
+    const uint32_t pn = ((bit ? 31u : 4096u) + vcg_mul24(31u, P)) >> 5;
+    ctx.st(h, pn);
     f.psig = isSig ? pn : f.psig;
     // renormalisation
     const bool sh = f.range < (1u << 24);
@@ -740,16 +747,14 @@ VCG_HD uint32_t vcg_efsm_step(vcg_efsm &f, Ctx &ctx, Lv &lv, uint8_t *out) {
         f.pos += sh ? 1u : 0u;                        // count: one byte per shift
     }
     f.range = sh ? f.range << 8 : f.range;
-    // symbol bookkeeping
-    const bool adv = (isSig & !bit) | (isTail & cz);
-    f.c = c + ((isEgp & bit) ? 1u : 0u) - ((isTail & !cz) ? 1u : 0u);
-    const uint32_t up = S + 1u < 13u ? S + 1u : 13u;
-    const uint32_t tn = bit ? up : (isSig ? 0u : 14u);
-    const uint32_t Sn = isTail ? (cz ? 0u : 14u) : tn;
-    f.S = Sn;
-    f.P = Sn == 0u ? f.psig : (Sn <= 12u ? R : VCG_P_BYPASS);
+    // symbol bookkeeping (the decoder's transitions)
+    const bool adv = (isSig & !bit) | (S == VCG_S_SIGN);
+    const uint32_t up = (bit || tail) ? S + 1u : 268u - (S > VCG_S_PFX ? S : VCG_S_PFX);
+    f.S = adv ? 0u : up;
+    f.P = adv ? f.psig : (bit ? R : VCG_P_BYPASS);
     f.p += adv ? 1u : 0u;
     f.v = adv ? vn : v;
+    if (!CHECK) return 0;
     return f.p == f.lim;
 }
 
@@ -774,6 +779,19 @@ VCG_HD void vcg_efsm_edge(vcg_efsm &f, const uint32_t *band_start, uint8_t *out)
     f.P = 2048u;
 }
 
+// VCG_EFSM_REFILL steps, tested for edges only when `check`
+template <bool WRITE, class Ctx, class Lv>
+VCG_HD void vcg_efsm_block(vcg_efsm &f, Ctx &ctx, Lv &lv, const uint32_t *band_start, uint8_t *out, bool check) {
+    if (check) {
+#pragma unroll
+        for (int k = 0; k < VCG_EFSM_REFILL; ++k)
+            if (vcg_efsm_step<true, WRITE>(f, ctx, lv, out)) vcg_efsm_edge<WRITE>(f, band_start, out);
+    } else {
+#pragma unroll
+        for (int k = 0; k < VCG_EFSM_REFILL; ++k) vcg_efsm_step<false, WRITE>(f, ctx, lv, out);
+    }
+}
+
 // the lane's level ring as a plain array (CPU reference)
 struct vcg_lv_array {
     int16_t v[32];
@@ -787,14 +805,16 @@ struct vcg_lv_array {
 };
 
 // Whole-atom encode with the state machine, driven as the kernels drive it
-// (CPU-side reference of the kernels' per-lane logic): a ring refill, then
-// VCG_EFSM_REFILL steps, the shared byte budget checked between groups.  Same
-// results as vcg_rc_encode_atom (out NULL: count only).
+// (CPU-side reference of the kernels' per-lane logic): a ring refill, then a
+// block of VCG_EFSM_REFILL steps, the shared byte budget checked between
+// blocks.  always_check tests for edges in every block, as a lane does when
+// another lane of its wave is near one.  Same results as vcg_rc_encode_atom
+// (out NULL: count only).
 VCG_HD uint32_t vcg_rc_encode_atom_fsm(const int16_t *L, uint32_t eob, const uint32_t *band_start, uint8_t *out,
                                        uint32_t cap, int *ovf, uint32_t *kb = nullptr, uint32_t budget = 0,
-                                       int *stopped = nullptr) {
+                                       int *stopped = nullptr, bool always_check = false) {
     vcg_ctx_array ctx;
-    for (int k = 0; k < 16; ++k) ctx.v[k] = 2048u;
+    for (uint32_t k = 0; k < VCG_CTX_SLOTS; ++k) ctx.v[k] = k == 13u || k == 15u ? VCG_P_BYPASS : 2048u;
     vcg_lv_array lv;
     vcg_efsm f;
     int stop = 0;
@@ -803,15 +823,14 @@ VCG_HD uint32_t vcg_rc_encode_atom_fsm(const int16_t *L, uint32_t eob, const uin
         vcg_efsm_init<true>(f, lv, L, eob, band_start, out, cap);
         while (!f.done) {
             vcg_efsm_refill(f, lv);
-            for (int k = 0; k < VCG_EFSM_REFILL; ++k)
-                if (vcg_efsm_step<true>(f, ctx, lv, out)) vcg_efsm_edge<true>(f, band_start, out);
+            vcg_efsm_block<true>(f, ctx, lv, band_start, out, always_check || vcg_efsm_near(f, VCG_EFSM_REFILL));
         }
     } else {
         vcg_efsm_init<false>(f, lv, L, eob, band_start, nullptr, cap);
         while (!f.done) {
             vcg_efsm_refill(f, lv);
-            for (int k = 0; k < VCG_EFSM_REFILL; ++k)
-                if (vcg_efsm_step<false>(f, ctx, lv, nullptr)) vcg_efsm_edge<false>(f, band_start, nullptr);
+            vcg_efsm_block<false>(f, ctx, lv, band_start, nullptr,
+                                  always_check || vcg_efsm_near(f, VCG_EFSM_REFILL));
             while (kb && !f.done && f.pos >= kbm) {
                 kbm += 1024u;
                 if (vcg_atomic_add(kb, 1024u) + 1024u > budget) { stop = 1; f.done = 1; f.lim = 0xFFFFFFFFu; }

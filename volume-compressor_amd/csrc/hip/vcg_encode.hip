@@ -229,13 +229,27 @@ __global__ __launch_bounds__(FDCT_THREADS) void k_quant_groups(const samp_group 
     if (tid == 0) eobs[row] = (uint32_t)seob;
 }
 
-// per-lane context memory (16 slots), slot-major over the 64 lanes of the
-// wave so a wave's access is conflict-free
+// per-lane context memory (VCG_CTX_SLOTS slots), slot-major over the 64
+// lanes of the wave so a wave's access is conflict-free.  m is the array (a
+// fixed LDS address), l4 the lane's byte offset (< 256); handles are byte
+// offsets, and sink (the sink slot's) is kept opaque to the compiler so that
+// picking a handle stays one select after the shift-or.
 struct enc_lds_ctx {
-    uint32_t *b;
-    __device__ __forceinline__ uint32_t ld(uint32_t k) const { return b[k * 64u]; }
-    __device__ __forceinline__ void st(uint32_t k, uint32_t v) { b[k * 64u] = v; }
+    uint32_t *m;
+    uint32_t l4, sink;
+    __device__ __forceinline__ uint32_t at(uint32_t S, bool isCtx) const { return isCtx ? (S << 8) | l4 : sink; }
+    __device__ __forceinline__ uint32_t ld_next(uint32_t h) const {
+        return *(const uint32_t *)((const char *)m + h + 256u);
+    }
+    __device__ __forceinline__ void st(uint32_t h, uint32_t v) { *(uint32_t *)((char *)m + h) = v; }
 };
+__device__ __forceinline__ enc_lds_ctx enc_ctx_init(uint32_t *ctxm) {
+    for (uint32_t k = 1; k < VCG_CTX_SLOTS; ++k)
+        ctxm[k * 64u + threadIdx.x] = k == 13u || k == 15u ? VCG_P_BYPASS : 2048u;
+    uint32_t sink = (14u << 8) | (threadIdx.x * 4u);
+    asm volatile("" : "+v"(sink));
+    return enc_lds_ctx{ctxm, threadIdx.x * 4u, sink};
+}
 // per-lane ring of four 8-level blocks (64 B), lanes 68 B apart so the lanes'
 // reads spread over the LDS banks
 #define ENC_LV_STRIDE 68u
@@ -257,7 +271,7 @@ __global__ __launch_bounds__(64) void k_rc_count_groups(const samp_group *__rest
                                                         const uint32_t *__restrict__ eobs,
                                                         const vcg_dtab *__restrict__ T, uint32_t *__restrict__ kb,
                                                         uint32_t *__restrict__ lens) {
-    __shared__ uint32_t ctxm[16 * 64];
+    __shared__ uint32_t ctxm[VCG_CTX_SLOTS * 64];
     __shared__ __align__(16) uint8_t lvm[64 * ENC_LV_STRIDE];
     __shared__ uint32_t bands[9];
     if (threadIdx.x < 9) bands[threadIdx.x] = T->band_start[threadIdx.x];
@@ -268,9 +282,8 @@ __global__ __launch_bounds__(64) void k_rc_count_groups(const samp_group *__rest
     const uint32_t gi = group_of(G, ng, r);
     const samp_group &g = G[gi];
     const bool zero = g.items[r - g.off].zero != 0;
-    enc_lds_ctx ctx{ctxm + threadIdx.x};
+    enc_lds_ctx ctx = enc_ctx_init(ctxm);
     enc_lds_lv lr{lvm + threadIdx.x * ENC_LV_STRIDE};
-    for (uint32_t k = 1; k <= 12; ++k) ctx.st(k, 2048u);
     vcg_efsm f;
     // lanes past the rows and all-zero atoms run as finished empty encodes
     vcg_efsm_init<false>(f, lr, lv + r * VCG_A3, live && !zero ? eobs[r] : 0u, bands, nullptr, VCG_RC_CAP);
@@ -280,9 +293,9 @@ __global__ __launch_bounds__(64) void k_rc_count_groups(const samp_group *__rest
     uint32_t *kg = kb + gi;
     while (__any(!f.done)) {
         vcg_efsm_refill(f, lr);
-#pragma unroll
-        for (int k = 0; k < VCG_EFSM_REFILL; ++k)
-            if (vcg_efsm_step<false>(f, ctx, lr, nullptr)) vcg_efsm_edge<false>(f, bands, nullptr);
+        // edge tests only where some lane of the wave can meet an edge (a
+        // wave-uniform branch)
+        vcg_efsm_block<false>(f, ctx, lr, bands, nullptr, __any(vcg_efsm_near(f, VCG_EFSM_REFILL)));
         // every 1024 bytes a lane adds to its group's total; past the budget
         // the group is abandoned
         if (!f.done && f.pos >= kbm) {
@@ -303,7 +316,7 @@ __global__ __launch_bounds__(64) void k_rc_encode(uint32_t n, const int16_t *__r
                                                   const vcg_dtab *__restrict__ T,
                                                   uint8_t *__restrict__ pay, uint32_t pay_cap,
                                                   uint32_t *__restrict__ lens) {
-    __shared__ uint32_t ctxm[16 * 64];
+    __shared__ uint32_t ctxm[VCG_CTX_SLOTS * 64];
     __shared__ __align__(16) uint8_t lvm[64 * ENC_LV_STRIDE];
     __shared__ uint32_t bands[9];
     if (threadIdx.x < 9) bands[threadIdx.x] = T->band_start[threadIdx.x];
@@ -316,9 +329,8 @@ __global__ __launch_bounds__(64) void k_rc_encode(uint32_t n, const int16_t *__r
     uint8_t *out = pay + (size_t)r * pay_cap;
     if (live && !zero) out[0] = (uint8_t)it.dc;
     const uint32_t cap = pay_cap - 1 < VCG_RC_CAP ? pay_cap - 1 : VCG_RC_CAP;
-    enc_lds_ctx ctx{ctxm + threadIdx.x};
+    enc_lds_ctx ctx = enc_ctx_init(ctxm);
     enc_lds_lv lr{lvm + threadIdx.x * ENC_LV_STRIDE};
-    for (uint32_t k = 1; k <= 12; ++k) ctx.st(k, 2048u);
     vcg_efsm f;
     // lanes past the items and all-zero atoms run as finished empty encodes
     // that write nothing
@@ -327,9 +339,7 @@ __global__ __launch_bounds__(64) void k_rc_encode(uint32_t n, const int16_t *__r
     if (!live || zero) f.done = 1;
     while (__any(!f.done)) {
         vcg_efsm_refill(f, lr);
-#pragma unroll
-        for (int k = 0; k < VCG_EFSM_REFILL; ++k)
-            if (vcg_efsm_step<true>(f, ctx, lr, out + 1)) vcg_efsm_edge<true>(f, bands, out + 1);
+        vcg_efsm_block<true>(f, ctx, lr, bands, out + 1, __any(vcg_efsm_near(f, VCG_EFSM_REFILL)));
     }
     if (!live) return;
     lens[i] = zero ? 0u : f.ovf ? 0xFFFFFFFFu : f.len + 1;
