@@ -230,13 +230,24 @@ __device__ __forceinline__ void store_atom(const dst_view &d, uint32_t ax, uint3
     }
 }
 
+// Workgroups are dealt to the 8 XCDs round robin (workgroup b to XCD b % 8),
+// and each XCD has its own L2.  Item i of a launch goes to workgroup b such
+// that every XCD takes a contiguous run of items, in order: atoms that are
+// neighbours along x then run side by side on one XCD, and their 32-byte row
+// pieces of a raster output meet in that XCD's L2 as whole lines.  A
+// bijection on [0, n).
+__device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
+    const uint32_t x = b & 7u, k = b >> 3, q = n >> 3, r = n & 7u;
+    return x * q + (x < r ? x : r) + k;
+}
+
 __global__ __launch_bounds__(IDCT_THREADS) void k_idct(dec_job j, const vcg_dtab *__restrict__ T,
                                                        const uint32_t *__restrict__ lists,
                                                        const vcg_item *__restrict__ items) {
     extern __shared__ int buf[];
     __shared__ unsigned long long l1sum;
     const int tid = threadIdx.x;
-    const uint32_t i = blockIdx.x;
+    const uint32_t i = xcd_item(blockIdx.x, gridDim.x);
     const uint32_t gi = j.base + i;
     int lod;
     uint32_t ax, ay, az;
