@@ -171,6 +171,7 @@ __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__res
 // inverse transform: phases from vcg_xform.hpp separated by barriers
 // ---------------------------------------------------------------------------
 #define IDCT_THREADS 512
+#define IDCT_PRE 10          // list entries per thread loaded ahead (5120 per atom)
 
 // where an item's voxels go: whole atom at out (box == false), or clipped to
 // a voxel box whose raster starts at out
@@ -252,6 +253,23 @@ __global__ __launch_bounds__(IDCT_THREADS) void k_idct(dec_job j, const vcg_dtab
     int lod;
     uint32_t ax, ay, az;
     item_atom(j, gi, lod, ax, ay, az);
+    // loads that do not wait on each other go out together: the item, the
+    // dequant tables of the atom's LOD (known from the job, not the item) and
+    // the list's first IDCT_PRE entries per thread (a list's allocation spans
+    // VCG_LIST_STRIDE entries whatever its count), so the chain of memory
+    // round trips before the scatter is two (item and list, then scan table)
+    // instead of four.  The tile and the tables stay within 133 KiB of LDS, so
+    // a transform workgroup fits on a CU beside four entropy workgroups.
+    const uint32_t *L = lists + (size_t)i * VCG_LIST_STRIDE;
+    uint32_t pre[IDCT_PRE];
+#pragma unroll
+    for (int q = 0; q < IDCT_PRE; ++q) pre[q] = L[tid + q * IDCT_THREADS];
+    __shared__ int32_t dqs[2 * VCG_NFREQ];
+    if (tid < VCG_NFREQ) {
+        const int o = j.mode == MODE_PAYLOAD ? 0 : lod * VCG_NFREQ;
+        dqs[tid] = j.dqs[o + tid];
+        dqs[VCG_NFREQ + tid] = j.dqb[o + tid];
+    }
     const vcg_item it = items[i];
     uint8_t *stage = (uint8_t *)buf;
     if (tid == 0 && j.status_out) j.status_out[gi] = it.status == 2 ? 1 : 0;
@@ -261,19 +279,23 @@ __global__ __launch_bounds__(IDCT_THREADS) void k_idct(dec_job j, const vcg_dtab
         store_atom(item_dst(j, gi), ax, ay, az, stage, tid);
         return;
     }
-    __shared__ int32_t dqs[2 * VCG_NFREQ];
-    {
-        const int32_t *st = j.dqs + (j.mode == MODE_PAYLOAD ? 0 : (int)it.lod * VCG_NFREQ);
-        const int32_t *bi = j.dqb + (j.mode == MODE_PAYLOAD ? 0 : (int)it.lod * VCG_NFREQ);
-        if (tid < VCG_NFREQ) { dqs[tid] = st[tid]; dqs[VCG_NFREQ + tid] = bi[tid]; }
-    }
+    const uint32_t cnt = it.cnt;
+    const int fmax = it.fmax;
+    uint32_t sidx[IDCT_PRE];
+#pragma unroll
+    for (int q = 0; q < IDCT_PRE; ++q)
+        sidx[q] = tid + q * IDCT_THREADS < (int)cnt ? T->scan[vcg_ent_pos(pre[q])] : 0u;
     vcg_tile_zero(buf, tid, IDCT_THREADS);
     if (tid == 0) l1sum = 0;
     __syncthreads();
-    const uint32_t cnt = it.cnt;
-    const uint32_t *L = lists + (size_t)i * VCG_LIST_STRIDE;
-    const int fmax = it.fmax;
-    const unsigned long long mine = vcg_scatter(buf, L, cnt, T->scan, dqs, dqs + VCG_NFREQ, tid, IDCT_THREADS);
+    const int32_t *st = dqs, *bi = dqs + VCG_NFREQ;
+    unsigned long long mine = 0;
+#pragma unroll
+    for (int q = 0; q < IDCT_PRE; ++q)
+        if (tid + q * IDCT_THREADS < (int)cnt) mine += vcg_scatter1(buf, pre[q], sidx[q], st, bi);
+    if (cnt > IDCT_PRE * IDCT_THREADS)
+        mine += vcg_scatter(buf, L + IDCT_PRE * IDCT_THREADS, cnt - IDCT_PRE * IDCT_THREADS, T->scan, st, bi, tid,
+                            IDCT_THREADS);
     if (mine) atomicAdd(&l1sum, mine);
     __syncthreads();
     // Every intermediate element is bounded by the atom's L2 norm (each 1-D

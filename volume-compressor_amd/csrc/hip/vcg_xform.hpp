@@ -170,9 +170,21 @@ VCG_HD void vcg_tile_zero(int *t, int tid, int nthr) {
     for (int k = tid; k < VCG_TILE_INTS / 4; k += nthr) q[k] = i4{0, 0, 0, 0};
 }
 
-// Dequantize entries [tid, cnt) step nthr into the tile (src/vc/vc.c:327-338);
-// returns this thread's sum of |coef| (for the 24-bit bound).  Entries go in
-// groups of 8 per thread so their list and scan-table loads overlap.
+// Dequantize list entry v, coefficient at raster index idx, into the tile
+// (src/vc/vc.c:327-338); returns |coef| (for the 24-bit bound).
+VCG_HD uint64_t vcg_scatter1(int *t, uint32_t v, uint32_t idx, const int32_t *st, const int32_t *bi) {
+    const int lv = vcg_ent_level(v);
+    const int f = vcg_freq(idx);
+    const uint32_t mag = (uint32_t)(lv < 0 ? -lv : lv);
+    int r = (int)(mag * (uint32_t)st[f] + (uint32_t)bi[f] + 128u) >> 8;
+    r = lv < 0 ? (int)(0u - (uint32_t)r) : r;
+    t[vcg_tidx((int)(idx >> 10), (int)((idx >> 5) & 31), (int)(idx & 31))] = r;
+    return (uint64_t)(r < 0 ? -(int64_t)r : (int64_t)r);
+}
+
+// Dequantize entries [tid, cnt) step nthr into the tile; returns this
+// thread's sum of |coef|.  Entries go in groups of 8 per thread so their list
+// and scan-table loads overlap.
 VCG_HD uint64_t vcg_scatter(int *t, const uint32_t *L, uint32_t cnt, const uint16_t *scan, const int32_t *st,
                             const int32_t *bi, int tid, int nthr) {
     uint64_t mine = 0;
@@ -186,13 +198,7 @@ VCG_HD uint64_t vcg_scatter(int *t, const uint32_t *L, uint32_t cnt, const uint1
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             if (e0 + q * un >= cnt) break;
-            const int lv = vcg_ent_level(v[q]);
-            const int f = vcg_freq(idx[q]);
-            const uint32_t mag = (uint32_t)(lv < 0 ? -lv : lv);
-            int r = (int)(mag * (uint32_t)st[f] + (uint32_t)bi[f] + 128u) >> 8;
-            r = lv < 0 ? (int)(0u - (uint32_t)r) : r;
-            t[vcg_tidx((int)(idx[q] >> 10), (int)((idx[q] >> 5) & 31), (int)(idx[q] & 31))] = r;
-            mine += (uint64_t)(r < 0 ? -(int64_t)r : (int64_t)r);
+            mine += vcg_scatter1(t, v[q], idx[q], st, bi);
         }
     }
     return mine;
