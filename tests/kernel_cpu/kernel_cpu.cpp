@@ -52,18 +52,15 @@ void kc_quant(const int32_t *c, size_t n, float q, float g, int32_t *out) {
 int kc_fdct(const uint8_t *vol, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t ax, uint32_t ay, uint32_t az,
             const int32_t cmat[32][32], const uint16_t *scan, const float *gstep, float q, int16_t *lv,
             int32_t *coef_raster, uint32_t *eob, uint32_t *dc_out) {
-    vcg_fwd_tab F;
-    for (int n = 0; n < 16; ++n)
-        for (int j = 0; j < 16; ++j) { F.e[n][j] = cmat[2 * j][n]; F.o[n][j] = cmat[2 * j + 1][n]; }
     int *t = (int *)calloc(VCG_TILE_INTS, sizeof(int));
     uint32_t sum = 0;
     for (int tid = 0; tid < NT; ++tid) sum += vcg_gather(t, vol, nx, ny, nz, ax, ay, az, tid, NT);
     if (sum == 0) { free(t); *eob = 0; *dc_out = 0; return 1; }
     const int dc = (int)((sum + 16384u) / 32768u);
     for (int tid = 0; tid < NT; ++tid) vcg_level_shift(t, dc, tid, NT);
-    for (int tid = 0; tid < NT; ++tid) vcg_fwd_pass_x(t, &F, tid, NT);
-    for (int tid = 0; tid < NT; ++tid) vcg_fwd_pass_y(t, &F, tid, NT);
-    for (int tid = 0; tid < NT; ++tid) vcg_fwd_pass_z(t, &F, tid, NT);
+    for (int tid = 0; tid < NT; ++tid) vcg_fwd_pass_x(t, cmat, tid, NT);
+    for (int tid = 0; tid < NT; ++tid) vcg_fwd_pass_y(t, cmat, tid, NT);
+    for (int tid = 0; tid < NT; ++tid) vcg_fwd_pass_z(t, cmat, tid, NT);
     if (coef_raster)
         for (int k = 0; k < 32768; ++k) coef_raster[k] = t[vcg_tidx(k >> 10, (k >> 5) & 31, k & 31)];
     int last = 0;

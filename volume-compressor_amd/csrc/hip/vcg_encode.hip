@@ -2,7 +2,7 @@
 //
 //   k_downsample : LOD l -> l+1, integer rounded mean of the in-bounds voxels
 //                  of each 2x2x2 cell (src/vc/vc.c:1562-1577).
-//   k_fdct       : one 256-thread workgroup per atom, atoms of one or many
+//   k_fdct       : one 512-thread workgroup per atom, atoms of one or many
 //                  volumes per launch.  Gathers the 32^3 atom (zero padded),
 //                  sums it (zero atom -> no payload, dc = (sum+16384)/32768,
 //                  src/vc/vc.c:599-604), runs the three forward
@@ -28,6 +28,7 @@
 #include "vcg_xform.hpp"
 
 #define FDCT_THREADS 256
+#define FDCT_TX 512          // k_fdct: two waves per SIMD, one line pair per thread and pass
 
 struct enc_item {
     uint32_t eob;
@@ -117,11 +118,10 @@ enum { FDCT_STORE_COEF = 0, FDCT_QUANT = 1 };
 // and items to items_out, both indexed by i; FDCT_STORE_COEF: coefficients
 // (scan order) and items to the volume's sample arrays, indexed by the
 // atom's place in its list.
-__global__ __launch_bounds__(FDCT_THREADS) void k_fdct(const enc_src *__restrict__ srcs,
+__global__ __launch_bounds__(FDCT_TX) void k_fdct(const enc_src *__restrict__ srcs,
                                                        const enc_seg *__restrict__ segs, uint32_t nseg,
                                                        uint64_t base, int mode,
                                                        const vcg_dtab *__restrict__ T,
-                                                       const vcg_fwd_tab *__restrict__ F,
                                                        int16_t *__restrict__ lv_out,
                                                        enc_item *__restrict__ items_out) {
     extern __shared__ int buf[];
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(FDCT_THREADS) void k_fdct(const enc_src *__restrict
     enc_item *items = store ? s.sitems + gi : items_out + i;
     if (tid == 0) { ssum = 0; seob = 0; }
     __syncthreads();
-    const uint32_t part = vcg_gather(buf, s.lod[lod], s.nx[lod], s.ny[lod], s.nz[lod], ax, ay, az, tid, FDCT_THREADS);
+    const uint32_t part = vcg_gather(buf, s.lod[lod], s.nx[lod], s.ny[lod], s.nz[lod], ax, ay, az, tid, FDCT_TX);
     if (part) atomicAdd(&ssum, part);
     __syncthreads();
     const uint32_t sum = ssum;
@@ -148,24 +148,24 @@ __global__ __launch_bounds__(FDCT_THREADS) void k_fdct(const enc_src *__restrict
         return;
     }
     const int dc = (int)((sum + 16384u) / 32768u);
-    vcg_level_shift(buf, dc, tid, FDCT_THREADS);
+    vcg_level_shift(buf, dc, tid, FDCT_TX);
     __syncthreads();
-    vcg_fwd_pass_x(buf, F, tid, FDCT_THREADS);
+    vcg_fwd_pass_x(buf, T->cmat, tid, FDCT_TX);
     __syncthreads();
-    vcg_fwd_pass_y(buf, F, tid, FDCT_THREADS);
+    vcg_fwd_pass_y(buf, T->cmat, tid, FDCT_TX);
     __syncthreads();
-    vcg_fwd_pass_z(buf, F, tid, FDCT_THREADS);
+    vcg_fwd_pass_z(buf, T->cmat, tid, FDCT_TX);
     __syncthreads();
     if (store) {
         int32_t *co = s.coef + gi * VCG_A3;
-        for (int p = tid; p < 32768; p += FDCT_THREADS) {
+        for (int p = tid; p < 32768; p += FDCT_TX) {
             const uint32_t idx = T->scan[p];
             co[p] = buf[vcg_tidx((int)(idx >> 10), (int)((idx >> 5) & 31), (int)(idx & 31))];
         }
         if (tid == 0) { enc_item it = {0, (uint32_t)dc, 0, (uint32_t)lod}; *items = it; }
         return;
     }
-    const int last = vcg_quant_scan(buf, T->scan, T->gstep, s.q[lod], lv_out + (size_t)i * VCG_A3, tid, FDCT_THREADS);
+    const int last = vcg_quant_scan(buf, T->scan, T->gstep, s.q[lod], lv_out + (size_t)i * VCG_A3, tid, FDCT_TX);
     if (last) atomicMax(&seob, last);
     __syncthreads();
     if (tid == 0) { enc_item it = {(uint32_t)seob, (uint32_t)dc, 0, (uint32_t)lod}; *items = it; }
@@ -386,19 +386,11 @@ __global__ void k_assemble(const uint8_t *__restrict__ pool, const uint64_t *__r
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-static vcg_fwd_tab *g_ftab = nullptr;
 static const size_t LDS_BYTES = (size_t)VCG_TILE_INTS * 4;
 
 static void enc_setup() {
     static bool done = false;
     if (done) return;
-    vcg_dtab h;
-    vcg_copy(&h, g_dtab, sizeof h, 2);
-    vcg_fwd_tab f;
-    for (int n = 0; n < 16; ++n)
-        for (int j = 0; j < 16; ++j) { f.e[n][j] = h.cmat[2 * j][n]; f.o[n][j] = h.cmat[2 * j + 1][n]; }
-    VCG_CHECK(hipMalloc(&g_ftab, sizeof f));
-    vcg_copy(g_ftab, &f, sizeof f, 1);
     VCG_CHECK(hipFuncSetAttribute((const void *)k_fdct, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_BYTES));
     done = true;
 }
@@ -641,8 +633,8 @@ extern "C" int vcg_enc_samples(int nv, vcg_enc *const *es, const uint32_t *const
     enc_src *dsrc; enc_seg *dseg;
     const uint64_t tot = upload_sources(c, nv, es, nullptr, true, host, seg, &dsrc, &dseg);
     if (tot) {
-        hipLaunchKernelGGL(k_fdct, dim3((uint32_t)tot), dim3(FDCT_THREADS), LDS_BYTES, c->s, dsrc, dseg,
-                           (uint32_t)seg.size(), (uint64_t)0, (int)FDCT_STORE_COEF, g_dtab, g_ftab,
+        hipLaunchKernelGGL(k_fdct, dim3((uint32_t)tot), dim3(FDCT_TX), LDS_BYTES, c->s, dsrc, dseg,
+                           (uint32_t)seg.size(), (uint64_t)0, (int)FDCT_STORE_COEF, g_dtab,
                            (int16_t *)nullptr, (enc_item *)nullptr);
         VCG_CHECK(hipGetLastError());
     }
@@ -763,8 +755,8 @@ extern "C" int vcg_enc_all(int nv, vcg_enc *const *es, const float (*qs)[VCG_NLO
     size_t sg = 0;
     for (uint64_t b = 0; b < tot; b += maxb) {
         const uint32_t n = (uint32_t)(tot - b < maxb ? tot - b : maxb);
-        hipLaunchKernelGGL(k_fdct, dim3(n), dim3(FDCT_THREADS), LDS_BYTES, c->s, dsrc, dseg,
-                           (uint32_t)(seg.size() - 1), b, (int)FDCT_QUANT, g_dtab, g_ftab, S.lv, S.items);
+        hipLaunchKernelGGL(k_fdct, dim3(n), dim3(FDCT_TX), LDS_BYTES, c->s, dsrc, dseg,
+                           (uint32_t)(seg.size() - 1), b, (int)FDCT_QUANT, g_dtab, S.lv, S.items);
         VCG_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_rc_encode, dim3((n + 63) / 64), dim3(64), 0, c->s, n, S.lv, S.items, g_dtab, S.pay,
                            (uint32_t)VCG_PAY_CAP, S.lens);

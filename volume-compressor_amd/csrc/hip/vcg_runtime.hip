@@ -42,8 +42,9 @@ extern "C" int vcg_init(const vcg_tables *t) {
                      lds_block, lds_optin, need);
             vc__panic(msg);
         }
-        // the inverse transform regroups its even-row sums by these exact
-        // symmetries of the basis (they hold for the Q14 table by construction)
+        // the forward and inverse transforms regroup their even-row sums by
+        // these exact symmetries of the basis (they hold for the Q14 table by
+        // construction)
         for (int k = 0; k < 32; ++k)
             for (int n = 0; n < 16; ++n) {
                 const int sgn = (k & 1) ? -1 : 1;

@@ -281,43 +281,142 @@ VCG_HD void vcg_inv_stage(uint8_t *stage, const uint8_t *vox, int tid) {
 // ===========================================================================
 // forward
 // ===========================================================================
-// Basis in forward order: E[n][j] = C[2j][n], O[n][j] = C[2j+1][n].
-struct vcg_fwd_tab { int32_t e[16][16]; int32_t o[16][16]; };
-
-// One forward line (element n at src[n*stride]):
+// Two forward lines (element n of line i at s_i[n*stride]), in place:
 //   s[n] = v[n]+v[31-n], d[n] = v[n]-v[31-n]  (n < 16)
 //   out[2j] = (sum_n C[2j][n] s[n] + 8192) >> 14, out[2j+1] likewise with d
-// (src/vc/vc.c:148-169).  Operands stay below 2^16 in magnitude (|v| <= 255 in
-// pass 1, <= 2039 in pass 2, <= 16293 in pass 3), so 24-bit multiplies are exact.
-VCG_HD void vcg_fwd_line(const vcg_fwd_tab *F, int *src, int stride) {
-    int ae[16], ao[16];
+// (src/vc/vc.c:148-169), i32 wrap.  The even outputs are regrouped exactly
+// (integer sums, mod 2^32) by the even-row symmetries of the basis,
+// C[2j][15-n] = (-1)^j C[2j][n], C[4m][7-n] = (-1)^m C[4m][n], C[8p][3-n] =
+// (-1)^p C[8p][n] (checked on the table at device init), as the inverse does:
+// 344 multiply-adds per line instead of 512.  Operands stay below 2^23 in
+// magnitude (|v| <= 255 in pass 1, <= 2039 in pass 2, <= 16293 in pass 3, and
+// the regrouped sums at most 16 times that), so 24-bit multiplies are exact.
+// Output rows go four at a time: the two lines share each basis row (a
+// workgroup-uniform scalar load), and the eight sums are independent chains.
+VCG_HD void vcg_fwd_line2(const int32_t (*C)[32], int *s0, int *s1, int stride) {
+    struct row16 { int32_t v[16]; };
+    struct row8 { int32_t v[8]; };
+    struct row4 { int32_t v[4]; };
+    struct row2 { int32_t v[2]; };
+    int d0[16], d1[16], sd0[8], sd1[8], t0[4], t1[4], u0[4], u1[4];
+    {
+        int a0[16], a1[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) { ae[j] = 0; ao[j] = 0; }
-#pragma unroll 1
-    for (int n = 0; n < 16; ++n) {
-        const int a = src[n * stride], b = src[(31 - n) * stride];
-        const int s = a + b, d = a - b;
+        for (int n = 0; n < 16; ++n) {
+            const int p0 = s0[n * stride], q0 = s0[(31 - n) * stride];
+            const int p1 = s1[n * stride], q1 = s1[(31 - n) * stride];
+            a0[n] = p0 + q0; d0[n] = p0 - q0;
+            a1[n] = p1 + q1; d1[n] = p1 - q1;
+        }
+        int b0[8], b1[8];
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            ae[j] += vcg_mul24(F->e[n][j], s);
-            ao[j] += vcg_mul24(F->o[n][j], d);
+        for (int n = 0; n < 8; ++n) {
+            b0[n] = a0[n] + a0[15 - n]; sd0[n] = a0[n] - a0[15 - n];       // sd: rows 4m+2
+            b1[n] = a1[n] + a1[15 - n]; sd1[n] = a1[n] - a1[15 - n];
+        }
+        int c0[4], c1[4];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            c0[n] = b0[n] + b0[7 - n]; t0[n] = b0[n] - b0[7 - n];           // t: rows 8p+4
+            c1[n] = b1[n] + b1[7 - n]; t1[n] = b1[n] - b1[7 - n];
+        }
+#pragma unroll
+        for (int n = 0; n < 2; ++n) {
+            u0[n] = c0[n] + c0[3 - n]; u0[2 + n] = c0[n] - c0[3 - n];       // u: rows 16q | 16q+8
+            u1[n] = c1[n] + c1[3 - n]; u1[2 + n] = c1[n] - c1[3 - n];
         }
     }
+    // odd rows 2j+1: sum_{n<16} C[2j+1][n] d[n]
+#pragma unroll 1
+    for (int j0 = 0; j0 < 16; j0 += 4) {
+        int a0[4], a1[4];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        src[(2 * j) * stride] = (int)((uint32_t)ae[j] + 8192u) >> 14;
-        src[(2 * j + 1) * stride] = (int)((uint32_t)ao[j] + 8192u) >> 14;
+        for (int q = 0; q < 4; ++q) {
+            const row16 r = *(const row16 *)C[2 * (j0 + q) + 1];
+            a0[q] = 8192; a1[q] = 8192;
+#pragma unroll
+            for (int n = 0; n < 16; ++n) {
+                a0[q] = vcg_acc<true>(a0[q], r.v[n], d0[n]);
+                a1[q] = vcg_acc<true>(a1[q], r.v[n], d1[n]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            s0[(2 * (j0 + q) + 1) * stride] = a0[q] >> 14;
+            s1[(2 * (j0 + q) + 1) * stride] = a1[q] >> 14;
+        }
+    }
+    // rows 4m+2: sum_{n<8} C[4m+2][n] sd[n]
+#pragma unroll 1
+    for (int m0 = 0; m0 < 8; m0 += 4) {
+        int a0[4], a1[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const row8 r = *(const row8 *)C[4 * (m0 + q) + 2];
+            a0[q] = 8192; a1[q] = 8192;
+#pragma unroll
+            for (int n = 0; n < 8; ++n) {
+                a0[q] = vcg_acc<true>(a0[q], r.v[n], sd0[n]);
+                a1[q] = vcg_acc<true>(a1[q], r.v[n], sd1[n]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            s0[(4 * (m0 + q) + 2) * stride] = a0[q] >> 14;
+            s1[(4 * (m0 + q) + 2) * stride] = a1[q] >> 14;
+        }
+    }
+    // rows 8p+4 from t (n < 4); rows 16q and 16q+8 from u (n < 2)
+    {
+        int a0[4], a1[4], g0[4], g1[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const row4 r = *(const row4 *)C[8 * p + 4];
+            a0[p] = 8192; a1[p] = 8192;
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                a0[p] = vcg_acc<true>(a0[p], r.v[n], t0[n]);
+                a1[p] = vcg_acc<true>(a1[p], r.v[n], t1[n]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {            // q < 2: row 16q from u[0..1]; else row 16(q-2)+8 from u[2..3]
+            const int k = q < 2 ? 16 * q : 16 * (q - 2) + 8, o = q < 2 ? 0 : 2;
+            const row2 r = *(const row2 *)C[k];
+            g0[q] = 8192; g1[q] = 8192;
+#pragma unroll
+            for (int n = 0; n < 2; ++n) {
+                g0[q] = vcg_acc<true>(g0[q], r.v[n], u0[o + n]);
+                g1[q] = vcg_acc<true>(g1[q], r.v[n], u1[o + n]);
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            s0[(8 * p + 4) * stride] = a0[p] >> 14;
+            s1[(8 * p + 4) * stride] = a1[p] >> 14;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = q < 2 ? 16 * q : 16 * (q - 2) + 8;
+            s0[k * stride] = g0[q] >> 14;
+            s1[k * stride] = g1[q] >> 14;
+        }
     }
 }
 
-VCG_HD void vcg_fwd_pass_x(int *t, const vcg_fwd_tab *F, int tid, int nthr) {
-    for (int line = tid; line < 1024; line += nthr) vcg_fwd_line(F, t + vcg_tidx(line >> 5, line & 31, 0), VCG_SX);
+// The passes give each thread the line pairs (line, line + nthr) for line =
+// tid, tid + 2*nthr, ... (nthr divides 512).
+VCG_HD void vcg_fwd_pass_x(int *t, const int32_t (*C)[32], int tid, int nthr) {
+    for (int l = tid; l < 1024; l += 2 * nthr)
+        vcg_fwd_line2(C, t + vcg_tidx(l >> 5, l & 31, 0), t + vcg_tidx((l + nthr) >> 5, (l + nthr) & 31, 0), VCG_SX);
 }
-VCG_HD void vcg_fwd_pass_y(int *t, const vcg_fwd_tab *F, int tid, int nthr) {
-    for (int line = tid; line < 1024; line += nthr) vcg_fwd_line(F, t + vcg_tidx(line >> 5, 0, line & 31), VCG_SY);
+VCG_HD void vcg_fwd_pass_y(int *t, const int32_t (*C)[32], int tid, int nthr) {
+    for (int l = tid; l < 1024; l += 2 * nthr)
+        vcg_fwd_line2(C, t + vcg_tidx(l >> 5, 0, l & 31), t + vcg_tidx((l + nthr) >> 5, 0, (l + nthr) & 31), VCG_SY);
 }
-VCG_HD void vcg_fwd_pass_z(int *t, const vcg_fwd_tab *F, int tid, int nthr) {
-    for (int line = tid; line < 1024; line += nthr) vcg_fwd_line(F, t + vcg_tidx(0, line >> 5, line & 31), VCG_SZ);
+VCG_HD void vcg_fwd_pass_z(int *t, const int32_t (*C)[32], int tid, int nthr) {
+    for (int l = tid; l < 1024; l += 2 * nthr)
+        vcg_fwd_line2(C, t + vcg_tidx(0, l >> 5, l & 31), t + vcg_tidx(0, (l + nthr) >> 5, (l + nthr) & 31), VCG_SZ);
 }
 
 // gather one zero-padded atom of an (nx,ny,nz) volume into the tile; returns
