@@ -3,6 +3,7 @@ vc_testhooks.h).  Thin: every call goes straight to the C entry point; there
 is no Python or CPU codec behind it.  Raises if libvc.so is not built.
 """
 import ctypes as C
+import time
 import os
 
 import numpy as np
@@ -138,16 +139,22 @@ def encode_q(vol, target_ratio, qs):
     return st, (_take(p, n.value) if st == VC_OK else None)
 
 
-def encode_device(dev_ptr, dims, target_ratio):
+def encode_device(dev_ptr, dims, target_ratio, timed=False):
+    """vcx_encode_device; with timed, also the seconds spent in the C call
+    (the archive's copy into a Python bytes object excluded)."""
     p, n = C.c_void_p(), C.c_size_t()
+    t0 = time.perf_counter()
     st = lib().vcx_encode_device(dev_ptr, Dims(*dims), C.c_float(target_ratio), C.byref(p), C.byref(n))
-    return st, (_take(p, n.value) if st == VC_OK else None)
+    t = time.perf_counter() - t0
+    r = (st, (_take(p, n.value) if st == VC_OK else None))
+    return r + (t,) if timed else r
 
 
-def encode_batch(vols, target_ratios, on_device=False, dims=None):
+def encode_batch(vols, target_ratios, on_device=False, dims=None, timed=False):
     """vcx_encode_batch: vols are numpy u8 volumes (z, y, x), or device
     pointers with dims [(nx, ny, nz)] when on_device.  Returns (rc, [(status,
-    archive bytes or None)])."""
+    archive bytes or None)]), and with timed also the seconds spent in the C
+    call (the archives' copies into Python bytes objects excluded)."""
     n = len(vols)
     vp = C.c_void_p
     if on_device:
@@ -162,8 +169,10 @@ def encode_batch(vols, target_ratios, on_device=False, dims=None):
     outs = (vp * n)()
     lens = np.zeros(n, np.uint64)
     st = np.zeros(n, np.int32)
+    t0 = time.perf_counter()
     rc = lib().vcx_encode_batch(n, C.cast(ptrs, vp), 1 if on_device else 0, C.cast(dd, vp), t.ctypes.data,
                                 C.cast(outs, vp), lens.ctypes.data, st.ctypes.data)
+    tc = time.perf_counter() - t0
     res = []
     for i in range(n):
         if st[i] == VC_OK:
@@ -171,7 +180,7 @@ def encode_batch(vols, target_ratios, on_device=False, dims=None):
         else:
             res.append((int(st[i]), None))
     del keep
-    return rc, res
+    return (rc, res, tc) if timed else (rc, res)
 
 
 def test_rc_rows(rows):
