@@ -180,7 +180,10 @@ VCG_HD int vcg_rc_decode_atom(const uint8_t *s, uint32_t n, const uint32_t *band
 // ring and advances cnt only on an emission; every 16 steps the caller copies
 // ring slots [flushed, flushed+6) to the list (at most 6 emissions fit in 16
 // steps: a coefficient takes at least 3 bins), so list stores are unmasked and
-// entries past cnt are overwritten by the next flush.
+// entries past cnt are overwritten by the next flush.  The position and cnt
+// share one register: position << 17 | cnt mod 64 << 8, so one add advances
+// both; the flush counts its new entries as cnt - flushed mod 8 and clears the
+// bits of cnt above its ring slot.
 // ---------------------------------------------------------------------------
 #ifndef VCG_ENTROPY_FSM
 #define VCG_ENTROPY_FSM 1
@@ -196,6 +199,7 @@ VCG_HD int vcg_rc_decode_atom(const uint8_t *s, uint32_t n, const uint32_t *band
 #define VCG_S_TAIL 128u
 #define VCG_S_SIGN 255u
 #define VCG_CTX_SLOTS 16u
+#define VCG_POS_MASK 0xFFFE0000u
 
 // mask keeping the stream bytes of the 8 at byte E (those before s[n])
 VCG_HD uint64_t vcg_tail_mask(uint32_t E, uint32_t n) {
@@ -220,9 +224,10 @@ struct vcg_fsm {
     uint64_t w;                  // window, wb valid bits, ending at stream byte E
     uint32_t wb, E;
     uint32_t P, S, psig, acc;
-    uint32_t pos17, lim17;       // scan position and next band edge / EOB, << 17 (mod 2^32)
+    uint32_t pc;                 // scan position << 17 (mod 2^32) | emitted count mod 64 << 8
+    uint32_t lim17;              // next band edge / EOB << 17 (mod 2^32)
     uint32_t eob;
-    uint32_t cnt8, fl8;          // emitted / flushed entry counts, << 8 (ring slot offsets)
+    uint32_t fl;                 // entries flushed to the list
     uint32_t band, bad;
 };
 
@@ -294,7 +299,7 @@ VCG_HD void vcg_fsm_init(vcg_fsm &f, vcg_look &A, vcg_look &B, const uint8_t *s,
     B = A;                                            // both at the window end (13)
     eob = eob > 32768u ? 32768u : eob;                // vc.c:549
     f.eob = eob;
-    f.P = 2048u; f.S = 0; f.psig = 2048u; f.acc = 0; f.pos17 = 0; f.cnt8 = 0; f.fl8 = 0;
+    f.P = 2048u; f.S = 0; f.psig = 2048u; f.acc = 0; f.pc = 0; f.fl = 0;
     f.band = 0; f.bad = 0;
     f.lim17 = (band_start[1] < eob ? band_start[1] : eob) << 17;   // EOB 32768 wraps to 0: met only at the end
     if (eob == 0) vcg_fsm_park(f);
@@ -303,7 +308,7 @@ VCG_HD void vcg_fsm_init(vcg_fsm &f, vcg_look &A, vcg_look &B, const uint8_t *s,
 // true when the lane may meet a band edge, EOB or the prefix cap within the
 // next k steps
 VCG_HD bool vcg_fsm_near(const vcg_fsm &f, uint32_t k) {
-    return (f.lim17 - f.pos17 <= (k << 17)) | (f.S - (VCG_S_BAD - k) < VCG_S_TAIL - VCG_S_BAD + k);
+    return (f.lim17 - (f.pc & VCG_POS_MASK) <= (k << 17)) | (f.S - (VCG_S_BAD - k) < VCG_S_TAIL - VCG_S_BAD + k);
 }
 
 // one bin.  Ctx: h = at(S, S < 13) the handle of the slot a bin in state S
@@ -333,22 +338,23 @@ VCG_HD uint32_t vcg_fsm_step(vcg_fsm &f, Ctx &ctx, Ring &ring) {
     const bool tail = S >= VCG_S_TAIL;
     const bool sgn = S == VCG_S_SIGN;
     const uint32_t acc2 = 2u * f.acc + (bit ? 0u - 10u : 0u - 11u);
-    ring.st(f.cnt8 & ((VCG_RING - 1u) << 8), f.pos17 | (acc2 & 0x1FFFFu));
-    f.cnt8 += sgn ? 256u : 0u;
+    ring.st(f.pc & ((VCG_RING - 1u) << 8), (f.pc & VCG_POS_MASK) | (acc2 & 0x1FFFFu));
     const bool adv = (isSig & !bit) | sgn;
-    f.pos17 += adv ? 0x20000u : 0u;
+    // one add moves the position (every new position) and the ring slot (an
+    // emission, which is also a new position)
+    f.pc += sgn ? 0x20100u : adv ? 0x20000u : 0u;
     f.acc = tail ? acc2 : (S < 12u ? S : 12u);
     const uint32_t up = (bit || tail) ? S + 1u : 268u - (S > VCG_S_PFX ? S : VCG_S_PFX);
     f.S = adv ? 0u : up;
     f.P = adv ? f.psig : (bit ? R : VCG_P_BYPASS);
     if (!CHECK) return 0;
-    return (f.pos17 == f.lim17) | (f.S - VCG_S_BAD < VCG_S_TAIL - VCG_S_BAD);
+    return ((f.pc & VCG_POS_MASK) == f.lim17) | (f.S - VCG_S_BAD < VCG_S_TAIL - VCG_S_BAD);
 }
 
 // band boundary / EOB / Exp-Golomb cap for a lane whose step returned nonzero
 VCG_HD void vcg_fsm_edge(vcg_fsm &f, const uint32_t *band_start) {
     const bool bad = f.S - VCG_S_BAD < VCG_S_TAIL - VCG_S_BAD;
-    const bool end = bad || f.pos17 == f.eob << 17;
+    const bool end = bad || (f.pc & VCG_POS_MASK) == f.eob << 17;
     // a new band starts from the reset context (vc.c:552-555) and its first
     // bin is a significance bin; a parked lane runs on the same values
     f.psig = 2048u;
@@ -381,11 +387,12 @@ VCG_HD void vcg_fsm_block(vcg_fsm &f, Ctx &ctx, Ring &ring, const uint32_t *band
 // slack entries past the list capacity).  Returns the number of real entries.
 template <class Ring>
 VCG_HD uint32_t vcg_fsm_flush(vcg_fsm &f, Ring &ring, uint32_t *out) {
-    const uint32_t pending = (f.cnt8 - f.fl8) >> 8;
-    uint32_t *o = out + (f.fl8 >> 8);
+    const uint32_t pending = ((f.pc >> 8) - f.fl) & (VCG_RING - 1u);    // at most VCG_FLUSH_N < VCG_RING
+    uint32_t *o = out + f.fl;
 #pragma unroll
-    for (uint32_t k = 0; k < VCG_FLUSH_N; ++k) o[k] = ring.ld((f.fl8 + (k << 8)) & ((VCG_RING - 1u) << 8));
-    f.fl8 = f.cnt8;
+    for (uint32_t k = 0; k < VCG_FLUSH_N; ++k) o[k] = ring.ld(((f.fl + k) << 8) & ((VCG_RING - 1u) << 8));
+    f.fl += pending;
+    f.pc &= ~0x1F800u;                                // keep the emission count field below bit 17
     return pending;
 }
 
@@ -425,7 +432,7 @@ VCG_HD int vcg_rc_decode_atom_fsm(const uint8_t *s, uint32_t n, const uint32_t *
         vcg_fsm_block(f, ctx, ring, band_start, always_check || vcg_fsm_near(f, VCG_REFILL_STEPS));
     }
     if (vcg_fsm_flush(f, ring, out) > VCG_FLUSH_N) over = 1;
-    *cnt_out = f.bad ? 0 : f.cnt8 >> 8;
+    *cnt_out = f.bad ? 0 : f.fl;
     return over ? 2 : (int)f.bad;
 }
 

@@ -149,9 +149,9 @@ __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__res
     }
     vcg_fsm_flush(f, ring, out);
     const int bad = (int)f.bad;
-    const uint32_t cnt = f.cnt8 >> 8;
+    const uint32_t cnt = f.fl;
     // the last entry (highest scan position) is still in the ring
-    const int fmax = cnt ? vcg_freq(T->scan[vcg_ent_pos(ring.ld((f.cnt8 - 256u) & ((VCG_RING - 1u) << 8)))]) : -1;
+    const int fmax = cnt ? vcg_freq(T->scan[vcg_ent_pos(ring.ld(((cnt - 1u) << 8) & ((VCG_RING - 1u) << 8)))]) : -1;
 #else
     (void)ctxm; (void)ringm; (void)bands;
     uint32_t cnt = 0;
