@@ -177,7 +177,7 @@ def req_bytes(r):
     return (x1 - x0) * (y1 - y0) * (z1 - z0)
 
 
-def serve_bench(pyvc, a, reqs, qps, seed=11, device_bytes=6 << 30):
+def serve_bench(pyvc, a, reqs, qps, seed=11, device_bytes=16 << 30):
     """Open-loop Poisson trace of `reqs` at `qps` through the continuous-batching
     server.  Latency = completion (server clock) - scheduled arrival.  Three
     served requests are checked against the plain per-call path."""

@@ -106,8 +106,11 @@ vc_status vcx_synth_device(uint64_t seed, vc_dims dims, int nsheets, void *dev_o
  * concurrent atom and region requests decoded together.
  *
  * vcx_server_create keeps the archive on the device and starts a worker
- * thread.  device_bytes bounds the output memory of the batches in flight
- * (three batch slots share it; a request larger than one slot is rejected).
+ * thread.  device_bytes bounds the device memory of the batches in flight:
+ * three batch slots share it at vcx_server_atom_bytes() per atom of slot
+ * capacity (output, request staging and decoder scratch, about 160 KiB),
+ * all of it allocated before create returns (NULL when it does not fit); a
+ * request larger than one slot is rejected.
  * Requests are admitted in arrival order as slots free up; batches overlap on
  * the device.  Submit returns a request id (>= 0) or -1 (invalid request or
  * too large).  Atom requests: quads = (lod, ax, ay, az) x n, out receives n
@@ -126,6 +129,7 @@ int64_t     vcx_server_submit_region(vcx_server *s, int lod, vc_box box, uint8_t
 size_t      vcx_server_poll(vcx_server *s, int64_t *ids, int32_t *status, uint64_t *t_done_ns, size_t max,
                             int wait_ms);
 size_t      vcx_server_capacity(const vcx_server *s);   /* atoms per batch slot */
+size_t      vcx_server_atom_bytes(void);                /* device bytes per atom of capacity */
 uint64_t    vcx_now_ns(void);
 void       *vcx_host_alloc(size_t bytes);               /* pinned host memory */
 void        vcx_host_free(void *p);

@@ -81,7 +81,7 @@ def test_server_mix_equals_alone(pyvc, served):
     rng = np.random.default_rng(3)
     reqs = _random_requests(a, rng, 28, 8)
     # a small budget: 48 atoms per batch slot, so the mix spans many batches
-    outs, status = _serve(pyvc, a, reqs, 3 * 48 * (A3 + 20))
+    outs, status = _serve(pyvc, a, reqs, 3 * 48 * pyvc.Server.atom_bytes())
     say("served", len(reqs), "requests")
     for k, (kind, r) in enumerate(reqs):
         assert status[k] == 0, (k, kind)
@@ -96,7 +96,7 @@ def test_server_mix_equals_alone(pyvc, served):
 
 
 def test_server_rejects_invalid(pyvc, served):
-    srv = pyvc.Server(served, 3 * 8 * (A3 + 20))
+    srv = pyvc.Server(served, 3 * 8 * pyvc.Server.atom_bytes())
     try:
         b = pyvc.PinnedBuffer(16 * A3)
         assert srv.capacity() == 8
@@ -137,7 +137,7 @@ def test_server_corrupt_atom(pyvc, oracle, golden_containers):
     reqs = [("atoms", np.array([[0, 0, 0, 0]], np.int32)),
             ("atoms", np.array([[0, 1, 0, 0], [0, 1, 1, 1]], np.int32)),
             ("region", (0, (0, 0, 0, 64, 64, 32)))]
-    outs, status = _serve(pyvc, a, reqs, 3 * 16 * (A3 + 20))
+    outs, status = _serve(pyvc, a, reqs, 3 * 16 * pyvc.Server.atom_bytes())
     assert status == {0: 0, 1: 1, 2: 1}
     assert np.array_equal(outs[0], good[0])
     assert not outs[1][:A3].any() and np.array_equal(outs[1][A3:], good[1])

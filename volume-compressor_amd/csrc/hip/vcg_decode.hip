@@ -546,3 +546,25 @@ extern "C" int vcg_list_launch(vcg_arc *a, size_t n, const int32_t *quads_pinned
     return 0;
 }
 
+extern "C" size_t vcg_slot_scratch_bytes(void) { return (size_t)VCG_LIST_STRIDE * 4 + sizeof(vcg_item); }
+
+extern "C" int vcg_reserve_slots(size_t n) {
+    if (n == 0 || n > MAX_BATCH) return -1;
+    vcg_ctx *c = vcg_get_ctx();
+    for (int k = 0; k < VCG_SLOTS; ++k) {
+        if (c->lists_items[k] >= n && c->items_cap[k] >= n) continue;
+        VCG_CHECK(hipStreamSynchronize(c->sE[k]));
+        if (c->lists[k]) VCG_CHECK(hipFree(c->lists[k]));
+        if (c->items[k]) VCG_CHECK(hipFree(c->items[k]));
+        c->lists[k] = nullptr; c->items[k] = nullptr;
+        c->lists_items[k] = c->items_cap[k] = 0;
+        if (hipMalloc((void **)&c->lists[k], n * (size_t)VCG_LIST_STRIDE * 4) != hipSuccess ||
+            hipMalloc((void **)&c->items[k], n * sizeof(vcg_item)) != hipSuccess) {
+            (void)hipGetLastError();
+            return -1;   // capacity 0: the next launch on the slot grows it
+        }
+        c->lists_items[k] = c->items_cap[k] = n;
+    }
+    return 0;
+}
+

@@ -72,6 +72,7 @@ struct vcx_server {
     pthread_mutex_t mu;
     pthread_cond_t cv_work, cv_done;
     int stop;
+    int ready;                  /* worker: 1 scratch reserved, -1 failed */
     srv_req *qhead, *qtail;
     int64_t next_id;
     size_t pending, inflight;
@@ -155,7 +156,14 @@ static void retire(vcx_server *s, srv_slot *b, uint64_t now) {
 
 static void *worker(void *arg) {
     vcx_server *s = (vcx_server *)arg;
+    /* the decoder scratch of a full batch in every slot, reserved on this
+     * thread (it owns the decode context) before the first request: no batch
+     * then waits on an allocation and the device synchronisation it takes */
+    const int ok = vcg_reserve_slots(s->cap_atoms) == 0;
     pthread_mutex_lock(&s->mu);
+    s->ready = ok ? 1 : -1;
+    pthread_cond_broadcast(&s->cv_done);
+    if (!ok) { pthread_mutex_unlock(&s->mu); return NULL; }
     for (;;) {
         /* retire finished batches */
         int retired = 0;
@@ -226,7 +234,7 @@ vcx_server *vcx_server_create(vc_archive *a, size_t device_bytes) {
     vc__require_gpu();
     vcg_arc *d = vc__arc_dev(a);
     if (!d) return NULL;
-    size_t per = device_bytes / SRV_SLOTS / ((size_t)VC_ATOM3 + 20);
+    size_t per = device_bytes / SRV_SLOTS / vcx_server_atom_bytes();
     if (per > SRV_MAX_BATCH) per = SRV_MAX_BATCH;
     if (per < 1) return NULL;
     vcx_server *s = (vcx_server *)calloc(1, sizeof *s);
@@ -251,6 +259,11 @@ vcx_server *vcx_server_create(vc_archive *a, size_t device_bytes) {
         }
     }
     if (pthread_create(&s->th, NULL, worker, s) != 0) { vcx_server_destroy(s); return NULL; }
+    pthread_mutex_lock(&s->mu);
+    while (!s->ready) pthread_cond_wait(&s->cv_done, &s->mu);
+    const int ready = s->ready;
+    pthread_mutex_unlock(&s->mu);
+    if (ready < 0) { vcx_server_destroy(s); return NULL; }
     return s;
 }
 
@@ -343,6 +356,11 @@ size_t vcx_server_poll(vcx_server *s, int64_t *ids, int32_t *status, uint64_t *t
 }
 
 size_t vcx_server_capacity(const vcx_server *s) { return s ? s->cap_atoms : 0; }
+
+/* output atom, quad, box placement, status word and decoder scratch */
+size_t vcx_server_atom_bytes(void) {
+    return (size_t)VC_ATOM3 + 16 + sizeof(vcg_place) + 4 + vcg_slot_scratch_bytes();
+}
 
 void *vcx_host_alloc(size_t bytes) { vc__require_gpu(); return vcg_host_alloc(bytes); }
 void vcx_host_free(void *p) { vcg_host_free(p); }

@@ -89,6 +89,14 @@ typedef struct { uint32_t x0, y0, z0, x1, y1, z1; uint64_t out_off; } vcg_place;
  * streams: the caller orders reuse of its buffers through that stream. */
 int vcg_list_launch(vcg_arc *a, size_t n, const int32_t *quads_pinned, int32_t *quads_dev, uint8_t *out_dev,
                     int32_t *status_dev, size_t nplace, int slot);
+/* Device scratch bytes per atom of a list launch's capacity (coefficient
+ * list and item record), and a reservation of that scratch in every slot of
+ * the calling thread's context for batches of up to n atoms (n <= 32768),
+ * so that later launches of that size allocate nothing.  0 or -1 (out of
+ * device memory: a slot left short grows on demand, as without the
+ * reservation). */
+size_t vcg_slot_scratch_bytes(void);
+int    vcg_reserve_slots(size_t n);
 /* Decode n raw payloads (no archive) at one base_q. */
 int vcg_decode_payloads(size_t n, const uint8_t *pay, const uint64_t *offs, const uint32_t *lens,
                         const int32_t dq_step[VCG_NFREQ], const int32_t dq_bias[VCG_NFREQ],

@@ -77,6 +77,7 @@ def lib():
         "vcx_server_submit_region": (C.c_int64, [vp, i32, Box, vp]),
         "vcx_server_poll": (sz, [vp, vp, vp, vp, sz, i32]),
         "vcx_server_capacity": (sz, [vp]),
+        "vcx_server_atom_bytes": (sz, []),
         "vcx_now_ns": (C.c_uint64, []),
         "vcx_host_alloc": (vp, [sz]),
         "vcx_host_free": (None, [vp]),
@@ -382,7 +383,12 @@ class PinnedBuffer:
 class Server:
     """vcx_server_*: continuous-batching request loop over an opened archive."""
 
-    def __init__(self, archive, device_bytes=4 << 30):
+    @staticmethod
+    def atom_bytes():
+        """Device bytes per atom of batch-slot capacity (three slots)."""
+        return lib().vcx_server_atom_bytes()
+
+    def __init__(self, archive, device_bytes=16 << 30):
         self.L = lib()
         self.archive = archive
         self.h = self.L.vcx_server_create(archive.h, device_bytes)
