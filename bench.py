@@ -427,7 +427,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the decode shards by atom "
-              "with no inter-GPU traffic, see DESIGN.md", file=sys.stderr)
+              "with no inter-GPU traffic", file=sys.stderr)
         sys.exit(2)
 
     pyvc = load_pyvc()

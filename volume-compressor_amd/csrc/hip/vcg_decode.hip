@@ -60,8 +60,9 @@ __device__ __forceinline__ void item_atom(const dec_job &j, uint32_t gi, int &lo
 
 // per-lane LDS array (context memory: VCG_CTX_SLOTS slots; output ring: 8 slots),
 // slot-major over the 64 lanes of the wave so a wave's access is conflict-free.
-// m is the array (a fixed LDS address), l4 the lane's byte offset (< 256), so
-// a slot's address is one shift-or and the array base goes into the
+// m is the workgroup's array (a fixed LDS address) and l4 the lane's byte
+// offset in it: the wave's part (a power of two in size) ORed with lane * 4,
+// so a slot's address is one shift-or and the array base goes into the
 // instruction's offset field.  Context handles are byte offsets; sink is the
 // sink slot's, kept opaque to the compiler so that picking a handle stays one
 // select after the shift-or.
@@ -84,22 +85,31 @@ struct lds_ring {
     __device__ __forceinline__ void st(uint32_t off, uint32_t v) { *(uint32_t *)((char *)m + (off | l4)) = v; }
 };
 
-__global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__restrict__ T,
+#ifndef ENT_WAVES
+#define ENT_WAVES 4
+#endif
+#define ENT_THREADS (64 * ENT_WAVES)
+
+__global__ __launch_bounds__(ENT_THREADS) void k_entropy(dec_job j, const vcg_dtab *__restrict__ T,
                                                 uint32_t *__restrict__ lists,
                                                 vcg_item *__restrict__ items,
                                                 uint32_t *__restrict__ status) {
     // issue priority over transform waves sharing the SIMD (the serial
     // decode is the critical path; the transform fills its idle issue cycles)
     __builtin_amdgcn_s_setprio(3);
-    __shared__ uint32_t ctxm[VCG_CTX_SLOTS * 64];
-    __shared__ uint32_t ringm[VCG_RING * 64];
+    __shared__ uint32_t ctxw[ENT_WAVES][VCG_CTX_SLOTS * 64];
+    __shared__ uint32_t ringw[ENT_WAVES][VCG_RING * 64];
     __shared__ uint32_t bands[9];
     // band table in LDS: a band edge then waits on LDS only, not on the
     // look-ahead loads in flight
     if (threadIdx.x < 9) bands[threadIdx.x] = T->band_start[threadIdx.x];
     __syncthreads();
-    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+    const uint32_t i = blockIdx.x * (uint32_t)ENT_THREADS + threadIdx.x;
     if (i >= j.n) return;
+    // a wave's arrays: its context slots at byte (wave << 12), its ring at
+    // (wave << 11), so a lane's offset ORs into the slot addressing
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t *ctxm = &ctxw[0][0], *ringm = &ringw[0][0];
     const uint32_t gi = j.base + i;
     int lod = 0;
     uint64_t off = 0;
@@ -127,12 +137,16 @@ __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__res
 #if VCG_ENTROPY_FSM
     // range-coder bytes [p+1, p+len); the device buffers are padded past
     // their end for the 16-byte look-ahead loads
-    uint32_t sink = (14u << 8) | (threadIdx.x * 4u);
+    const uint32_t cl4 = (wave << 12) | (lane * 4u);
+    uint32_t sink = (14u << 8) | cl4;
     asm volatile("" : "+v"(sink));
-    lds_ctx ctx{ctxm, threadIdx.x * 4u, sink};
-    lds_ring ring{ringm, threadIdx.x * 4u};
+    lds_ctx ctx{ctxm, cl4, sink};
+    // opaque, so that the slot offset and the lane offset stay one and-or
+    uint32_t rl4 = (wave << 11) | (lane * 4u);
+    asm volatile("" : "+v"(rl4));
+    lds_ring ring{ringm, rl4};
     for (uint32_t k = 1; k < VCG_CTX_SLOTS; ++k)
-        ctxm[k * 64u + threadIdx.x] = k == 13u || k == 15u ? VCG_P_BYPASS : 2048u;
+        ctxw[wave][k * 64u + lane] = k == 13u || k == 15u ? VCG_P_BYPASS : 2048u;
     const uint8_t *s = p + 1;
     const uint32_t n = len - 1;
     vcg_fsm f;
@@ -153,7 +167,7 @@ __global__ __launch_bounds__(64) void k_entropy(dec_job j, const vcg_dtab *__res
     // the last entry (highest scan position) is still in the ring
     const int fmax = cnt ? vcg_freq(T->scan[vcg_ent_pos(ring.ld(((cnt - 1u) << 8) & ((VCG_RING - 1u) << 8)))]) : -1;
 #else
-    (void)ctxm; (void)ringm; (void)bands;
+    (void)ctxm; (void)ringm; (void)bands; (void)lane; (void)wave;
     uint32_t cnt = 0;
     const int bad = vcg_rc_decode_atom(p + 1, len - 1, T->band_start, out, &cnt);
     const int fmax = cnt && !bad ? vcg_freq(T->scan[vcg_ent_pos(out[cnt - 1])]) : -1;
@@ -365,7 +379,7 @@ static int run_batch(vcg_ctx *c, dec_job j, uint32_t total, bool inputs_on_s, co
         if (up_bytes) VCG_CHECK(hipMemcpyAsync(up_dst, up_src, up_bytes, hipMemcpyHostToDevice, sE));
         j.base = 0;
         j.n = total;
-        hipLaunchKernelGGL(k_entropy, dim3((j.n + 63) / 64), dim3(64), 0, sE, j, g_dtab, c->lists[fixed_slot],
+        hipLaunchKernelGGL(k_entropy, dim3((j.n + ENT_THREADS - 1) / ENT_THREADS), dim3(ENT_THREADS), 0, sE, j, g_dtab, c->lists[fixed_slot],
                            c->items[fixed_slot], c->status_d);
         VCG_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_idct, dim3(j.n), dim3(IDCT_THREADS), LDS_BYTES, sE, j, g_dtab, c->lists[fixed_slot],
@@ -390,7 +404,7 @@ static int run_batch(vcg_ctx *c, dec_job j, uint32_t total, bool inputs_on_s, co
         if (inputs_on_s) VCG_CHECK(hipStreamWaitEvent(sE, c->evF, 0));
         if (up_bytes && sl != first) VCG_CHECK(hipStreamWaitEvent(sE, c->evI[first], 0));
         if (c->ev[0]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[0], sE));
-        hipLaunchKernelGGL(k_entropy, dim3((j.n + 63) / 64), dim3(64), 0, sE, j, g_dtab, c->lists[sl],
+        hipLaunchKernelGGL(k_entropy, dim3((j.n + ENT_THREADS - 1) / ENT_THREADS), dim3(ENT_THREADS), 0, sE, j, g_dtab, c->lists[sl],
                            c->items[sl], c->status_d);
         VCG_CHECK(hipGetLastError());
         if (c->ev[1]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[1], sE));
