@@ -6,7 +6,7 @@
 //               runs the adaptive binary range decoder and context coder
 //               (src/vc/vc.c:403-441, 491-560) and writes the nonzero levels
 //               as (scan position, level) entries to HBM scratch.
-//   k_idct    : one 256-thread workgroup per atom.  Scatters and dequantizes
+//   k_idct    : one 512-thread workgroup per atom.  Scatters and dequantizes
 //               the entries into a zeroed 32x32x(32+1) i32 LDS tile
 //               (src/vc/vc.c:327-338), runs the three inverse partial
 //               butterfly passes along x, y, z in place (src/vc/vc.c:172-229),

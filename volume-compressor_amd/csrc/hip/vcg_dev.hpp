@@ -31,8 +31,9 @@ struct vcg_item {
     int16_t fmax;     // highest coded frequency cz+cy+cx (-1: none)
 };
 
-// Coefficient entry: (scan position << 16) | (uint16_t)level.  Entries of one
-// item are written in increasing scan position.
+// Coefficient entry: scan position << 17 | (2m - 11 + sign) mod 2^17 for a
+// level of magnitude m (vcg_ent_make / vcg_ent_level).  Entries of one item
+// are written in increasing scan position.
 #define VCG_LIST_CAP 32768u
 // per-atom list stride in entries: padded past a power of two so the lists of
 // the 64 lanes of a wave (written concurrently) do not share cache sets
