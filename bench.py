@@ -19,7 +19,10 @@ Further sections, each beside its plain per-call counterpart:
                    continuous-batching request loop under open-loop Poisson
                    traces of atom requests (1-4096 atoms, log-uniform, LOD 0
                    with p = 0.7) and of region requests (log-uniform edges
-                   17-600 voxels, LOD 0-7), p50/p99 latency;
+                   17-600 voxels, LOD 0-7), p50/p99 latency; and the same
+                   archive rewritten into the canonical v2 layout in device
+                   memory (vcx_repack_device) beside the reference's own
+                   repack tool on the host;
   batched_encode : 256 device-resident volumes of uneven dims (axes uniform
                    in [40, 512]) at ratios {10, 20, 50, 100} in one
                    vcx_encode_batch call, against per-volume encodes.
@@ -303,8 +306,52 @@ def serving_section(pyvc, args):
     sr["rps_vs_plain"] = round(sr["achieved_rps"] / pr["rps"], 1)
     res["region_requests"] = {"batched": sr, "plain": pr}
     log(f"region serving: {sr}; plain {pr}")
+    res["repack"] = repack_bench(pyvc, a, arc, no_cpu=args.no_cpu)
+    log(f"repack: {res['repack']}")
     a.close()
     return res
+
+
+def repack_bench(pyvc, a, arc, reps=3, no_cpu=False):
+    """Container rewrite of the serving archive into device memory
+    (vcx_repack_device, a synchronous call, host wall clock).  The archive is
+    the encoder's canonical v2, so the image must equal it byte for byte.
+    Beside it, the reference's own tool (tools/vc_repack.c, built from source
+    into oracle/_ref) on the same file with 16 threads, when built."""
+    st, n = a.repack_device(None, 0)
+    assert st == 2 and n == len(arc), (st, n)
+    dev = pyvc.dev_alloc(n)
+    try:
+        ts = []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            st, _ = a.repack_device(dev, n)
+            ts.append(time.perf_counter() - t0)
+            assert st == 0, st
+        same = pyvc.d2h(dev, n).tobytes() == arc
+    finally:
+        pyvc.dev_free(dev)
+    t = min(ts[1:])
+    out = {"workload": "2048^3 ratio-50 archive (v2) rewritten into device memory", "bytes": n,
+           "device_s": round(t, 4), "GBps": round(n / t / 1e9, 2), "identical": bool(same)}
+    tool = os.path.join(ROOT, "oracle", "_ref", "vc_repack")
+    if not no_cpu and os.path.exists(tool):
+        import subprocess
+        import tempfile
+        d = tempfile.mkdtemp()
+        p_in, p_out = os.path.join(d, "in.vca"), os.path.join(d, "out.vca")
+        try:
+            open(p_in, "wb").write(arc)
+            t0 = time.perf_counter()
+            subprocess.run([tool, p_in, p_out, "--threads", "16"], check=True, stdout=subprocess.DEVNULL, timeout=120)
+            out["reference_tool_s"] = round(time.perf_counter() - t0, 3)
+            out["reference_tool_identical"] = open(p_out, "rb").read() == arc
+        finally:
+            for f in (p_in, p_out):
+                if os.path.exists(f):
+                    os.unlink(f)
+            os.rmdir(d)
+    return out
 
 
 def ratio_section(pyvc, hip, o, dvol, dout, n, ratios, steps=10):

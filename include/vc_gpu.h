@@ -100,6 +100,22 @@ vc_status vcx_decode_payloads(size_t n, const uint8_t *pay, const uint64_t *offs
  * memory. */
 vc_status vcx_synth_device(uint64_t seed, vc_dims dims, int nsheets, void *dev_out);
 
+/* ---- container rewrite into the canonical v2 layout ----
+ * Replaces tools/vc_repack.c (main, :184-291; a command-line tool in the
+ * reference, no library entry point): the same bytes that tool writes for
+ * the same input.  a: a v1 or v2 archive opened from memory (vc_open); its
+ * device copy is used.  Regions go coarse-LOD-first, each data region as one
+ * blob [512 KiB slot array][payloads in slot order] after the 4 KiB-aligned
+ * directory; payloads are copied verbatim.  Errors: VC_ERR_RANGE for NULL
+ * arguments or a streaming archive, VC_ERR_FORMAT for a present payload
+ * range outside the archive (the tool copies it unchecked), VC_ERR_OOM.
+ * vcx_repack returns a malloc'd image (the caller free()s it).
+ * vcx_repack_device writes it to dev_out; *out_len receives the image size,
+ * and a dev_out that is NULL or smaller (cap) gets VC_ERR_RANGE and nothing
+ * written. */
+vc_status vcx_repack(vc_archive *a, uint8_t **out, size_t *out_len);
+vc_status vcx_repack_device(vc_archive *a, void *dev_out, size_t cap, size_t *out_len);
+
 /* ---- serving: device-resident request loop with continuous batching ----
  * No counterpart in the reference (its vc_decode_atom / vc_decode_region are
  * one call per request); callers are request servers that want many

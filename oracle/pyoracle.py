@@ -63,6 +63,7 @@ class Oracle:
         L.vco_decode_atoms_omp.argtypes = [vp, C.c_int, vp, sz, C.c_int, vp]
         L.vco_decode_atoms_omp.restype = C.c_double
         L.vco_synth.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, vp]
+        L.vco_repack.argtypes = [vp, sz, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]
         self.libc = C.CDLL("libc.so.6")
         self.libc.free.argtypes = [C.c_void_p]
 
@@ -190,6 +191,13 @@ class Oracle:
         qs = np.zeros(8, np.float32)
         self.L.vco_lod_schedule(C.c_float(q0), nlod, qs.ctypes.data)
         return qs
+
+    def repack(self, buf):
+        """canonical v2 image of an archive (tools/vc_repack.c): (status, bytes or None)"""
+        b = np.frombuffer(bytes(buf), np.uint8).copy()
+        p, n = C.c_void_p(), C.c_size_t()
+        st = self.L.vco_repack(b.ctypes.data, b.size, C.byref(p), C.byref(n))
+        return st, (self._take(p, n.value) if st == 0 else None)
 
     def open(self, buf):
         return OracleArchive(self, buf)

@@ -708,6 +708,98 @@ int vco_coverage(const vco_archive *a, int lod, u32 az, u32 ay, u32 ax, u64 *off
     return f == 2 ? 1 : 0;
 }
 
+/* Container rewrite into the canonical v2 layout, serially
+ * (tools/vc_repack.c:184-291): regions coarse-LOD-first; pass 1 classifies
+ * each region from its atoms' coverage and sizes its blob (:96-121); a
+ * prefix sum places the blobs after the 4 KiB-aligned directory (:244-256);
+ * pass 2 writes each data region's slot array, payloads in slot order with
+ * dc = first payload byte (:131-162); header fields carried over (:272-281).
+ * A present payload range outside the archive, which the tool copies
+ * unchecked, is VCO_ERR_FORMAT here. */
+int vco_repack(const u8 *buf, size_t len, u8 **out, size_t *out_len) {
+    if (!out || !out_len) return VCO_ERR_RANGE;
+    *out = NULL; *out_len = 0;
+    vco_archive *a = vco_open(buf, len);
+    if (!a) return VCO_ERR_FORMAT;
+    u64 prefix[VCO_NLOD + 1], nreg = 0;                            /* directory index base per LOD */
+    for (int l = 0; l < a->nlod; ++l) {
+        u32 nrz, nry, nrx; region_grid(a->nx, a->ny, a->nz, l, &nrz, &nry, &nrx);
+        prefix[l] = nreg;
+        nreg += (u64)nrz * nry * nrx;
+    }
+    prefix[a->nlod] = nreg;
+    u64 *blen = (u64 *)calloc(nreg ? nreg : 1, 8), *boff = (u64 *)calloc(nreg ? nreg : 1, 8);
+    u32 *rflag = (u32 *)calloc(nreg ? nreg : 1, 4);
+    u64 *order = (u64 *)calloc(nreg ? nreg : 1, 8);
+    int *olod = (int *)calloc(nreg ? nreg : 1, sizeof(int));
+    u32 *orz = (u32 *)calloc(nreg ? nreg : 1, 4), *ory = (u32 *)calloc(nreg ? nreg : 1, 4),
+        *orx = (u32 *)calloc(nreg ? nreg : 1, 4);
+    int st = VCO_OK;
+    u64 ni = 0;
+    for (int l = a->nlod - 1; l >= 0; --l) {
+        u32 nrz, nry, nrx; region_grid(a->nx, a->ny, a->nz, l, &nrz, &nry, &nrx);
+        for (u32 rz = 0; rz < nrz; ++rz) for (u32 ry = 0; ry < nry; ++ry) for (u32 rx = 0; rx < nrx; ++rx) {
+            order[ni] = prefix[l] + ((u64)rz * nry + ry) * nrx + rx;
+            olod[ni] = l; orz[ni] = rz; ory[ni] = ry; orx[ni] = rx;
+            ni++;
+        }
+    }
+    for (u64 i = 0; i < ni && st == VCO_OK; ++i) {                 /* pass 1 */
+        u64 pay = 0; int anyp = 0, anyz = 0;
+        for (u32 dz = 0; dz < 32; ++dz) for (u32 dy = 0; dy < 32; ++dy) for (u32 dx = 0; dx < 32; ++dx) {
+            u64 o; u32 n;
+            int c = vco_coverage(a, olod[i], orz[i] * 32 + dz, ory[i] * 32 + dy, orx[i] * 32 + dx, &o, &n);
+            if (c == 2) {
+                if (o >= len || n > len - o) st = VCO_ERR_FORMAT;
+                anyp = 1; pay += n;
+            } else if (c == 1) {
+                anyz = 1;
+            }
+        }
+        rflag[order[i]] = anyp ? 2u : anyz ? 1u : 0u;
+        if (anyp) blen[order[i]] = L2_BYTES + pay;
+    }
+    const u64 dir_off = HDR, blobs = (dir_off + nreg * 16 + 4095) & ~4095ull;
+    u64 cur = blobs;
+    for (u64 i = 0; i < ni; ++i)
+        if (rflag[order[i]] == 2) { boff[order[i]] = cur; cur += blen[order[i]]; }
+    u8 *o8 = st == VCO_OK ? (u8 *)calloc(cur, 1) : NULL;
+    if (st == VCO_OK && !o8) st = VCO_ERR_OOM;
+    for (u64 i = 0; i < ni && st == VCO_OK; ++i) {                 /* pass 2 */
+        const u64 ri = order[i];
+        u8 *e = o8 + dir_off + ri * 16;
+        w32(e + 12, rflag[ri]);
+        if (rflag[ri] != 2) continue;
+        w64(e, boff[ri]); w32(e + 8, (u32)blen[ri]);
+        u64 pc = boff[ri] + L2_BYTES;
+        for (u32 dz = 0; dz < 32; ++dz) for (u32 dy = 0; dy < 32; ++dy) for (u32 dx = 0; dx < 32; ++dx) {
+            u64 so; u32 n;
+            int c = vco_coverage(a, olod[i], orz[i] * 32 + dz, ory[i] * 32 + dy, orx[i] * 32 + dx, &so, &n);
+            u8 *slot = o8 + boff[ri] + ((u64)(dz * 32 + dy) * 32 + dx) * SLOT;
+            if (c == 2) {
+                w64(slot, pc); w32(slot + 8, n); slot[12] = 1; slot[13] = buf[so];
+                memcpy(o8 + pc, buf + so, n);
+                pc += n;
+            } else if (c == 1) {
+                slot[12] = 2;
+            }
+        }
+    }
+    if (st == VCO_OK) {
+        w32(o8, MAGIC); w32(o8 + 4, 2); w32(o8 + 8, A); w32(o8 + 16, R_ATOMS); w32(o8 + 20, (u32)a->nlod);
+        memcpy(o8 + 24, buf + 24, 12);                 /* nx, ny, nz */
+        memcpy(o8 + 36, buf + 36, 4);                  /* target ratio */
+        memcpy(o8 + 72, buf + 72, 4 * VCO_NLOD);       /* base_q */
+        w64(o8 + 40, dir_off); w64(o8 + 48, nreg); w64(o8 + 56, cur);
+        *out = o8; *out_len = (size_t)cur;
+    } else {
+        free(o8);
+    }
+    free(blen); free(boff); free(rflag); free(order); free(olod); free(orz); free(ory); free(orx);
+    vco_close(a);
+    return st;
+}
+
 int vco_decode_atom(vco_archive *a, int lod, int ax, int ay, int az, u8 out[A3]) {  /* vc.c:1404-1421 */
     if (!a || lod < 0 || lod >= a->nlod || !out) return VCO_ERR_RANGE;
     u32 x, y, z; vco_lod_dims(a->nx, a->ny, a->nz, lod, &x, &y, &z);

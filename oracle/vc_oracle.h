@@ -88,6 +88,8 @@ void vco_close(vco_archive *a);
 int  vco_nlod_of(const vco_archive *a);
 int  vco_decode_atom(vco_archive *a, int lod, int ax, int ay, int az, uint8_t out[VCO_A3]);
 /* coverage: 0 ABSENT, 1 KNOWN_ZERO, 2 PRESENT (vc.c:1423-1430) */
+/* canonical v2 image of any archive (tools/vc_repack.c); *out malloc'd */
+int  vco_repack(const uint8_t *buf, size_t len, uint8_t **out, size_t *out_len);
 int  vco_coverage(const vco_archive *a, int lod, uint32_t az, uint32_t ay, uint32_t ax,
                   uint64_t *off, uint32_t *len);
 /* box = x0,y0,z0,x1,y1,z1 */

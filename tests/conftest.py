@@ -65,6 +65,21 @@ def golden_containers():
     return out
 
 
+@pytest.fixture(scope="session")
+def golden_repack():
+    """container-rewrite cases: {name: (meta, {"v1": bytes, "v2": bytes})},
+    v2 from tools/vc_repack.c on the reference writer's v1"""
+    meta = json.load(open(os.path.join(GOLDEN, "repack.json")))
+    out = {}
+    for name, m in meta.items():
+        b = {}
+        for tag in ("v1", "v2"):
+            with gzip.open(os.path.join(GOLDEN, f"{name}.{tag}.vca.gz"), "rb") as f:
+                b[tag] = f.read()
+        out[name] = (m, b)
+    return out
+
+
 def payloads_of(g, qi=None):
     """[(atom index, q index, payload bytes, recon sha)] from atoms.npz"""
     blob = g["blob"].tobytes()

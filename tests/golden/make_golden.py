@@ -13,8 +13,13 @@ inputs plus the reference's outputs for them.
   containers.json + *.vca.gz
                   whole archives: v1 from the reference writer (single thread,
                   LOD-major raster append) and v2 from tools/vc_repack.c on it
+  repack.json + repack_*.vca.gz
+                  container-rewrite cases: a v1 archive of several regions per
+                  LOD from the reference writer, atoms appended in a shuffled
+                  order with some left absent, some marked zero and one region
+                  all zero, and v2 from tools/vc_repack.c on it
 
-usage: python tests/golden/make_golden.py
+usage: python tests/golden/make_golden.py [--only repack]
 """
 import gzip
 import hashlib
@@ -217,14 +222,65 @@ def make_containers(o, ref):
     json.dump(meta, open(os.path.join(OUT, "containers.json"), "w"), indent=1)
 
 
+def make_repack(o, ref):
+    meta = {}
+    name, (nz, ny, nx), seed = "repack_multi", (40, 72, 1100), 31
+    vol = o.synth(seed, nx, ny, nz, 6)
+    nlod = o.nlod(nx, ny, nz)
+    qs = list(o.lod_schedule(8.0, nlod))
+    lods = [vol]
+    for _ in range(1, nlod):
+        lods.append(o.downsample2x(lods[-1]))
+    rng = np.random.default_rng(seed)
+    atoms, absent, zero = [], 0, 0
+    for lod, v in enumerate(lods):
+        lz, ly, lx = v.shape
+        for az in range((lz + 31) // 32):
+            for ay in range((ly + 31) // 32):
+                for ax in range((lx + 31) // 32):
+                    if lod == 0 and ax >= 32:          # LOD0's second x region: every atom marked zero
+                        atoms.append((lod, az, ay, ax, None)); zero += 1
+                        continue
+                    u = rng.random()
+                    if u < 0.1:
+                        absent += 1
+                        continue
+                    if u < 0.2:
+                        atoms.append((lod, az, ay, ax, None)); zero += 1
+                        continue
+                    a = np.zeros((32, 32, 32), np.uint8)
+                    sl = v[az * 32:az * 32 + 32, ay * 32:ay * 32 + 32, ax * 32:ax * 32 + 32]
+                    a[:sl.shape[0], :sl.shape[1], :sl.shape[2]] = sl
+                    atoms.append((lod, az, ay, ax, a))
+    order = rng.permutation(len(atoms))
+    p1, p2 = "/tmp/golden_rp_v1.vca", "/tmp/golden_rp_v2.vca"
+    ref.write_v1(p1, (nx, ny, nz), 10.0, qs, [atoms[i] for i in order])
+    subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "vc_repack"), p1, p2], stdout=subprocess.DEVNULL)
+    b1, b2 = open(p1, "rb").read(), open(p2, "rb").read()
+    os.unlink(p1); os.unlink(p2)
+    for tag, b in (("v1", b1), ("v2", b2)):
+        with gzip.open(os.path.join(OUT, f"{name}.{tag}.vca.gz"), "wb", 9) as f:
+            f.write(b)
+    meta[name] = {"shape_zyx": [nz, ny, nx], "seed": seed, "nlod": nlod, "qs": [float(x) for x in qs],
+                  "atoms_written": len(atoms), "absent": absent, "zero_marked": zero,
+                  "v1_sha": hashlib.sha256(b1).hexdigest(), "v2_sha": hashlib.sha256(b2).hexdigest(),
+                  "v1_len": len(b1), "v2_len": len(b2)}
+    print(name, len(b1), len(b2), "atoms", len(atoms), "absent", absent, "zero", zero)
+    json.dump(meta, open(os.path.join(OUT, "repack.json"), "w"), indent=1)
+
+
 def main():
     o = Oracle()
     ref = Ref("strict")
+    if sys.argv[1:] == ["--only", "repack"]:
+        make_repack(o, ref)
+        return
     ref_fast = Ref("fast_v4" if "fast_v4" in __import__("pyoracle").ref_variants() else "fast_v3")
     make_atoms(o, ref, ref_fast)
     make_corrupt(o, ref)
     make_downsample(o, ref)
     make_containers(o, ref)
+    make_repack(o, ref)
 
 
 if __name__ == "__main__":

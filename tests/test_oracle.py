@@ -323,3 +323,66 @@ def test_oracle_reader_matches_reference(oracle, ref, golden_containers):
                         s1, v1 = ra.decode_atom(0, ax, ay, az)
                         s2, v2 = oa.decode_atom(0, ax, ay, az)
                         assert s1 == s2 and np.array_equal(v1, v2)
+
+
+# ---------------------------------------------------------------- container rewrite
+def _v2_dir(b):
+    """(directory entries [(off, len, flags)], total length field) of a v2 image"""
+    import struct
+    dir_off, cnt, tot = struct.unpack_from("<QQQ", b, 40)
+    return [struct.unpack_from("<QII", b, dir_off + 16 * k) for k in range(cnt)], tot
+
+
+def test_golden_repack_fixture(oracle, golden_repack):
+    """The rewrite fixture is what its metadata says: several regions per LOD,
+    a region of zero-marked atoms that becomes a ZERO entry, absent atoms, and
+    v1 and v2 that decode alike."""
+    for name, (m, b) in golden_repack.items():
+        assert hashlib.sha256(b["v1"]).hexdigest() == m["v1_sha"]
+        assert hashlib.sha256(b["v2"]).hexdigest() == m["v2_sha"]
+        ents, tot = _v2_dir(b["v2"])
+        assert tot == len(b["v2"]) == m["v2_len"]
+        flags = [f for _, _, f in ents]
+        assert flags.count(2) >= 2 and flags.count(1) >= 1, flags
+        a1, a2 = oracle.open(b["v1"]), oracle.open(b["v2"])
+        nz, ny, nx = m["shape_zyx"]
+        seen = {0: 0, 1: 0, 2: 0}
+        for az in range((nz + 31) // 32):
+            for ay in range((ny + 31) // 32):
+                for ax in range((nx + 31) // 32):
+                    c1, c2 = a1.coverage(0, az, ay, ax), a2.coverage(0, az, ay, ax)
+                    assert c1[0] == c2[0] and c1[2] == c2[2]
+                    seen[c1[0]] += 1
+                    s1, v1 = a1.decode_atom(0, ax, ay, az)
+                    s2, v2 = a2.decode_atom(0, ax, ay, az)
+                    assert s1 == s2 == 0 and np.array_equal(v1, v2)
+        assert all(seen[k] > 0 for k in seen), seen
+
+
+def test_oracle_repack_equals_reference_tool(oracle, golden_containers, golden_repack):
+    """vco_repack reproduces tools/vc_repack.c's output on every golden v1 (and
+    maps each canonical v2 to itself)."""
+    cases = dict(golden_containers)
+    cases.update(golden_repack)
+    for name, (m, b) in cases.items():
+        st, out = oracle.repack(b["v1"])
+        assert st == 0 and out == b["v2"], name
+        st, out = oracle.repack(b["v2"])
+        assert st == 0 and out == b["v2"], name
+
+
+def test_oracle_repack_rejects_payload_outside_archive(oracle, golden_repack):
+    """A planted fault: one present slot's length made to run past the end."""
+    import struct
+    m, b = golden_repack["repack_multi"]
+    buf = bytearray(b["v2"])
+    ents, _ = _v2_dir(buf)
+    off = next(o for o, _, f in ents if f == 2)
+    for s in range(32768):
+        so = off + 16 * s
+        if buf[so + 12] == 1:
+            po, = struct.unpack_from("<Q", buf, so)
+            struct.pack_into("<I", buf, so + 8, len(buf) - po + 1)
+            break
+    st, out = oracle.repack(bytes(buf))
+    assert st == 1 and out is None

@@ -92,12 +92,11 @@ static uint8_t resolve_slot(const vc_archive *a, uint64_t so, uint64_t *off, uin
     return f;
 }
 
-uint8_t vc__resolve(const vc_archive *a, int lod, uint32_t az, uint32_t ay, uint32_t ax, uint64_t *off,
-                    uint32_t *len) {
+int vc__region_block(const vc_archive *a, int lod, uint32_t rz, uint32_t ry, uint32_t rx, uint64_t *blk_out) {
     uint64_t blk;
+    *blk_out = 0;
     if (a->version == 2) {
         uint32_t nrz, nry, nrx; vc__region_grid(a->dims0, lod, &nrz, &nry, &nrx);
-        uint32_t rz = az / R_ATOMS, ry = ay / R_ATOMS, rx = ax / R_ATOMS;
         if (rz >= nrz || ry >= nry || rx >= nrx) return AF_ABSENT;
         uint64_t ri = a->region_prefix[lod] + ((uint64_t)rz * nry + ry) * nrx + rx;
         if (ri >= a->region_prefix[a->nlod]) return AF_ABSENT;
@@ -108,7 +107,7 @@ uint8_t vc__resolve(const vc_archive *a, int lod, uint32_t az, uint32_t ay, uint
         if (f == 1) return AF_ZERO;
         blk = rd64(e);
     } else {
-        uint64_t key = vc__region_key(lod, az / R_ATOMS, ay / R_ATOMS, ax / R_ATOMS);
+        uint64_t key = vc__region_key(lod, rz, ry, rx);
         uint64_t mask = a->l1_cap - 1, i = vc__mix64(key) & mask;
         blk = 0;
         for (uint64_t probe = 0; probe < a->l1_cap; ++probe) {
@@ -123,6 +122,15 @@ uint8_t vc__resolve(const vc_archive *a, int lod, uint32_t az, uint32_t ay, uint
         if (blk == ZERO_REGION) return AF_ZERO;
     }
     if (blk > a->len || a->len - blk < L2_BLOCK) return AF_ABSENT;     /* corrupt guard */
+    *blk_out = blk;
+    return AF_PRESENT;
+}
+
+uint8_t vc__resolve(const vc_archive *a, int lod, uint32_t az, uint32_t ay, uint32_t ax, uint64_t *off,
+                    uint32_t *len) {
+    uint64_t blk;
+    const int kind = vc__region_block(a, lod, az / R_ATOMS, ay / R_ATOMS, ax / R_ATOMS, &blk);
+    if (kind != AF_PRESENT) return (uint8_t)kind;
     uint64_t si = ((uint64_t)(az & (R_ATOMS - 1)) * R_ATOMS + (ay & (R_ATOMS - 1))) * R_ATOMS + (ax & (R_ATOMS - 1));
     return resolve_slot(a, blk + si * ATOM_SLOT, off, len);
 }

@@ -75,6 +75,10 @@ struct vc_archive {
     pthread_mutex_t mu;
     vcg_arc *dev;                       /* device copy, created on first GPU decode */
 };
+/* a region's slot array (src/vc/vc.c:1349-1360,1376-1400): AF_PRESENT with
+ * its offset in *blk, AF_ZERO (all-zero region) or AF_ABSENT (also for a
+ * block failing the corrupt guard) */
+int vc__region_block(const vc_archive *a, int lod, uint32_t rz, uint32_t ry, uint32_t rx, uint64_t *blk);
 /* resolve (lod,az,ay,ax) -> slot flags, payload range (src/vc/vc.c:1365-1402) */
 uint8_t vc__resolve(const vc_archive *a, int lod, uint32_t az, uint32_t ay, uint32_t ax,
                     uint64_t *off, uint32_t *len);

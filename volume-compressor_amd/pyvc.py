@@ -71,6 +71,8 @@ def lib():
         "vcx_h2d": (i32, [vp, vp, sz]),
         "vcx_d2h": (i32, [vp, vp, sz]),
         "vcx_copy_kernel": (i32, [vp, vp, sz]),
+        "vcx_repack": (i32, [vp, P(vp), P(sz)]),
+        "vcx_repack_device": (i32, [vp, vp, sz, P(sz)]),
         "vcx_server_create": (vp, [vp, sz]),
         "vcx_server_destroy": (None, [vp]),
         "vcx_server_submit_atoms": (C.c_int64, [vp, sz, vp, vp]),
@@ -238,6 +240,18 @@ class Archive:
 
     def version(self):
         return self.L.vc_archive_version(self.h)
+
+    def repack(self):
+        """vcx_repack: (status, canonical v2 image bytes or None)."""
+        p, n = C.c_void_p(), C.c_size_t()
+        st = self.L.vcx_repack(self.h, C.byref(p), C.byref(n))
+        return st, (_take(p, n.value) if st == VC_OK else None)
+
+    def repack_device(self, dev_out, cap):
+        """vcx_repack_device: (status, image size)."""
+        n = C.c_size_t()
+        st = self.L.vcx_repack_device(self.h, dev_out, cap, C.byref(n))
+        return st, n.value
 
     def lod_dims(self, lod):
         d = Dims()
