@@ -332,8 +332,15 @@ def repack_bench(pyvc, a, arc, reps=3, no_cpu=False):
     finally:
         pyvc.dev_free(dev)
     t = min(ts[1:])
+    # payload bytes: data blobs less their 512 KiB slot arrays (v2 directory)
+    import struct
+    dir_off, cnt = struct.unpack_from("<QQ", arc, 40)
+    ents = np.frombuffer(arc, np.uint32, 4 * cnt, dir_off).reshape(cnt, 4)
+    data = ents[:, 3] == 2
+    payload = int(ents[data, 2].astype(np.int64).sum()) - int(data.sum()) * 32768 * 16
     out = {"workload": "2048^3 ratio-50 archive (v2) rewritten into device memory", "bytes": n,
-           "device_s": round(t, 4), "GBps": round(n / t / 1e9, 2), "identical": bool(same)}
+           "payload_bytes": payload, "data_regions": int(data.sum()),
+           "device_s": round(t, 5), "GBps": round(n / t / 1e9, 2), "identical": bool(same)}
     tool = os.path.join(ROOT, "oracle", "_ref", "vc_repack")
     if not no_cpu and os.path.exists(tool):
         import subprocess
@@ -466,6 +473,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-ratios", action="store_true")
     ap.add_argument("--no-encode-batch", action="store_true")
+    ap.add_argument("--repack-only", action="store_true",
+                    help="profiler run: only the container rewrite of the 2048^3 ratio-50 archive, its JSON on stdout")
     ap.add_argument("--encode-batch-only", action="store_true",
                     help="profiler run: only the batched-encode section (256 volumes), its JSON on stdout")
     ap.add_argument("--profile", action="store_true",
@@ -482,6 +491,19 @@ def main():
     assert pyvc.device_init() == 0, "no HIP device"
     if args.encode_batch_only:
         print(json.dumps(encode_batch_section(pyvc, nseq=0)), flush=True)
+        return
+    if args.repack_only:
+        n = args.serve_size
+        dvol = pyvc.dev_alloc(n ** 3)
+        try:
+            assert pyvc.synth_device(3, (n, n, n), args.serve_sheets, dvol) == 0
+            st, arc = pyvc.encode_device(dvol, (n, n, n), 50.0)
+            assert st == 0, st
+        finally:
+            pyvc.dev_free(dvol)
+        a = pyvc.Archive(arc)
+        print(json.dumps(repack_bench(pyvc, a, arc, reps=5, no_cpu=True)), flush=True)
+        a.close()
         return
     L = pyvc.lib()
     n = args.size

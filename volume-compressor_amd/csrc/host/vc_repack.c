@@ -26,11 +26,14 @@ typedef struct {
     uint64_t blobs_off, total_len, njobs;
     uint8_t *head;                       /* header, directory, zero pad: blobs_off bytes */
     size_t ndata;
-    uint64_t *blk, *boff, *jbase;        /* per data region, coarse-first order */
+    uint64_t *blk, *boff;                /* per data region, coarse-first order */
+    uint64_t *cdst, *cjob;               /* per plane of a data region: first payload, first job */
 } rp_plan;
 
+#define PLANES 32u                       /* 1024-slot planes per slot array */
+
 static void plan_free(rp_plan *p) {
-    free(p->head); free(p->blk); free(p->boff); free(p->jbase);
+    free(p->head); free(p->blk); free(p->boff); free(p->cdst); free(p->cjob);
     memset(p, 0, sizeof *p);
 }
 
@@ -52,13 +55,14 @@ static vc_status plan_build(vc_archive *a, vcg_arc *d, rp_plan *p) {
     uint8_t *kind = (uint8_t *)malloc(nreg ? nreg : 1);
     uint64_t *rblk = (uint64_t *)malloc((nreg ? nreg : 1) * 8);
     uint64_t *bl = (uint64_t *)malloc((nreg ? nreg : 1) * 8);      /* slot arrays to size */
-    uint64_t *sums = (uint64_t *)malloc((nreg ? nreg : 1) * 24);
+    uint64_t *sums = (uint64_t *)malloc((nreg ? nreg : 1) * PLANES * 24);
     p->head = (uint8_t *)calloc(1, p->blobs_off);
     p->blk = (uint64_t *)malloc((nreg ? nreg : 1) * 8);
     p->boff = (uint64_t *)malloc((nreg ? nreg : 1) * 8);
-    p->jbase = (uint64_t *)malloc((nreg ? nreg : 1) * 8);
+    p->cdst = (uint64_t *)malloc((nreg ? nreg : 1) * PLANES * 8);
+    p->cjob = (uint64_t *)malloc((nreg ? nreg : 1) * PLANES * 8);
     vc_status st = VC_OK;
-    if (!order || !kind || !rblk || !bl || !sums || !p->head || !p->blk || !p->boff || !p->jbase) {
+    if (!order || !kind || !rblk || !bl || !sums || !p->head || !p->blk || !p->boff || !p->cdst || !p->cjob) {
         st = VC_ERR_OOM;
         goto done;
     }
@@ -85,14 +89,22 @@ static vc_status plan_build(vc_archive *a, vcg_arc *d, rp_plan *p) {
         uint8_t *e = dir + ri * V2_DIR_ENTRY;
         uint32_t flags = kind[ri] == AF_ZERO ? V2_RGN_ZERO : V2_RGN_ABSENT;
         if (kind[ri] == AF_PRESENT) {
-            const uint64_t bytes = sums[3 * b], np = sums[3 * b + 1], nz = sums[3 * b + 2];
+            const uint64_t *sp = sums + 3 * PLANES * b;
+            uint64_t bytes = 0, np = 0, nz = 0;
+            for (uint32_t z = 0; z < PLANES; ++z) { bytes += sp[3 * z]; np += sp[3 * z + 1]; nz += sp[3 * z + 2]; }
             b++;
             if (np) {
                 const uint64_t blen = L2_BLOCK + bytes;
                 wr64(e, cursor);
                 wr32(e + 8, (uint32_t)blen);
                 flags = V2_RGN_DATA;
-                p->blk[p->ndata] = rblk[ri]; p->boff[p->ndata] = cursor; p->jbase[p->ndata] = jcur;
+                /* the planes' payloads follow one another in slot order */
+                uint64_t pc = cursor + L2_BLOCK, jc = jcur;
+                for (uint32_t z = 0; z < PLANES; ++z) {
+                    p->cdst[PLANES * p->ndata + z] = pc; p->cjob[PLANES * p->ndata + z] = jc;
+                    pc += sp[3 * z]; jc += sp[3 * z + 1];
+                }
+                p->blk[p->ndata] = rblk[ri]; p->boff[p->ndata] = cursor;
                 p->ndata++;
                 cursor += blen;
                 jcur += np;
@@ -121,7 +133,7 @@ done:
 
 static vc_status plan_write(vc_archive *a, vcg_arc *d, const rp_plan *p, uint8_t *dev_out) {
     if (vcg_copy(dev_out, p->head, p->blobs_off, 1) != 0) return VC_ERR_IO;
-    if (vcg_repack_write(d, p->ndata, p->blk, p->boff, p->jbase, p->njobs, dev_out) != 0) return VC_ERR_OOM;
+    if (vcg_repack_write(d, p->ndata, p->blk, p->boff, p->cdst, p->cjob, p->njobs, dev_out) != 0) return VC_ERR_OOM;
     (void)a;
     return VC_OK;
 }

@@ -98,15 +98,16 @@ int vcg_list_launch(vcg_arc *a, size_t n, const int32_t *quads_pinned, int32_t *
 size_t vcg_slot_scratch_bytes(void);
 int    vcg_reserve_slots(size_t n);
 /* Container rewrite (vcg_repack.hip), synchronous on the calling thread's
- * stream.  size: for the nblk slot arrays at archive offsets blk[], sums[3k ..
- * 3k+2] = payload bytes, present atoms, zero atoms; 0, 1 (a present payload
- * range outside the archive) or -1 (out of device memory).  write: for the
- * ndata data regions, the v2 slot array from blk[k] at dev_out + boff[k],
- * payloads after it in slot order (copy jobs numbered from jbase[k], njobs in
- * all); 0 or -1. */
+ * stream.  A slot array (32768 slots) is taken as 32 planes of 1024 slots.
+ * size: for the nblk slot arrays at archive offsets blk[], per plane p = 32k
+ * + z: sums[3p .. 3p+2] = payload bytes, present atoms, zero atoms; returns 0,
+ * 1 (a present payload range outside the archive) or -1 (out of device
+ * memory).  write: for the ndata data regions, the v2 slot array from blk[k]
+ * at dev_out + boff[k]; plane p's payloads placed from cdst[p] on in slot
+ * order, its copy jobs numbered from cjob[p] (njobs in all); 0 or -1. */
 int vcg_repack_size(vcg_arc *a, size_t nblk, const uint64_t *blk, uint64_t *sums);
-int vcg_repack_write(vcg_arc *a, size_t ndata, const uint64_t *blk, const uint64_t *boff, const uint64_t *jbase,
-                     uint64_t njobs, uint8_t *dev_out);
+int vcg_repack_write(vcg_arc *a, size_t ndata, const uint64_t *blk, const uint64_t *boff, const uint64_t *cdst,
+                     const uint64_t *cjob, uint64_t njobs, uint8_t *dev_out);
 /* Decode n raw payloads (no archive) at one base_q. */
 int vcg_decode_payloads(size_t n, const uint8_t *pay, const uint64_t *offs, const uint32_t *lens,
                         const int32_t dq_step[VCG_NFREQ], const int32_t dq_bias[VCG_NFREQ],
