@@ -100,6 +100,23 @@ vc_status vcx_decode_payloads(size_t n, const uint8_t *pay, const uint64_t *offs
  * memory. */
 vc_status vcx_synth_device(uint64_t seed, vc_dims dims, int nsheets, void *dev_out);
 
+/* ---- streaming archives decoded from HBM ----
+ * Extends vc_open_streaming (src/vc/vc.h:108-118; src/vc/vc.c:1325-1338):
+ * for a v2 archive, every decode call (vc_decode_atom, vc_decode_region,
+ * vc_decode_lod, vcx_decode_atoms and the device variants) first makes the
+ * regions it touches resident in a region-blob cache in HBM, fetching each
+ * missing region blob with ONE call of the read function
+ * (vc_region_blob_range's range), then decodes there.  Results are the bytes
+ * the in-memory archive gives.  Regions that cannot be served from their
+ * blob (a payload outside it, a failing read) and v1 archives keep the
+ * per-atom read path.  vcx_stream_cache sets the cache size in bytes (0:
+ * the archive size, at most 8 GiB) and empties the cache; call it when no
+ * other thread decodes from the archive.  A call whose blobs exceed the
+ * cache uses the per-atom path.  vcx_stream_stats: blob reads and cache
+ * evictions so far. */
+vc_status vcx_stream_cache(vc_archive *a, size_t bytes);
+vc_status vcx_stream_stats(const vc_archive *a, uint64_t *blob_reads, uint64_t *evictions);
+
 /* ---- container rewrite into the canonical v2 layout ----
  * Replaces tools/vc_repack.c (main, :184-291; a command-line tool in the
  * reference, no library entry point): the same bytes that tool writes for

@@ -232,6 +232,78 @@ extern "C" vcg_arc *vcg_arc_create(const uint8_t *buf, const vcg_geom *g) {
     return a;
 }
 
+// Region-blob arena of a streaming archive: g->len bytes of which the first
+// head_len come from head (header and directory); blobs arrive by vcg_arc_put.
+extern "C" vcg_arc *vcg_arc_create_arena(const vcg_geom *g, const uint8_t *head, uint64_t head_len) {
+    if (head_len > g->len) return nullptr;
+    vcg_ctx *c = vcg_get_ctx();
+    vcg_arc *a = new vcg_arc();
+    a->host = *g;
+    if (hipMalloc(&a->buf, g->len + 64) != hipSuccess ||
+        hipMalloc(&a->geom, sizeof(vcg_geom)) != hipSuccess) {
+        (void)hipGetLastError();
+        if (a->buf) (void)hipFree(a->buf);
+        delete a;
+        return nullptr;
+    }
+    VCG_CHECK(hipMemsetAsync(a->buf + g->len, 0, 64, c->s));
+    VCG_CHECK(hipMemcpyAsync(a->buf, head, head_len, hipMemcpyHostToDevice, c->s));
+    VCG_CHECK(hipMemcpyAsync(a->geom, g, sizeof(vcg_geom), hipMemcpyHostToDevice, c->s));
+    VCG_CHECK(hipStreamSynchronize(c->s));
+    return a;
+}
+
+extern "C" int vcg_arc_put(vcg_arc *a, uint64_t off, const void *src, uint64_t n) {
+    if (off > a->host.len || n > a->host.len - off) return -1;
+    vcg_ctx *c = vcg_get_ctx();
+    VCG_CHECK(hipMemcpyAsync(a->buf + off, src, n, hipMemcpyHostToDevice, c->s));
+    VCG_CHECK(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+// A blob read from file offset file_off now sits at arena offset at: point its
+// present slots at the arena.  A slot whose payload the reader would reject
+// (outside the file, or longer than the streaming bound src/vc/vc.c:1276-1277)
+// gets the invalid offset, which the decoder reports as a corrupt atom; one
+// whose payload lies in the file but outside this blob sets *irr.
+__global__ __launch_bounds__(1024) void k_blob_rebase(uint8_t *__restrict__ blob, uint64_t at, uint64_t file_off,
+                                                      uint32_t blob_len, uint64_t file_len, uint32_t *irr) {
+    const uint32_t s = blockIdx.x * 1024u + threadIdx.x;
+    uint4 *p = (uint4 *)(blob + (uint64_t)s * 16u);
+    uint4 v = *p;
+    if ((v.w & 0xFFu) != 1u) return;
+    const uint64_t o = (uint64_t)v.x | (uint64_t)v.y << 32;
+    const uint32_t l = v.z;
+    if (o == ~0ull || l < 1u) return;
+    uint64_t n;
+    if (o > file_len || (uint64_t)l > file_len - o || l > VCG_RC_CAP) {
+        n = ~0ull;
+    } else if (o < file_off + 32768ull * 16u || o + l > file_off + blob_len) {
+        atomicOr(irr, 1u);
+        return;
+    } else {
+        n = o - file_off + at;
+    }
+    v.x = (uint32_t)n; v.y = (uint32_t)(n >> 32);
+    *p = v;
+}
+
+extern "C" int vcg_arc_rebase(vcg_arc *a, uint64_t at, uint64_t file_off, uint32_t blob_len, uint64_t file_len) {
+    if ((at & 15u) || blob_len < 32768u * 16u || at > a->host.len || blob_len > a->host.len - at) return -1;
+    vcg_ctx *c = vcg_get_ctx();
+    uint32_t *dirr = nullptr, irr = 0;
+    if (hipMalloc(&dirr, 4) != hipSuccess) { (void)hipGetLastError(); return -1; }
+    VCG_CHECK(hipMemsetAsync(dirr, 0, 4, c->s));
+    hipLaunchKernelGGL(k_blob_rebase, dim3(32), dim3(1024), 0, c->s, a->buf + at, at, file_off, blob_len, file_len, dirr);
+    VCG_CHECK(hipGetLastError());
+    VCG_CHECK(hipMemcpyAsync(&irr, dirr, 4, hipMemcpyDeviceToHost, c->s));
+    VCG_CHECK(hipStreamSynchronize(c->s));
+    VCG_CHECK(hipFree(dirr));
+    return irr ? 1 : 0;
+}
+
+extern "C" void vcg_device_sync(void) { VCG_CHECK(hipDeviceSynchronize()); }
+
 extern "C" void vcg_arc_destroy(vcg_arc *a) {
     if (!a) return;
     VCG_CHECK(hipDeviceSynchronize());

@@ -64,6 +64,7 @@ vc_archive *vc_open_streaming(vc_read_fn read, void *ud, uint64_t total_len) {
 void vc_close(vc_archive *a) {
     if (!a) return;
     if (a->dev) vcg_arc_destroy(a->dev);
+    vc__stream_free(a);
     pthread_mutex_destroy(&a->mu);
     free(a);
 }
@@ -266,12 +267,18 @@ vc_status vcx_decode_atoms(vc_archive *a, size_t n, const int32_t *q, uint8_t *o
         if (q[4 * i] < 0 || q[4 * i] >= a->nlod || !atom_in_range(a, q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]))
             return VC_ERR_RANGE;
     if (n == 0) return VC_OK;
-    if (!a->buf) return vc__decode_atoms_host(a, n, q, out, per);
-    vcg_arc *d = vc__arc_dev(a);
-    if (!d) return VC_ERR_OOM;
+    vcg_arc *d;
+    if (!a->buf) {
+        d = vc__stream_begin(a, 0, NULL, n, q);
+        if (!d) return vc__decode_atoms_host(a, n, q, out, per);
+    } else {
+        d = vc__arc_dev(a);
+        if (!d) return VC_ERR_OOM;
+    }
     int32_t *st = (int32_t *)malloc(n * 4);
-    if (!st) return VC_ERR_OOM;
+    if (!st) { if (!a->buf) vc__stream_end(a); return VC_ERR_OOM; }
     int rc = vcg_decode_list(d, n, q, out, st);
+    if (!a->buf) vc__stream_end(a);
     for (size_t i = 0; i < n; ++i) if (per) per[i] = st[i] ? VC_ERR_FORMAT : VC_OK;
     free(st);
     return rc ? VC_ERR_FORMAT : VC_OK;
@@ -332,10 +339,16 @@ vc_status vc_decode_region(vc_archive *a, int lod, vc_box box, uint8_t *out) {
     if (!a || lod < 0 || lod >= a->nlod || !out) return VC_ERR_RANGE;
     vc_status s = region_check(a, lod, box);
     if (s != VC_OK) return s;
-    if (!a->buf) return region_host(a, lod, box, out);
+    vcg_box b = {box.x0, box.y0, box.z0, box.x1, box.y1, box.z1};
+    if (!a->buf) {
+        vcg_arc *d = vc__stream_begin(a, lod, &box, 0, NULL);
+        if (!d) return region_host(a, lod, box, out);
+        const int rc = vcg_decode_box(d, lod, &b, out, 1, 1);
+        vc__stream_end(a);
+        return rc ? VC_ERR_FORMAT : VC_OK;
+    }
     vcg_arc *d = vc__arc_dev(a);
     if (!d) return VC_ERR_OOM;
-    vcg_box b = {box.x0, box.y0, box.z0, box.x1, box.y1, box.z1};
     return vcg_decode_box(d, lod, &b, out, 1, 1) ? VC_ERR_FORMAT : VC_OK;
 }
 
@@ -351,10 +364,25 @@ vc_status vcx_decode_region_device(vc_archive *a, int lod, vc_box box, void *dev
     if (!a || lod < 0 || lod >= a->nlod || !dev_out) return VC_ERR_RANGE;
     vc_status s = region_check(a, lod, box);
     if (s != VC_OK) return s;
-    if (!a->buf) return VC_ERR_RANGE;    /* streaming archives have no device copy */
+    vcg_box b = {box.x0, box.y0, box.z0, box.x1, box.y1, box.z1};
+    if (!a->buf) {
+        vcg_arc *d = vc__stream_begin(a, lod, &box, 0, NULL);
+        if (d) {
+            const int rc = vcg_decode_box(d, lod, &b, (uint8_t *)dev_out, 0, sync);
+            vc__stream_end(a);
+            return rc ? VC_ERR_FORMAT : VC_OK;
+        }
+        /* host-resolved decode, then one copy into the device buffer */
+        const size_t nb = (size_t)(box.x1 - box.x0) * (box.y1 - box.y0) * (box.z1 - box.z0);
+        uint8_t *tmp = (uint8_t *)malloc(nb);
+        if (!tmp) return VC_ERR_OOM;
+        vc_status st = region_host(a, lod, box, tmp);
+        vcg_copy(dev_out, tmp, nb, 1);
+        free(tmp);
+        return st;
+    }
     vcg_arc *d = vc__arc_dev(a);
     if (!d) return VC_ERR_OOM;
-    vcg_box b = {box.x0, box.y0, box.z0, box.x1, box.y1, box.z1};
     return vcg_decode_box(d, lod, &b, (uint8_t *)dev_out, 0, sync) ? VC_ERR_FORMAT : VC_OK;
 }
 

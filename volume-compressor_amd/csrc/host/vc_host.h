@@ -74,6 +74,9 @@ struct vc_archive {
     float base_q[VC_NLOD];
     pthread_mutex_t mu;
     vcg_arc *dev;                       /* device copy, created on first GPU decode */
+    void *sdev;                         /* streaming v2: region-blob arena in HBM (vc_stream.c) */
+    int sdev_failed;
+    uint64_t stream_cache;              /* arena bytes requested (0: default) */
 };
 /* a region's slot array (src/vc/vc.c:1349-1360,1376-1400): AF_PRESENT with
  * its offset in *blk, AF_ZERO (all-zero region) or AF_ABSENT (also for a
@@ -86,6 +89,12 @@ int     vc__rd(const vc_archive *a, uint64_t off, uint32_t len, uint8_t *dst);
 vcg_arc *vc__arc_dev(vc_archive *a);   /* NULL on allocation failure */
 vc_status vc__region_check(const vc_archive *a, int lod, vc_box b);
 int     vc__atom_ok(const vc_archive *a, int lod, int ax, int ay, int az);
+/* streaming v2 archives: the arena holding every region the call touches
+ * (a box of LOD lod, or n quads), with the arena lock held until
+ * vc__stream_end; NULL: decode through the host-resolved path instead */
+vcg_arc *vc__stream_begin(vc_archive *a, int lod, const vc_box *box, size_t n, const int32_t *quads);
+void     vc__stream_end(vc_archive *a);
+void     vc__stream_free(vc_archive *a);
 /* host-resolved batched decode of whole atoms (streaming archives, writer views) */
 vc_status vc__decode_atoms_host(vc_archive *a, size_t n, const int32_t *quads, uint8_t *out,
                                 vc_status *per_atom);

@@ -66,6 +66,14 @@ void  vcg_host_free(void *p);
 typedef struct vcg_arc vcg_arc;
 vcg_arc *vcg_arc_create(const uint8_t *buf, const vcg_geom *g);   /* copies buf to HBM */
 void     vcg_arc_destroy(vcg_arc *a);
+/* Streaming archives: an arena of g->len bytes in HBM, head (header and
+ * directory, head_len bytes) at its start; region blobs are put into it and
+ * rebased there (present slots then address the arena; 0 done, 1 a payload
+ * lies in the file outside the blob, -1 bad arguments). */
+vcg_arc *vcg_arc_create_arena(const vcg_geom *g, const uint8_t *head, uint64_t head_len);
+int      vcg_arc_put(vcg_arc *a, uint64_t off, const void *src, uint64_t n);   /* synchronous; 0 / -1 */
+int      vcg_arc_rebase(vcg_arc *a, uint64_t at, uint64_t file_off, uint32_t blob_len, uint64_t file_len);
+void     vcg_device_sync(void);                                              /* every stream of the device */
 
 /* Decode every atom overlapping `box` (voxels of LOD `lod`) into out (box
  * raster).  out_host: out is host memory (staged through HBM).  sync: wait

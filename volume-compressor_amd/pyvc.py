@@ -72,6 +72,8 @@ def lib():
         "vcx_d2h": (i32, [vp, vp, sz]),
         "vcx_copy_kernel": (i32, [vp, vp, sz]),
         "vcx_repack": (i32, [vp, P(vp), P(sz)]),
+        "vcx_stream_cache": (i32, [vp, sz]),
+        "vcx_stream_stats": (i32, [vp, P(u64), P(u64)]),
         "vcx_repack_device": (i32, [vp, vp, sz, P(sz)]),
         "vcx_server_create": (vp, [vp, sz]),
         "vcx_server_destroy": (None, [vp]),
@@ -325,6 +327,16 @@ class StreamingArchive(Archive):
         self.h = self.L.vc_open_streaming(self._cb, None, len(self.src))
         if not self.h:
             raise ValueError("vc_open_streaming rejected the archive")
+
+    def set_cache(self, nbytes):
+        """vcx_stream_cache: region-blob cache size in HBM (0: default)"""
+        return self.L.vcx_stream_cache(self.h, nbytes)
+
+    def stats(self):
+        """vcx_stream_stats: (blob reads, cache evictions)"""
+        r, e = C.c_uint64(), C.c_uint64()
+        self.L.vcx_stream_stats(self.h, C.byref(r), C.byref(e))
+        return r.value, e.value
 
 
 # ---- writer -------------------------------------------------------------------
