@@ -22,7 +22,9 @@ Further sections, each beside its plain per-call counterpart:
                    17-600 voxels, LOD 0-7), p50/p99 latency; and the same
                    archive rewritten into the canonical v2 layout in device
                    memory (vcx_repack_device) beside the reference's own
-                   repack tool on the host;
+                   repack tool on the host; and opened with vc_open_streaming
+                   (LOD0 from the region-blob cache, cold and warm, and LOD1
+                   by per-atom reads);
   batched_encode : 256 device-resident volumes of uneven dims (axes uniform
                    in [40, 512]) at ratios {10, 20, 50, 100} in one
                    vcx_encode_batch call, against per-volume encodes.
@@ -308,8 +310,54 @@ def serving_section(pyvc, args):
     log(f"region serving: {sr}; plain {pr}")
     res["repack"] = repack_bench(pyvc, a, arc, no_cpu=args.no_cpu)
     log(f"repack: {res['repack']}")
+    res["streaming"] = stream_bench(pyvc, a, arc)
+    log(f"streaming: {res['streaming']}")
     a.close()
     return res
+
+
+def stream_bench(pyvc, a, arc):
+    """The serving archive opened with vc_open_streaming over a Python read
+    function on host memory: LOD0 decoded into device memory from the
+    region-blob cache, cold (every blob fetched by one read) and warm, beside
+    the in-memory archive; and LOD1 decoded by per-atom reads (the path the
+    cache replaces) with a cache too small for any blob."""
+    n = a.lod_dims(0)[1]
+    nb = n[0] * n[1] * n[2]
+    dev = pyvc.dev_alloc(nb)
+    out = {"workload": "2048^3 ratio-50 archive, vc_open_streaming over host memory", "archive_bytes": len(arc)}
+    try:
+        s = pyvc.StreamingArchive(arc)
+        t0 = time.perf_counter()
+        assert s.decode_lod_device(0, dev, 1) == 0
+        t_cold = time.perf_counter() - t0
+        reads, _ = s.stats()
+        t0 = time.perf_counter()
+        assert s.decode_lod_device(0, dev, 1) == 0
+        t_warm = time.perf_counter() - t0
+        got = pyvc.d2h(dev, nb)
+        t0 = time.perf_counter()
+        assert a.decode_lod_device(0, dev, 1) == 0
+        t_flat = time.perf_counter() - t0
+        same = np.array_equal(got, pyvc.d2h(dev, nb))
+        out.update({"lod0_cold_s": round(t_cold, 4), "lod0_warm_s": round(t_warm, 4), "lod0_in_memory_s": round(t_flat, 4),
+                    "lod0_cold_GBps": round(nb / t_cold / 1e9, 2), "blob_reads": reads, "read_calls": s.reads,
+                    "identical": bool(same)})
+        s.close()
+        s = pyvc.StreamingArchive(arc)
+        s.set_cache(1 << 20)                      # no blob fits: per-atom reads
+        d1 = s.lod_dims(1)[1]
+        t0 = time.perf_counter()
+        st, v1 = s.decode_lod(1)
+        t_atoms = time.perf_counter() - t0
+        st2, f1 = a.decode_lod(1)
+        out.update({"lod1_per_atom_reads_s": round(t_atoms, 3), "lod1_per_atom_read_calls": s.reads,
+                    "lod1_GBps_per_atom_reads": round(d1[0] * d1[1] * d1[2] / t_atoms / 1e9, 3),
+                    "lod1_identical": bool(st == st2 and np.array_equal(v1, f1))})
+        s.close()
+    finally:
+        pyvc.dev_free(dev)
+    return out
 
 
 def repack_bench(pyvc, a, arc, reps=3, no_cpu=False):
