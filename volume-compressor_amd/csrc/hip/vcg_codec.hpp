@@ -324,7 +324,7 @@ VCG_HD uint32_t vcg_fsm_step(vcg_fsm &f, Ctx &ctx, Ring &ring) {
     const uint32_t R = ctx.ld_next(a);
     // binary decision (vc.c:414-441)
     const uint32_t r0 = (uint32_t)(((uint64_t)f.range * (P << 20)) >> 32);
-    const bool bit = f.code >= r0;
+    const bool nbit = f.code < r0, bit = !nbit;
     f.code = bit ? f.code - r0 : f.code;
     f.range = (bit && isCtx) ? f.range - r0 : r0;
     // adaptation (vc.c:420-431): bit 1: P - (P>>5), bit 0: P + ((4096-P)>>5);
@@ -339,7 +339,7 @@ VCG_HD uint32_t vcg_fsm_step(vcg_fsm &f, Ctx &ctx, Ring &ring) {
     const bool sgn = S == VCG_S_SIGN;
     const uint32_t acc2 = 2u * f.acc + (bit ? 0u - 10u : 0u - 11u);
     ring.st(f.pc & ((VCG_RING - 1u) << 8), (f.pc & VCG_POS_MASK) | (acc2 & 0x1FFFFu));
-    const bool adv = (isSig & !bit) | sgn;
+    const bool adv = (isSig & nbit) | sgn;
     // one add moves the position (every new position) and the ring slot (an
     // emission, which is also a new position)
     f.pc += sgn ? 0x20100u : adv ? 0x20000u : 0u;
