@@ -8,7 +8,7 @@
  * volume-compressor, src/vc/vc.c unless noted) and cites the lines it follows.
  * Functions with no reference counterpart (the reference has no vc_encode,
  * vc_decode_lod, rate control or synthetic generator) restate the definitions
- * this build pins for them; their parity is "unpinned" by reference output and
+ * the library pins for them; their parity is "unpinned" by reference output and
  * is instead checked against the reference's primitives they compose (the
  * v1 writer + tools/vc_repack.c layout, vc_test_downsample2x) and against
  * closed forms.

@@ -1,7 +1,7 @@
 /* vc_encode.c — whole-volume encode.
  *
  * vc_encode is sketched but absent in the reference (README.md:51-52,
- * plan.txt:206-207); this build defines it by composing the reference's own
+ * plan.txt:206-207); here it is defined by composing the reference's own
  * primitives:
  *   pyramid   : LOD l+1 = integer rounded mean of the in-bounds voxels of each
  *               2x2x2 cell of LOD l (vc_test_downsample2x, src/vc/vc.c:1562-1577),
