@@ -116,3 +116,44 @@ def test_stream_device_api(pyvc, golden_repack):
         pyvc.dev_free(dev)
     st, want = f.decode_lod(0)
     assert st == 0 and np.array_equal(got, want)
+
+
+def test_stream_concurrent_callers(pyvc, golden_repack):
+    """Four host threads decode random boxes from one streaming archive whose
+    cache holds two blobs (so calls evict while others have work in flight);
+    every result equals the in-memory archive's."""
+    import threading
+    m, b = golden_repack["repack_multi"]
+    ents = _dir(b["v2"])
+    big = max(ln for _, ln, fl in ents if fl not in (0, 1))
+    flat, s = pyvc.Archive(b["v2"]), pyvc.StreamingArchive(b["v2"])
+    assert s.set_cache(4096 + 16 * len(ents) + 2 * big + 64) == 0
+    nlod = _nlod(flat)
+    rng = np.random.default_rng(12)
+    jobs = []
+    for _ in range(48):
+        lod = int(rng.integers(0, nlod))
+        _, (nx, ny, nz) = flat.lod_dims(lod)
+        x0, y0, z0 = (int(rng.integers(0, d)) for d in (nx, ny, nz))
+        box = (x0, y0, z0, min(nx, x0 + 1 + int(rng.integers(0, 300))), min(ny, y0 + 1 + int(rng.integers(0, 40))),
+               min(nz, z0 + 1 + int(rng.integers(0, 40))))
+        jobs.append((lod, box, flat.decode_region(lod, box)))
+    errors = []
+
+    def work(k):
+        try:
+            for lod, box, (st0, want) in jobs[k::4]:
+                st, got = s.decode_region(lod, box)
+                if st != st0 or not np.array_equal(got, want):
+                    errors.append((k, lod, box))
+        except Exception as e:          # noqa: BLE001 - reported below
+            errors.append((k, repr(e)))
+
+    th = [threading.Thread(target=work, args=(k,)) for k in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in th)
+    assert not errors, errors[:3]
+    say("stats", s.stats())
