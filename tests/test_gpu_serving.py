@@ -143,3 +143,29 @@ def test_server_corrupt_atom(pyvc, oracle, golden_containers):
     assert not outs[1][:A3].any() and np.array_equal(outs[1][A3:], good[1])
     st, ref = a.decode_region(0, (0, 0, 0, 64, 64, 32))
     assert st == 1 and np.array_equal(outs[2], ref.ravel())
+
+
+def test_recycled_context_starts_clean(pyvc, golden_containers):
+    """The server's worker decodes a corrupt atom (reported per request, so
+    its thread's status word is never collected) and exits; the next host
+    thread that starts decoding picks up that thread's context and must not
+    see the stale status: a valid decode there returns VC_OK."""
+    import threading
+    m, b = golden_containers["synth_64x64x64"]
+    buf = bytearray(b["v2"])
+    a0 = pyvc.Archive(bytes(buf))
+    c, off, ln = a0.payload_range(0, 0, 0, 1)
+    a0.close()
+    idx = bytes(buf).find(off.to_bytes(8, "little") + ln.to_bytes(4, "little") + b"\x01")
+    assert idx > 0
+    buf[idx:idx + 8] = (len(buf) - 2).to_bytes(8, "little")
+    bad = pyvc.Archive(bytes(buf))
+    outs, status = _serve(pyvc, bad, [("atoms", np.array([[0, 1, 0, 0]], np.int32))], 3 * 4 * pyvc.Server.atom_bytes())
+    assert status == {0: 1}
+    good = pyvc.Archive(b["v2"])
+    want = good.decode_atom(0, 1, 1, 0)
+    got = []
+    t = threading.Thread(target=lambda: got.append(good.decode_atom(0, 1, 1, 0)))
+    t.start()
+    t.join(timeout=60)
+    assert got and got[0][0] == 0 == want[0] and np.array_equal(got[0][1], want[1])

@@ -82,11 +82,25 @@ struct ctx_holder {
 };
 static thread_local ctx_holder tl_ctx;
 
+__global__ void k_reset_word(uint32_t *w);
+
 vcg_ctx *vcg_get_ctx() {
     if (tl_ctx.c) return tl_ctx.c;
+    vcg_ctx *r = nullptr;
     {
         std::lock_guard<std::mutex> g(g_pool_mu);
-        if (!g_pool.empty()) { tl_ctx.c = g_pool.back(); g_pool.pop_back(); return tl_ctx.c; }
+        if (!g_pool.empty()) { r = g_pool.back(); g_pool.pop_back(); }
+    }
+    if (r) {
+        // a recycled context starts clean: its last owner's work has finished
+        // and its status word no longer holds that owner's corrupt atoms (list
+        // launches report per item and never collect the word)
+        for (int k = 0; k < VCG_SLOTS; ++k) VCG_CHECK(hipStreamSynchronize(r->sE[k]));
+        hipLaunchKernelGGL(k_reset_word, dim3(1), dim3(1), 0, r->s, r->status_d);
+        VCG_CHECK(hipGetLastError());
+        VCG_CHECK(hipStreamSynchronize(r->s));
+        tl_ctx.c = r;
+        return r;
     }
     vcg_ctx *c = new vcg_ctx();
     VCG_CHECK(hipSetDevice(0));
