@@ -92,10 +92,12 @@ vcg_ctx *vcg_get_ctx() {
         if (!g_pool.empty()) { r = g_pool.back(); g_pool.pop_back(); }
     }
     if (r) {
-        // a recycled context starts clean: its last owner's work has finished
-        // and its status word no longer holds that owner's corrupt atoms (list
-        // launches report per item and never collect the word)
+        // a recycled context starts clean: its last owner's work has finished,
+        // its status word no longer holds that owner's corrupt atoms (list
+        // launches report per item and never collect the word), and the
+        // owner's stage events (which it may have destroyed) are dropped
         for (int k = 0; k < VCG_SLOTS; ++k) VCG_CHECK(hipStreamSynchronize(r->sE[k]));
+        for (int k = 0; k < 3; ++k) r->ev[k] = nullptr;
         hipLaunchKernelGGL(k_reset_word, dim3(1), dim3(1), 0, r->s, r->status_d);
         VCG_CHECK(hipGetLastError());
         VCG_CHECK(hipStreamSynchronize(r->s));
