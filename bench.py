@@ -25,6 +25,8 @@ Further sections, each beside its plain per-call counterpart:
                    repack tool on the host; and opened with vc_open_streaming
                    (LOD0 from the region-blob cache, cold and warm, and LOD1
                    by per-atom reads);
+  downscale      : the exporter's within-cell 2x downscale (cbox) of the
+                   1024^3 volume on the device (HBM-bound);
   batched_encode : 256 device-resident volumes of uneven dims (axes uniform
                    in [40, 512]) at ratios {10, 20, 50, 100} in one
                    vcx_encode_batch call, against per-volume encodes.
@@ -409,6 +411,27 @@ def repack_bench(pyvc, a, arc, reps=3, no_cpu=False):
     return out
 
 
+def ds_section(pyvc, hip, dvol, dout, n, reps=10):
+    """vcx_downscale2x_device (cbox, alpha 0.5, the exporter's default) of the
+    device-resident n^3 volume into dout, back to back on the library stream:
+    n^3 bytes read and n^3/8 written per call (HBM-bound)."""
+    L = pyvc.lib()
+    stream = L.vcx_stream()
+    d = pyvc.Dims(n, n, n)
+    assert L.vcx_downscale2x_device(dvol, d, 1, C.c_float(0.5), dout, 1) == 0
+    e0, e1 = hip.event(), hip.event()
+    hip.L.hipStreamSynchronize(stream)
+    hip.record(e0, stream)
+    for _ in range(reps):
+        L.vcx_downscale2x_device(dvol, d, 1, C.c_float(0.5), dout, 0)
+    hip.record(e1, stream)
+    ms = hip.ms(e0, e1) / reps
+    assert L.vcx_sync() == 0
+    moved = n ** 3 + (n // 2) ** 3
+    return {"workload": f"{n}^3 u8 -> {n // 2}^3, cbox alpha 0.5", "ms": round(ms, 4),
+            "GBps_moved": round(moved / ms / 1e6, 1), "frac_of_hbm": round(moved / ms / 1e6 / 8000, 3)}
+
+
 def ratio_section(pyvc, hip, o, dvol, dout, n, ratios, steps=10):
     """The same volume encoded at other target ratios: encode time, then
     back-to-back device LOD0 decodes (GB/s) and 16 sampled atoms against the
@@ -664,6 +687,11 @@ def main():
     ratios = None
     if not (args.no_ratios or args.profile):
         ratios = ratio_section(pyvc, hip, o, dvol, dout, n, [r for r in (10.0, 50.0, 100.0) if r != args.ratio])
+    # ---- the exporter's within-cell downscale of the same volume (HBM-bound) ----
+    ds = None
+    if not args.profile:
+        ds = ds_section(pyvc, hip, dvol, dout, n)
+        log(f"downscale: {ds}")
     pyvc.dev_free(dout)
     pyvc.dev_free(dvol)
     dout = dvol = None
@@ -720,6 +748,7 @@ def main():
         "serving": served,
         "encode_s": round(t_enc, 3),
         "other_ratios": ratios,
+        "downscale": ds,
         "batched_encode": enc_batch,
         "cpu_baseline": cpu,
     }

@@ -100,6 +100,20 @@ vc_status vcx_decode_payloads(size_t n, const uint8_t *pay, const uint64_t *offs
  * memory. */
 vc_status vcx_synth_device(uint64_t seed, vc_dims dims, int nsheets, void *dev_out);
 
+/* ---- the exporter's within-cell 2x downscale ----
+ * Replaces ds_downscale2x of tools/downscale.h:48-93 (a header used by the
+ * exporter tools/vc_export_stream.c:429-458,618) for device-resident data:
+ * dev_out (ceil(nx/2) x ceil(ny/2) x ceil(nz/2) bytes) receives each output
+ * voxel from its own 2x2x2 cell of dev_in, the odd boundary edge-replicated,
+ * by method VCX_DS_BOX (mean) or VCX_DS_CBOX (the mean pushed by alpha, 0..1,
+ * toward the cell's voxel of largest deviation); an all-zero cell gives 0.
+ * Float arithmetic in source order, as the reference compiles it without
+ * contraction.  sync: wait; otherwise ordered on vcx_stream().
+ * VC_ERR_RANGE for NULL pointers, a zero dimension, an unknown method or a
+ * NaN alpha. */
+enum { VCX_DS_BOX = 0, VCX_DS_CBOX = 1 };
+vc_status vcx_downscale2x_device(const void *dev_in, vc_dims in, int method, float alpha, void *dev_out, int sync);
+
 /* ---- streaming archives decoded from HBM ----
  * Extends vc_open_streaming (src/vc/vc.h:108-118; src/vc/vc.c:1325-1338):
  * for a v2 archive, every decode call (vc_decode_atom, vc_decode_region,

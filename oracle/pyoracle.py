@@ -64,6 +64,7 @@ class Oracle:
         L.vco_decode_atoms_omp.restype = C.c_double
         L.vco_synth.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, vp]
         L.vco_repack.argtypes = [vp, sz, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]
+        L.vco_ds2x.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_float, vp]
         self.libc = C.CDLL("libc.so.6")
         self.libc.free.argtypes = [C.c_void_p]
 
@@ -149,6 +150,14 @@ class Oracle:
         vol = np.ascontiguousarray(vol, np.uint8)
         out = np.zeros(((nz + 1) // 2, (ny + 1) // 2, (nx + 1) // 2), np.uint8)
         self.L.vco_downsample2x(vol.ctypes.data, nx, ny, nz, out.ctypes.data)
+        return out
+
+    def ds2x(self, vol, method, alpha):
+        """the exporter's within-cell 2x downscale (tools/downscale.h); method 0 box, 1 cbox"""
+        nz, ny, nx = vol.shape
+        vol = np.ascontiguousarray(vol, np.uint8)
+        out = np.zeros(((nz + 1) // 2, (ny + 1) // 2, (nx + 1) // 2), np.uint8)
+        self.L.vco_ds2x(vol.ctypes.data, nx, ny, nz, method, C.c_float(alpha), out.ctypes.data)
         return out
 
     def nlod(self, nx, ny, nz):
@@ -255,6 +264,24 @@ class OracleArchive:
 # ---------------------------------------------------------------------------
 # Reference build (oracle/_ref): the reference's own vc.c, compiled in place.
 # ---------------------------------------------------------------------------
+def ds_ref(variant="strict"):
+    """tools/downscale.h from the reference tree, built by oracle/ref.mk (None if absent):
+    a function (vol, method, alpha) -> downscaled volume"""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_ref", f"libdsref_{variant}.so")
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    L.ref_ds2x.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_void_p]
+
+    def run(vol, method, alpha):
+        nz, ny, nx = vol.shape
+        vol = np.ascontiguousarray(vol, np.uint8)
+        out = np.zeros(((nz + 1) // 2, (ny + 1) // 2, (nx + 1) // 2), np.uint8)
+        L.ref_ds2x(vol.ctypes.data, nx, ny, nz, method, C.c_float(alpha), out.ctypes.data)
+        return out
+    return run
+
+
 def ref_variants():
     if not os.path.isdir(REF_DIR):
         return {}

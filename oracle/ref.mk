@@ -19,6 +19,9 @@
 #   libvcref_base.so               the reference flags with no -march (x86-64
 #                                  baseline, no FMA instructions available).
 #   vc_repack                      tools/vc_repack.c linked to libvcref_strict.so.
+#   libdsref_fast.so / _strict.so  tools/downscale.h (the exporter's within-cell
+#                                  downscale) through oracle/ds_ref_shim.c, with
+#                                  the reference flags and with strict float.
 REF      ?= /root/reference
 OUT      ?= oracle/_ref
 SRC       = $(REF)/src/vc/vc.c
@@ -27,7 +30,7 @@ FASTF     = -O3 -ffp-contract=fast -ffast-math -funroll-all-loops -fopenmp
 LIBS      = -lm -lpthread
 
 all: $(OUT)/libvcref_fast_v3.so $(OUT)/libvcref_fast_v4.so $(OUT)/libvcref_strict.so \
-     $(OUT)/libvcref_base.so $(OUT)/vc_repack
+     $(OUT)/libvcref_base.so $(OUT)/vc_repack $(OUT)/libdsref_fast.so $(OUT)/libdsref_strict.so
 
 $(OUT):
 	mkdir -p $(OUT)
@@ -48,7 +51,13 @@ $(OUT)/vc_repack: $(REF)/tools/vc_repack.c $(OUT)/libvcref_strict.so | $(OUT)
 	gcc -std=c2x -O2 $(REF)/tools/vc_repack.c -o $@ -L$(OUT) -l:libvcref_strict.so \
 	    -Wl,-rpath,'$$ORIGIN' $(LIBS)
 
+$(OUT)/libdsref_fast.so: oracle/ds_ref_shim.c $(REF)/tools/downscale.h | $(OUT)
+	gcc -std=c2x -fPIC -shared $(FASTF) -I$(REF)/tools oracle/ds_ref_shim.c -o $@
+
+$(OUT)/libdsref_strict.so: oracle/ds_ref_shim.c $(REF)/tools/downscale.h | $(OUT)
+	gcc -std=c2x -fPIC -shared -O2 -ffp-contract=off -I$(REF)/tools oracle/ds_ref_shim.c -o $@
+
 clean:
-	rm -f $(OUT)/libvcref_*.so $(OUT)/vc_repack
+	rm -f $(OUT)/libvcref_*.so $(OUT)/libdsref_*.so $(OUT)/vc_repack
 
 .PHONY: all clean

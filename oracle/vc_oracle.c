@@ -359,6 +359,41 @@ void vco_downsample2x(const u8 *in, u32 nx, u32 ny, u32 nz, u8 *out) {
         out[((size_t)z * oy + y) * ox + x] = (u8)((acc + n / 2) / (n ? n : 1));
     }
 }
+/* within-cell 2x downscale of the exporter, tools/downscale.h:32-93: each
+ * output voxel from its own 2x2x2 cell, the odd boundary edge-replicated;
+ * method 0 box (float mean, +0.5, truncate), 1 cbox (the mean pushed by alpha
+ * toward the first voxel of largest deviation); an all-zero cell is 0.
+ * Strict float in source order (no contraction). */
+static u8 ds_u8(float v) { return (u8)(v < 0 ? 0 : v > 255 ? 255 : (v + 0.5f)); }
+void vco_ds2x(const u8 *in, u32 nx, u32 ny, u32 nz, int method, float alpha, u8 *out) {
+    u32 ox = (nx + 1) / 2, oy = (ny + 1) / 2, oz = (nz + 1) / 2;
+    if (ox < 1) ox = 1;
+    if (oy < 1) oy = 1;
+    if (oz < 1) oz = 1;
+    for (u32 z = 0; z < oz; ++z) for (u32 y = 0; y < oy; ++y) for (u32 x = 0; x < ox; ++x) {
+        u8 c[8]; int n = 0; unsigned acc = 0;
+        for (u32 dz = 0; dz < 2; ++dz) { u32 zz = 2 * z + dz; if (zz >= nz) zz = nz - 1;
+        for (u32 dy = 0; dy < 2; ++dy) { u32 yy = 2 * y + dy; if (yy >= ny) yy = ny - 1;
+        for (u32 dx = 0; dx < 2; ++dx) { u32 xx = 2 * x + dx; if (xx >= nx) xx = nx - 1;
+            c[n] = in[((size_t)zz * ny + yy) * nx + xx]; acc |= c[n]; n++; } } }
+        u8 r = 0;
+        if (acc) {
+            const float mean = (float)(c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7]) * 0.125f;
+            if (method == 0) {
+                r = ds_u8(mean);
+            } else {
+                float best = mean, bestdev = -1.0f;
+                for (int i = 0; i < 8; ++i) {
+                    const float d = (float)c[i] - mean, ad = d < 0 ? -d : d;
+                    if (ad > bestdev) { bestdev = ad; best = c[i]; }
+                }
+                r = ds_u8(mean + alpha * (best - mean));
+            }
+        }
+        out[((size_t)z * oy + y) * ox + x] = r;
+    }
+}
+
 /* atom gather with zero padding outside dims, tests/test_realdata.c:108-113 */
 void vco_gather_atom(const u8 *vol, u32 nx, u32 ny, u32 nz, u32 az, u32 ay, u32 ax, u8 atom[A3]) {
     for (u32 z = 0; z < A; ++z) for (u32 y = 0; y < A; ++y) for (u32 x = 0; x < A; ++x) {

@@ -60,6 +60,8 @@ int  vco_decode_payload(const uint8_t *pay, uint32_t len, float base_q, uint8_t 
 
 /* ---- pyramid (vc.c:1562-1577) ---- */
 void vco_downsample2x(const uint8_t *in, uint32_t nx, uint32_t ny, uint32_t nz, uint8_t *out);
+/* the exporter's within-cell downscale (tools/downscale.h): method 0 box, 1 cbox */
+void vco_ds2x(const uint8_t *in, uint32_t nx, uint32_t ny, uint32_t nz, int method, float alpha, uint8_t *out);
 int  vco_nlod(uint32_t nx, uint32_t ny, uint32_t nz);                    /* vc.c:1001-1006 */
 void vco_lod_dims(uint32_t nx, uint32_t ny, uint32_t nz, int lod,
                   uint32_t *ox, uint32_t *oy, uint32_t *oz);               /* vc.c:733-739 */

@@ -19,7 +19,11 @@ inputs plus the reference's outputs for them.
                   order with some left absent, some marked zero and one region
                   all zero, and v2 from tools/vc_repack.c on it
 
-usage: python tests/golden/make_golden.py [--only repack]
+  ds.json         SHA-256 of the exporter's within-cell downscale
+                  (tools/downscale.h, box and cbox at several alphas) on
+                  seeded inputs of odd and even shapes
+
+usage: python tests/golden/make_golden.py [--only repack|ds]
 """
 import gzip
 import hashlib
@@ -222,6 +226,36 @@ def make_containers(o, ref):
     json.dump(meta, open(os.path.join(OUT, "containers.json"), "w"), indent=1)
 
 
+DS_SHAPES = [(2, 64, 96), (7, 9, 11), (1, 1, 1), (33, 17, 5), (2, 130, 66), (40, 41, 43)]
+DS_ALPHAS = [0.0, 0.3, 0.5, 0.77, 1.0]
+
+
+def ds_input(shape, kind, seed):
+    """seeded downscale input: uniform bytes, 80% zeros, or sparse bright sheets on a dim background"""
+    rng = np.random.default_rng(seed)
+    v = rng.integers(0, 256, shape).astype(np.uint8)
+    if kind == "sparse":
+        v[rng.random(shape) < 0.8] = 0
+    elif kind == "sheet":
+        v = np.where(rng.random(shape) < 0.1, 200, 12).astype(np.uint8)
+    return v
+
+
+def make_ds():
+    from pyoracle import ds_ref
+    run = ds_ref("strict")
+    cases = []
+    for si, shp in enumerate(DS_SHAPES):
+        for ki, kind in enumerate(("rand", "sparse", "sheet")):
+            v = ds_input(shp, kind, 100 * si + ki)
+            for m in (0, 1):
+                for a in DS_ALPHAS:
+                    cases.append({"shape_zyx": list(shp), "kind": kind, "seed": 100 * si + ki, "method": m,
+                                  "alpha": a, "sha": sha(run(v, m, a))})
+    json.dump(cases, open(os.path.join(OUT, "ds.json"), "w"), indent=0)
+    print("ds cases", len(cases))
+
+
 def make_repack(o, ref):
     meta = {}
     name, (nz, ny, nx), seed = "repack_multi", (40, 72, 1100), 31
@@ -275,12 +309,16 @@ def main():
     if sys.argv[1:] == ["--only", "repack"]:
         make_repack(o, ref)
         return
+    if sys.argv[1:] == ["--only", "ds"]:
+        make_ds()
+        return
     ref_fast = Ref("fast_v4" if "fast_v4" in __import__("pyoracle").ref_variants() else "fast_v3")
     make_atoms(o, ref, ref_fast)
     make_corrupt(o, ref)
     make_downsample(o, ref)
     make_containers(o, ref)
     make_repack(o, ref)
+    make_ds()
 
 
 if __name__ == "__main__":

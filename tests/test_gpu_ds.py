@@ -1,0 +1,47 @@
+"""GPU: the exporter's within-cell 2x downscale (vcx_downscale2x_device)
+against the reference's tools/downscale.h (golden SHA-256 from its build) and
+the oracle's restatement."""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def test_ds2x_matches_reference(pyvc):
+    sys.path.insert(0, GOLDEN)
+    from make_golden import ds_input
+    for c in json.load(open(os.path.join(GOLDEN, "ds.json"))):
+        v = ds_input(tuple(c["shape_zyx"]), c["kind"], c["seed"])
+        st, out = pyvc.downscale2x(v, c["method"], c["alpha"])
+        assert st == 0 and sha(out) == c["sha"], c
+
+
+def test_ds2x_large_matches_oracle(pyvc, oracle):
+    vol = pyvc.synth(9, (514, 300, 131), 12)
+    for m, a in ((0, 0.0), (1, 0.5), (1, 1.0)):
+        st, out = pyvc.downscale2x(vol, m, a)
+        assert st == 0 and np.array_equal(out, oracle.ds2x(vol, m, a)), (m, a)
+
+
+def test_ds2x_argument_errors(pyvc):
+    import ctypes as C
+    L = pyvc.lib()
+    d = pyvc.dev_alloc(64)
+    try:
+        assert L.vcx_downscale2x_device(d, pyvc.Dims(4, 4, 4), 2, C.c_float(0.5), d, 1) == 2
+        assert L.vcx_downscale2x_device(d, pyvc.Dims(0, 4, 4), 0, C.c_float(0.5), d, 1) == 2
+        assert L.vcx_downscale2x_device(d, pyvc.Dims(4, 4, 4), 1, C.c_float(float("nan")), d, 1) == 2
+        assert L.vcx_downscale2x_device(None, pyvc.Dims(4, 4, 4), 0, C.c_float(0.5), d, 1) == 2
+    finally:
+        pyvc.dev_free(d)

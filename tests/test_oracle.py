@@ -386,3 +386,24 @@ def test_oracle_repack_rejects_payload_outside_archive(oracle, golden_repack):
             break
     st, out = oracle.repack(bytes(buf))
     assert st == 1 and out is None
+
+
+# ---------------------------------------------------------------- exporter downscale
+def test_oracle_ds2x_golden(oracle):
+    """vco_ds2x against the reference's tools/downscale.h (strict build) on
+    every golden case, and against the built reference variants directly."""
+    import json
+    import os
+    import sys
+    from conftest import GOLDEN
+    sys.path.insert(0, GOLDEN)
+    from make_golden import ds_input
+    from pyoracle import ds_ref
+    cases = json.load(open(os.path.join(GOLDEN, "ds.json")))
+    refs = [r for r in (ds_ref("strict"), ds_ref("fast")) if r]
+    for c in cases:
+        v = ds_input(tuple(c["shape_zyx"]), c["kind"], c["seed"])
+        got = oracle.ds2x(v, c["method"], c["alpha"])
+        assert sha(got) == c["sha"], c
+        for r in refs:
+            assert np.array_equal(got, r(v, c["method"], c["alpha"])), c

@@ -111,6 +111,44 @@ __global__ void k_downsample(const uint8_t *__restrict__ in, uint32_t nx, uint32
     }
 }
 
+// The exporter's within-cell 2x downscale (tools/downscale.h:32-93): output
+// voxel o from its own 2x2x2 cell, the odd boundary edge-replicated.  method 0
+// box (float mean + 0.5, truncated), 1 cbox (the mean pushed by alpha toward
+// the first voxel of largest deviation); an all-zero cell gives 0.  Float in
+// source order (the file is built without contraction).
+__device__ __forceinline__ uint8_t ds_u8(float v) { return (uint8_t)(v < 0.0f ? 0.0f : v > 255.0f ? 255.0f : v + 0.5f); }
+__global__ void k_ds2x(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz, int method, float alpha,
+                       uint8_t *__restrict__ out) {
+    const uint32_t ox = (nx + 1) / 2, oy = (ny + 1) / 2, oz = (nz + 1) / 2;
+    const uint64_t total = (uint64_t)ox * oy * oz;
+    for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
+         o += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = (uint32_t)(o % ox), y = (uint32_t)((o / ox) % oy), z = (uint32_t)(o / ((uint64_t)ox * oy));
+        const uint32_t x0 = 2 * x, x1 = 2 * x + 1 < nx ? 2 * x + 1 : nx - 1;
+        const uint32_t y0 = 2 * y, y1 = 2 * y + 1 < ny ? 2 * y + 1 : ny - 1;
+        const uint32_t z0 = 2 * z, z1 = 2 * z + 1 < nz ? 2 * z + 1 : nz - 1;
+        const uint8_t *r00 = in + ((uint64_t)z0 * ny + y0) * nx, *r01 = in + ((uint64_t)z0 * ny + y1) * nx;
+        const uint8_t *r10 = in + ((uint64_t)z1 * ny + y0) * nx, *r11 = in + ((uint64_t)z1 * ny + y1) * nx;
+        const uint32_t c[8] = {r00[x0], r00[x1], r01[x0], r01[x1], r10[x0], r10[x1], r11[x0], r11[x1]};
+        uint8_t r = 0;
+        if (c[0] | c[1] | c[2] | c[3] | c[4] | c[5] | c[6] | c[7]) {
+            const float mean = (float)(c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7]) * 0.125f;
+            if (method == 0) {
+                r = ds_u8(mean);
+            } else {
+                float best = mean, bestdev = -1.0f;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const float d = (float)c[i] - mean, ad = d < 0.0f ? -d : d;
+                    if (ad > bestdev) { bestdev = ad; best = (float)c[i]; }
+                }
+                r = ds_u8(mean + alpha * (best - mean));
+            }
+        }
+        out[o] = r;
+    }
+}
+
 enum { FDCT_STORE_COEF = 0, FDCT_QUANT = 1 };
 
 // Block i handles flat atom base + i of the segments (in FDCT_STORE_COEF
@@ -467,6 +505,18 @@ extern "C" int vcg_downsample(const void *in, const uint32_t d[3], void *out) {
     hipLaunchKernelGGL(k_downsample, dim3((uint32_t)blocks), dim3(256), 0, c->s, (const uint8_t *)in, d[0], d[1],
                        d[2], (uint8_t *)out);
     VCG_CHECK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int vcg_ds2x(const void *in, const uint32_t d[3], int method, float alpha, void *out, int sync) {
+    vcg_ctx *c = vcg_get_ctx();
+    const uint64_t total = (uint64_t)((d[0] + 1) / 2) * ((d[1] + 1) / 2) * ((d[2] + 1) / 2);
+    uint64_t blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_ds2x, dim3((uint32_t)blocks), dim3(256), 0, c->s, (const uint8_t *)in, d[0], d[1], d[2],
+                       method, alpha, (uint8_t *)out);
+    VCG_CHECK(hipGetLastError());
+    if (sync) VCG_CHECK(hipStreamSynchronize(c->s));
     return 0;
 }
 

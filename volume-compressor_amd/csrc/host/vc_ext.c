@@ -83,6 +83,16 @@ vc_status vcx_synth_device(uint64_t seed, vc_dims dims, int nsheets, void *dev_o
     return rc ? VC_ERR_OOM : VC_OK;
 }
 
+/* ---- the exporter's within-cell downscale (tools/downscale.h:32-93) ---- */
+vc_status vcx_downscale2x_device(const void *dev_in, vc_dims in, int method, float alpha, void *dev_out, int sync) {
+    if (!dev_in || !dev_out || !in.nx || !in.ny || !in.nz) return VC_ERR_RANGE;
+    if ((method != VCX_DS_BOX && method != VCX_DS_CBOX) || !(alpha == alpha)) return VC_ERR_RANGE;
+    vc__require_gpu();
+    const uint32_t d[3] = {in.nx, in.ny, in.nz};
+    vcg_ds2x(dev_in, d, method, alpha, dev_out, sync);
+    return VC_OK;
+}
+
 /* ---- test hooks (src/vc/vc.c:1557-1591), run on the GPU ---- */
 uint8_t *vc_test_downsample2x(const uint8_t *vol, vc_dims in, vc_dims *out) {
     vc__require_gpu();

@@ -166,6 +166,9 @@ int vcg_encode_atoms(size_t n, const uint8_t *vox, float base_q, uint8_t *out, s
                      uint32_t *lens);
 /* Device downsample for a volume already on device (LOD l -> l+1). */
 int vcg_downsample(const void *dev_in, const uint32_t dims[3], void *dev_out);
+/* the exporter's within-cell downscale (tools/downscale.h), method 0 box,
+ * 1 cbox; on the calling thread's stream (sync: wait) */
+int vcg_ds2x(const void *dev_in, const uint32_t dims[3], int method, float alpha, void *dev_out, int sync);
 
 /* ---- synthetic generator ---- */
 typedef struct { int32_t axis, c0, tu, tv, a1, a2, w1, w2, p1, p2, half, inten; } vcg_sheet;

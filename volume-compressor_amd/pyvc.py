@@ -73,6 +73,7 @@ def lib():
         "vcx_copy_kernel": (i32, [vp, vp, sz]),
         "vcx_repack": (i32, [vp, P(vp), P(sz)]),
         "vcx_stream_cache": (i32, [vp, sz]),
+        "vcx_downscale2x_device": (i32, [vp, Dims, i32, f32, vp, i32]),
         "vcx_stream_stats": (i32, [vp, P(u64), P(u64)]),
         "vcx_repack_device": (i32, [vp, vp, sz, P(sz)]),
         "vcx_server_create": (vp, [vp, sz]),
@@ -479,6 +480,21 @@ def h2d(dev, arr):
 
 def synth_device(seed, dims, nsheets, dev_out):
     return lib().vcx_synth_device(seed, Dims(*dims), nsheets, dev_out)
+
+
+def downscale2x(vol, method, alpha):
+    """vcx_downscale2x_device on a host volume (z, y, x): (status, output)"""
+    vol = _u8(vol)
+    nz, ny, nx = vol.shape
+    o = ((nz + 1) // 2, (ny + 1) // 2, (nx + 1) // 2)
+    di, do = dev_alloc(vol.size), dev_alloc(o[0] * o[1] * o[2])
+    try:
+        h2d(di, vol)
+        st = lib().vcx_downscale2x_device(di, Dims(nx, ny, nz), method, C.c_float(alpha), do, 1)
+        return st, (d2h(do, o[0] * o[1] * o[2]).reshape(o) if st == VC_OK else None)
+    finally:
+        dev_free(di)
+        dev_free(do)
 
 
 def synth(seed, dims, nsheets):
