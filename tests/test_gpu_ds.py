@@ -28,10 +28,13 @@ def test_ds2x_matches_reference(pyvc):
 
 
 def test_ds2x_large_matches_oracle(pyvc, oracle):
-    vol = pyvc.synth(9, (514, 300, 131), 12)
-    for m, a in ((0, 0.0), (1, 0.5), (1, 1.0)):
-        st, out = pyvc.downscale2x(vol, m, a)
-        assert st == 0 and np.array_equal(out, oracle.ds2x(vol, m, a)), (m, a)
+    """odd dims (one voxel per thread, edge replication) and even dims with x
+    a multiple of 8 (four voxels per thread from 8-byte row loads)"""
+    for dims in ((514, 300, 131), (256, 96, 64)):
+        vol = pyvc.synth(9, dims, 12)
+        for m, a in ((0, 0.0), (1, 0.5), (1, 1.0)):
+            st, out = pyvc.downscale2x(vol, m, a)
+            assert st == 0 and np.array_equal(out, oracle.ds2x(vol, m, a)), (dims, m, a)
 
 
 def test_ds2x_argument_errors(pyvc):

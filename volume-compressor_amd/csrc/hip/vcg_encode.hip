@@ -117,6 +117,43 @@ __global__ void k_downsample(const uint8_t *__restrict__ in, uint32_t nx, uint32
 // the first voxel of largest deviation); an all-zero cell gives 0.  Float in
 // source order (the file is built without contraction).
 __device__ __forceinline__ uint8_t ds_u8(float v) { return (uint8_t)(v < 0.0f ? 0.0f : v > 255.0f ? 255.0f : v + 0.5f); }
+__device__ __forceinline__ uint8_t ds_cell(const uint32_t c[8], int method, float alpha) {
+    if (!(c[0] | c[1] | c[2] | c[3] | c[4] | c[5] | c[6] | c[7])) return 0;
+    const float mean = (float)(c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7]) * 0.125f;
+    if (method == 0) return ds_u8(mean);
+    float best = mean, bestdev = -1.0f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float d = (float)c[i] - mean, ad = d < 0.0f ? -d : d;
+        if (ad > bestdev) { bestdev = ad; best = (float)c[i]; }
+    }
+    return ds_u8(mean + alpha * (best - mean));
+}
+// even dims, nx a multiple of 8 and 8-byte aligned rows: four output voxels
+// per thread from one 8-byte load of each of the cell's four input rows
+__global__ void k_ds2x_v(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz, int method,
+                         float alpha, uint8_t *__restrict__ out) {
+    const uint32_t ox = nx / 2, oy = ny / 2, oz = nz / 2, ox4 = ox / 4;
+    const uint64_t total = (uint64_t)ox4 * oy * oz;
+    for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
+         o += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t x4 = (uint32_t)(o % ox4), y = (uint32_t)((o / ox4) % oy), z = (uint32_t)(o / ((uint64_t)ox4 * oy));
+        const uint64_t b00 = ((uint64_t)(2 * z) * ny + 2 * y) * nx + 8u * x4;
+        const uint2 r00 = *(const uint2 *)(in + b00), r01 = *(const uint2 *)(in + b00 + nx);
+        const uint2 r10 = *(const uint2 *)(in + b00 + (uint64_t)ny * nx), r11 = *(const uint2 *)(in + b00 + (uint64_t)ny * nx + nx);
+        uint32_t packed = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t w00 = k < 2 ? r00.x : r00.y, w01 = k < 2 ? r01.x : r01.y;
+            const uint32_t w10 = k < 2 ? r10.x : r10.y, w11 = k < 2 ? r11.x : r11.y;
+            const uint32_t sh = 16u * (k & 1);
+            const uint32_t c[8] = {(w00 >> sh) & 255u, (w00 >> (sh + 8)) & 255u, (w01 >> sh) & 255u, (w01 >> (sh + 8)) & 255u,
+                                   (w10 >> sh) & 255u, (w10 >> (sh + 8)) & 255u, (w11 >> sh) & 255u, (w11 >> (sh + 8)) & 255u};
+            packed |= (uint32_t)ds_cell(c, method, alpha) << (8 * k);
+        }
+        *(uint32_t *)(out + ((uint64_t)z * oy + y) * ox + 4u * x4) = packed;
+    }
+}
 __global__ void k_ds2x(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz, int method, float alpha,
                        uint8_t *__restrict__ out) {
     const uint32_t ox = (nx + 1) / 2, oy = (ny + 1) / 2, oz = (nz + 1) / 2;
@@ -130,22 +167,7 @@ __global__ void k_ds2x(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny,
         const uint8_t *r00 = in + ((uint64_t)z0 * ny + y0) * nx, *r01 = in + ((uint64_t)z0 * ny + y1) * nx;
         const uint8_t *r10 = in + ((uint64_t)z1 * ny + y0) * nx, *r11 = in + ((uint64_t)z1 * ny + y1) * nx;
         const uint32_t c[8] = {r00[x0], r00[x1], r01[x0], r01[x1], r10[x0], r10[x1], r11[x0], r11[x1]};
-        uint8_t r = 0;
-        if (c[0] | c[1] | c[2] | c[3] | c[4] | c[5] | c[6] | c[7]) {
-            const float mean = (float)(c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7]) * 0.125f;
-            if (method == 0) {
-                r = ds_u8(mean);
-            } else {
-                float best = mean, bestdev = -1.0f;
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const float d = (float)c[i] - mean, ad = d < 0.0f ? -d : d;
-                    if (ad > bestdev) { bestdev = ad; best = (float)c[i]; }
-                }
-                r = ds_u8(mean + alpha * (best - mean));
-            }
-        }
-        out[o] = r;
+        out[o] = ds_cell(c, method, alpha);
     }
 }
 
@@ -510,11 +532,18 @@ extern "C" int vcg_downsample(const void *in, const uint32_t d[3], void *out) {
 
 extern "C" int vcg_ds2x(const void *in, const uint32_t d[3], int method, float alpha, void *out, int sync) {
     vcg_ctx *c = vcg_get_ctx();
-    const uint64_t total = (uint64_t)((d[0] + 1) / 2) * ((d[1] + 1) / 2) * ((d[2] + 1) / 2);
+    const bool vec = d[0] % 8 == 0 && d[1] % 2 == 0 && d[2] % 2 == 0 && ((uintptr_t)in & 7) == 0 &&
+                     ((uintptr_t)out & 3) == 0;
+    const uint64_t total = vec ? (uint64_t)(d[0] / 8) * (d[1] / 2) * (d[2] / 2)
+                               : (uint64_t)((d[0] + 1) / 2) * ((d[1] + 1) / 2) * ((d[2] + 1) / 2);
     uint64_t blocks = (total + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_ds2x, dim3((uint32_t)blocks), dim3(256), 0, c->s, (const uint8_t *)in, d[0], d[1], d[2],
-                       method, alpha, (uint8_t *)out);
+    if (vec)
+        hipLaunchKernelGGL(k_ds2x_v, dim3((uint32_t)blocks), dim3(256), 0, c->s, (const uint8_t *)in, d[0], d[1],
+                           d[2], method, alpha, (uint8_t *)out);
+    else
+        hipLaunchKernelGGL(k_ds2x, dim3((uint32_t)blocks), dim3(256), 0, c->s, (const uint8_t *)in, d[0], d[1], d[2],
+                           method, alpha, (uint8_t *)out);
     VCG_CHECK(hipGetLastError());
     if (sync) VCG_CHECK(hipStreamSynchronize(c->s));
     return 0;
