@@ -544,6 +544,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-ratios", action="store_true")
     ap.add_argument("--no-encode-batch", action="store_true")
+    ap.add_argument("--downscale-only", action="store_true",
+                    help="profiler run: only the within-cell downscale of the 1024^3 volume, its JSON on stdout")
     ap.add_argument("--repack-only", action="store_true",
                     help="profiler run: only the container rewrite of the 2048^3 ratio-50 archive, its JSON on stdout")
     ap.add_argument("--encode-batch-only", action="store_true",
@@ -562,6 +564,16 @@ def main():
     assert pyvc.device_init() == 0, "no HIP device"
     if args.encode_batch_only:
         print(json.dumps(encode_batch_section(pyvc, nseq=0)), flush=True)
+        return
+    if args.downscale_only:
+        n = args.size
+        dvol, dout = pyvc.dev_alloc(n ** 3), pyvc.dev_alloc((n // 2) ** 3)
+        try:
+            assert pyvc.synth_device(2, (n, n, n), args.sheets, dvol) == 0
+            print(json.dumps(ds_section(pyvc, hip, dvol, dout, n)), flush=True)
+        finally:
+            pyvc.dev_free(dvol)
+            pyvc.dev_free(dout)
         return
     if args.repack_only:
         n = args.serve_size
