@@ -129,15 +129,34 @@ __global__ __launch_bounds__(64 * RP_COPY_WAVES) void k_repack_copy(const uint8_
     const rp_job jb = jobs[j];
     const uint8_t *s = arc + jb.src;
     uint8_t *d = out + jb.dst;
-    // 16-byte pieces where source and destination share their alignment
-    uint32_t head = (uint32_t)((16u - ((uintptr_t)d & 15u)) & 15u);
-    if ((((uintptr_t)s ^ (uintptr_t)d) & 15u) != 0 || head >= jb.len) head = jb.len;
+    if ((((uintptr_t)s ^ (uintptr_t)d) & 15u) == 0) {
+        // 16-byte pieces where source and destination share their alignment
+        uint32_t head = (uint32_t)((16u - ((uintptr_t)d & 15u)) & 15u);
+        if (head > jb.len) head = jb.len;
+        for (uint32_t b = lane; b < head; b += 64u) d[b] = s[b];
+        const uint32_t nq = (jb.len - head) >> 4;
+        const uint4 *sq = (const uint4 *)(s + head);
+        uint4 *dq = (uint4 *)(d + head);
+        for (uint32_t q = lane; q < nq; q += 64u) dq[q] = sq[q];
+        for (uint32_t b = head + 16u * nq + lane; b < jb.len; b += 64u) d[b] = s[b];
+        return;
+    }
+    // otherwise aligned destination words, each from the two aligned source
+    // words it straddles (the archive buffer is padded past its end, so the
+    // second word of the last piece stays inside the allocation)
+    uint32_t head = (uint32_t)((4u - ((uintptr_t)d & 3u)) & 3u);
+    if (head > jb.len) head = jb.len;
     for (uint32_t b = lane; b < head; b += 64u) d[b] = s[b];
-    const uint32_t nq = (jb.len - head) >> 4;
-    const uint4 *sq = (const uint4 *)(s + head);
-    uint4 *dq = (uint4 *)(d + head);
-    for (uint32_t q = lane; q < nq; q += 64u) dq[q] = sq[q];
-    for (uint32_t b = head + 16u * nq + lane; b < jb.len; b += 64u) d[b] = s[b];
+    const uint8_t *sp = s + head;
+    const uint32_t r = (uint32_t)((uintptr_t)sp & 3u);
+    const uint32_t *sw = (const uint32_t *)(sp - r);
+    uint32_t *dw = (uint32_t *)(d + head);
+    const uint32_t nw = (jb.len - head) >> 2;
+    for (uint32_t k = lane; k < nw; k += 64u) {
+        const uint32_t lo = sw[k];
+        dw[k] = r ? __builtin_amdgcn_alignbyte(sw[k + 1], lo, r) : lo;
+    }
+    for (uint32_t b = head + 4u * nw + lane; b < jb.len; b += 64u) d[b] = s[b];
 }
 
 // ---------------------------------------------------------------------------
