@@ -126,8 +126,10 @@ vc_status vcx_downscale2x_device(const void *dev_in, vc_dims in, int method, flo
  * per-atom read path.  vcx_stream_cache sets the cache size in bytes (0:
  * the archive size, at most 8 GiB) and empties the cache; call it when no
  * other thread decodes from the archive.  A call whose blobs exceed the
- * cache uses the per-atom path.  vcx_stream_stats: blob reads and cache
- * evictions so far. */
+ * cache uses the per-atom path.  Decode calls on one streaming archive
+ * serialize their cache updates and launches; the read function is called
+ * with that lock held, so it must not decode from the same archive.
+ * vcx_stream_stats: blob reads and cache evictions so far. */
 vc_status vcx_stream_cache(vc_archive *a, size_t bytes);
 vc_status vcx_stream_stats(const vc_archive *a, uint64_t *blob_reads, uint64_t *evictions);
 
