@@ -8,7 +8,7 @@
 //   k_repack_slots : one workgroup per plane of a data region: its v2 slots,
 //                    output offsets assigned in slot order from the plane's
 //                    base (tools/vc_repack.c:131-162).
-//   k_repack_copy  : one wave per present atom: its payload, verbatim.
+//   k_repack_copy  : a wave per present atom at a time: its payload, verbatim.
 #include <hip/hip_runtime.h>
 #include <string.h>
 #include "vcg_dev.hpp"
@@ -120,13 +120,9 @@ __global__ __launch_bounds__(RP_THREADS) void k_repack_slots(const uint8_t *__re
 
 // one wave per payload (a payload is a few KiB at most)
 #define RP_COPY_WAVES 4
-__global__ __launch_bounds__(64 * RP_COPY_WAVES) void k_repack_copy(const uint8_t *__restrict__ arc,
-                                                                    const rp_job *__restrict__ jobs,
-                                                                    uint64_t njobs, uint8_t *__restrict__ out) {
-    const uint64_t j = (uint64_t)blockIdx.x * RP_COPY_WAVES + (threadIdx.x >> 6);
-    if (j >= njobs) return;
-    const uint32_t lane = threadIdx.x & 63u;
-    const rp_job jb = jobs[j];
+#define RP_COPY_GRID 4096u                 // 16 workgroups per CU
+__device__ __forceinline__ void rp_copy_job(const uint8_t *__restrict__ arc, const rp_job jb, uint32_t lane,
+                                            uint8_t *__restrict__ out) {
     const uint8_t *s = arc + jb.src;
     uint8_t *d = out + jb.dst;
     if ((((uintptr_t)s ^ (uintptr_t)d) & 15u) == 0) {
@@ -157,6 +153,17 @@ __global__ __launch_bounds__(64 * RP_COPY_WAVES) void k_repack_copy(const uint8_
         dw[k] = r ? __builtin_amdgcn_alignbyte(sw[k + 1], lo, r) : lo;
     }
     for (uint32_t b = head + 4u * nw + lane; b < jb.len; b += 64u) d[b] = s[b];
+}
+
+// waves stride over the jobs: a grid of a few thousand workgroups instead of
+// one short-lived wave per payload
+__global__ __launch_bounds__(64 * RP_COPY_WAVES) void k_repack_copy(const uint8_t *__restrict__ arc,
+                                                                    const rp_job *__restrict__ jobs,
+                                                                    uint64_t njobs, uint8_t *__restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * RP_COPY_WAVES;
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint64_t j = (uint64_t)blockIdx.x * RP_COPY_WAVES + (threadIdx.x >> 6); j < njobs; j += stride)
+        rp_copy_job(arc, jobs[j], lane, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -216,11 +223,10 @@ extern "C" int vcg_repack_write(vcg_arc *a, size_t ndata, const uint64_t *blk, c
                            dboff + b / RP_CHUNKS, dcdst + b, dcjob + b, dev_out, djobs);
         VCG_CHECK(hipGetLastError());
     }
-    const uint64_t per = 0x7FFFFFFFull * RP_COPY_WAVES;   // grid x limit
-    for (uint64_t b = 0; b < njobs; b += per) {
-        const uint64_t n = njobs - b < per ? njobs - b : per;
-        hipLaunchKernelGGL(k_repack_copy, dim3((uint32_t)((n + RP_COPY_WAVES - 1) / RP_COPY_WAVES)),
-                           dim3(64 * RP_COPY_WAVES), 0, c->s, arc, djobs + b, n, dev_out);
+    if (njobs) {
+        const uint64_t want = (njobs + RP_COPY_WAVES - 1) / RP_COPY_WAVES;
+        hipLaunchKernelGGL(k_repack_copy, dim3((uint32_t)(want < RP_COPY_GRID ? want : RP_COPY_GRID)),
+                           dim3(64 * RP_COPY_WAVES), 0, c->s, arc, djobs, njobs, dev_out);
         VCG_CHECK(hipGetLastError());
     }
     VCG_CHECK(hipStreamSynchronize(c->s));
