@@ -162,8 +162,19 @@ __global__ __launch_bounds__(64 * RP_COPY_WAVES) void k_repack_copy(const uint8_
                                                                     uint64_t njobs, uint8_t *__restrict__ out) {
     const uint64_t stride = (uint64_t)gridDim.x * RP_COPY_WAVES;
     const uint32_t lane = threadIdx.x & 63u;
-    for (uint64_t j = (uint64_t)blockIdx.x * RP_COPY_WAVES + (threadIdx.x >> 6); j < njobs; j += stride)
-        rp_copy_job(arc, jobs[j], lane, out);
+    uint64_t j = (uint64_t)blockIdx.x * RP_COPY_WAVES + (threadIdx.x >> 6);
+    if (j >= njobs) return;
+    rp_job jb = jobs[j];
+    for (;;) {
+        // the next job's record is in flight while this payload is copied
+        const uint64_t jn = j + stride;
+        rp_job nx = jb;
+        if (jn < njobs) nx = jobs[jn];
+        rp_copy_job(arc, jb, lane, out);
+        if (jn >= njobs) break;
+        jb = nx;
+        j = jn;
+    }
 }
 
 // ---------------------------------------------------------------------------
