@@ -131,13 +131,15 @@ __device__ __forceinline__ uint8_t ds_cell(const uint32_t c[8], int method, floa
 }
 // even dims, nx a multiple of 8 and 8-byte aligned rows: four output voxels
 // per thread from one 8-byte load of each of the cell's four input rows
-__global__ void k_ds2x_v(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz, int method,
-                         float alpha, uint8_t *__restrict__ out) {
+// (index math in IT: 32-bit when the output count allows, 64-bit otherwise)
+template <typename IT>
+__device__ __forceinline__ void ds2x_v_body(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
+                                            int method, float alpha, uint8_t *__restrict__ out) {
     const uint32_t ox = nx / 2, oy = ny / 2, oz = nz / 2, ox4 = ox / 4;
-    const uint64_t total = (uint64_t)ox4 * oy * oz;
-    for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
-         o += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t x4 = (uint32_t)(o % ox4), y = (uint32_t)((o / ox4) % oy), z = (uint32_t)(o / ((uint64_t)ox4 * oy));
+    const IT total = (IT)ox4 * oy * oz;
+    for (IT o = (IT)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += (IT)gridDim.x * blockDim.x) {
+        const IT row = o / ox4;
+        const uint32_t x4 = (uint32_t)(o - row * ox4), z = (uint32_t)(row / oy), y = (uint32_t)(row - (IT)z * oy);
         const uint64_t b00 = ((uint64_t)(2 * z) * ny + 2 * y) * nx + 8u * x4;
         const uint2 r00 = *(const uint2 *)(in + b00), r01 = *(const uint2 *)(in + b00 + nx);
         const uint2 r10 = *(const uint2 *)(in + b00 + (uint64_t)ny * nx), r11 = *(const uint2 *)(in + b00 + (uint64_t)ny * nx + nx);
@@ -153,6 +155,14 @@ __global__ void k_ds2x_v(const uint8_t *__restrict__ in, uint32_t nx, uint32_t n
         }
         *(uint32_t *)(out + ((uint64_t)z * oy + y) * ox + 4u * x4) = packed;
     }
+}
+__global__ void k_ds2x_v(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz, int method,
+                         float alpha, uint8_t *__restrict__ out) {
+    // the grid-stride sum stays below 2^32 too: the grid is at most 2^24 threads
+    if ((uint64_t)(nx / 8) * (ny / 2) * (nz / 2) + (1ull << 24) < (1ull << 32))
+        ds2x_v_body<uint32_t>(in, nx, ny, nz, method, alpha, out);
+    else
+        ds2x_v_body<uint64_t>(in, nx, ny, nz, method, alpha, out);
 }
 __global__ void k_ds2x(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz, int method, float alpha,
                        uint8_t *__restrict__ out) {
