@@ -88,13 +88,27 @@ __device__ __forceinline__ uint32_t seg_of(const enc_seg *seg, uint32_t n, uint6
 }
 
 // ---------------------------------------------------------------------------
-__global__ void k_downsample(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
-                             uint8_t *__restrict__ out) {
+// Output voxel o of an ox x oy x oz raster as (x, y, z), in the index type IT.
+// The 2x kernels below run grid-stride loops over at most 2^24 threads and
+// take IT = uint32_t when the output count plus that stays below 2^32 (64-bit
+// division costs several times the 32-bit one).
+template <typename IT>
+__device__ __forceinline__ void ds_split(IT o, uint32_t ox, uint32_t oy, uint32_t &x, uint32_t &y, uint32_t &z) {
+    const IT row = o / ox;
+    x = (uint32_t)(o - row * ox);
+    z = (uint32_t)(row / oy);
+    y = (uint32_t)(row - (IT)z * oy);
+}
+__host__ __device__ __forceinline__ bool ds_small(uint64_t total) { return total + (1ull << 24) < (1ull << 32); }
+
+template <typename IT>
+__device__ __forceinline__ void downsample_body(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny,
+                                                uint32_t nz, uint8_t *__restrict__ out) {
     const uint32_t ox = (nx + 1) / 2, oy = (ny + 1) / 2, oz = (nz + 1) / 2;
-    const uint64_t total = (uint64_t)ox * oy * oz;
-    for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
-         o += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t x = (uint32_t)(o % ox), y = (uint32_t)((o / ox) % oy), z = (uint32_t)(o / ((uint64_t)ox * oy));
+    const IT total = (IT)ox * oy * oz;
+    for (IT o = (IT)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += (IT)gridDim.x * blockDim.x) {
+        uint32_t x, y, z;
+        ds_split<IT>(o, ox, oy, x, y, z);
         uint32_t acc = 0, n = 0;
         for (uint32_t dz = 0; dz < 2; ++dz) {
             uint32_t zz = 2 * z + dz; if (zz >= nz) continue;
@@ -109,6 +123,13 @@ __global__ void k_downsample(const uint8_t *__restrict__ in, uint32_t nx, uint32
         }
         out[o] = (uint8_t)((acc + n / 2) / n);
     }
+}
+__global__ void k_downsample(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
+                             uint8_t *__restrict__ out) {
+    if (ds_small((uint64_t)((nx + 1) / 2) * ((ny + 1) / 2) * ((nz + 1) / 2)))
+        downsample_body<uint32_t>(in, nx, ny, nz, out);
+    else
+        downsample_body<uint64_t>(in, nx, ny, nz, out);
 }
 
 // The exporter's within-cell 2x downscale (tools/downscale.h:32-93): output
@@ -131,15 +152,14 @@ __device__ __forceinline__ uint8_t ds_cell(const uint32_t c[8], int method, floa
 }
 // even dims, nx a multiple of 8 and 8-byte aligned rows: four output voxels
 // per thread from one 8-byte load of each of the cell's four input rows
-// (index math in IT: 32-bit when the output count allows, 64-bit otherwise)
 template <typename IT>
 __device__ __forceinline__ void ds2x_v_body(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
                                             int method, float alpha, uint8_t *__restrict__ out) {
     const uint32_t ox = nx / 2, oy = ny / 2, oz = nz / 2, ox4 = ox / 4;
     const IT total = (IT)ox4 * oy * oz;
     for (IT o = (IT)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += (IT)gridDim.x * blockDim.x) {
-        const IT row = o / ox4;
-        const uint32_t x4 = (uint32_t)(o - row * ox4), z = (uint32_t)(row / oy), y = (uint32_t)(row - (IT)z * oy);
+        uint32_t x4, y, z;
+        ds_split<IT>(o, ox4, oy, x4, y, z);
         const uint64_t b00 = ((uint64_t)(2 * z) * ny + 2 * y) * nx + 8u * x4;
         const uint2 r00 = *(const uint2 *)(in + b00), r01 = *(const uint2 *)(in + b00 + nx);
         const uint2 r10 = *(const uint2 *)(in + b00 + (uint64_t)ny * nx), r11 = *(const uint2 *)(in + b00 + (uint64_t)ny * nx + nx);
@@ -158,19 +178,19 @@ __device__ __forceinline__ void ds2x_v_body(const uint8_t *__restrict__ in, uint
 }
 __global__ void k_ds2x_v(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz, int method,
                          float alpha, uint8_t *__restrict__ out) {
-    // the grid-stride sum stays below 2^32 too: the grid is at most 2^24 threads
-    if ((uint64_t)(nx / 8) * (ny / 2) * (nz / 2) + (1ull << 24) < (1ull << 32))
+    if (ds_small((uint64_t)(nx / 8) * (ny / 2) * (nz / 2)))
         ds2x_v_body<uint32_t>(in, nx, ny, nz, method, alpha, out);
     else
         ds2x_v_body<uint64_t>(in, nx, ny, nz, method, alpha, out);
 }
-__global__ void k_ds2x(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz, int method, float alpha,
-                       uint8_t *__restrict__ out) {
+template <typename IT>
+__device__ __forceinline__ void ds2x_body(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
+                                          int method, float alpha, uint8_t *__restrict__ out) {
     const uint32_t ox = (nx + 1) / 2, oy = (ny + 1) / 2, oz = (nz + 1) / 2;
-    const uint64_t total = (uint64_t)ox * oy * oz;
-    for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
-         o += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t x = (uint32_t)(o % ox), y = (uint32_t)((o / ox) % oy), z = (uint32_t)(o / ((uint64_t)ox * oy));
+    const IT total = (IT)ox * oy * oz;
+    for (IT o = (IT)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += (IT)gridDim.x * blockDim.x) {
+        uint32_t x, y, z;
+        ds_split<IT>(o, ox, oy, x, y, z);
         const uint32_t x0 = 2 * x, x1 = 2 * x + 1 < nx ? 2 * x + 1 : nx - 1;
         const uint32_t y0 = 2 * y, y1 = 2 * y + 1 < ny ? 2 * y + 1 : ny - 1;
         const uint32_t z0 = 2 * z, z1 = 2 * z + 1 < nz ? 2 * z + 1 : nz - 1;
@@ -179,6 +199,13 @@ __global__ void k_ds2x(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny,
         const uint32_t c[8] = {r00[x0], r00[x1], r01[x0], r01[x1], r10[x0], r10[x1], r11[x0], r11[x1]};
         out[o] = ds_cell(c, method, alpha);
     }
+}
+__global__ void k_ds2x(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz, int method, float alpha,
+                       uint8_t *__restrict__ out) {
+    if (ds_small((uint64_t)((nx + 1) / 2) * ((ny + 1) / 2) * ((nz + 1) / 2)))
+        ds2x_body<uint32_t>(in, nx, ny, nz, method, alpha, out);
+    else
+        ds2x_body<uint64_t>(in, nx, ny, nz, method, alpha, out);
 }
 
 enum { FDCT_STORE_COEF = 0, FDCT_QUANT = 1 };
