@@ -674,13 +674,15 @@ def main():
     assert L.vcx_sync() == 0
     slot_bytes = 16 * natoms
     region_bytes = 16 * ((nax + 31) // 32) ** 3
-    alg_k1 = pay_bytes + slot_bytes + region_bytes           # entropy: payload + index reads
-    alg_k2 = nvox                                            # transform: voxel writes
-    alg_step = pay_bytes + slot_bytes + region_bytes + nvox  # whole decode (BASELINE.md formula)
+    # algorithmic bytes of one decode of the LOD0 (32768 atoms: one launch of
+    # each kernel): every touched atom's payload + its 16-byte slot, 16 bytes
+    # per touched index region, and the voxels written (1 byte each)
+    alg_step = pay_bytes + slot_bytes + region_bytes + nvox
     if k1_ms >= k2_ms:
-        dom, dom_ms, dom_bytes = "k_entropy", k1_ms, alg_k1
+        dom, dom_ms = "k_entropy", k1_ms
     else:
-        dom, dom_ms, dom_bytes = "k_idct", k2_ms, alg_k2
+        dom, dom_ms = "k_idct", k2_ms
+    dom_bytes = alg_step
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
     log(f"step {ms_step:.3f} ms ({gbs:.1f} GB/s); k_entropy {k1_ms:.3f} ms, k_idct {k2_ms:.3f} ms")
 
