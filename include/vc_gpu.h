@@ -49,6 +49,30 @@ vc_status vcx_sync(void);
 vc_status vcx_decode_atoms(vc_archive *a, size_t n, const int32_t *lod_ax_ay_az,
                            uint8_t *out, vc_status *per_atom);
 
+/* ---- allocation-free batched atom decode ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): vcx_decode_atoms with inputs and outputs in device
+ * memory, enqueued on the caller's stream, calling no hipMalloc, hipFree or
+ * other allocating routine.  The decoder's scratch is a static device array
+ * of VCX_NOALLOC_MAX_ATOMS atoms (about 8.4 MB per GPU, held until the
+ * process ends); larger n is VC_ERR_RANGE (loop over chunks on one stream).
+ * Every call on a GPU shares that array: calls that overlap on one GPU (on
+ * different streams, or from different threads) race and must be ordered by
+ * the caller.
+ * quads_dev: n (lod, ax, ay, az) quads in device memory, checked on the
+ * device; out_dev: n * VC_ATOM3 bytes, atom i at i * VC_ATOM3, the bytes
+ * vc_decode_atom returns; status_dev (optional, n int32): 0 OK, 1 corrupt
+ * payload (atom zeroed; vc_decode_atom's VC_ERR_FORMAT), 2 quad out of range
+ * (atom zeroed; VC_ERR_RANGE).  stream: a hipStream_t (NULL: the null stream)
+ * of the library's device.  Returns after enqueueing: VC_OK, or VC_ERR_RANGE
+ * for NULL pointers, n > VCX_NOALLOC_MAX_ATOMS, a streaming archive, or an
+ * archive without its device copy yet (any earlier device decode of the
+ * archive, e.g. one vc_decode_atom, makes it). */
+#define VCX_NOALLOC_MAX_ATOMS 64
+vc_status vcx_decode_atoms_noalloc(vc_archive *a, size_t n, const int32_t *quads_dev, uint8_t *out_dev,
+                                   int32_t *status_dev, void *stream);
+
 /* Stage events: while set, each decode call records events[0] before its
  * entropy kernel, events[1] between the entropy and transform kernels and
  * events[2] after the transform kernel, on the calling thread's stream.

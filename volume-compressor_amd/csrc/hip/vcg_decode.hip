@@ -31,6 +31,7 @@ struct dec_job {
     const int32_t *dqs;         // [8][94] (box/list: geom tables; payload: [94])
     const int32_t *dqb;
     int mode, lod;
+    int check;                          // list mode: quads come unchecked (device memory)
     uint32_t ax0, ay0, az0, nax, nay;   // box mode atom range
     uint32_t base, n;                   // item range of this launch
     const int32_t *quads;               // list mode: (lod | placement << 8, ax, ay, az)
@@ -43,7 +44,11 @@ struct dec_job {
     uint32_t bx0, by0, bz0, bx1, by1, bz1;   // box mode: voxel box
 };
 
-__device__ __forceinline__ void item_atom(const dec_job &j, uint32_t gi, int &lod, uint32_t &ax,
+// Returns false for an unchecked list quad outside the archive (lod not
+// below nlod or an atom coordinate outside that LOD's atom grid, the range
+// vc_decode_atom rejects, src/vc/vc.c:1406-1409); its lod and coordinates
+// then read as 0 so that no table is indexed out of range.
+__device__ __forceinline__ bool item_atom(const dec_job &j, uint32_t gi, int &lod, uint32_t &ax,
                                           uint32_t &ay, uint32_t &az) {
     if (j.mode == MODE_BOX) {
         lod = j.lod;
@@ -51,11 +56,18 @@ __device__ __forceinline__ void item_atom(const dec_job &j, uint32_t gi, int &lo
         ay = j.ay0 + (gi / j.nax) % j.nay;
         az = j.az0 + gi / (j.nax * j.nay);
     } else if (j.mode == MODE_LIST) {
-        lod = j.quads[4 * gi] & 0xFF; ax = (uint32_t)j.quads[4 * gi + 1];
+        const int32_t w = j.quads[4 * gi];
+        lod = w & 0xFF; ax = (uint32_t)j.quads[4 * gi + 1];
         ay = (uint32_t)j.quads[4 * gi + 2]; az = (uint32_t)j.quads[4 * gi + 3];
+        if (j.check && (w < 0 || w >= j.geom->nlod || ax >= j.geom->acx[w] || ay >= j.geom->acy[w] ||
+                        az >= j.geom->acz[w])) {
+            lod = 0; ax = ay = az = 0;
+            return false;
+        }
     } else {
         lod = 0; ax = ay = az = 0;
     }
+    return true;
 }
 
 // per-lane LDS array (context memory: VCG_CTX_SLOTS slots; output ring: 8 slots),
@@ -120,13 +132,13 @@ __global__ __launch_bounds__(ENT_THREADS) void k_entropy(dec_job j, const vcg_dt
         r = len >= 1 ? 1 : 0;
     } else {
         uint32_t ax, ay, az;
-        item_atom(j, gi, lod, ax, ay, az);
-        r = vcg_resolve(j.arc, j.geom, lod, az, ay, ax, &off, &len);
+        r = item_atom(j, gi, lod, ax, ay, az) ? vcg_resolve(j.arc, j.geom, lod, az, ay, ax, &off, &len) : 3;
     }
     vcg_item it;
     it.cnt = 0; it.lod = (uint16_t)lod; it.dc = 0; it.fmax = -1;
     if (r != 1) {
-        it.status = r == 2 ? 2u : 0u;
+        // 0: zeros; 2: corrupt slot (zeros, status word); 3: out of range (zeros)
+        it.status = r == 2 ? 2u : r == 3 ? 3u : 0u;
         items[i] = it;
         if (r == 2) atomicOr(status, 1u);
         return;
@@ -286,7 +298,7 @@ __global__ __launch_bounds__(IDCT_THREADS) void k_idct(dec_job j, const vcg_dtab
     }
     const vcg_item it = items[i];
     uint8_t *stage = (uint8_t *)buf;
-    if (tid == 0 && j.status_out) j.status_out[gi] = it.status == 2 ? 1 : 0;
+    if (tid == 0 && j.status_out) j.status_out[gi] = it.status == 2 ? 1 : it.status == 3 ? 2 : 0;
     if (it.status != 1) {
         for (int v = tid; v < (int)VCG_A3 / 4; v += IDCT_THREADS) ((uint32_t *)stage)[v] = 0;
         __syncthreads();
@@ -582,3 +594,73 @@ extern "C" int vcg_reserve_slots(size_t n) {
     return 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// allocation-free list decode (vcx_decode_atoms_noalloc)
+// ---------------------------------------------------------------------------
+// The decoder's scratch for up to VCG_NOALLOC_MAX atoms lives in one
+// file-scope device array: [unused 256 B][items][coefficient lists].  Every
+// device holds its own copy, at its own address, until the process ends.  In
+// host code `scratch` only names the variable: its device address comes from
+// hipGetSymbolAddress, once per device at set-up (vcg_noalloc_setup, called
+// from vcg_init with the device current), and only that address is passed to
+// kernels.  Nothing in it needs clearing: k_entropy writes every item and
+// list entry the transform reads.  Calls that overlap on one device share
+// the array and race.
+#define NA_ITEMS_OFF 256u
+#define NA_LISTS_OFF (NA_ITEMS_OFF + VCG_NOALLOC_MAX * 16u)
+#define NA_BYTES (NA_LISTS_OFF + (size_t)VCG_NOALLOC_MAX * VCG_LIST_STRIDE * 4u)
+static_assert(sizeof(vcg_item) == 16, "item record size");
+static_assert(NA_LISTS_OFF % 256 == 0, "list alignment");
+static __device__ __align__(256) uint8_t scratch[NA_BYTES];
+
+#define NA_MAX_DEVICES 64
+static uint8_t *g_scratch_dev[NA_MAX_DEVICES];
+
+extern "C" int vcg_noalloc_setup(void) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NA_MAX_DEVICES) { (void)hipGetLastError(); return -1; }
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) { (void)hipGetLastError(); return -1; }
+    g_scratch_dev[dev] = (uint8_t *)p;
+    return 0;
+}
+
+extern "C" size_t vcg_noalloc_bytes(void) { return NA_BYTES; }
+
+extern "C" int vcg_list_launch_noalloc(vcg_arc *a, size_t n, const int32_t *quads_dev, uint8_t *out_dev,
+                                       int32_t *status_dev, void *stream) {
+    if (n == 0 || n > VCG_NOALLOC_MAX) return -1;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NA_MAX_DEVICES) { (void)hipGetLastError(); return -1; }
+    uint8_t *base = g_scratch_dev[dev];
+    if (!base) return -1;
+    ensure_attr();
+    const vcg_geom *gd = vcg_arc_geom_dev(a);
+    dec_job j;
+    memset(&j, 0, sizeof j);
+    j.arc = vcg_arc_buf(a);
+    j.geom = gd;
+    j.dqs = &gd->dq_step[0][0];
+    j.dqb = &gd->dq_bias[0][0];
+    j.mode = MODE_LIST;
+    j.check = 1;
+    j.quads = quads_dev;
+    j.out = out_dev;
+    j.status_out = status_dev;
+    j.base = 0;
+    j.n = (uint32_t)n;
+    hipStream_t s = (hipStream_t)stream;
+    // the word k_entropy ORs corrupt atoms into sits in the unused head; this
+    // path reports per atom and never reads it
+    uint32_t *word = (uint32_t *)base;
+    vcg_item *items = (vcg_item *)(base + NA_ITEMS_OFF);
+    uint32_t *lists = (uint32_t *)(base + NA_LISTS_OFF);
+    hipLaunchKernelGGL(k_entropy, dim3((j.n + ENT_THREADS - 1) / ENT_THREADS), dim3(ENT_THREADS), 0, s, j, g_dtab,
+                       lists, items, word);
+    if (hipGetLastError() != hipSuccess) return -1;
+    hipLaunchKernelGGL(k_idct, dim3(j.n), dim3(IDCT_THREADS), LDS_BYTES, s, j, g_dtab, lists, items);
+    if (hipGetLastError() != hipSuccess) return -1;
+    return 0;
+}

@@ -25,7 +25,7 @@ extern vcg_dtab *g_dtab;  // device pointer
 // Per-item result of the entropy stage.
 struct vcg_item {
     uint32_t cnt;     // coefficient entries written
-    uint32_t status;  // 0: all-zero output, 1: coefficients, 2: corrupt (zeros + FORMAT)
+    uint32_t status;  // 0: all-zero output, 1: coefficients, 2: corrupt (zeros + FORMAT), 3: out of range (zeros)
     uint32_t dc;      // payload dc byte
     uint16_t lod;     // dequant table
     int16_t fmax;     // highest coded frequency cz+cy+cx (-1: none)

@@ -61,6 +61,8 @@ extern "C" int vcg_init(const vcg_tables *t) {
         memcpy(h.scan, t->scan, sizeof h.scan);
         for (int q = 0; q < VCG_A3; ++q)
             h.fscan[q] = (uint8_t)((h.scan[q] >> 10) + ((h.scan[q] >> 5) & 31) + (h.scan[q] & 31));
+        // the allocation-free decode's static scratch: its device address
+        if (vcg_noalloc_setup() != 0) vc__panic("hipGetSymbolAddress failed for the decoder's static scratch");
         VCG_CHECK(hipMalloc(&g_dtab, sizeof(vcg_dtab)));
         // through this thread's stream: the library never touches the null
         // stream, which would take one of the device's few hardware queues

@@ -284,6 +284,22 @@ vc_status vcx_decode_atoms(vc_archive *a, size_t n, const int32_t *q, uint8_t *o
     return rc ? VC_ERR_FORMAT : VC_OK;
 }
 
+_Static_assert(VCX_NOALLOC_MAX_ATOMS == VCG_NOALLOC_MAX, "no-alloc batch limit");
+
+/* No counterpart in the reference: vcx_decode_atoms for callers that cannot
+ * allocate (device-resident quads and output, static decoder scratch). */
+vc_status vcx_decode_atoms_noalloc(vc_archive *a, size_t n, const int32_t *quads_dev, uint8_t *out_dev,
+                                   int32_t *status_dev, void *stream) {
+    if (!a || !a->buf) return VC_ERR_RANGE;
+    if (n == 0) return VC_OK;
+    if (!quads_dev || !out_dev || n > VCX_NOALLOC_MAX_ATOMS) return VC_ERR_RANGE;
+    pthread_mutex_lock(&a->mu);
+    vcg_arc *d = a->dev;
+    pthread_mutex_unlock(&a->mu);
+    if (!d) return VC_ERR_RANGE;               /* creating the device copy would allocate */
+    return vcg_list_launch_noalloc(d, n, quads_dev, out_dev, status_dev, stream) == 0 ? VC_OK : VC_ERR_RANGE;
+}
+
 /* src/vc/vc.c:1404-1421 */
 vc_status vc_decode_atom(vc_archive *a, int lod, int ax, int ay, int az, uint8_t out[VC_ATOM3]) {
     if (!a || lod < 0 || lod >= a->nlod || !out) return VC_ERR_RANGE;

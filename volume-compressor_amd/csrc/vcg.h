@@ -105,6 +105,18 @@ int vcg_list_launch(vcg_arc *a, size_t n, const int32_t *quads_pinned, int32_t *
  * reservation). */
 size_t vcg_slot_scratch_bytes(void);
 int    vcg_reserve_slots(size_t n);
+/* Allocation-free list decode: at most VCG_NOALLOC_MAX quads in device
+ * memory, checked on the device (status 2 = out of range, 1 = corrupt, 0 =
+ * ok into status_dev when given), whole atoms to out_dev, enqueued on
+ * `stream` (a hipStream_t; NULL = the null stream).  Its scratch is a static
+ * device array of vcg_noalloc_bytes() bytes whose address vcg_noalloc_setup
+ * looks up (vcg_init calls it).  0, or -1 (n out of [1, VCG_NOALLOC_MAX],
+ * no set-up on the current device, or a launch error). */
+#define VCG_NOALLOC_MAX 64u
+int    vcg_noalloc_setup(void);
+size_t vcg_noalloc_bytes(void);
+int    vcg_list_launch_noalloc(vcg_arc *a, size_t n, const int32_t *quads_dev, uint8_t *out_dev,
+                               int32_t *status_dev, void *stream);
 /* Container rewrite (vcg_repack.hip), synchronous on the calling thread's
  * stream.  A slot array (32768 slots) is taken as 32 planes of 1024 slots.
  * size: for the nblk slot arrays at archive offsets blk[], per plane p = 32k

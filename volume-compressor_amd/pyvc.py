@@ -90,6 +90,7 @@ def lib():
         "vcx_decode_region_device": (i32, [vp, i32, Box, vp, i32]),
         "vcx_sync": (i32, []),
         "vcx_decode_atoms": (i32, [vp, sz, vp, vp, vp]),
+        "vcx_decode_atoms_noalloc": (i32, [vp, sz, vp, vp, vp, vp]),
         "vcx_set_stage_events": (None, [vp]),
         "vcx_encode_q": (i32, [vp, Dims, f32, vp, P(vp), P(sz)]),
         "vcx_encode_device": (i32, [vp, Dims, f32, P(vp), P(sz)]),
