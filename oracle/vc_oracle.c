@@ -350,6 +350,8 @@ int vco_nlod(u32 nx, u32 ny, u32 nz) {                    /* vc.c:1001-1006 */
 /* integer rounded mean over the in-bounds voxels of each 2x2x2 cell, vc.c:1562-1577 */
 void vco_downsample2x(const u8 *in, u32 nx, u32 ny, u32 nz, u8 *out) {
     u32 ox = (nx + 1) / 2, oy = (ny + 1) / 2, oz = (nz + 1) / 2;
+    /* output planes are independent */
+    #pragma omp parallel for schedule(static)
     for (u32 z = 0; z < oz; ++z) for (u32 y = 0; y < oy; ++y) for (u32 x = 0; x < ox; ++x) {
         u32 acc = 0, n = 0;
         for (u32 dz = 0; dz < 2; ++dz) { u32 zz = 2 * z + dz; if (zz >= nz) continue;
@@ -858,20 +860,29 @@ int vco_decode_region(vco_archive *a, int lod, const u32 b[6], u8 *out) {
     if (b[3] > nx || b[4] > ny || b[5] > nz) return VCO_ERR_RANGE;
     if (b[0] >= b[3] || b[1] >= b[4] || b[2] >= b[5]) return VCO_ERR_RANGE;
     u32 ox = b[3] - b[0], oy = b[4] - b[1];
+    const u32 ax0 = b[0] / 32, ay0 = b[1] / 32, az0 = b[2] / 32;
+    const u32 nax = (b[3] - 1) / 32 - ax0 + 1, nay = (b[4] - 1) / 32 - ay0 + 1, naz = (b[5] - 1) / 32 - az0 + 1;
+    const i64 natoms = (i64)nax * nay * naz;
     int st = VCO_OK;
-    u8 *atom = (u8 *)malloc(A3);
-    for (u32 az = b[2] / 32; az <= (b[5] - 1) / 32; ++az)
-    for (u32 ay = b[1] / 32; ay <= (b[4] - 1) / 32; ++ay)
-    for (u32 ax = b[0] / 32; ax <= (b[3] - 1) / 32; ++ax) {
-        int s = vco_decode_atom(a, lod, (int)ax, (int)ay, (int)az, atom);
-        if (s != VCO_OK && st == VCO_OK) st = s;
-        for (u32 z = 0; z < 32; ++z) { u32 vz = az * 32 + z; if (vz < b[2] || vz >= b[5]) continue;
-        for (u32 y = 0; y < 32; ++y) { u32 vy = ay * 32 + y; if (vy < b[1] || vy >= b[4]) continue;
-        for (u32 x = 0; x < 32; ++x) { u32 vx = ax * 32 + x; if (vx < b[0] || vx >= b[3]) continue;
-            out[((size_t)(vz - b[2]) * oy + (vy - b[1])) * ox + (vx - b[0])] = atom[(z * 32 + y) * 32 + x];
-        } } }
+    /* atoms are independent and write disjoint parts of out: decoded in
+     * parallel; every failing atom is VCO_ERR_FORMAT, so the first error is
+     * any error */
+    #pragma omp parallel reduction(max:st)
+    {
+        u8 *atom = (u8 *)malloc(A3);
+        #pragma omp for schedule(dynamic, 4)
+        for (i64 k = 0; k < natoms; ++k) {
+            const u32 ax = ax0 + (u32)(k % nax), ay = ay0 + (u32)((k / nax) % nay), az = az0 + (u32)(k / ((i64)nax * nay));
+            int s = vco_decode_atom(a, lod, (int)ax, (int)ay, (int)az, atom);
+            if (s > st) st = s;
+            for (u32 z = 0; z < 32; ++z) { u32 vz = az * 32 + z; if (vz < b[2] || vz >= b[5]) continue;
+            for (u32 y = 0; y < 32; ++y) { u32 vy = ay * 32 + y; if (vy < b[1] || vy >= b[4]) continue;
+            for (u32 x = 0; x < 32; ++x) { u32 vx = ax * 32 + x; if (vx < b[0] || vx >= b[3]) continue;
+                out[((size_t)(vz - b[2]) * oy + (vy - b[1])) * ox + (vx - b[0])] = atom[(z * 32 + y) * 32 + x];
+            } } }
+        }
+        free(atom);
     }
-    free(atom);
     return st;
 }
 
