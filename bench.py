@@ -135,6 +135,86 @@ def cpu_baseline(arc, atoms_zyx, budget_s=12.0):
                       f"({reps * n * 32768 / 2**20:.0f} MiB out), {t:.1f} s, {label}, OpenMP schedule(dynamic)"}
 
 
+def _ref_lib():
+    """(path of the reference build used as the CPU baseline, kind) or (None, None)"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    v = pyoracle.ref_variants()
+    if not v:
+        return None
+    return v.get("fast_v4") if pyoracle.host_has_avx512() and "fast_v4" in v else v.get("fast_v3", v.get("strict"))
+
+
+def _threads():
+    t = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    return min(t, os.cpu_count() or t)
+
+
+def cpu_serve_bench(arc, reqs, qps, seed=11):
+    """The same open-loop trace (same requests, same Poisson gaps as
+    serve_bench) served on the host cores by the reference's own
+    vc_decode_atom / vc_decode_region (oracle/_ref): a pool of one worker per
+    core takes whole requests in arrival order."""
+    lib = _ref_lib()
+    if not lib:
+        return None
+    L = C.CDLL(os.path.join(ROOT, "oracle", "libcpubench.so"))
+    L.cpu_serve.restype = C.c_double
+    L.cpu_serve.argtypes = [C.c_char_p, C.c_void_p, C.c_size_t, C.c_size_t] + [C.c_void_p] * 6 + \
+        [C.c_int, C.c_void_p, C.POINTER(C.c_long)]
+    n = len(reqs)
+    gaps = np.random.default_rng(seed).exponential(1.0 / qps, n)
+    t_arr = np.ascontiguousarray(2e-3 + np.cumsum(gaps))
+    kind = np.array([0 if k == "atoms" else 1 for k, _ in reqs], np.int32)
+    nat = np.array([len(x) if k == "atoms" else 0 for k, x in reqs], np.int32)
+    qoff = np.concatenate([[0], np.cumsum(nat)[:-1]]).astype(np.int64)
+    quads = np.ascontiguousarray(np.concatenate([x for k, x in reqs if k == "atoms"] or [np.zeros((1, 4), np.int32)]),
+                                 np.int32)
+    boxes = np.zeros((n, 7), np.int32)
+    for r, (k, x) in enumerate(reqs):
+        if k == "region":
+            boxes[r] = (x[0],) + tuple(x[1])
+    arr = np.frombuffer(arc, np.uint8)
+    t_done = np.zeros(n)
+    bad = C.c_long()
+    th = _threads()
+    span = L.cpu_serve(lib.encode(), arr.ctypes.data, arr.size, n, kind.ctypes.data, qoff.ctypes.data,
+                       nat.ctypes.data, quads.ctypes.data, boxes.ctypes.data, t_arr.ctypes.data, th,
+                       t_done.ctypes.data, C.byref(bad))
+    assert span > 0 and bad.value == 0, (span, bad.value)
+    lat = (t_done - t_arr) * 1e3
+    out = {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+           "requests": n, "offered_qps": qps, "achieved_rps": round(float(n / (span - t_arr[0])), 1), "cores": th,
+           "kind": "reference", "lib": os.path.basename(lib)}
+    if kind[0] == 0:
+        out["atoms_per_s"] = round(float(nat.sum() / (span - t_arr[0])), 1)
+    return out
+
+
+def cpu_encode_bench(vols, targets, budget_s=12.0):
+    """vc_encode on the host cores: the oracle's OpenMP encoder (the reference
+    ships no whole-volume encoder; this restates it from the reference's
+    primitives), four volumes at a time, on as many of the volumes as fit the
+    budget (at least one).  GB/s of LOD0 input."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from concurrent.futures import ThreadPoolExecutor
+    from pyoracle import Oracle
+    o = Oracle()
+    t0 = time.perf_counter()
+    done, nbytes = 0, 0
+    with ThreadPoolExecutor(min(4, len(vols))) as ex:
+        while done < len(vols) and (done == 0 or time.perf_counter() - t0 < budget_s):
+            k = min(len(vols), done + 4)
+            for st, _ in ex.map(lambda i: o.encode(vols[i], targets[i]), range(done, k)):
+                assert st == 0
+            nbytes += sum(int(v.size) for v in vols[done:k])
+            done = k
+    t = time.perf_counter() - t0
+    return {"value": round(nbytes / t / 1e9, 4), "unit": "GB/s", "cores": _threads(), "kind": "port",
+            "sample": f"{done} of {len(vols)} volumes, {nbytes / 1e9:.3f} GB, {t:.1f} s, oracle/liboracle.so "
+                      "vco_encode (OpenMP), 4 volumes at a time"}
+
+
 def lod_count(a):
     nlod = 0
     while nlod < 8 and a.lod_dims(nlod)[0] == 0:
@@ -255,10 +335,50 @@ def serve_bench(pyvc, a, reqs, qps, seed=11, device_bytes=16 << 30):
     return out
 
 
-def plain_latency(pyvc, a, reqs, limit_s=4.0):
+def qps_sweep(pyvc, a, reqs, rates):
+    """The same trace offered at increasing rates: p50/p99 and achieved rps
+    per rate (the knee is the highest rate still served at about the offered
+    rate)."""
+    out = []
+    for q in rates:
+        r = serve_bench(pyvc, a, reqs, float(q))
+        out.append({k: r[k] for k in ("offered_qps", "achieved_rps", "p50", "p99", "errors")})
+        log(f"  sweep {q}: {out[-1]}")
+    return out
+
+
+def atom_floor(pyvc, a, sample=3000, seed=13):
+    """Per LOD, the heaviest atom of a random sample (largest payload, i.e.
+    most serial range-coder bins) decoded alone through vc_decode_atom: the
+    latency no request containing that atom can beat, since one lane decodes
+    one atom serially."""
+    L = pyvc.lib()
+    rng = np.random.default_rng(seed)
+    out = np.zeros(32768, np.uint8)
+    res = []
+    for lod in range(lod_count(a)):
+        _, (nx, ny, nz) = a.lod_dims(lod)
+        ax, ay, az = (nx + 31) // 32, (ny + 31) // 32, (nz + 31) // 32
+        best = (-1, None)
+        for _ in range(min(sample, ax * ay * az)):
+            c = (int(rng.integers(0, az)), int(rng.integers(0, ay)), int(rng.integers(0, ax)))
+            ln = a.payload_range(lod, *c)[2]
+            if ln > best[0]:
+                best = (ln, c)
+        z, y, x = best[1]
+        ts = []
+        for _ in range(6):
+            t0 = time.perf_counter()
+            L.vc_decode_atom(a.h, lod, x, y, z, out.ctypes.data)
+            ts.append((time.perf_counter() - t0) * 1e3)
+        res.append({"lod": lod, "max_payload_bytes": best[0], "decode_ms": round(min(ts[1:]), 3)})
+    return res
+
+
+def plain_latency(pyvc, a, reqs, limit_s=4.0, min_n=5):
     """The plain per-call path: vc_decode_atom per atom / vc_decode_region per
     box, host wall clock per request, on the first requests of the trace until
-    `limit_s` is spent."""
+    `limit_s` is spent and at least min_n requests are done."""
     L = pyvc.lib()
     out = np.zeros(32768, np.uint8)
     lat, t_all = [], time.perf_counter()
@@ -270,7 +390,7 @@ def plain_latency(pyvc, a, reqs, limit_s=4.0):
         else:
             a.decode_region(x[0], x[1])
         lat.append((time.perf_counter() - t0) * 1e3)
-        if time.perf_counter() - t_all > limit_s and len(lat) >= 5:
+        if time.perf_counter() - t_all > limit_s and len(lat) >= min_n:
             break
     lat = np.array(lat)
     return {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
@@ -301,15 +421,21 @@ def serving_section(pyvc, args):
     res = {"workload": f"{n}^3 u8 synthetic, {args.serve_sheets} sheets, seed 3, target_ratio 50",
            "archive_bytes": len(arc), "encode_s": round(t_enc, 3)}
     sa = serve_bench(pyvc, a, areqs, args.qps)
-    pa = plain_latency(pyvc, a, areqs)
+    pa = plain_latency(pyvc, a, areqs, limit_s=15.0, min_n=12)
     sa["rps_vs_plain"] = round(sa["achieved_rps"] / pa["rps"], 1)
-    res["atom_requests"] = {"batched": sa, "plain": pa}
-    log(f"atom serving: {sa}; plain {pa}")
+    ca = None if args.no_cpu else cpu_serve_bench(arc, areqs, args.qps)
+    res["atom_requests"] = {"batched": sa, "plain": pa, "cpu": ca,
+                            "sweep": qps_sweep(pyvc, a, areqs, (200, 400, 800, 1600, 3200))}
+    log(f"atom serving: {sa}; plain {pa}; cpu {ca}")
     sr = serve_bench(pyvc, a, rreqs, args.region_qps)
-    pr = plain_latency(pyvc, a, rreqs)
+    pr = plain_latency(pyvc, a, rreqs, limit_s=8.0, min_n=50)
     sr["rps_vs_plain"] = round(sr["achieved_rps"] / pr["rps"], 1)
-    res["region_requests"] = {"batched": sr, "plain": pr}
-    log(f"region serving: {sr}; plain {pr}")
+    cr = None if args.no_cpu else cpu_serve_bench(arc, rreqs, args.region_qps)
+    res["region_requests"] = {"batched": sr, "plain": pr, "cpu": cr,
+                              "sweep": qps_sweep(pyvc, a, rreqs, (200, 400, 800, 1600, 3200))}
+    log(f"region serving: {sr}; plain {pr}; cpu {cr}")
+    res["atom_floor"] = atom_floor(pyvc, a)
+    log(f"atom floor: {res['atom_floor']}")
     res["repack"] = repack_bench(pyvc, a, arc, no_cpu=args.no_cpu)
     log(f"repack: {res['repack']}")
     res["streaming"] = stream_bench(pyvc, a, arc)
@@ -432,6 +558,41 @@ def ds_section(pyvc, hip, dvol, dout, n, reps=10):
             "GBps_moved": round(moved / ms / 1e6, 1), "frac_of_hbm": round(moved / ms / 1e6 / 8000, 3)}
 
 
+def transfer_section(pyvc, a, dout, n, reps=3):
+    """Host<->device copies of the LOD0 volume's size (pageable numpy memory
+    and pinned memory), and vc_decode_lod LOD0 into host memory (the drop-in
+    call: device decode plus the copy out), against the device-resident rate."""
+    L = pyvc.lib()
+    nb = n ** 3
+    host = np.ones(nb, np.uint8)
+    pin = pyvc.PinnedBuffer(nb)
+    pin.a[:] = 1
+    out = {"bytes": nb}
+
+    def best(f):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            f()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+    try:
+        out["h2d_pageable_GBps"] = round(nb / best(lambda: L.vcx_h2d(dout, host.ctypes.data, nb)) / 1e9, 2)
+        out["h2d_pinned_GBps"] = round(nb / best(lambda: L.vcx_h2d(dout, pin.p, nb)) / 1e9, 2)
+        out["d2h_pageable_GBps"] = round(nb / best(lambda: L.vcx_d2h(host.ctypes.data, dout, nb)) / 1e9, 2)
+        out["d2h_pinned_GBps"] = round(nb / best(lambda: L.vcx_d2h(pin.p, dout, nb)) / 1e9, 2)
+        d = pyvc.Dims()
+        t = best(lambda: L.vc_decode_lod(a.h, 0, host.ctypes.data, C.byref(d)))
+        out["decode_lod_host_pageable_ms"] = round(t * 1e3, 3)
+        out["decode_lod_host_pageable_GBps"] = round(nb / t / 1e9, 2)
+        t = best(lambda: L.vc_decode_lod(a.h, 0, pin.p, C.byref(d)))
+        out["decode_lod_host_pinned_ms"] = round(t * 1e3, 3)
+        out["decode_lod_host_pinned_GBps"] = round(nb / t / 1e9, 2)
+    finally:
+        pin.free()
+    return out
+
+
 def ratio_section(pyvc, hip, o, dvol, dout, n, ratios, steps=10):
     """The same volume encoded at other target ratios: encode time, then
     back-to-back device LOD0 decodes (GB/s) and 16 sampled atoms against the
@@ -473,7 +634,7 @@ def ratio_section(pyvc, hip, o, dvol, dout, n, ratios, steps=10):
     return out
 
 
-def encode_batch_section(pyvc, nvol=256, nseq=16, seed=2024):
+def encode_batch_section(pyvc, nvol=256, nseq=16, seed=2024, with_cpu=False):
     """Batched encode service: nvol device-resident volumes with each axis
     uniform in [40, 512], sheets scaled with the volume's edge (20 per 128
     voxels), target ratios uniform over {10, 20, 50, 100}; one
@@ -502,6 +663,12 @@ def encode_batch_section(pyvc, nvol=256, nseq=16, seed=2024):
             st, arc, tk = pyvc.encode_device(ptrs[k], dims[k], targets[k], timed=True)
             t_seq += tk
             same &= st == 0 and arc == res[k][1]
+        cpu = None
+        if with_cpu:
+            hv = [pyvc.d2h(ptrs[k], dims[k][0] * dims[k][1] * dims[k][2]).reshape(dims[k][2], dims[k][1], dims[k][0])
+                  for k in range(64)]
+            cpu = cpu_encode_bench(hv, targets[:64])
+            del hv
     finally:
         for p in ptrs:
             pyvc.dev_free(p)
@@ -510,7 +677,7 @@ def encode_batch_section(pyvc, nvol=256, nseq=16, seed=2024):
            "batched_s": round(t_batch, 3), "batched_GBps": round(tot / t_batch / 1e9, 3),
            "per_call_GBps": round(sub / t_seq / 1e9, 3) if nseq else None, "per_call_volumes": nseq,
            "batched_vs_per_call": round((tot / t_batch) / (sub / t_seq), 2) if nseq else None,
-           "identical_to_per_call": bool(same)}
+           "identical_to_per_call": bool(same), "cpu_baseline": cpu}
     log(f"batched encode: {out}")
     return out
 
@@ -526,6 +693,19 @@ def pmc_traffic(kernel, size, ratio):
         if len(w) == 3 and w[0] == "traffic_bytes" and w[1] == kernel:
             return int(w[2])
     return None
+
+
+def sq_counters(size, ratio):
+    """{kernel: VALU wave-instructions per launch} from the committed rocprofv3
+    SQ-counter summary of this workload, or {}."""
+    f = os.path.join(ROOT, "profiles", f"decode_lod0_{size}_ratio{ratio:g}_sq_counters.md")
+    out = {}
+    if os.path.exists(f):
+        for line in open(f):
+            w = line.split()
+            if len(w) == 3 and w[0] == "valu_insts":
+                out[w[1]] = int(w[2])
+    return out
 
 
 def main():
@@ -697,6 +877,19 @@ def main():
     lat = np.array(lat[10:])
     p50, p99 = (float(np.percentile(lat, 50)), float(np.percentile(lat, 99))) if lat.size else (None, None)
 
+    # ---- host transfers and the host-destination drop-in call ----
+    xfer = None if args.profile else transfer_section(pyvc, a, dout, n)
+    if xfer:
+        log(f"transfers: {xfer}")
+    # ---- vc_encode on the host cores (the oracle's OpenMP encoder) ----
+    cpu_enc = None
+    if not (args.no_cpu or args.profile):
+        hv = pyvc.d2h(dvol, nvox).reshape(n, n, n)
+        cpu_enc = cpu_encode_bench([hv], [args.ratio])
+        cpu_enc["gpu_GBps"] = round(nvox / t_enc / 1e9, 3)
+        del hv
+        log(f"cpu encode {n}^3: {cpu_enc}")
+
     # ---- the same volume at the other target ratios ----
     ratios = None
     if not (args.no_ratios or args.profile):
@@ -718,7 +911,7 @@ def main():
     # ---- batched encode service ----
     enc_batch = None
     if not (args.no_encode_batch or args.profile):
-        enc_batch = encode_batch_section(pyvc)
+        enc_batch = encode_batch_section(pyvc, with_cpu=not args.no_cpu)
 
     # ---- CPU baseline ----
     cpu = None
@@ -727,6 +920,7 @@ def main():
         cpu = cpu_baseline(arc, zyx)
         log(f"cpu baseline: {cpu}")
 
+    vi = sq_counters(n, args.ratio)
     res = {
         "metric": METRIC,
         "value": round(gbs, 3),
@@ -749,7 +943,12 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
                      "traffic": pmc_traffic(dom, n, args.ratio),
-                     "algorithmic_bytes": dom_bytes, "avg_ms": round(dom_ms, 4)},
+                     "algorithmic_bytes": dom_bytes, "avg_ms": round(dom_ms, 4),
+                     # the kernel's real limit: share of the chip's VALU issue
+                     # slots it uses (profiled instruction count x 4 cycles over
+                     # 1024 SIMDs at 2.4 GHz for this run's launch time)
+                     "valu_issue_frac": (round(vi[dom] * 4 / (1024 * 2.4e9 * dom_ms / 1e3), 4)
+                                         if dom in vi else None)},
         "step_roofline": {"achieved": round(alg_step / (ms_step / 1e3) / 1e9, 3), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(alg_step / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 6),
                           "algorithmic_bytes": alg_step},
@@ -764,7 +963,9 @@ def main():
         "other_ratios": ratios,
         "downscale": ds,
         "batched_encode": enc_batch,
+        "transfers": xfer,
         "cpu_baseline": cpu,
+        "cpu_encode": cpu_enc,
     }
     if args.profile:
         # 1 GiB device copy by a plain kernel (k_copy16): byte-counter reference
