@@ -8,7 +8,7 @@
 #include <string.h>
 #include "vcg_xform.hpp"
 
-static const int NT = 256;   // threads per workgroup in the forward kernel
+static const int NT = 512;   // threads per workgroup in the forward kernel
 static const int NTI = 512;  // threads per workgroup in the inverse kernel
 
 extern "C" {
@@ -48,17 +48,44 @@ void kc_quant(const int32_t *c, size_t n, float q, float g, int32_t *out) {
     for (size_t i = 0; i < n; ++i) out[i] = vcg_quant1(c[i], q, g);
 }
 
-// k_fdct on one atom of an (nx,ny,nz) volume: returns 1 for an all-zero atom
+// k_fdct on one atom of an (nx,ny,nz) volume: returns 1 for an all-zero atom.
+// Pass 1 follows the kernel's matrix-core form lane by lane: each z-plane's
+// 64 lanes load their 16-voxel rows (vcg_fwd1_row), the voxels become int8
+// by flipping their top bit, and each output register (k, y) of the 32x32x32
+// product is the sum over both lane halves and all 16 bytes of A byte x B
+// byte, exactly what v_mfma_i32_32x32x32_i8 computes; then passes 2 and 3 as
+// the kernel runs them.
 int kc_fdct(const uint8_t *vol, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t ax, uint32_t ay, uint32_t az,
             const int32_t cmat[32][32], const uint16_t *scan, const float *gstep, float q, int16_t *lv,
             int32_t *coef_raster, uint32_t *eob, uint32_t *dc_out) {
     int *t = (int *)calloc(VCG_TILE_INTS, sizeof(int));
+    static int8_t fa[2][64][16];
+    static int32_t rowsum[32];
+    static uint32_t raw[32][64][4];
+    vcg_fwd1_tables(cmat, fa, rowsum);
     uint32_t sum = 0;
-    for (int tid = 0; tid < NT; ++tid) sum += vcg_gather(t, vol, nx, ny, nz, ax, ay, az, tid, NT);
+    for (int z = 0; z < 32; ++z)
+        for (int l = 0; l < 64; ++l) {
+            vcg_fwd1_row(vol, nx, ny, nz, ax, ay, az, z, l & 31, l >> 5, raw[z][l]);
+            for (int e = 0; e < 16; ++e) sum += (raw[z][l][e >> 2] >> (8 * (e & 3))) & 0xFFu;
+        }
     if (sum == 0) { free(t); *eob = 0; *dc_out = 0; return 1; }
     const int dc = (int)((sum + 16384u) / 32768u);
-    for (int tid = 0; tid < NT; ++tid) vcg_level_shift(t, dc, tid, NT);
-    for (int tid = 0; tid < NT; ++tid) vcg_fwd_pass_x(t, cmat, tid, NT);
+    for (int z = 0; z < 32; ++z)
+        for (int l = 0; l < 64; ++l) {
+            const int y = l & 31, h = l >> 5;
+            for (int r = 0; r < 16; ++r) {
+                const int k = vcg_fwd1_k(r, h);
+                int d[2] = {0, 0};
+                for (int p = 0; p < 2; ++p)
+                    for (int h2 = 0; h2 < 2; ++h2)
+                        for (int e = 0; e < 16; ++e) {
+                            const int8_t b = (int8_t)(((raw[z][y + 32 * h2][e >> 2] ^ 0x80808080u) >> (8 * (e & 3))) & 0xFFu);
+                            d[p] += fa[p][k + 32 * h2][e] * b;
+                        }
+                t[vcg_tidx(z, y, k)] = vcg_fwd1_out(d[0], d[1], (128 - dc) * rowsum[k] + 8192);
+            }
+        }
     for (int tid = 0; tid < NT; ++tid) vcg_fwd_pass_y(t, cmat, tid, NT);
     for (int tid = 0; tid < NT; ++tid) vcg_fwd_pass_z(t, cmat, tid, NT);
     if (coef_raster)

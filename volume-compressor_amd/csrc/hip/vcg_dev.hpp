@@ -19,6 +19,8 @@ struct vcg_dtab {
     uint32_t band_start[9];
     uint16_t scan[VCG_A3];
     uint8_t  fscan[VCG_A3];   // frequency cz+cy+cx of scan position p (non-decreasing in p)
+    int8_t   fa[2][64][16];   // forward pass 1 on MFMA: per-lane basis planes (vcg_fwd1_tables)
+    int32_t  rowsum[32];      // basis row sums
 };
 extern vcg_dtab *g_dtab;  // device pointer
 

@@ -5,8 +5,9 @@
 //   k_fdct       : one 512-thread workgroup per atom, atoms of one or many
 //                  volumes per launch.  Gathers the 32^3 atom (zero padded),
 //                  sums it (zero atom -> no payload, dc = (sum+16384)/32768,
-//                  src/vc/vc.c:599-604), runs the three forward
-//                  partial-butterfly passes along x, y, z in LDS
+//                  src/vc/vc.c:599-604), runs forward pass 1 (along x) as
+//                  exact int8 MFMA products straight from the voxel bytes,
+//                  passes 2 and 3 as partial butterflies in LDS
 //                  (src/vc/vc.c:142-171, 212-218) and either stores raster
 //                  coefficients (rate-control sample) or quantizes them
 //                  (src/vc/vc.c:293-306, strict IEEE in source order) and
@@ -26,6 +27,9 @@
 #include <vector>
 #include "vcg_dev.hpp"
 #include "vcg_xform.hpp"
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
 
 #define FDCT_THREADS 256
 #define FDCT_TX 512          // k_fdct: two waves per SIMD, one line pair per thread and pass
@@ -236,7 +240,22 @@ __global__ __launch_bounds__(FDCT_TX) void k_fdct(const enc_src *__restrict__ sr
     enc_item *items = store ? s.sitems + gi : items_out + i;
     if (tid == 0) { ssum = 0; seob = 0; }
     __syncthreads();
-    const uint32_t part = vcg_gather(buf, s.lod[lod], s.nx[lod], s.ny[lod], s.nz[lod], ax, ay, az, tid, FDCT_TX);
+    // pass 1 on the matrix cores (vcg_xform.hpp): wave w takes z-planes w,
+    // w+8, w+16, w+24; lane (y, h) loads the 16 voxels x = 16h.. of row (z, y)
+    // of each, which also give the atom's sum (and so dc) before any product
+    const int wv = tid >> 6, ln = tid & 63, y = ln & 31, hh = ln >> 5;
+    v4i raw[4];
+    uint32_t part = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        uint32_t w[4];
+        vcg_fwd1_row(s.lod[lod], s.nx[lod], s.ny[lod], s.nz[lod], ax, ay, az, wv + 8 * b, y, hh, w);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            part = __builtin_amdgcn_sad_u8(w[q], 0u, part);
+            raw[b][q] = (int)(w[q] ^ 0x80808080u);     // v - 128 as int8
+        }
+    }
     if (part) atomicAdd(&ssum, part);
     __syncthreads();
     const uint32_t sum = ssum;
@@ -245,9 +264,21 @@ __global__ __launch_bounds__(FDCT_TX) void k_fdct(const enc_src *__restrict__ sr
         return;
     }
     const int dc = (int)((sum + 16384u) / 32768u);
-    vcg_level_shift(buf, dc, tid, FDCT_TX);
-    __syncthreads();
-    vcg_fwd_pass_x(buf, T->cmat, tid, FDCT_TX);
+    {
+        const v4i ah = *(const v4i *)T->fa[0][ln], al = *(const v4i *)T->fa[1][ln];
+        int corr[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) corr[r] = (128 - dc) * T->rowsum[vcg_fwd1_k(r, hh)] + 8192;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const v16i zero = {};
+            const v16i dh = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, raw[b], zero, 0, 0, 0);
+            const v16i dl = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, raw[b], zero, 0, 0, 0);
+            const int z = wv + 8 * b;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) buf[vcg_tidx(z, y, vcg_fwd1_k(r, hh))] = vcg_fwd1_out(dh[r], dl[r], corr[r]);
+        }
+    }
     __syncthreads();
     vcg_fwd_pass_y(buf, T->cmat, tid, FDCT_TX);
     __syncthreads();

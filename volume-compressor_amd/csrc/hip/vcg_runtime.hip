@@ -7,6 +7,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include "vcg_dev.hpp"
+#include "vcg_xform.hpp"
 
 vcg_dtab *g_dtab = nullptr;
 static std::once_flag g_init_once;
@@ -61,6 +62,7 @@ extern "C" int vcg_init(const vcg_tables *t) {
         memcpy(h.scan, t->scan, sizeof h.scan);
         for (int q = 0; q < VCG_A3; ++q)
             h.fscan[q] = (uint8_t)((h.scan[q] >> 10) + ((h.scan[q] >> 5) & 31) + (h.scan[q] & 31));
+        vcg_fwd1_tables(h.cmat, h.fa, h.rowsum);
         // the allocation-free decode's static scratch: its device address
         if (vcg_noalloc_setup() != 0) vc__panic("hipGetSymbolAddress failed for the decoder's static scratch");
         VCG_CHECK(hipMalloc(&g_dtab, sizeof(vcg_dtab)));

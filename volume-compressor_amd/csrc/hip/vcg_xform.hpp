@@ -404,12 +404,9 @@ VCG_HD void vcg_fwd_line2(const int32_t (*C)[32], int *s0, int *s1, int stride) 
     }
 }
 
-// The passes give each thread the line pairs (line, line + nthr) for line =
-// tid, tid + 2*nthr, ... (nthr divides 512).
-VCG_HD void vcg_fwd_pass_x(int *t, const int32_t (*C)[32], int tid, int nthr) {
-    for (int l = tid; l < 1024; l += 2 * nthr)
-        vcg_fwd_line2(C, t + vcg_tidx(l >> 5, l & 31, 0), t + vcg_tidx((l + nthr) >> 5, (l + nthr) & 31, 0), VCG_SX);
-}
+// Passes 2 and 3 (pass 1 runs on the matrix cores, below) give each thread
+// the line pairs (line, line + nthr) for line = tid, tid + 2*nthr, ... (nthr
+// divides 512).
 VCG_HD void vcg_fwd_pass_y(int *t, const int32_t (*C)[32], int tid, int nthr) {
     for (int l = tid; l < 1024; l += 2 * nthr)
         vcg_fwd_line2(C, t + vcg_tidx(l >> 5, 0, l & 31), t + vcg_tidx((l + nthr) >> 5, 0, (l + nthr) & 31), VCG_SY);
@@ -419,29 +416,67 @@ VCG_HD void vcg_fwd_pass_z(int *t, const int32_t (*C)[32], int tid, int nthr) {
         vcg_fwd_line2(C, t + vcg_tidx(0, l >> 5, l & 31), t + vcg_tidx(0, (l + nthr) >> 5, (l + nthr) & 31), VCG_SZ);
 }
 
-// gather one zero-padded atom of an (nx,ny,nz) volume into the tile; returns
-// this thread's voxel sum.  Thread handles runs of 4 consecutive x.
-VCG_HD uint32_t vcg_gather(int *t, const uint8_t *vol, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t ax,
-                           uint32_t ay, uint32_t az, int tid, int nthr) {
-    uint32_t part = 0;
-    for (int r = tid; r < 32 * 32 * 8; r += nthr) {
-        const int z = r >> 8, y = (r >> 3) & 31, x4 = (r & 7) * 4;
-        const uint32_t vz = az * 32 + z, vy = ay * 32 + y, vx = ax * 32 + x4;
-        int v[4] = {0, 0, 0, 0};
-        if (vz < nz && vy < ny) {
-            const uint8_t *row = vol + ((uint64_t)vz * ny + vy) * nx;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) if (vx + q < nx) v[q] = row[vx + q];
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) { t[vcg_tidx(z, y, x4 + q)] = v[q]; part += (uint32_t)v[q]; }
+// ---------------------------------------------------------------------------
+// forward pass 1 on the matrix cores (v_mfma_i32_32x32x32_i8)
+// ---------------------------------------------------------------------------
+// Pass 1 (along x) of the forward transform, out[k] = (sum_n C[k][n] (v[n] -
+// dc) + 8192) >> 14, is computed exactly as an int8 matrix product:
+//   v - dc = (v - 128) + (128 - dc), and v - 128 is int8 (the voxel's byte
+//   with its top bit flipped), so  sum_n C[k][n](v[n]-dc)
+//     = sum_n C[k][n](v[n]-128) + (128-dc) rowsum[k];
+//   |C| <= 4091 is not int8, but C = 64 Ch + Cl with Cl = ((C+32) mod 64) -
+//   32 in [-32, 31] and Ch in [-64, 64] is: two int8 products, i32 sums.
+// All terms stay far inside i32 (|sum| <= 32 * 4091 * 128 plus 127 * 92672),
+// so the result equals the reference's butterfly (src/vc/vc.c:142-171),
+// whose regrouping is exact, bit for bit.
+//
+// One 32x32x32 product covers a z-plane: 32 output frequencies k (rows) by
+// 32 lines y (columns), K = the 32 voxels n of a line.  Lane l = j + 32 h of
+// a wave supplies 16 bytes of each operand, byte e pairing with byte e of
+// the other operand's lane of the same h: A = Ch or Cl row k = j, n = 16h +
+// e; B = line y = j, n = 16h + e, i.e. 16 consecutive voxels of the row,
+// one 16-byte load.  Output register r of lane l is D[k][y] with y = l mod
+// 32 and k = 8 (r / 4) + 4 (l / 32) + r mod 4.
+
+// per-lane A operands (the two basis planes) and row sums, built once
+VCG_HD void vcg_fwd1_tables(const int32_t (*C)[32], int8_t fa[2][64][16], int32_t rowsum[32]) {
+    for (int k = 0; k < 32; ++k) {
+        int32_t s = 0;
+        for (int n = 0; n < 32; ++n) s += C[k][n];
+        rowsum[k] = s;
     }
-    return part;
+    for (int l = 0; l < 64; ++l)
+        for (int e = 0; e < 16; ++e) {
+            const int c = C[l & 31][16 * (l >> 5) + e];
+            const int lo = ((c + 32) & 63) - 32;
+            fa[0][l][e] = (int8_t)((c - lo) / 64);
+            fa[1][l][e] = (int8_t)lo;
+        }
 }
 
-VCG_HD void vcg_level_shift(int *t, int dc, int tid, int nthr) {
-    for (int k = tid; k < 32768; k += nthr) t[vcg_tidx(k >> 10, (k >> 5) & 31, k & 31)] -= dc;
+// the 16 voxels (x = 16h .. 16h+15 of atom row (z, y)) lane (y, h) feeds
+// for one z-plane, zero outside the volume, little-endian in w[0..3]
+VCG_HD void vcg_fwd1_row(const uint8_t *vol, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t ax, uint32_t ay,
+                         uint32_t az, int z, int y, int h, uint32_t w[4]) {
+    w[0] = w[1] = w[2] = w[3] = 0;
+    const uint32_t vz = az * 32 + (uint32_t)z, vy = ay * 32 + (uint32_t)y, vx = ax * 32 + 16 * (uint32_t)h;
+    if (vz >= nz || vy >= ny || vx >= nx) return;
+    const uint8_t *p = vol + ((uint64_t)vz * ny + vy) * nx + vx;
+    if (vx + 16 <= nx && ((uintptr_t)p & 3) == 0) {
+        const uint32_t *q = (const uint32_t *)p;
+        w[0] = q[0]; w[1] = q[1]; w[2] = q[2]; w[3] = q[3];
+        return;
+    }
+    const uint32_t m = nx - vx < 16 ? nx - vx : 16;
+    for (uint32_t e = 0; e < m; ++e) w[e >> 2] |= (uint32_t)p[e] << (8 * (e & 3));
 }
+
+// output register r of lane half h: frequency row k
+VCG_HD int vcg_fwd1_k(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
+
+// combine the two plane products of output (k, y) into the pass-1 value;
+// corr = (128 - dc) rowsum[k] + 8192
+VCG_HD int vcg_fwd1_out(int dh, int dl, int corr) { return (dh * 64 + dl + corr) >> 14; }
 
 // quantize the tile's coefficients in scan order into lv[p]; returns this
 // thread's max (p+1) over nonzero levels (the EOB candidate)
