@@ -264,7 +264,25 @@ def req_bytes(r):
     return (x1 - x0) * (y1 - y0) * (z1 - z0)
 
 
-def serve_bench(pyvc, a, reqs, qps, seed=11, device_bytes=16 << 30):
+def tune_section(pyvc, a, device_bytes=16 << 30):
+    """vcx_server_autotune on the serving archive: the launch-shape table
+    (per LOD class and batch-size bucket, default and tuned ms of one
+    captured batch of that size), then the tuning time."""
+    srv = pyvc.Server(a, device_bytes)
+    try:
+        t0 = time.perf_counter()
+        assert srv.autotune() == 0
+        t = time.perf_counter() - t0
+        rows = srv.tuning()
+    finally:
+        srv.close()
+    table = [{"class": "lod0" if c == 0 else "coarse", "bucket": b, "ent_waves": w, "ent_groups": ge,
+              "idct_groups": gt, "default_ms": round(md, 4), "tuned_ms": round(mt, 4),
+              "gain": round(md / mt, 3) if mt > 0 else None} for c, b, w, ge, gt, md, mt in rows]
+    return {"tune_s": round(t, 2), "table": table}
+
+
+def serve_bench(pyvc, a, reqs, qps, seed=11, device_bytes=16 << 30, tune=True):
     """Open-loop Poisson trace of `reqs` at `qps` through the continuous-batching
     server.  Latency = completion (server clock) - scheduled arrival.  Three
     served requests are checked against the plain per-call path."""
@@ -275,6 +293,8 @@ def serve_bench(pyvc, a, reqs, qps, seed=11, device_bytes=16 << 30):
     offs = np.cumsum([0] + sizes)
     arena = pyvc.PinnedBuffer(int(offs[-1]))
     srv = pyvc.Server(a, device_bytes)
+    if tune:
+        assert srv.autotune() == 0
     ids, arrive = {}, np.zeros(nreq)
     done_t, bad = {}, 0
     try:
@@ -420,7 +440,11 @@ def serving_section(pyvc, args):
     plain_latency(pyvc, a, areqs[:1], 0.0)
     res = {"workload": f"{n}^3 u8 synthetic, {args.serve_sheets} sheets, seed 3, target_ratio 50",
            "archive_bytes": len(arc), "encode_s": round(t_enc, 3)}
+    res["autotune"] = tune_section(pyvc, a)
+    log(f"autotune: {res['autotune']}")
     sa = serve_bench(pyvc, a, areqs, args.qps)
+    sa["untuned"] = {k: v for k, v in serve_bench(pyvc, a, areqs, args.qps, tune=False).items()
+                     if k in ("p50", "p99", "achieved_rps")}
     pa = plain_latency(pyvc, a, areqs, limit_s=15.0, min_n=12)
     sa["rps_vs_plain"] = round(sa["achieved_rps"] / pa["rps"], 1)
     ca = None if args.no_cpu else cpu_serve_bench(arc, areqs, args.qps)

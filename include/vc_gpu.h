@@ -202,6 +202,24 @@ int64_t     vcx_server_submit_region(vcx_server *s, int lod, vc_box box, uint8_t
 size_t      vcx_server_poll(vcx_server *s, int64_t *ids, int32_t *status, uint64_t *t_done_ns, size_t max,
                             int wait_ms);
 size_t      vcx_server_capacity(const vcx_server *s);   /* atoms per batch slot */
+/* Batches run as persistent kernels (entropy waves claiming 64 atoms at a
+ * time, transform workgroups one atom at a time, the batch size read from
+ * device memory), captured once per batch slot, LOD class (LOD0 only / any
+ * coarser LOD) and size bucket (up to 64, 512, 4096, 32768 atoms) as a
+ * hipGraph and replayed per batch.  vcx_server_autotune waits until no batch
+ * is in flight, then times every candidate launch shape per class and bucket
+ * (entropy workgroups of 1, 2 or 4 waves, enough of them for all atoms at
+ * once or for half, 256 / 512 / 1024 transform workgroups or one per atom)
+ * on sample batches of the archive's own atoms, keeps the fastest and
+ * rebuilds the captured launches; results do not depend on the shape.
+ * vcx_server_tuning writes the table (one line per class and bucket: class,
+ * bucket, shape, default ms, tuned ms) into buf (NUL-terminated, at most
+ * cap bytes) and returns its full length. */
+vc_status   vcx_server_autotune(vcx_server *s);
+/* Test hook: serve every later batch with one launch shape (ent_waves 1, 2
+ * or 4; any positive group counts), waiting for the batches in flight. */
+vc_status   vcx_test_server_shape(vcx_server *s, int ent_waves, uint32_t ent_groups, uint32_t idct_groups);
+size_t      vcx_server_tuning(vcx_server *s, char *buf, size_t cap);
 size_t      vcx_server_atom_bytes(void);                /* device bytes per atom of capacity */
 uint64_t    vcx_now_ns(void);
 void       *vcx_host_alloc(size_t bytes);               /* pinned host memory */

@@ -102,26 +102,15 @@ struct lds_ring {
 #endif
 #define ENT_THREADS (64 * ENT_WAVES)
 
-__global__ __launch_bounds__(ENT_THREADS) void k_entropy(dec_job j, const vcg_dtab *__restrict__ T,
-                                                uint32_t *__restrict__ lists,
-                                                vcg_item *__restrict__ items,
-                                                uint32_t *__restrict__ status) {
-    // issue priority over transform waves sharing the SIMD (the serial
-    // decode is the critical path; the transform fills its idle issue cycles)
-    __builtin_amdgcn_s_setprio(3);
-    __shared__ uint32_t ctxw[ENT_WAVES][VCG_CTX_SLOTS * 64];
-    __shared__ uint32_t ringw[ENT_WAVES][VCG_RING * 64];
-    __shared__ uint32_t bands[9];
-    // band table in LDS: a band edge then waits on LDS only, not on the
-    // look-ahead loads in flight
-    if (threadIdx.x < 9) bands[threadIdx.x] = T->band_start[threadIdx.x];
-    __syncthreads();
-    const uint32_t i = blockIdx.x * (uint32_t)ENT_THREADS + threadIdx.x;
-    if (i >= j.n) return;
-    // a wave's arrays: its context slots at byte (wave << 12), its ring at
-    // (wave << 11), so a lane's offset ORs into the slot addressing
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    uint32_t *ctxm = &ctxw[0][0], *ringm = &ringw[0][0];
+// One lane's atom (item i of the launch): resolve it through the device copy
+// of the archive's index, range-decode its payload and write its coefficient
+// list and item record.  ctxm / ringm: the workgroup's context and output-ring
+// LDS arrays, a wave's part at byte (wave << 12) and (wave << 11), so a lane's
+// offset ORs into the slot addressing.
+__device__ __forceinline__ void ent_atom(const dec_job &j, uint32_t i, const vcg_dtab *__restrict__ T,
+                                         uint32_t *__restrict__ lists, vcg_item *__restrict__ items,
+                                         uint32_t *__restrict__ status, uint32_t *ctxm, uint32_t *ringm,
+                                         const uint32_t *bands, uint32_t lane, uint32_t wave) {
     const uint32_t gi = j.base + i;
     int lod = 0;
     uint64_t off = 0;
@@ -158,7 +147,7 @@ __global__ __launch_bounds__(ENT_THREADS) void k_entropy(dec_job j, const vcg_dt
     asm volatile("" : "+v"(rl4));
     lds_ring ring{ringm, rl4};
     for (uint32_t k = 1; k < VCG_CTX_SLOTS; ++k)
-        ctxw[wave][k * 64u + lane] = k == 13u || k == 15u ? VCG_P_BYPASS : 2048u;
+        ctxm[(wave * VCG_CTX_SLOTS + k) * 64u + lane] = k == 13u || k == 15u ? VCG_P_BYPASS : 2048u;
     const uint8_t *s = p + 1;
     const uint32_t n = len - 1;
     vcg_fsm f;
@@ -191,6 +180,25 @@ __global__ __launch_bounds__(ENT_THREADS) void k_entropy(dec_job j, const vcg_dt
         it.status = 1; it.cnt = cnt; it.fmax = (int16_t)fmax;
     }
     items[i] = it;
+}
+
+__global__ __launch_bounds__(ENT_THREADS) void k_entropy(dec_job j, const vcg_dtab *__restrict__ T,
+                                                uint32_t *__restrict__ lists,
+                                                vcg_item *__restrict__ items,
+                                                uint32_t *__restrict__ status) {
+    // issue priority over transform waves sharing the SIMD (the serial
+    // decode is the critical path; the transform fills its idle issue cycles)
+    __builtin_amdgcn_s_setprio(3);
+    __shared__ uint32_t ctxw[ENT_WAVES][VCG_CTX_SLOTS * 64];
+    __shared__ uint32_t ringw[ENT_WAVES][VCG_RING * 64];
+    __shared__ uint32_t bands[9];
+    // band table in LDS: a band edge then waits on LDS only, not on the
+    // look-ahead loads in flight
+    if (threadIdx.x < 9) bands[threadIdx.x] = T->band_start[threadIdx.x];
+    __syncthreads();
+    const uint32_t i = blockIdx.x * (uint32_t)ENT_THREADS + threadIdx.x;
+    if (i >= j.n) return;
+    ent_atom(j, i, T, lists, items, status, &ctxw[0][0], &ringw[0][0], bands, threadIdx.x & 63u, threadIdx.x >> 6);
 }
 
 // ---------------------------------------------------------------------------
@@ -268,13 +276,13 @@ __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
     return x * q + (x < r ? x : r) + k;
 }
 
-__global__ __launch_bounds__(IDCT_THREADS) void k_idct(dec_job j, const vcg_dtab *__restrict__ T,
-                                                       const uint32_t *__restrict__ lists,
-                                                       const vcg_item *__restrict__ items) {
-    extern __shared__ int buf[];
-    __shared__ unsigned long long l1sum;
+// One workgroup's item i of the launch: scatter and dequantize its list into
+// the LDS tile buf, run the three inverse passes and store the voxels.  Every
+// thread of the workgroup calls it with the same i (it holds barriers).
+__device__ __forceinline__ void idct_item(const dec_job &j, uint32_t i, const vcg_dtab *__restrict__ T,
+                                          const uint32_t *__restrict__ lists, const vcg_item *__restrict__ items,
+                                          int *buf, unsigned long long *l1sum, int32_t *dqs) {
     const int tid = threadIdx.x;
-    const uint32_t i = xcd_item(blockIdx.x, gridDim.x);
     const uint32_t gi = j.base + i;
     int lod;
     uint32_t ax, ay, az;
@@ -290,7 +298,6 @@ __global__ __launch_bounds__(IDCT_THREADS) void k_idct(dec_job j, const vcg_dtab
     uint32_t pre[IDCT_PRE];
 #pragma unroll
     for (int q = 0; q < IDCT_PRE; ++q) pre[q] = L[tid + q * IDCT_THREADS];
-    __shared__ int32_t dqs[2 * VCG_NFREQ];
     if (tid < VCG_NFREQ) {
         const int o = j.mode == MODE_PAYLOAD ? 0 : lod * VCG_NFREQ;
         dqs[tid] = j.dqs[o + tid];
@@ -312,7 +319,7 @@ __global__ __launch_bounds__(IDCT_THREADS) void k_idct(dec_job j, const vcg_dtab
     for (int q = 0; q < IDCT_PRE; ++q)
         sidx[q] = tid + q * IDCT_THREADS < (int)cnt ? T->scan[vcg_ent_pos(pre[q])] : 0u;
     vcg_tile_zero(buf, tid, IDCT_THREADS);
-    if (tid == 0) l1sum = 0;
+    if (tid == 0) *l1sum = 0;
     __syncthreads();
     const int32_t *st = dqs, *bi = dqs + VCG_NFREQ;
     unsigned long long mine = 0;
@@ -322,13 +329,13 @@ __global__ __launch_bounds__(IDCT_THREADS) void k_idct(dec_job j, const vcg_dtab
     if (cnt > IDCT_PRE * IDCT_THREADS)
         mine += vcg_scatter(buf, L + IDCT_PRE * IDCT_THREADS, cnt - IDCT_PRE * IDCT_THREADS, T->scan, st, bi, tid,
                             IDCT_THREADS);
-    if (mine) atomicAdd(&l1sum, mine);
+    if (mine) atomicAdd(l1sum, mine);
     __syncthreads();
     // Every intermediate element is bounded by the atom's L2 norm (each 1-D
     // pass is orthonormal up to Q14 rounding, sigma_max <= 1.001, plus <= 0.5
     // rounding per element), and L2 <= L1.  With L1 <= 8e6 all MAC operands
     // stay below 8.02e6 < 2^23, so the 24-bit multiply path is exact.
-    const bool fast = l1sum <= 8000000ull;
+    const bool fast = *l1sum <= 8000000ull;
     const int32_t (*C)[32] = T->cmat;
     const int dc = (int)it.dc;
     uint8_t vox[64];
@@ -349,6 +356,71 @@ __global__ __launch_bounds__(IDCT_THREADS) void k_idct(dec_job j, const vcg_dtab
     vcg_inv_stage(stage, vox, tid);
     __syncthreads();
     store_atom(item_dst(j, gi), ax, ay, az, stage, tid);
+}
+
+__global__ __launch_bounds__(IDCT_THREADS) void k_idct(dec_job j, const vcg_dtab *__restrict__ T,
+                                                       const uint32_t *__restrict__ lists,
+                                                       const vcg_item *__restrict__ items) {
+    extern __shared__ int buf[];
+    __shared__ unsigned long long l1sum;
+    __shared__ int32_t dqs[2 * VCG_NFREQ];
+    idct_item(j, xcd_item(blockIdx.x, gridDim.x), T, lists, items, buf, &l1sum, dqs);
+}
+
+// ---------------------------------------------------------------------------
+// persistent batch kernels (server batches replayed as hipGraphs)
+// ---------------------------------------------------------------------------
+// A batch's size and its two claim counters live in device memory (a 16-byte
+// header uploaded with the batch's quads), so one captured launch sequence
+// serves every batch of up to its capacity.  Entropy waves claim 64 atoms
+// (one per lane) at a time until the batch is exhausted; transform
+// workgroups claim one atom at a time.  Both loops end once a claim passes n,
+// which every wave reaches.
+struct vcg_qhdr {
+    uint32_t n;        // atoms in the batch
+    uint32_t ent;      // next unclaimed atom of the entropy pass
+    uint32_t idct;     // next unclaimed atom of the transform
+    uint32_t pad;
+};
+
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_entropy_p(dec_job j, const vcg_dtab *__restrict__ T,
+                                                      uint32_t *__restrict__ lists, vcg_item *__restrict__ items,
+                                                      uint32_t *__restrict__ status, vcg_qhdr *__restrict__ q) {
+    __builtin_amdgcn_s_setprio(3);
+    __shared__ uint32_t ctxw[W][VCG_CTX_SLOTS * 64];
+    __shared__ uint32_t ringw[W][VCG_RING * 64];
+    __shared__ uint32_t bands[9];
+    if (threadIdx.x < 9) bands[threadIdx.x] = T->band_start[threadIdx.x];
+    __syncthreads();
+    const uint32_t n = q->n;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&q->ent, 64u);
+        base = __shfl(base, 0, 64);
+        if (base >= n) break;
+        const uint32_t i = base + lane;
+        if (i < n) ent_atom(j, i, T, lists, items, status, &ctxw[0][0], &ringw[0][0], bands, lane, wave);
+    }
+}
+
+__global__ __launch_bounds__(IDCT_THREADS) void k_idct_p(dec_job j, const vcg_dtab *__restrict__ T,
+                                                         const uint32_t *__restrict__ lists,
+                                                         const vcg_item *__restrict__ items, vcg_qhdr *__restrict__ q) {
+    extern __shared__ int buf[];
+    __shared__ unsigned long long l1sum;
+    __shared__ int32_t dqs[2 * VCG_NFREQ];
+    __shared__ uint32_t cur;
+    const uint32_t n = q->n;
+    for (;;) {
+        __syncthreads();                 // the previous atom is done with the tile and cur
+        if (threadIdx.x == 0) cur = atomicAdd(&q->idct, 1u);
+        __syncthreads();
+        const uint32_t i = cur;
+        if (i >= n) break;
+        idct_item(j, i, T, lists, items, buf, &l1sum, dqs);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -372,33 +444,10 @@ static void ensure_attr() {
 // waits for them too; the transform always waits for earlier work on c->s (it
 // writes the caller's output).  up_*: an input upload done on the first slot
 // stream instead (pinned source).  Returns the scratch slot of the last batch.
-// fixed_slot >= 0 (one batch, total <= MAX_BATCH): everything on that slot's
-// stream, no ordering against c->s and no join into it.
 static int run_batch(vcg_ctx *c, dec_job j, uint32_t total, bool inputs_on_s, const void *up_src = nullptr,
-                     void *up_dst = nullptr, size_t up_bytes = 0, int fixed_slot = -1) {
+                     void *up_dst = nullptr, size_t up_bytes = 0) {
     ensure_attr();
     uint32_t maxb = total < MAX_BATCH ? total : MAX_BATCH;
-    if (fixed_slot >= 0) {
-        hipStream_t sE = c->sE[fixed_slot];
-        // only this slot's scratch, and only after this slot's earlier work
-        // (other slots' work is not ordered against c->s here)
-        if (c->lists_items[fixed_slot] < maxb || c->items_cap[fixed_slot] < maxb) {
-            VCG_CHECK(hipStreamSynchronize(sE));
-            vcg_grow((void **)&c->lists[fixed_slot], &c->lists_items[fixed_slot], maxb,
-                     (size_t)VCG_LIST_STRIDE * 4);
-            vcg_grow((void **)&c->items[fixed_slot], &c->items_cap[fixed_slot], maxb, sizeof(vcg_item));
-        }
-        if (up_bytes) VCG_CHECK(hipMemcpyAsync(up_dst, up_src, up_bytes, hipMemcpyHostToDevice, sE));
-        j.base = 0;
-        j.n = total;
-        hipLaunchKernelGGL(k_entropy, dim3((j.n + ENT_THREADS - 1) / ENT_THREADS), dim3(ENT_THREADS), 0, sE, j, g_dtab, c->lists[fixed_slot],
-                           c->items[fixed_slot], c->status_d);
-        VCG_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_idct, dim3(j.n), dim3(IDCT_THREADS), LDS_BYTES, sE, j, g_dtab, c->lists[fixed_slot],
-                           c->items[fixed_slot]);
-        VCG_CHECK(hipGetLastError());
-        return fixed_slot;
-    }
     for (int k = 0; k < VCG_SLOTS; ++k) {
         vcg_grow((void **)&c->lists[k], &c->lists_items[k], maxb, (size_t)VCG_LIST_STRIDE * 4);
         vcg_grow((void **)&c->items[k], &c->items_cap[k], maxb, sizeof(vcg_item));
@@ -552,26 +601,6 @@ extern "C" int vcg_decode_payloads(size_t n, const uint8_t *pay, const uint64_t 
 // asynchronous launchers (serving loop): enqueue and return; completion is
 // observed through an event recorded on the calling thread's stream
 // ---------------------------------------------------------------------------
-extern "C" int vcg_list_launch(vcg_arc *a, size_t n, const int32_t *quads_pinned, int32_t *quads_dev,
-                               uint8_t *out_dev, int32_t *status_dev, size_t nplace, int slot) {
-    vcg_ctx *c = vcg_get_ctx();
-    if (n == 0 || n > MAX_BATCH || slot < 0 || slot >= VCG_SLOTS) return -1;
-    const vcg_geom *gd = vcg_arc_geom_dev(a);
-    dec_job j;
-    memset(&j, 0, sizeof j);
-    j.arc = vcg_arc_buf(a);
-    j.geom = gd;
-    j.dqs = &gd->dq_step[0][0];
-    j.dqb = &gd->dq_bias[0][0];
-    j.mode = MODE_LIST;
-    j.quads = quads_dev;
-    j.places = nplace ? (const vcg_place *)(quads_dev + 4 * n) : nullptr;
-    j.out = out_dev;
-    j.status_out = status_dev;
-    run_batch(c, j, (uint32_t)n, false, quads_pinned, quads_dev, n * 16 + nplace * sizeof(vcg_place), slot);
-    return 0;
-}
-
 extern "C" size_t vcg_slot_scratch_bytes(void) { return (size_t)VCG_LIST_STRIDE * 4 + sizeof(vcg_item); }
 
 extern "C" int vcg_reserve_slots(size_t n) {
@@ -663,4 +692,106 @@ extern "C" int vcg_list_launch_noalloc(vcg_arc *a, size_t n, const int32_t *quad
     hipLaunchKernelGGL(k_idct, dim3(j.n), dim3(IDCT_THREADS), LDS_BYTES, s, j, g_dtab, lists, items);
     if (hipGetLastError() != hipSuccess) return -1;
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// captured persistent batch decode (server slots)
+// ---------------------------------------------------------------------------
+struct vcg_qgraph {
+    hipGraphExec_t exec = nullptr;
+    hipStream_t s = nullptr;
+    uint8_t *qdev = nullptr;
+    size_t cap = 0;
+    int slot = 0;
+    uint32_t *lists = nullptr;           // the slot scratch the graph was captured with
+    vcg_item *items = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+};
+
+static_assert(sizeof(vcg_qhdr) == 16, "batch header");
+
+extern "C" vcg_qgraph *vcg_qgraph_create(vcg_arc *a, int slot, const vcg_qcfg *cfg, uint8_t *qdev, size_t cap,
+                                         uint8_t *out_dev, int32_t *status_dev) {
+    if (slot < 0 || slot >= VCG_SLOTS || cap == 0 || cap > MAX_BATCH || !cfg) return nullptr;
+    const int W = cfg->ent_waves;
+    if ((W != 1 && W != 2 && W != 4) || cfg->ent_groups == 0 || cfg->idct_groups == 0) return nullptr;
+    vcg_ctx *c = vcg_get_ctx();
+    if (c->lists_items[slot] < cap || c->items_cap[slot] < cap) return nullptr;   // reserve first
+    ensure_attr();
+    static bool attr_p = false;
+    if (!attr_p) {
+        VCG_CHECK(hipFuncSetAttribute((const void *)k_idct_p, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)LDS_BYTES));
+        attr_p = true;
+    }
+    const vcg_geom *gd = vcg_arc_geom_dev(a);
+    dec_job j;
+    memset(&j, 0, sizeof j);
+    j.arc = vcg_arc_buf(a);
+    j.geom = gd;
+    j.dqs = &gd->dq_step[0][0];
+    j.dqb = &gd->dq_bias[0][0];
+    j.mode = MODE_LIST;
+    j.quads = (const int32_t *)(qdev + 16);
+    j.places = (const vcg_place *)(qdev + 16 + cap * 16);
+    j.out = out_dev;
+    j.status_out = status_dev;
+    vcg_qhdr *hq = (vcg_qhdr *)qdev;
+    vcg_qgraph *g = new vcg_qgraph();
+    g->s = c->sE[slot]; g->qdev = qdev; g->cap = cap; g->slot = slot;
+    g->lists = c->lists[slot]; g->items = c->items[slot];
+    hipGraph_t graph = nullptr;
+    VCG_CHECK(hipStreamBeginCapture(g->s, hipStreamCaptureModeThreadLocal));
+    const dim3 ge(cfg->ent_groups), gt(cfg->idct_groups);
+    if (W == 1)
+        hipLaunchKernelGGL(k_entropy_p<1>, ge, dim3(64), 0, g->s, j, g_dtab, g->lists, g->items, c->status_d, hq);
+    else if (W == 2)
+        hipLaunchKernelGGL(k_entropy_p<2>, ge, dim3(128), 0, g->s, j, g_dtab, g->lists, g->items, c->status_d, hq);
+    else
+        hipLaunchKernelGGL(k_entropy_p<4>, ge, dim3(256), 0, g->s, j, g_dtab, g->lists, g->items, c->status_d, hq);
+    hipLaunchKernelGGL(k_idct_p, gt, dim3(IDCT_THREADS), LDS_BYTES, g->s, j, g_dtab, g->lists, g->items, hq);
+    VCG_CHECK(hipStreamEndCapture(g->s, &graph));
+    VCG_CHECK(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0));
+    VCG_CHECK(hipGraphDestroy(graph));
+    VCG_CHECK(hipEventCreate(&g->t0));
+    VCG_CHECK(hipEventCreate(&g->t1));
+    return g;
+}
+
+extern "C" int vcg_qgraph_launch(vcg_qgraph *g, const uint8_t *qpin, size_t n, size_t nplace) {
+    if (!g || n == 0 || n > g->cap || nplace > g->cap) return -1;
+    vcg_ctx *c = vcg_get_ctx();
+    // the graph holds the slot scratch it was captured with
+    if (c->lists[g->slot] != g->lists || c->items[g->slot] != g->items)
+        vc__panic("decoder scratch of a captured batch slot was reallocated");
+    // header (n, claim counters at 0) and quads in one copy, placements in another
+    VCG_CHECK(hipMemcpyAsync(g->qdev, qpin, 16 + n * 16, hipMemcpyHostToDevice, g->s));
+    if (nplace)
+        VCG_CHECK(hipMemcpyAsync(g->qdev + 16 + g->cap * 16, qpin + 16 + g->cap * 16, nplace * sizeof(vcg_place),
+                                 hipMemcpyHostToDevice, g->s));
+    VCG_CHECK(hipGraphLaunch(g->exec, g->s));
+    return 0;
+}
+
+extern "C" float vcg_qgraph_time(vcg_qgraph *g, const uint8_t *qpin, size_t n, size_t nplace, int reps) {
+    float best = 1e30f;
+    for (int r = 0; r < reps; ++r) {
+        VCG_CHECK(hipEventRecord(g->t0, g->s));
+        if (vcg_qgraph_launch(g, qpin, n, nplace) != 0) return -1.0f;
+        VCG_CHECK(hipEventRecord(g->t1, g->s));
+        VCG_CHECK(hipEventSynchronize(g->t1));
+        float ms = 0;
+        VCG_CHECK(hipEventElapsedTime(&ms, g->t0, g->t1));
+        if (ms < best) best = ms;
+    }
+    return best;
+}
+
+extern "C" void vcg_qgraph_destroy(vcg_qgraph *g) {
+    if (!g) return;
+    VCG_CHECK(hipStreamSynchronize(g->s));
+    VCG_CHECK(hipGraphExecDestroy(g->exec));
+    VCG_CHECK(hipEventDestroy(g->t0));
+    VCG_CHECK(hipEventDestroy(g->t1));
+    delete g;
 }

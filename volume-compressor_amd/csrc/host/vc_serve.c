@@ -25,6 +25,7 @@
  * every atom is decoded independently (the batch only shares launches).
  */
 #include <pthread.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -38,6 +39,10 @@
 #define SRV_COARSE_SLOTS 1
 #endif
 #define SRV_MAX_BATCH 32768u
+/* launch-shape table: one captured persistent launch per LOD class and batch
+ * size bucket (a batch takes the smallest bucket holding it) */
+#define SRV_NB 4
+static const size_t srv_bucket[SRV_NB] = {64, 512, 4096, 32768};
 
 typedef struct srv_req {
     int64_t id;
@@ -59,9 +64,10 @@ typedef struct {
     void *ev;
     srv_req *reqs;              /* admitted requests, in batch order */
     size_t n;                   /* atoms in the batch */
-    int32_t *q_pin, *st_pin;    /* pinned staging: quads, then box placements */
-    int32_t *q_dev, *st_dev;
+    uint8_t *q_pin, *q_dev;     /* [16-byte header][cap quads][cap placements], pinned / device */
+    int32_t *st_pin, *st_dev;
     uint8_t *out_dev;
+    vcg_qgraph *g[2][SRV_NB];   /* captured launches, built on first use */
 } srv_slot;
 
 struct vcx_server {
@@ -80,6 +86,12 @@ struct vcx_server {
     int64_t *d_id; int32_t *d_st; uint64_t *d_t;
     size_t d_cap, d_head, d_len;
     srv_slot slot[SRV_SLOTS];
+    /* launch shapes per (class, bucket): the built-in default until
+     * vcx_server_autotune measures better ones */
+    vcg_qcfg cfg[2][SRV_NB];
+    float ms_default[2][SRV_NB], ms_tuned[2][SRV_NB];
+    int tune_req, tune_done, tuned;
+    int rebuild;                /* shapes changed: recapture before the next batch */
 };
 
 uint64_t vcx_now_ns(void) {
@@ -106,12 +118,60 @@ static void push_done(vcx_server *s, int64_t id, int32_t st, uint64_t t) {
     s->d_len++;
 }
 
+static int bucket_of(size_t n) {
+    int k = 0;
+    while (k + 1 < SRV_NB && n > srv_bucket[k]) ++k;
+    return k;
+}
+
+/* the built-in launch shape for batches of up to S atoms: the one-shot
+ * kernels' geometry (four-wave entropy workgroups covering every atom at
+ * once, one transform workgroup per atom) */
+static vcg_qcfg default_cfg(size_t S) {
+    vcg_qcfg c;
+    c.ent_waves = 4;
+    c.ent_groups = (uint32_t)((S + 255) / 256);
+    c.idct_groups = (uint32_t)S;
+    return c;
+}
+
+static size_t slot_qbytes(size_t cap) { return 16 + cap * 16 + cap * sizeof(vcg_place); }
+
+static vcg_qgraph *slot_graph(vcx_server *s, srv_slot *b, int cls, int bk) {
+    if (!b->g[cls][bk]) {
+        const int k = (int)(b - s->slot);
+        b->g[cls][bk] = vcg_qgraph_create(s->dev, k, &s->cfg[cls][bk], b->q_dev, s->cap_atoms, b->out_dev, b->st_dev);
+        if (!b->g[cls][bk]) vc__panic("server: capturing a batch launch failed");
+    }
+    return b->g[cls][bk];
+}
+
+/* capture every reachable (slot, class, bucket) launch up front */
+static void prebuild(vcx_server *s) {
+    for (int k = 0; k < SRV_SLOTS; ++k)
+        for (int c = 0; c < 2; ++c)
+            for (int bk = 0; bk < SRV_NB; ++bk)
+                if (bk == 0 || srv_bucket[bk - 1] < s->cap_atoms) (void)slot_graph(s, &s->slot[k], c, bk);
+}
+
+static void drop_graphs(vcx_server *s) {
+    for (int k = 0; k < SRV_SLOTS; ++k)
+        for (int c = 0; c < 2; ++c)
+            for (int bk = 0; bk < SRV_NB; ++bk) {
+                vcg_qgraph_destroy(s->slot[k].g[c][bk]);
+                s->slot[k].g[c][bk] = NULL;
+            }
+}
+
 /* worker: enqueue one admitted batch (no lock held) */
 static void launch(vcx_server *s, srv_slot *b) {
-    vcg_place *pl = (vcg_place *)(b->q_pin + 4 * b->n);
+    const size_t cap = s->cap_atoms;
+    vcg_place *pl = (vcg_place *)(b->q_pin + 16 + cap * 16);
+    uint32_t hdr[4] = {(uint32_t)b->n, 0, 0, 0};
+    memcpy(b->q_pin, hdr, 16);
     size_t np = 0;
     for (srv_req *r = b->reqs; r; r = r->next) {
-        int32_t *q = b->q_pin + 4 * r->first;
+        int32_t *q = (int32_t *)(b->q_pin + 16) + 4 * r->first;
         if (r->kind == 0) { memcpy(q, r->quads, r->n * 16); continue; }
         /* a box: its atoms, clipped into the raster at the request's output space */
         const vc_box bx = r->box;
@@ -129,12 +189,86 @@ static void launch(vcx_server *s, srv_slot *b) {
      * this slot's scratch: batches in other slots neither wait for it nor
      * delay it, and the slot's next batch follows its copies */
     const int k = (int)(b - s->slot);
-    if (vcg_list_launch(s->dev, b->n, b->q_pin, b->q_dev, b->out_dev, b->st_dev, np, k) != 0)
+    if (vcg_qgraph_launch(slot_graph(s, b, b->cls, bucket_of(b->n)), b->q_pin, b->n, np) != 0)
         vc__panic("server batch launch rejected");
     for (srv_req *r = b->reqs; r; r = r->next)
         vcg_copy_async(r->out, b->out_dev + r->first * (size_t)VC_ATOM3, r->out_bytes, 2, k);
     vcg_copy_async(b->st_pin, b->st_dev, b->n * 4, 2, k);
     vcg_event_record(b->ev, k);
+}
+
+/* Launch-shape candidates for batches of up to S atoms: entropy workgroups
+ * of 1, 2 or 4 waves (64 to 256 atoms per workgroup in flight), enough of
+ * them for every atom at once or for half (waves then take two rounds of 64),
+ * and 256, 512 or 1024 transform workgroups (one, two or four per CU
+ * claiming atoms in turn) or one per atom.  The default is among them. */
+static int tune_candidates(size_t S, vcg_qcfg *out) {
+    static const int ws[3] = {1, 2, 4};
+    int n = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int half = 0; half < 2; ++half) {
+            uint32_t ge = (uint32_t)((S + 64u * ws[a] - 1) / (64u * ws[a]));
+            if (half) { if (ge < 2) continue; ge = (ge + 1) / 2; }
+            const uint32_t gts[4] = {256, 512, 1024, (uint32_t)S};
+            for (int t = 0; t < 4; ++t) {
+                if (gts[t] > S || (t < 3 && gts[t] == S)) continue;
+                out[n].ent_waves = ws[a]; out[n].ent_groups = ge; out[n].idct_groups = gts[t];
+                n++;
+            }
+        }
+    return n;
+}
+
+/* S random atoms of LOD class cls (0: LOD0; 1: LODs 1..nlod-1, uniform)
+ * into slot b's pinned batch buffer */
+static void tune_sample(const vcx_server *s, srv_slot *b, int cls, size_t S, unsigned *seed) {
+    uint32_t hdr[4] = {(uint32_t)S, 0, 0, 0};
+    memcpy(b->q_pin, hdr, 16);
+    int32_t *q = (int32_t *)(b->q_pin + 16);
+    for (size_t i = 0; i < S; ++i) {
+        const int lod = cls == 0 ? 0 : 1 + (int)(rand_r(seed) % (unsigned)(s->a->nlod - 1));
+        const vc_dims d = vc__lod_dims(s->a->dims0, lod);
+        q[4 * i] = lod;
+        q[4 * i + 1] = (int32_t)((unsigned)rand_r(seed) % ((d.nx + VC_A - 1) / VC_A));
+        q[4 * i + 2] = (int32_t)((unsigned)rand_r(seed) % ((d.ny + VC_A - 1) / VC_A));
+        q[4 * i + 3] = (int32_t)((unsigned)rand_r(seed) % ((d.nz + VC_A - 1) / VC_A));
+    }
+}
+
+/* worker, no batch in flight: time every candidate shape per (class,
+ * bucket) on sample batches of the archive's own atoms (the fastest of three
+ * replays each) and keep the fastest; the captured launches are rebuilt on
+ * next use */
+static void autotune(vcx_server *s) {
+    drop_graphs(s);
+    srv_slot *b = &s->slot[0];
+    unsigned seed = 20261204u;
+    vcg_qcfg cand[24];
+    for (int cls = 0; cls < 2; ++cls)
+        for (int bk = 0; bk < SRV_NB; ++bk) {
+            const size_t S = srv_bucket[bk] < s->cap_atoms ? srv_bucket[bk] : s->cap_atoms;
+            if ((cls == 1 && s->a->nlod < 2) || (bk > 0 && srv_bucket[bk - 1] >= s->cap_atoms)) {
+                s->ms_default[cls][bk] = s->ms_tuned[cls][bk] = 0.0f;     /* class or bucket never used */
+                continue;
+            }
+            tune_sample(s, b, cls, S, &seed);
+            const vcg_qcfg def = default_cfg(S);
+            const int nc = tune_candidates(S, cand);
+            float best = 1e30f;
+            for (int c = 0; c < nc; ++c) {
+                vcg_qgraph *g = vcg_qgraph_create(s->dev, 0, &cand[c], b->q_dev, s->cap_atoms, b->out_dev, b->st_dev);
+                if (!g) continue;
+                const float ms = vcg_qgraph_time(g, b->q_pin, S, 0, 3);
+                vcg_qgraph_destroy(g);
+                if (ms < 0) continue;
+                if (cand[c].ent_waves == def.ent_waves && cand[c].ent_groups == def.ent_groups &&
+                    cand[c].idct_groups == def.idct_groups)
+                    s->ms_default[cls][bk] = ms;
+                if (ms < best) { best = ms; s->cfg[cls][bk] = cand[c]; }
+            }
+            s->ms_tuned[cls][bk] = best;
+        }
+    s->tuned = 1;
 }
 
 static void retire(vcx_server *s, srv_slot *b, uint64_t now) {
@@ -160,11 +294,31 @@ static void *worker(void *arg) {
      * thread (it owns the decode context) before the first request: no batch
      * then waits on an allocation and the device synchronisation it takes */
     const int ok = vcg_reserve_slots(s->cap_atoms) == 0;
+    if (ok) prebuild(s);
     pthread_mutex_lock(&s->mu);
     s->ready = ok ? 1 : -1;
     pthread_cond_broadcast(&s->cv_done);
     if (!ok) { pthread_mutex_unlock(&s->mu); return NULL; }
     for (;;) {
+        if (s->rebuild && s->inflight == 0) {
+            s->rebuild = 0;
+            pthread_mutex_unlock(&s->mu);
+            drop_graphs(s);
+            prebuild(s);
+            pthread_mutex_lock(&s->mu);
+            continue;
+        }
+        /* a tuning request runs once no batch is in flight */
+        if (s->tune_req && s->inflight == 0) {
+            pthread_mutex_unlock(&s->mu);
+            autotune(s);
+            prebuild(s);
+            pthread_mutex_lock(&s->mu);
+            s->tune_req = 0;
+            s->tune_done = 1;
+            pthread_cond_broadcast(&s->cv_done);
+            continue;
+        }
         /* retire finished batches */
         int retired = 0;
         for (int k = 0; k < SRV_SLOTS; ++k) {
@@ -213,7 +367,7 @@ static void *worker(void *arg) {
             pthread_mutex_lock(&s->mu);
             continue;
         }
-        if (s->stop && !s->qhead && s->inflight == 0) break;
+        if (s->stop && !s->qhead && s->inflight == 0 && !s->tune_req) break;
         /* queued requests that no free slot may take wait for a retirement */
         if (s->inflight == 0) {
             pthread_cond_wait(&s->cv_work, &s->mu);
@@ -226,6 +380,7 @@ static void *worker(void *arg) {
         }
     }
     pthread_mutex_unlock(&s->mu);
+    drop_graphs(s);
     return NULL;
 }
 
@@ -240,6 +395,9 @@ vcx_server *vcx_server_create(vc_archive *a, size_t device_bytes) {
     vcx_server *s = (vcx_server *)calloc(1, sizeof *s);
     if (!s) return NULL;
     s->a = a; s->dev = d; s->cap_atoms = per;
+    for (int c = 0; c < 2; ++c)
+        for (int bk = 0; bk < SRV_NB; ++bk)
+            s->cfg[c][bk] = default_cfg(srv_bucket[bk] < per ? srv_bucket[bk] : per);
     pthread_mutex_init(&s->mu, NULL);
     pthread_cond_init(&s->cv_work, NULL);
     pthread_cond_init(&s->cv_done, NULL);
@@ -247,9 +405,9 @@ vcx_server *vcx_server_create(vc_archive *a, size_t device_bytes) {
         srv_slot *b = &s->slot[k];
         b->ev = vcg_event_create();
         /* quads plus at most one box placement per atom */
-        b->q_pin = (int32_t *)vcg_host_alloc(per * (16 + sizeof(vcg_place)));
+        b->q_pin = (uint8_t *)vcg_host_alloc(slot_qbytes(per));
         b->st_pin = (int32_t *)vcg_host_alloc(per * 4);
-        b->q_dev = (int32_t *)vcg_alloc(per * (16 + sizeof(vcg_place)));
+        b->q_dev = (uint8_t *)vcg_alloc(slot_qbytes(per));
         b->st_dev = (int32_t *)vcg_alloc(per * 4);
         b->out_dev = (uint8_t *)vcg_alloc(per * (size_t)VC_ATOM3);
         if (!b->q_pin || !b->st_pin || !b->q_dev || !b->st_dev || !b->out_dev) {
@@ -356,6 +514,59 @@ size_t vcx_server_poll(vcx_server *s, int64_t *ids, int32_t *status, uint64_t *t
 }
 
 size_t vcx_server_capacity(const vcx_server *s) { return s ? s->cap_atoms : 0; }
+
+vc_status vcx_server_autotune(vcx_server *s) {
+    if (!s) return VC_ERR_RANGE;
+    pthread_mutex_lock(&s->mu);
+    if (s->stop) { pthread_mutex_unlock(&s->mu); return VC_ERR_RANGE; }
+    s->tune_done = 0;
+    s->tune_req = 1;
+    pthread_cond_signal(&s->cv_work);
+    while (!s->tune_done) pthread_cond_wait(&s->cv_done, &s->mu);
+    pthread_mutex_unlock(&s->mu);
+    return VC_OK;
+}
+
+vc_status vcx_test_server_shape(vcx_server *s, int ent_waves, uint32_t ent_groups, uint32_t idct_groups) {
+    if (!s || (ent_waves != 1 && ent_waves != 2 && ent_waves != 4) || !ent_groups || !idct_groups) return VC_ERR_RANGE;
+    pthread_mutex_lock(&s->mu);
+    while (s->inflight || s->qhead) {           /* drain: the slots' graphs are rebuilt */
+        pthread_mutex_unlock(&s->mu);
+        struct timespec ts = {0, 200000};
+        nanosleep(&ts, NULL);
+        pthread_mutex_lock(&s->mu);
+    }
+    for (int c = 0; c < 2; ++c)
+        for (int bk = 0; bk < SRV_NB; ++bk) {
+            s->cfg[c][bk].ent_waves = ent_waves;
+            s->cfg[c][bk].ent_groups = ent_groups;
+            s->cfg[c][bk].idct_groups = idct_groups;
+        }
+    s->rebuild = 1;
+    pthread_mutex_unlock(&s->mu);
+    return VC_OK;
+}
+
+size_t vcx_server_tuning(vcx_server *s, char *buf, size_t cap) {
+    if (!s) return 0;
+    char tmp[4096];
+    size_t n = 0;
+    pthread_mutex_lock(&s->mu);
+    n += (size_t)snprintf(tmp + n, sizeof tmp - n, "# tuned %d; class bucket ent_waves ent_groups idct_groups default_ms tuned_ms\n",
+                          s->tuned);
+    for (int c = 0; c < 2; ++c)
+        for (int bk = 0; bk < SRV_NB; ++bk)
+            n += (size_t)snprintf(tmp + n, sizeof tmp - n, "%d %zu %d %u %u %.4f %.4f\n", c, srv_bucket[bk],
+                                  s->cfg[c][bk].ent_waves, s->cfg[c][bk].ent_groups, s->cfg[c][bk].idct_groups,
+                                  (double)s->ms_default[c][bk], (double)s->ms_tuned[c][bk]);
+    pthread_mutex_unlock(&s->mu);
+    if (buf && cap) {
+        const size_t m = n < cap - 1 ? n : cap - 1;
+        memcpy(buf, tmp, m);
+        buf[m] = 0;
+    }
+    return n;
+}
 
 /* output atom, quad, box placement, status word and decoder scratch */
 size_t vcx_server_atom_bytes(void) {

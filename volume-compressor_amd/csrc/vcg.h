@@ -86,17 +86,6 @@ int vcg_decode_list(vcg_arc *a, size_t n, const int32_t *quads, uint8_t *out, in
  * box [x0,x1) x [y0,y1) x [z0,z1) of its LOD and written into the box raster
  * at out_dev + out_off. */
 typedef struct { uint32_t x0, y0, z0, x1, y1, z1; uint64_t out_off; } vcg_place;
-/* enqueue only: n <= 32768 quads from pinned memory (staged into quads_dev),
- * per-atom 1 = corrupt / 0 to status_dev.  Item k's atom goes whole to
- * out_dev + k * 32768, unless nplace > 0 and its quad's lod word carries a
- * placement: lod | (p + 1) << 8 sends it, clipped, to box placement p.  The
- * nplace placements follow the quads in the same pinned/device buffers
- * (16-byte aligned, at quads + 4 * n).  The whole batch (upload, entropy,
- * transform) runs in order on the stream of scratch slot `slot`, after
- * whatever was enqueued there before and independent of the thread's other
- * streams: the caller orders reuse of its buffers through that stream. */
-int vcg_list_launch(vcg_arc *a, size_t n, const int32_t *quads_pinned, int32_t *quads_dev, uint8_t *out_dev,
-                    int32_t *status_dev, size_t nplace, int slot);
 /* Device scratch bytes per atom of a list launch's capacity (coefficient
  * list and item record), and a reservation of that scratch in every slot of
  * the calling thread's context for batches of up to n atoms (n <= 32768),
@@ -105,6 +94,28 @@ int vcg_list_launch(vcg_arc *a, size_t n, const int32_t *quads_pinned, int32_t *
  * reservation). */
 size_t vcg_slot_scratch_bytes(void);
 int    vcg_reserve_slots(size_t n);
+/* Captured persistent batch decode (server slots).  A graph decodes batches
+ * of up to cap quads on the stream of scratch slot `slot`: the entropy
+ * kernel with cfg->ent_waves (1, 2 or 4) waves per workgroup and
+ * cfg->ent_groups workgroups, whose waves claim 64 atoms at a time until the
+ * batch is exhausted, then the transform with cfg->idct_groups workgroups
+ * claiming one atom at a time.  qdev (device) and the caller's pinned qpin
+ * share one layout: a 16-byte header {n, 0, 0, 0}, cap quads (lod |
+ * placement << 8, ax, ay, az; a placement p sends the item's atom clipped
+ * to box placement p - 1, else whole to out_dev + item * 32768), cap
+ * placements (vcg_place); status_dev gets 1 per corrupt atom, else 0.
+ * The slot's scratch must be reserved
+ * for cap atoms first (vcg_reserve_slots) and stay so.  launch: upload the
+ * header, n quads and nplace placements, then replay the graph (enqueue
+ * only).  time: reps launches, each timed by events on the slot stream
+ * (synchronous), the fastest in ms (the autotuner's measurement). */
+typedef struct { int32_t ent_waves; uint32_t ent_groups; uint32_t idct_groups; } vcg_qcfg;
+typedef struct vcg_qgraph vcg_qgraph;
+vcg_qgraph *vcg_qgraph_create(vcg_arc *a, int slot, const vcg_qcfg *cfg, uint8_t *qdev, size_t cap,
+                              uint8_t *out_dev, int32_t *status_dev);
+int   vcg_qgraph_launch(vcg_qgraph *g, const uint8_t *qpin, size_t n, size_t nplace);
+float vcg_qgraph_time(vcg_qgraph *g, const uint8_t *qpin, size_t n, size_t nplace, int reps);
+void  vcg_qgraph_destroy(vcg_qgraph *g);
 /* Allocation-free list decode: at most VCG_NOALLOC_MAX quads in device
  * memory, checked on the device (status 2 = out of range, 1 = corrupt, 0 =
  * ok into status_dev when given), whole atoms to out_dev, enqueued on

@@ -82,6 +82,9 @@ def lib():
         "vcx_server_submit_region": (C.c_int64, [vp, i32, Box, vp]),
         "vcx_server_poll": (sz, [vp, vp, vp, vp, sz, i32]),
         "vcx_server_capacity": (sz, [vp]),
+        "vcx_server_autotune": (i32, [vp]),
+        "vcx_server_tuning": (sz, [vp, vp, sz]),
+        "vcx_test_server_shape": (i32, [vp, i32, u32, u32]),
         "vcx_server_atom_bytes": (sz, []),
         "vcx_now_ns": (C.c_uint64, []),
         "vcx_host_alloc": (vp, [sz]),
@@ -425,6 +428,25 @@ class Server:
 
     def capacity(self):
         return self.L.vcx_server_capacity(self.h)
+
+    def autotune(self):
+        return self.L.vcx_server_autotune(self.h)
+
+    def tuning(self):
+        """the launch-shape table: [(class, bucket, ent_waves, ent_groups,
+        idct_groups, default_ms, tuned_ms)]"""
+        buf = C.create_string_buffer(4096)
+        self.L.vcx_server_tuning(self.h, buf, 4096)
+        rows = []
+        for line in buf.value.decode().splitlines():
+            if line.startswith("#") or not line.strip():
+                continue
+            w = line.split()
+            rows.append(tuple(int(x) for x in w[:5]) + tuple(float(x) for x in w[5:]))
+        return rows
+
+    def set_shape(self, ent_waves, ent_groups, idct_groups):
+        return self.L.vcx_test_server_shape(self.h, ent_waves, ent_groups, idct_groups)
 
     def submit_atoms(self, quads, out_ptr):
         q = np.ascontiguousarray(quads, np.int32).reshape(-1, 4)
