@@ -367,6 +367,40 @@ def qps_sweep(pyvc, a, reqs, rates):
     return out
 
 
+def dest_bench(pyvc, a, n=4096, reps=5, seed=19):
+    """One request of n random LOD0 atoms at a time through the tuned server,
+    its output in pinned host memory vs device memory (no PCIe copy): the
+    median latency of each."""
+    rng = np.random.default_rng(seed)
+    _, (nx, ny, nz) = a.lod_dims(0)
+    q = np.stack([np.zeros(n, np.int64), rng.integers(0, (nx + 31) // 32, n), rng.integers(0, (ny + 31) // 32, n),
+                  rng.integers(0, (nz + 31) // 32, n)], 1).astype(np.int32)
+    pin = pyvc.PinnedBuffer(n * 32768)
+    dev = pyvc.dev_alloc(n * 32768)
+    srv = pyvc.Server(a)
+    out = {"atoms": n}
+    try:
+        assert srv.autotune() == 0
+        for name, ptr in (("pinned_host", pin.p), ("device", dev)):
+            lat = []
+            for _ in range(reps + 1):
+                t0 = pyvc.now_ns()
+                i = srv.submit_atoms(q, ptr)
+                assert i >= 0
+                while True:
+                    got, st, t = srv.poll(wait_ms=100)
+                    if len(got):
+                        assert int(st[0]) == 0
+                        lat.append((int(t[0]) - t0) / 1e6)
+                        break
+            out[f"{name}_ms"] = round(float(np.median(lat[1:])), 3)
+    finally:
+        srv.close()
+        pin.free()
+        pyvc.dev_free(dev)
+    return out
+
+
 def atom_floor(pyvc, a, sample=3000, seed=13):
     """Per LOD, the heaviest atom of a random sample (largest payload, i.e.
     most serial range-coder bins) decoded alone through vc_decode_atom: the
@@ -458,6 +492,8 @@ def serving_section(pyvc, args):
     res["region_requests"] = {"batched": sr, "plain": pr, "cpu": cr,
                               "sweep": qps_sweep(pyvc, a, rreqs, (200, 400, 800, 1600, 3200))}
     log(f"region serving: {sr}; plain {pr}; cpu {cr}")
+    res["destination_4096_atoms"] = dest_bench(pyvc, a)
+    log(f"4096-atom request by destination: {res['destination_4096_atoms']}")
     res["atom_floor"] = atom_floor(pyvc, a)
     log(f"atom floor: {res['atom_floor']}")
     res["repack"] = repack_bench(pyvc, a, arc, no_cpu=args.no_cpu)

@@ -182,15 +182,22 @@ vc_status vcx_repack_device(vc_archive *a, void *dev_out, size_t cap, size_t *ou
  * thread.  device_bytes bounds the device memory of the batches in flight:
  * three batch slots share it at vcx_server_atom_bytes() per atom of slot
  * capacity (output, request staging and decoder scratch, about 160 KiB),
- * all of it allocated before create returns (NULL when it does not fit); a
- * request larger than one slot is rejected.
+ * all of it allocated before create returns (NULL when it does not fit).
+ * device_bytes = 0 admits by free device memory: half of what hipMemGetInfo
+ * reports free at create, at most full batches (32768 atoms) in every slot.
  * Requests are admitted in arrival order as slots free up; batches overlap on
- * the device.  Submit returns a request id (>= 0) or -1 (invalid request or
- * too large).  Atom requests: quads = (lod, ax, ay, az) x n, out receives n
- * atoms of VC_ATOM3 bytes; region requests: out receives the box raster as
- * vc_decode_region writes it.  out must stay valid until the request
- * completes; pinned memory (vcx_host_alloc) keeps the result copy
- * asynchronous.  A request decodes to the same bytes alone or in any mix.
+ * the device.  A request larger than one slot is not rejected but served in
+ * parts over consecutive batches and reported once, when its last part
+ * completes: an atom list in chunks of a slot's capacity, a box in z-slabs
+ * of whole atom layers (rejected only if one atom layer of the box exceeds a
+ * slot).  Submit returns a request id (>= 0) or -1 (invalid request).  Atom
+ * requests: quads = (lod, ax, ay, az) x n, out receives n atoms of VC_ATOM3
+ * bytes; region requests: out receives the box raster as vc_decode_region
+ * writes it.  out may be host or device memory (device memory: the result
+ * stays on the device, no PCIe copy) and must stay valid until the request
+ * completes; for host memory, pinned memory (vcx_host_alloc) keeps the
+ * result copy asynchronous.  A request decodes to the same bytes alone or in
+ * any mix.
  * vcx_server_poll returns up to max completions (id, status as vc_status,
  * completion time on CLOCK_MONOTONIC in ns, see vcx_now_ns), waiting up to
  * wait_ms for the first one.  Destroy waits for every submitted request. */

@@ -113,3 +113,59 @@ def test_autotune_table_and_results(pyvc, arc_pair):
     finally:
         srv.close()
     _check(oa, reqs, outs, done)
+
+
+def test_oversize_requests_split_and_device_destination(pyvc, arc_pair):
+    """Slots of 40 atoms: a 150-atom request and boxes of more atoms than a
+    slot are served in parts (one completion each), a box whose single atom
+    layer exceeds a slot is rejected, and results land in device memory when
+    out is a device pointer; every byte equals the oracle's decode."""
+    _, a, oa = arc_pair
+    srv = pyvc.Server(a, 40 * pyvc.Server.atom_bytes() * 3)
+    assert srv.capacity() == 40
+    rng = np.random.default_rng(9)
+    try:
+        _, (nx, ny, nz) = a.lod_dims(0)
+        q = np.stack([np.zeros(150, np.int64), rng.integers(0, (nx + 31) // 32, 150),
+                      rng.integers(0, (ny + 31) // 32, 150), rng.integers(0, (nz + 31) // 32, 150)], 1).astype(np.int32)
+        reqs = [("atoms", q),
+                ("region", (0, (5, 17, 3, 101, 60, 220))),       # 4 x 2 atoms a layer, 7 layers: 2 parts
+                ("region", (0, (0, 0, 0, 288, 64, 224))),        # 9 x 2 atoms a layer, 7 layers: 4 parts
+                ("region", (1, (3, 5, 7, 140, 100, 111))),       # LOD1, 5 x 4 atoms a layer, 4 layers: 2 parts
+                ("atoms", q[:7])]
+        outs, done = _serve(pyvc, srv, reqs)
+        _check(oa, reqs, outs, done)
+        # one atom layer of 7 x 8 atoms > 40: rejected
+        big = pyvc.PinnedBuffer(8)
+        assert srv.submit_region(0, (0, 0, 0, 224, 256, 10), big.p) == -1
+        big.free()
+        # device destinations (4 and 6 parts)
+        d1 = pyvc.dev_alloc(150 * A3)
+        box = (33, 40, 10, 200, 190, 170)
+        nb = (box[3] - box[0]) * (box[4] - box[1]) * (box[5] - box[2])
+        d2 = pyvc.dev_alloc(nb)
+        try:
+            ids = {srv.submit_atoms(q, d1): 0, srv.submit_region(0, box, d2): 1}
+            st = {}
+            for _ in range(400):
+                got, s_, _ = srv.poll(wait_ms=50)
+                for g, x in zip(got, s_):
+                    st[ids[int(g)]] = int(x)
+                if len(st) == 2:
+                    break
+            assert st == {0: 0, 1: 0}
+            _check(oa, [("atoms", q), ("region", (0, box))], [pyvc.d2h(d1, 150 * A3), pyvc.d2h(d2, nb)], {0: 0, 1: 0})
+        finally:
+            pyvc.dev_free(d1)
+            pyvc.dev_free(d2)
+    finally:
+        srv.close()
+
+
+def test_capacity_from_free_memory(pyvc, arc_pair):
+    _, a, _ = arc_pair
+    srv = pyvc.Server(a, 0)
+    try:
+        assert 0 < srv.capacity() <= 32768
+    finally:
+        srv.close()

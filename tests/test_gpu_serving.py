@@ -100,7 +100,8 @@ def test_server_rejects_invalid(pyvc, served):
     try:
         b = pyvc.PinnedBuffer(16 * A3)
         assert srv.capacity() == 8
-        assert srv.submit_atoms(np.zeros((9, 4), np.int32), b.p) == -1          # larger than a slot
+        # 160 x 128 voxels: one atom layer of 5 x 4 atoms exceeds the 8-atom slot
+        assert srv.submit_region(0, (0, 0, 0, 160, 128, 3), b.p) == -1
         assert srv.submit_atoms(np.array([[0, 99, 0, 0]], np.int32), b.p) == -1  # outside the grid
         assert srv.submit_atoms(np.array([[9, 0, 0, 0]], np.int32), b.p) == -1   # no such LOD
         assert srv.submit_region(0, (5, 5, 5, 5, 9, 9), b.p) == -1               # empty box

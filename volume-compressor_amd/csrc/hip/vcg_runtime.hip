@@ -163,8 +163,18 @@ static hipStream_t slot_stream(vcg_ctx *c, int slot) {
 
 extern "C" int vcg_copy_async(void *dst, const void *src, size_t bytes, int kind, int slot) {
     vcg_ctx *c = vcg_get_ctx();
-    hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    hipMemcpyKind k = kind == 0 ? hipMemcpyDefault : kind == 1 ? hipMemcpyHostToDevice
+                    : kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
     VCG_CHECK(hipMemcpyAsync(dst, src, bytes, k, slot_stream(c, slot)));
+    return 0;
+}
+
+extern "C" int vcg_mem_info(size_t *free_bytes, size_t *total_bytes) {
+    vcg_get_ctx();                      // the library's device is current
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) { (void)hipGetLastError(); return -1; }
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
     return 0;
 }
 

@@ -55,8 +55,17 @@ typedef struct srv_req {
     uint8_t *out;
     size_t out_bytes;
     size_t first;               /* index of its first atom in the batch */
+    struct srv_parent *parent;  /* a part of a request split over several batches, or NULL */
     struct srv_req *next;
 } srv_req;
+
+/* a request larger than one batch slot, served as parts in consecutive
+ * batches and reported once, when its last part completes */
+typedef struct srv_parent {
+    int64_t id;
+    size_t left;                /* parts not yet retired */
+    int32_t st;
+} srv_parent;
 
 typedef struct {
     int busy;
@@ -191,8 +200,10 @@ static void launch(vcx_server *s, srv_slot *b) {
     const int k = (int)(b - s->slot);
     if (vcg_qgraph_launch(slot_graph(s, b, b->cls, bucket_of(b->n)), b->q_pin, b->n, np) != 0)
         vc__panic("server batch launch rejected");
+    /* by pointer: pinned or pageable host memory, or device memory (a
+     * device destination costs no PCIe transfer) */
     for (srv_req *r = b->reqs; r; r = r->next)
-        vcg_copy_async(r->out, b->out_dev + r->first * (size_t)VC_ATOM3, r->out_bytes, 2, k);
+        vcg_copy_async(r->out, b->out_dev + r->first * (size_t)VC_ATOM3, r->out_bytes, 0, k);
     vcg_copy_async(b->st_pin, b->st_dev, b->n * 4, 2, k);
     vcg_event_record(b->ev, k);
 }
@@ -276,7 +287,16 @@ static void retire(vcx_server *s, srv_slot *b, uint64_t now) {
     while (r) {
         int32_t st = 0;
         for (size_t k = 0; k < r->n; ++k) st |= b->st_pin[r->first + k];
-        push_done(s, r->id, st ? VC_ERR_FORMAT : VC_OK, now);
+        if (r->parent) {
+            srv_parent *p = r->parent;
+            p->st |= st;
+            if (--p->left == 0) {
+                push_done(s, p->id, p->st ? VC_ERR_FORMAT : VC_OK, now);
+                free(p);
+            }
+        } else {
+            push_done(s, r->id, st ? VC_ERR_FORMAT : VC_OK, now);
+        }
         srv_req *nx = r->next;
         free(r->quads);
         free(r);
@@ -389,6 +409,15 @@ vcx_server *vcx_server_create(vc_archive *a, size_t device_bytes) {
     vc__require_gpu();
     vcg_arc *d = vc__arc_dev(a);
     if (!d) return NULL;
+    if (device_bytes == 0) {
+        /* admission by free device memory: half of what is free now, at
+         * most what full batches in every slot take */
+        size_t fr = 0;
+        if (vcg_mem_info(&fr, NULL) != 0) return NULL;
+        device_bytes = fr / 2;
+        const size_t most = (size_t)SRV_SLOTS * SRV_MAX_BATCH * vcx_server_atom_bytes();
+        if (device_bytes > most) device_bytes = most;
+    }
     size_t per = device_bytes / SRV_SLOTS / vcx_server_atom_bytes();
     if (per > SRV_MAX_BATCH) per = SRV_MAX_BATCH;
     if (per < 1) return NULL;
@@ -433,7 +462,13 @@ void vcx_server_destroy(vcx_server *s) {
     pthread_cond_broadcast(&s->cv_work);
     pthread_mutex_unlock(&s->mu);
     if (running) pthread_join(s->th, NULL);
-    for (srv_req *r = s->qhead; r;) { srv_req *nx = r->next; free(r->quads); free(r); r = nx; }
+    for (srv_req *r = s->qhead; r;) {
+        srv_req *nx = r->next;
+        if (r->parent && --r->parent->left == 0) free(r->parent);
+        free(r->quads);
+        free(r);
+        r = nx;
+    }
     for (int k = 0; k < SRV_SLOTS; ++k) {
         srv_slot *b = &s->slot[k];
         vcg_event_destroy(b->ev);
@@ -447,45 +482,92 @@ void vcx_server_destroy(vcx_server *s) {
     free(s);
 }
 
-static int64_t enqueue(vcx_server *s, srv_req *r) {
+/* queue the parts of one request (one part: the request itself) in order,
+ * under one id */
+static int64_t enqueue(vcx_server *s, srv_req **parts, size_t np, srv_parent *par) {
     pthread_mutex_lock(&s->mu);
-    if (s->stop) { pthread_mutex_unlock(&s->mu); free(r->quads); free(r); return -1; }
-    r->id = s->next_id++;
-    r->next = NULL;
-    if (s->qtail) s->qtail->next = r; else s->qhead = r;
-    s->qtail = r;
-    int64_t id = r->id;
+    if (s->stop) {
+        pthread_mutex_unlock(&s->mu);
+        for (size_t k = 0; k < np; ++k) { free(parts[k]->quads); free(parts[k]); }
+        free(par);
+        return -1;
+    }
+    const int64_t id = s->next_id++;
+    if (par) { par->id = id; par->left = np; }
+    for (size_t k = 0; k < np; ++k) {
+        srv_req *r = parts[k];
+        r->id = id;
+        r->parent = par;
+        r->next = NULL;
+        if (s->qtail) s->qtail->next = r; else s->qhead = r;
+        s->qtail = r;
+    }
     pthread_cond_signal(&s->cv_work);
     pthread_mutex_unlock(&s->mu);
     return id;
 }
 
-int64_t vcx_server_submit_atoms(vcx_server *s, size_t n, const int32_t *quads, uint8_t *out) {
-    if (!s || !n || !quads || !out || n > s->cap_atoms) return -1;
-    for (size_t i = 0; i < n; ++i)
-        if (!vc__atom_ok(s->a, quads[4 * i], quads[4 * i + 1], quads[4 * i + 2], quads[4 * i + 3])) return -1;
-    srv_req *r = (srv_req *)calloc(1, sizeof *r);
-    if (!r) return -1;
-    r->quads = (int32_t *)malloc(n * 16);
-    if (!r->quads) { free(r); return -1; }
-    memcpy(r->quads, quads, n * 16);
-    r->kind = 0; r->n = n; r->out = out; r->out_bytes = n * (size_t)VC_ATOM3;
-    for (size_t i = 0; i < n; ++i) if (quads[4 * i] != 0) r->cls = 1;
-    return enqueue(s, r);
+static void free_parts(srv_req **parts, size_t np) {
+    for (size_t k = 0; k < np; ++k) if (parts[k]) { free(parts[k]->quads); free(parts[k]); }
+    free(parts);
 }
 
+/* A list longer than one batch slot goes as parts of at most a slot's
+ * capacity, each writing its atoms at their place in out. */
+int64_t vcx_server_submit_atoms(vcx_server *s, size_t n, const int32_t *quads, uint8_t *out) {
+    if (!s || !n || !quads || !out) return -1;
+    for (size_t i = 0; i < n; ++i)
+        if (!vc__atom_ok(s->a, quads[4 * i], quads[4 * i + 1], quads[4 * i + 2], quads[4 * i + 3])) return -1;
+    const size_t cap = s->cap_atoms, np = (n + cap - 1) / cap;
+    srv_req **parts = (srv_req **)calloc(np, sizeof *parts);
+    srv_parent *par = np > 1 ? (srv_parent *)calloc(1, sizeof *par) : NULL;
+    if (!parts || (np > 1 && !par)) { free(parts); free(par); return -1; }
+    for (size_t k = 0; k < np; ++k) {
+        const size_t i0 = k * cap, m = n - i0 < cap ? n - i0 : cap;
+        srv_req *r = (srv_req *)calloc(1, sizeof *r);
+        if (!r || !(r->quads = (int32_t *)malloc(m * 16))) { free(r); free_parts(parts, np); free(par); return -1; }
+        parts[k] = r;
+        memcpy(r->quads, quads + 4 * i0, m * 16);
+        r->kind = 0; r->n = m; r->out = out + i0 * (size_t)VC_ATOM3; r->out_bytes = m * (size_t)VC_ATOM3;
+        for (size_t i = 0; i < m; ++i) if (r->quads[4 * i] != 0) r->cls = 1;
+    }
+    const int64_t id = enqueue(s, parts, np, par);
+    free(parts);
+    return id;
+}
+
+/* A box of more atoms than one batch slot goes as z-slabs of whole atom
+ * layers (the first and last clipped to the box): each slab's raster is the
+ * contiguous piece of the box raster at its first z.  Rejected only when one
+ * atom layer of the box exceeds a slot. */
 int64_t vcx_server_submit_region(vcx_server *s, int lod, vc_box box, uint8_t *out) {
     if (!s || !out || lod < 0 || lod >= s->a->nlod) return -1;
     if (vc__region_check(s->a, lod, box) != VC_OK) return -1;
-    const size_t nax = (box.x1 - 1) / VC_A - box.x0 / VC_A + 1, nay = (box.y1 - 1) / VC_A - box.y0 / VC_A + 1,
-                 naz = (box.z1 - 1) / VC_A - box.z0 / VC_A + 1;
-    const size_t bytes = (size_t)(box.x1 - box.x0) * (box.y1 - box.y0) * (box.z1 - box.z0);
-    if (nax * nay * naz > s->cap_atoms || bytes > s->cap_atoms * (size_t)VC_ATOM3) return -1;
-    srv_req *r = (srv_req *)calloc(1, sizeof *r);
-    if (!r) return -1;
-    r->kind = 1; r->n = nax * nay * naz; r->lod = lod; r->box = box; r->out = out; r->out_bytes = bytes;
-    r->cls = lod != 0;
-    return enqueue(s, r);
+    const size_t nax = (box.x1 - 1) / VC_A - box.x0 / VC_A + 1, nay = (box.y1 - 1) / VC_A - box.y0 / VC_A + 1;
+    const uint32_t az0 = box.z0 / VC_A, az1 = (box.z1 - 1) / VC_A + 1;
+    const size_t layer = nax * nay, cap = s->cap_atoms;
+    if (layer > cap) return -1;
+    const uint32_t per = (uint32_t)(cap / layer);                 /* atom layers per part */
+    const size_t np = (az1 - az0 + per - 1) / per;
+    const uint64_t plane = (uint64_t)(box.x1 - box.x0) * (box.y1 - box.y0);
+    srv_req **parts = (srv_req **)calloc(np, sizeof *parts);
+    srv_parent *par = np > 1 ? (srv_parent *)calloc(1, sizeof *par) : NULL;
+    if (!parts || (np > 1 && !par)) { free(parts); free(par); return -1; }
+    for (size_t k = 0; k < np; ++k) {
+        const uint32_t la = az0 + (uint32_t)k * per, lb = la + per < az1 ? la + per : az1;
+        vc_box b = box;
+        b.z0 = la * VC_A > box.z0 ? la * VC_A : box.z0;
+        b.z1 = lb * VC_A < box.z1 ? lb * VC_A : box.z1;
+        srv_req *r = (srv_req *)calloc(1, sizeof *r);
+        if (!r) { free_parts(parts, np); free(par); return -1; }
+        parts[k] = r;
+        r->kind = 1; r->n = layer * (lb - la); r->lod = lod; r->box = b;
+        r->out = out + (uint64_t)(b.z0 - box.z0) * plane; r->out_bytes = (uint64_t)(b.z1 - b.z0) * plane;
+        r->cls = lod != 0;
+    }
+    const int64_t id = enqueue(s, parts, np, par);
+    free(parts);
+    return id;
 }
 
 size_t vcx_server_poll(vcx_server *s, int64_t *ids, int32_t *status, uint64_t *t_done_ns, size_t max, int wait_ms) {

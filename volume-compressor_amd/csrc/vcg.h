@@ -54,7 +54,8 @@ int   vcg_copy_kernel(void *dst, const void *src, size_t bytes); /* 16-B aligned
 int   vcg_sync(void);                       /* wait; returns accumulated vc status */
 /* asynchronous helpers (serving loop): on the calling thread's stream
  * (slot -1) or on the stream of its scratch slot `slot` (0 .. VCG_SLOTS-1) */
-int   vcg_copy_async(void *dst, const void *src, size_t bytes, int kind, int slot);
+int   vcg_copy_async(void *dst, const void *src, size_t bytes, int kind, int slot); /* kind 0: by pointer */
+int   vcg_mem_info(size_t *free_bytes, size_t *total_bytes);   /* hipMemGetInfo; 0 / -1 */
 void *vcg_event_create(void);
 void  vcg_event_destroy(void *e);
 void  vcg_event_record(void *e, int slot);
