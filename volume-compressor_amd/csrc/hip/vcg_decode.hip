@@ -13,6 +13,7 @@
 //               adds dc, clamps and stores u8 voxels to the destination.
 #include <hip/hip_runtime.h>
 #include <string.h>
+#include <mutex>
 #include "vcg_dev.hpp"
 #include "vcg_xform.hpp"
 
@@ -428,6 +429,7 @@ __global__ __launch_bounds__(IDCT_THREADS) void k_idct_p(dec_job j, const vcg_dt
 // ---------------------------------------------------------------------------
 static const size_t LDS_BYTES = (size_t)VCG_TILE_INTS * 4;
 static const uint32_t MAX_BATCH = 32768;   // items per launch pair (scratch: 4 GiB)
+static const uint64_t VCG_HOST_SLAB = 1ull << 30;   // host-destination staging per slab
 
 static void ensure_attr() {
     static bool done = false;
@@ -444,14 +446,60 @@ static void ensure_attr() {
 // waits for them too; the transform always waits for earlier work on c->s (it
 // writes the caller's output).  up_*: an input upload done on the first slot
 // stream instead (pinned source).  Returns the scratch slot of the last batch.
+// Decoder scratch (coefficient lists and item records) is one pool per
+// device, shared by every host thread: VCG_SCRATCH_BLOCKS blocks taken in
+// turn, each batch's launches ordered after the previous user of its block
+// by that block's last-use event (a wait on the GPU, not on the host).  So
+// the scratch stays at most VCG_SCRATCH_BLOCKS x the largest batch however
+// many threads decode.  A block grows (after its last user finished) when a
+// batch needs more atoms than it holds.
+#define VCG_SCRATCH_BLOCKS 3
+struct scratch_blk {
+    uint32_t *lists = nullptr;
+    vcg_item *items = nullptr;
+    size_t cap = 0;
+    hipEvent_t last = nullptr;
+    bool used = false;
+};
+static std::mutex g_blk_mu;
+static scratch_blk g_blk[VCG_SCRATCH_BLOCKS];
+static int g_blk_next = 0;
+
+// the next block with room for n atoms (grown if needed), its last user
+// waited for on stream sE; returns it with g_blk_mu held
+static scratch_blk *blk_take(hipStream_t sE, size_t n) {
+    g_blk_mu.lock();
+    scratch_blk *b = &g_blk[g_blk_next];
+    g_blk_next = (g_blk_next + 1) % VCG_SCRATCH_BLOCKS;
+    if (!b->last) VCG_CHECK(hipEventCreateWithFlags(&b->last, hipEventDisableTiming));
+    if (b->cap < n) {
+        if (b->used) VCG_CHECK(hipEventSynchronize(b->last));
+        if (b->lists) VCG_CHECK(hipFree(b->lists));
+        if (b->items) VCG_CHECK(hipFree(b->items));
+        b->lists = nullptr; b->items = nullptr; b->cap = 0;
+        if (hipMalloc((void **)&b->lists, n * (size_t)VCG_LIST_STRIDE * 4) != hipSuccess ||
+            hipMalloc((void **)&b->items, n * sizeof(vcg_item)) != hipSuccess) {
+            (void)hipGetLastError();
+            g_blk_mu.unlock();
+            vc__panic("out of device memory (decoder scratch)");
+        }
+        b->cap = n;
+        b->used = false;
+    }
+    if (b->used) VCG_CHECK(hipStreamWaitEvent(sE, b->last, 0));
+    return b;
+}
+// after the block's last launch on sE: record its use, release the pool
+static void blk_give(scratch_blk *b, hipStream_t sE) {
+    VCG_CHECK(hipEventRecord(b->last, sE));
+    b->used = true;
+    g_blk_mu.unlock();
+}
+
 static int run_batch(vcg_ctx *c, dec_job j, uint32_t total, bool inputs_on_s, const void *up_src = nullptr,
                      void *up_dst = nullptr, size_t up_bytes = 0) {
     ensure_attr();
     uint32_t maxb = total < MAX_BATCH ? total : MAX_BATCH;
-    for (int k = 0; k < VCG_SLOTS; ++k) {
-        vcg_grow((void **)&c->lists[k], &c->lists_items[k], maxb, (size_t)VCG_LIST_STRIDE * 4);
-        vcg_grow((void **)&c->items[k], &c->items_cap[k], maxb, sizeof(vcg_item));
-    }
     VCG_CHECK(hipEventRecord(c->evF, c->s));
     const int first = c->slot;
     if (up_bytes) VCG_CHECK(hipMemcpyAsync(up_dst, up_src, up_bytes, hipMemcpyHostToDevice, c->sE[first]));
@@ -464,16 +512,17 @@ static int run_batch(vcg_ctx *c, dec_job j, uint32_t total, bool inputs_on_s, co
         // entropy: after the transform that last used this slot (same stream)
         if (inputs_on_s) VCG_CHECK(hipStreamWaitEvent(sE, c->evF, 0));
         if (up_bytes && sl != first) VCG_CHECK(hipStreamWaitEvent(sE, c->evI[first], 0));
+        scratch_blk *blk = blk_take(sE, maxb);
         if (c->ev[0]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[0], sE));
-        hipLaunchKernelGGL(k_entropy, dim3((j.n + ENT_THREADS - 1) / ENT_THREADS), dim3(ENT_THREADS), 0, sE, j, g_dtab, c->lists[sl],
-                           c->items[sl], c->status_d);
+        hipLaunchKernelGGL(k_entropy, dim3((j.n + ENT_THREADS - 1) / ENT_THREADS), dim3(ENT_THREADS), 0, sE, j, g_dtab,
+                           blk->lists, blk->items, c->status_d);
         VCG_CHECK(hipGetLastError());
         if (c->ev[1]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[1], sE));
         // transform: after its entropy pass (same stream) and earlier work on c->s
         if (!inputs_on_s) VCG_CHECK(hipStreamWaitEvent(sE, c->evF, 0));
-        hipLaunchKernelGGL(k_idct, dim3(j.n), dim3(IDCT_THREADS), LDS_BYTES, sE, j, g_dtab, c->lists[sl],
-                           c->items[sl]);
+        hipLaunchKernelGGL(k_idct, dim3(j.n), dim3(IDCT_THREADS), LDS_BYTES, sE, j, g_dtab, blk->lists, blk->items);
         VCG_CHECK(hipGetLastError());
+        blk_give(blk, sE);
         if (c->ev[2]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[2], sE));
         VCG_CHECK(hipEventRecord(c->evI[sl], sE));
         c->last_slot = sl;
@@ -505,17 +554,40 @@ extern "C" int vcg_decode_box(vcg_arc *a, int lod, const vcg_box *box, uint8_t *
     j.bx0 = box->x0; j.by0 = box->y0; j.bz0 = box->z0;
     j.bx1 = box->x1; j.by1 = box->y1; j.bz1 = box->z1;
     (void)g;
-    size_t bytes = (size_t)(box->x1 - box->x0) * (box->y1 - box->y0) * (box->z1 - box->z0);
-    if (out_host) {
-        vcg_grow((void **)&c->stage, &c->stage_cap, bytes, 1);
-        j.out = c->stage;
-    } else {
+    if (!out_host) {
         j.out = out;
+        run_batch(c, j, j.nax * j.nay * naz, false);
+        return sync ? vcg_sync() : 0;
     }
-    run_batch(c, j, j.nax * j.nay * naz, false);
-    if (out_host) VCG_CHECK(hipMemcpyAsync(out, c->stage, bytes, hipMemcpyDeviceToHost, c->s));
-    if (sync || out_host) return vcg_sync();
-    return 0;
+    // Host destination: decoded into device staging, then copied.  Boxes of
+    // more than VCG_HOST_SLAB bytes go as z-slabs of whole atom layers (the
+    // first and last clipped to the box) through two alternating staging
+    // buffers, so a thread's staging stays bounded and a slab's copy overlaps
+    // the next slab's decode; a slab's raster is the contiguous piece of the
+    // box raster at its first z.  The transform of slab k+2 waits for earlier
+    // work on c->s (run_batch), which includes slab k's copy out of the same
+    // buffer.
+    const uint64_t plane = (uint64_t)(box->x1 - box->x0) * (box->y1 - box->y0);
+    const size_t bytes = plane * (box->z1 - box->z0);
+    uint32_t per = naz;                                     // atom layers per slab
+    if (bytes > VCG_HOST_SLAB) {
+        per = (uint32_t)(VCG_HOST_SLAB / (plane * 32));
+        if (per == 0) per = 1;
+    }
+    const size_t slab = per >= naz ? bytes : (size_t)per * 32 * plane;
+    vcg_grow((void **)&c->stage, &c->stage_cap, per >= naz ? bytes : 2 * slab, 1);
+    for (uint32_t k = 0, la = j.az0; la < j.az0 + naz; ++k, la += per) {
+        const uint32_t lb = la + per < j.az0 + naz ? la + per : j.az0 + naz;
+        dec_job jj = j;
+        jj.az0 = la;
+        jj.bz0 = la * 32 > box->z0 ? la * 32 : box->z0;
+        jj.bz1 = lb * 32 < box->z1 ? lb * 32 : box->z1;
+        jj.out = c->stage + (k & 1) * slab;
+        run_batch(c, jj, j.nax * j.nay * (lb - la), false);
+        VCG_CHECK(hipMemcpyAsync(out + (uint64_t)(jj.bz0 - box->z0) * plane, jj.out, (uint64_t)(jj.bz1 - jj.bz0) * plane,
+                                 hipMemcpyDeviceToHost, c->s));
+    }
+    return vcg_sync();
 }
 
 extern "C" int vcg_decode_list(vcg_arc *a, size_t n, const int32_t *quads, uint8_t *out, int32_t *per_item) {
@@ -603,25 +675,6 @@ extern "C" int vcg_decode_payloads(size_t n, const uint8_t *pay, const uint64_t 
 // ---------------------------------------------------------------------------
 extern "C" size_t vcg_slot_scratch_bytes(void) { return (size_t)VCG_LIST_STRIDE * 4 + sizeof(vcg_item); }
 
-extern "C" int vcg_reserve_slots(size_t n) {
-    if (n == 0 || n > MAX_BATCH) return -1;
-    vcg_ctx *c = vcg_get_ctx();
-    for (int k = 0; k < VCG_SLOTS; ++k) {
-        if (c->lists_items[k] >= n && c->items_cap[k] >= n) continue;
-        VCG_CHECK(hipStreamSynchronize(c->sE[k]));
-        if (c->lists[k]) VCG_CHECK(hipFree(c->lists[k]));
-        if (c->items[k]) VCG_CHECK(hipFree(c->items[k]));
-        c->lists[k] = nullptr; c->items[k] = nullptr;
-        c->lists_items[k] = c->items_cap[k] = 0;
-        if (hipMalloc((void **)&c->lists[k], n * (size_t)VCG_LIST_STRIDE * 4) != hipSuccess ||
-            hipMalloc((void **)&c->items[k], n * sizeof(vcg_item)) != hipSuccess) {
-            (void)hipGetLastError();
-            return -1;   // capacity 0: the next launch on the slot grows it
-        }
-        c->lists_items[k] = c->items_cap[k] = n;
-    }
-    return 0;
-}
 
 
 // ---------------------------------------------------------------------------
@@ -703,7 +756,7 @@ struct vcg_qgraph {
     uint8_t *qdev = nullptr;
     size_t cap = 0;
     int slot = 0;
-    uint32_t *lists = nullptr;           // the slot scratch the graph was captured with
+    uint32_t *lists = nullptr;           // the caller's scratch the graph was captured with
     vcg_item *items = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
 };
@@ -711,12 +764,12 @@ struct vcg_qgraph {
 static_assert(sizeof(vcg_qhdr) == 16, "batch header");
 
 extern "C" vcg_qgraph *vcg_qgraph_create(vcg_arc *a, int slot, const vcg_qcfg *cfg, uint8_t *qdev, size_t cap,
-                                         uint8_t *out_dev, int32_t *status_dev) {
+                                         uint8_t *out_dev, int32_t *status_dev, void *scratch) {
     if (slot < 0 || slot >= VCG_SLOTS || cap == 0 || cap > MAX_BATCH || !cfg) return nullptr;
     const int W = cfg->ent_waves;
     if ((W != 1 && W != 2 && W != 4) || cfg->ent_groups == 0 || cfg->idct_groups == 0) return nullptr;
+    if (!scratch) return nullptr;
     vcg_ctx *c = vcg_get_ctx();
-    if (c->lists_items[slot] < cap || c->items_cap[slot] < cap) return nullptr;   // reserve first
     ensure_attr();
     static bool attr_p = false;
     if (!attr_p) {
@@ -739,7 +792,9 @@ extern "C" vcg_qgraph *vcg_qgraph_create(vcg_arc *a, int slot, const vcg_qcfg *c
     vcg_qhdr *hq = (vcg_qhdr *)qdev;
     vcg_qgraph *g = new vcg_qgraph();
     g->s = c->sE[slot]; g->qdev = qdev; g->cap = cap; g->slot = slot;
-    g->lists = c->lists[slot]; g->items = c->items[slot];
+    // the caller's scratch: cap list strides, then cap item records
+    g->lists = (uint32_t *)scratch;
+    g->items = (vcg_item *)((uint8_t *)scratch + cap * (size_t)VCG_LIST_STRIDE * 4);
     hipGraph_t graph = nullptr;
     VCG_CHECK(hipStreamBeginCapture(g->s, hipStreamCaptureModeThreadLocal));
     const dim3 ge(cfg->ent_groups), gt(cfg->idct_groups);
@@ -760,10 +815,6 @@ extern "C" vcg_qgraph *vcg_qgraph_create(vcg_arc *a, int slot, const vcg_qcfg *c
 
 extern "C" int vcg_qgraph_launch(vcg_qgraph *g, const uint8_t *qpin, size_t n, size_t nplace) {
     if (!g || n == 0 || n > g->cap || nplace > g->cap) return -1;
-    vcg_ctx *c = vcg_get_ctx();
-    // the graph holds the slot scratch it was captured with
-    if (c->lists[g->slot] != g->lists || c->items[g->slot] != g->items)
-        vc__panic("decoder scratch of a captured batch slot was reallocated");
     // header (n, claim counters at 0) and quads in one copy, placements in another
     VCG_CHECK(hipMemcpyAsync(g->qdev, qpin, 16 + n * 16, hipMemcpyHostToDevice, g->s));
     if (nplace)

@@ -58,8 +58,6 @@ struct vcg_ctx {
     hipStream_t s = nullptr, sE[VCG_SLOTS] = {};
     hipEvent_t evF = nullptr, evI[VCG_SLOTS] = {};
     int slot = 0, last_slot = 0;
-    uint32_t *lists[VCG_SLOTS] = {};  size_t lists_items[VCG_SLOTS] = {};
-    vcg_item *items[VCG_SLOTS] = {};  size_t items_cap[VCG_SLOTS] = {};
     uint8_t  *stage = nullptr;  size_t stage_cap = 0;
     int32_t  *quads = nullptr;  size_t quads_cap = 0;
     int32_t  *stat = nullptr;   size_t stat_cap = 0;

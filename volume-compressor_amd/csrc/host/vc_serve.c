@@ -10,8 +10,15 @@
  * by the transform kernel (a box of n atoms needs at most n atoms of output
  * space).  Finished batches are retired by polling their events, and each
  * request is reported with its status and completion time.  Results are
- * copied into the caller's buffer (pinned memory from vcx_host_alloc keeps
- * the copies asynchronous).
+ * copied into the caller's buffer (host or device memory; pinned memory from
+ * vcx_host_alloc keeps host copies asynchronous).  A request larger than one
+ * slot is served as parts over consecutive batches and reported once.
+ *
+ * A batch runs as two persistent kernels (entropy waves claiming 64 atoms at
+ * a time, transform workgroups one atom at a time, the batch size read from
+ * device memory), captured once per slot, LOD class and size bucket as a
+ * hipGraph and replayed; launch shapes come from a table that
+ * vcx_server_autotune fills by timing candidates on the archive's own atoms.
  *
  * Batches are LOD-aware.  Coarser LODs are quantized finer, so their atoms
  * carry several times the bins of an LOD0 atom, and a batch lasts as long as
@@ -76,6 +83,7 @@ typedef struct {
     uint8_t *q_pin, *q_dev;     /* [16-byte header][cap quads][cap placements], pinned / device */
     int32_t *st_pin, *st_dev;
     uint8_t *out_dev;
+    void *scratch;              /* decoder scratch of this slot's batches */
     vcg_qgraph *g[2][SRV_NB];   /* captured launches, built on first use */
 } srv_slot;
 
@@ -87,7 +95,7 @@ struct vcx_server {
     pthread_mutex_t mu;
     pthread_cond_t cv_work, cv_done;
     int stop;
-    int ready;                  /* worker: 1 scratch reserved, -1 failed */
+    int ready;                  /* worker: 1 once its launches are captured */
     srv_req *qhead, *qtail;
     int64_t next_id;
     size_t pending, inflight;
@@ -149,7 +157,8 @@ static size_t slot_qbytes(size_t cap) { return 16 + cap * 16 + cap * sizeof(vcg_
 static vcg_qgraph *slot_graph(vcx_server *s, srv_slot *b, int cls, int bk) {
     if (!b->g[cls][bk]) {
         const int k = (int)(b - s->slot);
-        b->g[cls][bk] = vcg_qgraph_create(s->dev, k, &s->cfg[cls][bk], b->q_dev, s->cap_atoms, b->out_dev, b->st_dev);
+        b->g[cls][bk] = vcg_qgraph_create(s->dev, k, &s->cfg[cls][bk], b->q_dev, s->cap_atoms, b->out_dev, b->st_dev,
+                                          b->scratch);
         if (!b->g[cls][bk]) vc__panic("server: capturing a batch launch failed");
     }
     return b->g[cls][bk];
@@ -267,7 +276,8 @@ static void autotune(vcx_server *s) {
             const int nc = tune_candidates(S, cand);
             float best = 1e30f;
             for (int c = 0; c < nc; ++c) {
-                vcg_qgraph *g = vcg_qgraph_create(s->dev, 0, &cand[c], b->q_dev, s->cap_atoms, b->out_dev, b->st_dev);
+                vcg_qgraph *g = vcg_qgraph_create(s->dev, 0, &cand[c], b->q_dev, s->cap_atoms, b->out_dev, b->st_dev,
+                                                  b->scratch);
                 if (!g) continue;
                 const float ms = vcg_qgraph_time(g, b->q_pin, S, 0, 3);
                 vcg_qgraph_destroy(g);
@@ -310,15 +320,12 @@ static void retire(vcx_server *s, srv_slot *b, uint64_t now) {
 
 static void *worker(void *arg) {
     vcx_server *s = (vcx_server *)arg;
-    /* the decoder scratch of a full batch in every slot, reserved on this
-     * thread (it owns the decode context) before the first request: no batch
-     * then waits on an allocation and the device synchronisation it takes */
-    const int ok = vcg_reserve_slots(s->cap_atoms) == 0;
-    if (ok) prebuild(s);
+    /* every slot's launches captured before the first request (the slots'
+     * scratch was allocated with the server), on this thread's streams */
+    prebuild(s);
     pthread_mutex_lock(&s->mu);
-    s->ready = ok ? 1 : -1;
+    s->ready = 1;
     pthread_cond_broadcast(&s->cv_done);
-    if (!ok) { pthread_mutex_unlock(&s->mu); return NULL; }
     for (;;) {
         if (s->rebuild && s->inflight == 0) {
             s->rebuild = 0;
@@ -439,7 +446,8 @@ vcx_server *vcx_server_create(vc_archive *a, size_t device_bytes) {
         b->q_dev = (uint8_t *)vcg_alloc(slot_qbytes(per));
         b->st_dev = (int32_t *)vcg_alloc(per * 4);
         b->out_dev = (uint8_t *)vcg_alloc(per * (size_t)VC_ATOM3);
-        if (!b->q_pin || !b->st_pin || !b->q_dev || !b->st_dev || !b->out_dev) {
+        b->scratch = vcg_alloc(per * vcg_slot_scratch_bytes());
+        if (!b->q_pin || !b->st_pin || !b->q_dev || !b->st_dev || !b->out_dev || !b->scratch) {
             s->stop = 1;
             vcx_server_destroy(s);
             return NULL;
@@ -448,9 +456,7 @@ vcx_server *vcx_server_create(vc_archive *a, size_t device_bytes) {
     if (pthread_create(&s->th, NULL, worker, s) != 0) { vcx_server_destroy(s); return NULL; }
     pthread_mutex_lock(&s->mu);
     while (!s->ready) pthread_cond_wait(&s->cv_done, &s->mu);
-    const int ready = s->ready;
     pthread_mutex_unlock(&s->mu);
-    if (ready < 0) { vcx_server_destroy(s); return NULL; }
     return s;
 }
 
@@ -473,7 +479,7 @@ void vcx_server_destroy(vcx_server *s) {
         srv_slot *b = &s->slot[k];
         vcg_event_destroy(b->ev);
         vcg_host_free(b->q_pin); vcg_host_free(b->st_pin);
-        vcg_free(b->q_dev); vcg_free(b->st_dev); vcg_free(b->out_dev);
+        vcg_free(b->q_dev); vcg_free(b->st_dev); vcg_free(b->out_dev); vcg_free(b->scratch);
     }
     free(s->d_id); free(s->d_st); free(s->d_t);
     pthread_cond_destroy(&s->cv_done);

@@ -87,14 +87,9 @@ int vcg_decode_list(vcg_arc *a, size_t n, const int32_t *quads, uint8_t *out, in
  * box [x0,x1) x [y0,y1) x [z0,z1) of its LOD and written into the box raster
  * at out_dev + out_off. */
 typedef struct { uint32_t x0, y0, z0, x1, y1, z1; uint64_t out_off; } vcg_place;
-/* Device scratch bytes per atom of a list launch's capacity (coefficient
- * list and item record), and a reservation of that scratch in every slot of
- * the calling thread's context for batches of up to n atoms (n <= 32768),
- * so that later launches of that size allocate nothing.  0 or -1 (out of
- * device memory: a slot left short grows on demand, as without the
- * reservation). */
+/* Device decoder scratch bytes per atom of a batch's capacity (coefficient
+ * list and item record). */
 size_t vcg_slot_scratch_bytes(void);
-int    vcg_reserve_slots(size_t n);
 /* Captured persistent batch decode (server slots).  A graph decodes batches
  * of up to cap quads on the stream of scratch slot `slot`: the entropy
  * kernel with cfg->ent_waves (1, 2 or 4) waves per workgroup and
@@ -105,15 +100,15 @@ int    vcg_reserve_slots(size_t n);
  * placement << 8, ax, ay, az; a placement p sends the item's atom clipped
  * to box placement p - 1, else whole to out_dev + item * 32768), cap
  * placements (vcg_place); status_dev gets 1 per corrupt atom, else 0.
- * The slot's scratch must be reserved
- * for cap atoms first (vcg_reserve_slots) and stay so.  launch: upload the
+ * scratch: cap x vcg_slot_scratch_bytes() bytes of device memory the graph
+ * keeps using (its batches must not overlap).  launch: upload the
  * header, n quads and nplace placements, then replay the graph (enqueue
  * only).  time: reps launches, each timed by events on the slot stream
  * (synchronous), the fastest in ms (the autotuner's measurement). */
 typedef struct { int32_t ent_waves; uint32_t ent_groups; uint32_t idct_groups; } vcg_qcfg;
 typedef struct vcg_qgraph vcg_qgraph;
 vcg_qgraph *vcg_qgraph_create(vcg_arc *a, int slot, const vcg_qcfg *cfg, uint8_t *qdev, size_t cap,
-                              uint8_t *out_dev, int32_t *status_dev);
+                              uint8_t *out_dev, int32_t *status_dev, void *scratch);
 int   vcg_qgraph_launch(vcg_qgraph *g, const uint8_t *qpin, size_t n, size_t nplace);
 float vcg_qgraph_time(vcg_qgraph *g, const uint8_t *qpin, size_t n, size_t nplace, int reps);
 void  vcg_qgraph_destroy(vcg_qgraph *g);
