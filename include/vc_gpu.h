@@ -138,6 +138,27 @@ vc_status vcx_synth_device(uint64_t seed, vc_dims dims, int nsheets, void *dev_o
 enum { VCX_DS_BOX = 0, VCX_DS_CBOX = 1 };
 vc_status vcx_downscale2x_device(const void *dev_in, vc_dims in, int method, float alpha, void *dev_out, int sync);
 
+/* ---- the exporter's pyramid cascade ----
+ * Replaces the compute path of tools/vc_export_stream.c (a command-line tool
+ * in the reference: zarr/S3 source, pyramid cascade :395-466, tile worker
+ * :473-593, coarse tail :595-626, main :658-826) for a u8 volume in host or
+ * device memory (src dims, x fastest) whose source chunks (chunk = cx, cy, cz
+ * voxels; NULL: 128^3) are all present.  Writes the v1 archive at path:
+ * dims rounded up to multiples of 1024 (zero padding); base_q[l] = q0 *
+ * q_falloff^l, raised to 0.05 before each level; regions past the source
+ * marked zero; per unit (xy tile of `tile` voxels, 0: 1024; z band of 128
+ * voxels) in the tool's unit order, LOD0..2 from the within-cell 2x
+ * downscale (method VCX_DS_BOX / VCX_DS_CBOX, alpha as vcx_downscale2x_device)
+ * appended slab by slab as the tool's cascade fills them, all-zero atoms
+ * marked zero, bands past the source's chunk grid stamped zero; the coarser
+ * LODs from the archive's own decoded 2x2x2 parents.  The file equals what
+ * the reference writer produces driven in that order (the tool run with one
+ * compute thread).  The pyramid, the atom encode and the tail's decode and
+ * downscale run on the GPU.  VC_ERR_RANGE for bad arguments, VC_ERR_IO if
+ * the file cannot be created, VC_ERR_OOM. */
+vc_status vcx_export_volume(const uint8_t *vol, vc_dims src, const uint32_t chunk[3], const char *path, float q0,
+                            float q_falloff, int method, float alpha, uint32_t tile);
+
 /* ---- streaming archives decoded from HBM ----
  * Extends vc_open_streaming (src/vc/vc.h:108-118; src/vc/vc.c:1325-1338):
  * for a v2 archive, every decode call (vc_decode_atom, vc_decode_region,

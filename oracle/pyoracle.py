@@ -321,6 +321,9 @@ class Ref:
         L.vc_set_base_q.argtypes = [vp, C.c_int, C.c_float]
         L.vc_append_atom.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, vp]
         L.vc_mark_zero_atom.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.vc_mark_zero_region.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.vc_writer_decode_2x2x2.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, vp,
+                                             C.POINTER(C.c_int)]
         L.vc_writer_close.argtypes = [vp]
         L.vc_open.argtypes = [vp, C.c_size_t]
         L.vc_open.restype = vp
@@ -373,6 +376,127 @@ class Ref:
             if st != 0:
                 self.L.vc_writer_close(w)
                 raise RuntimeError(f"append failed {st}")
+        self.L.vc_writer_close(w)
+
+    def export_volume(self, oracle, vol, path, q0=1.0, q_falloff=0.5, method=1, alpha=0.5, tile=0,
+                      chunk=(128, 128, 128)):
+        """The exporter's single-threaded order (tools/vc_export_stream.c) on a
+        local volume whose source chunks are all present, driving this
+        reference writer: padded dims (:688-694), base_q schedule (:711-719),
+        prepass of regions past the source (:778-790, :338-352), units in
+        claim order (:360-367) with the empty-band stamp (:505-522) or the
+        plane cascade (push_plane / flush_slab / finish_cube, :395-466), then
+        the coarse tail from the writer's own decoded parents (:603-626).
+        The downscale is the oracle's restatement of tools/downscale.h."""
+        A = 32
+        REG = 1024
+        nz, ny, nx = vol.shape
+        px, py, pz = [(d + REG - 1) // REG * REG for d in (nx, ny, nz)]
+        w = self.L.vc_create(path.encode(), _Dims(px, py, pz), C.c_float(1.0))
+        if not w:
+            raise RuntimeError("vc_create failed")
+        nlod = 1
+        d = (px, py, pz)
+        for l in range(1, 8):
+            d = tuple((v + 1) // 2 for v in d)
+            nlod = l + 1
+            if max(d) <= 1:
+                break
+        q = np.float32(q0)
+        for l in range(nlod):
+            if q < np.float32(0.05):
+                q = np.float32(0.05)
+            self.L.vc_set_base_q(w, l, C.c_float(float(q)))
+            q = np.float32(q * np.float32(q_falloff))
+        lx, ly, lz = px, py, pz
+        for L in range(nlod):
+            nrz = ((lz + A - 1) // A + 31) // 32
+            nry = ((ly + A - 1) // A + 31) // 32
+            nrx = ((lx + A - 1) // A + 31) // 32
+            for rz in range(nrz):
+                for ry in range(nry):
+                    for rx in range(nrx):
+                        if (rx * REG << L) >= nx or (ry * REG << L) >= ny or (rz * REG << L) >= nz:
+                            self.L.vc_mark_zero_region(w, L, rz, ry, rx)
+            lx, ly, lz = (lx + 1) // 2, (ly + 1) // 2, (lz + 1) // 2
+        fine = min(nlod - 1, 2)
+        band = A << fine
+        sb = tile if tile else 1024
+        sb = min(sb, A << (nlod - 1))
+        sb = (sb + band - 1) // band * band
+        # the padded pyramid LOD0..fine
+        lv = [np.zeros((pz, py, px), np.uint8)]
+        lv[0][:nz, :ny, :nx] = vol
+        for l in range(1, fine + 1):
+            lv.append(oracle.ds2x(lv[-1], method, alpha))
+
+        def put(L, az, ay, ax, atom):
+            a = np.ascontiguousarray(atom)
+            if not a.any():
+                st = self.L.vc_mark_zero_atom(w, L, az, ay, ax)
+            else:
+                st = self.L.vc_append_atom(w, L, az, ay, ax, a.ctypes.data)
+            if st != 0:
+                raise RuntimeError(f"append {L} {az} {ay} {ax}: {st}")
+
+        ntx, nty, nbz = (px + sb - 1) // sb, (py + sb - 1) // sb, (pz + band - 1) // band
+        ncx, ncy, ncz = [(d + c - 1) // c for d, c in zip((nx, ny, nz), chunk)]
+        for ti in range(ntx * nty * nbz):
+            bz, txy = ti % nbz, ti // nbz
+            ty, tx = txy // ntx, txy % ntx
+            vx0, vy0, vz0 = tx * sb, ty * sb, bz * band
+            vx1, vy1, vz1 = min(vx0 + sb, px), min(vy0 + sb, py), min(vz0 + band, pz)
+            if vx0 // chunk[0] >= ncx or vy0 // chunk[1] >= ncy or vz0 // chunk[2] >= ncz:
+                for L in range(fine + 1):
+                    az0, ay0, ax0 = (vz0 >> L) // A, (vy0 >> L) // A, (vx0 >> L) // A
+                    for az in range(((vz1 - vz0) >> L) // A):
+                        for ay in range(((vy1 - vy0) >> L) // A):
+                            for ax in range(((vx1 - vx0) >> L) // A):
+                                self.L.vc_mark_zero_atom(w, L, az0 + az, ay0 + ay, ax0 + ax)
+                continue
+            # the plane cascade: a level's slab flushes after its 32nd plane;
+            # every second plane of a level feeds one plane of the next
+            slabfill, pairfill, zatom = [0] * 8, [0] * 8, [(vz0 >> L) // A for L in range(8)]
+
+            def push(L):
+                slabfill[L] += 1
+                if slabfill[L] == A:
+                    v = lv[L]
+                    z0 = zatom[L] * A
+                    for ay in range(((vy1 - vy0) >> L) // A):
+                        for ax in range(((vx1 - vx0) >> L) // A):
+                            y0, x0 = (vy0 >> L) + ay * A, (vx0 >> L) + ax * A
+                            put(L, zatom[L], y0 // A, x0 // A, v[z0:z0 + A, y0:y0 + A, x0:x0 + A])
+                    zatom[L] += 1
+                    slabfill[L] = 0
+                if L + 1 <= fine:
+                    pairfill[L] += 1
+                    if pairfill[L] == 2:
+                        push(L + 1)
+                        pairfill[L] = 0
+            for _ in range(vz1 - vz0):
+                push(0)
+        # coarse tail
+        out8 = np.zeros((8, A3), np.uint8)
+        anyp = C.c_int(0)
+        for L in range(fine + 1, nlod):
+            dl = [px, py, pz]
+            for _ in range(L):
+                dl = [(v + 1) // 2 for v in dl]
+            dcx, dcy, dcz = [(v + A - 1) // A for v in dl]
+            for daz in range(dcz):
+                for day in range(dcy):
+                    for dax in range(dcx):
+                        self.L.vc_writer_decode_2x2x2(w, L - 1, 2 * daz, 2 * day, 2 * dax, out8.ctypes.data,
+                                                      C.byref(anyp))
+                        if not anyp.value:
+                            self.L.vc_mark_zero_atom(w, L, daz, day, dax)
+                            continue
+                        blk = np.zeros((64, 64, 64), np.uint8)
+                        for c in range(8):
+                            dz, dy, dx = c >> 2, (c >> 1) & 1, c & 1
+                            blk[dz * A:dz * A + A, dy * A:dy * A + A, dx * A:dx * A + A] = out8[c].reshape(A, A, A)
+                        put(L, daz, day, dax, oracle.ds2x(blk, method, alpha))
         self.L.vc_writer_close(w)
 
     def payloads(self, vox_list, q):

@@ -372,6 +372,8 @@ void vco_ds2x(const u8 *in, u32 nx, u32 ny, u32 nz, int method, float alpha, u8 
     if (ox < 1) ox = 1;
     if (oy < 1) oy = 1;
     if (oz < 1) oz = 1;
+    /* output planes are independent */
+    #pragma omp parallel for schedule(static)
     for (u32 z = 0; z < oz; ++z) for (u32 y = 0; y < oy; ++y) for (u32 x = 0; x < ox; ++x) {
         u8 c[8]; int n = 0; unsigned acc = 0;
         for (u32 dz = 0; dz < 2; ++dz) { u32 zz = 2 * z + dz; if (zz >= nz) zz = nz - 1;

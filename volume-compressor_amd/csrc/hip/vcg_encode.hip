@@ -653,6 +653,46 @@ extern "C" vcg_enc *vcg_enc_begin(const uint8_t *vol, int on_dev, const uint32_t
     return e;
 }
 
+// An encoder over level volumes the caller already built (device memory,
+// dims[l] = nx, ny, nz of level l): atom numbering and gathers as for a
+// pyramid, nothing computed or freed here.
+extern "C" vcg_enc *vcg_enc_begin_levels(const uint8_t *const *lods, const uint32_t (*dims)[3], int nlod) {
+    if (nlod < 1 || nlod > VCG_NLOD) return nullptr;
+    vcg_get_ctx();
+    enc_setup();
+    vcg_enc *e = new vcg_enc();
+    memset(&e->src, 0, sizeof e->src);
+    e->arena = tl_arena.open;
+    uint64_t acc = 0;
+    for (int l = 0; l < nlod; ++l) {
+        const uint32_t x = dims[l][0], y = dims[l][1], z = dims[l][2];
+        e->src.nx[l] = x; e->src.ny[l] = y; e->src.nz[l] = z;
+        e->src.acx[l] = (x + 31) / 32; e->src.acy[l] = (y + 31) / 32;
+        e->src.prefix[l] = acc;
+        acc += (uint64_t)((x + 31) / 32) * ((y + 31) / 32) * ((z + 31) / 32);
+        e->src.lod[l] = lods[l];
+    }
+    e->src.prefix[nlod] = acc;
+    e->src.nlod = nlod;
+    e->total = acc;
+    return e;
+}
+
+// dst (dd = padded dims, device) = src (sd dims, host or device by pointer)
+// with zeros beyond sd; on the calling thread's stream, synchronous
+extern "C" int vcg_pad_volume(const uint8_t *src, const uint32_t sd[3], uint8_t *dst, const uint32_t dd[3]) {
+    if (sd[0] > dd[0] || sd[1] > dd[1] || sd[2] > dd[2]) return -1;
+    vcg_ctx *c = vcg_get_ctx();
+    const size_t plane = (size_t)dd[0] * dd[1];
+    if (sd[0] != dd[0] || sd[1] != dd[1] || sd[2] != dd[2])
+        VCG_CHECK(hipMemsetAsync(dst, 0, plane * dd[2], c->s));
+    for (uint32_t z = 0; z < sd[2]; ++z)
+        VCG_CHECK(hipMemcpy2DAsync(dst + z * plane, dd[0], src + (size_t)z * sd[0] * sd[1], sd[0], sd[0], sd[1],
+                                   hipMemcpyDefault, c->s));
+    VCG_CHECK(hipStreamSynchronize(c->s));
+    return 0;
+}
+
 // scratch shared by the encode launches of this thread
 struct enc_scratch {
     int16_t *lv = nullptr; size_t lv_cap = 0;

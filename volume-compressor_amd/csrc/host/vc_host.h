@@ -99,4 +99,11 @@ void     vc__stream_free(vc_archive *a);
 vc_status vc__decode_atoms_host(vc_archive *a, size_t n, const int32_t *quads, uint8_t *out,
                                 vc_status *per_atom);
 
+/* writer internals for the exporter cascade (vc_export.c) */
+vc_status vc__writer_store(vc_writer *w, int lod, uint32_t az, uint32_t ay, uint32_t ax, const uint8_t *pay,
+                           uint32_t len);
+vc_status vc__writer_decode_atoms(vc_writer *w, size_t n, const int32_t *quads, uint8_t *out);
+int       vc__writer_nlod(const vc_writer *w);
+float     vc__writer_q(vc_writer *w, int lod);
+
 #endif

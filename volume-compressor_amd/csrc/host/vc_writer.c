@@ -352,3 +352,33 @@ void vc_writer_close(vc_writer *w) {
     pthread_mutex_destroy(&w->mu);
     free(w);
 }
+
+/* ---- internal: the exporter cascade (vc_export.c) ---- */
+/* store one atom whose payload was encoded elsewhere (len > 0) */
+vc_status vc__writer_store(vc_writer *w, int lod, uint32_t az, uint32_t ay, uint32_t ax, const uint8_t *pay,
+                           uint32_t len) {
+    if (!w || lod < 0 || lod >= w->nlod || !pay || len == 0) return VC_ERR_RANGE;
+    pthread_mutex_lock(&w->mu);
+    vc_status s = store_atom(w, lod, az, ay, ax, pay, len);
+    pthread_mutex_unlock(&w->mu);
+    return s;
+}
+
+/* decode n atoms (quads) from the live file, as vc_writer_decode_atom does
+ * one by one: absent and zero atoms read as zeros */
+vc_status vc__writer_decode_atoms(vc_writer *w, size_t n, const int32_t *quads, uint8_t *out) {
+    if (!w) return VC_ERR_RANGE;
+    pthread_mutex_lock(&w->mu);
+    vc_archive a; view(w, &a);
+    vc_status s = vc__decode_atoms_host(&a, n, quads, out, NULL);
+    pthread_mutex_unlock(&w->mu);
+    return s;
+}
+
+int vc__writer_nlod(const vc_writer *w) { return w ? w->nlod : 0; }
+float vc__writer_q(vc_writer *w, int lod) {
+    pthread_mutex_lock(&w->mu);
+    const float q = w->base_q[lod];
+    pthread_mutex_unlock(&w->mu);
+    return q;
+}

@@ -149,6 +149,12 @@ void vcg_enc_arena_open(void);
 void vcg_enc_arena_close(void);
 /* Upload (or adopt) a LOD0 volume and build the nlod-level pyramid on device. */
 vcg_enc *vcg_enc_begin(const uint8_t *vol, int vol_on_device, const uint32_t dims[3], int nlod);
+/* An encoder over nlod level volumes already on the device (dims[l] = nx,
+ * ny, nz of level l), encoded by vcg_enc_all like a pyramid. */
+vcg_enc *vcg_enc_begin_levels(const uint8_t *const *lods, const uint32_t (*dims)[3], int nlod);
+/* dst (device, dims dd) = src (host or device, dims sd <= dd) zero-padded;
+ * synchronous.  0 / -1 */
+int vcg_pad_volume(const uint8_t *src, const uint32_t sd[3], uint8_t *dst, const uint32_t dd[3]);
 /* Rate-control samples of nv encoders in one launch: encoder v's ns[v] listed
  * LOD0 atoms (zyx triples) are transformed and their coefficients kept on
  * device (once per encoder); zero_masks[v][i] = 1 for its all-zero atoms. */

@@ -103,6 +103,7 @@ def lib():
         "vcx_encode_atoms": (i32, [sz, vp, f32, vp, sz, vp]),
         "vcx_decode_payloads": (i32, [sz, vp, vp, vp, f32, vp, vp]),
         "vcx_synth_device": (i32, [u64, Dims, i32, vp]),
+        "vcx_export_volume": (i32, [vp, Dims, vp, C.c_char_p, f32, f32, i32, f32, u32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -531,6 +532,20 @@ def synth(seed, dims, nsheets):
         return d2h(d, n).reshape(nz, ny, nx)
     finally:
         dev_free(d)
+
+
+def export_volume(vol, path, q0=1.0, q_falloff=0.5, method=1, alpha=0.5, tile=0, chunk=(128, 128, 128),
+                  dims=None):
+    """vcx_export_volume: vol a numpy u8 volume (z, y, x) or a device pointer
+    with dims (nx, ny, nz); returns the status."""
+    ch = np.ascontiguousarray(chunk, np.uint32)
+    if dims is None:
+        vol = _u8(vol)
+        ptr, d = vol.ctypes.data, dims_of(vol)
+    else:
+        ptr, d = vol, Dims(*dims)
+    return lib().vcx_export_volume(ptr, d, ch.ctypes.data, path.encode(), C.c_float(q0), C.c_float(q_falloff),
+                                   method, C.c_float(alpha), tile)
 
 
 def test_downsample2x(vol):
