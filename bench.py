@@ -653,6 +653,45 @@ def transfer_section(pyvc, a, dout, n, reps=3):
     return out
 
 
+def export_section(pyvc, dvol, n, no_cpu=False):
+    """vcx_export_volume of the device-resident n^3 volume (the exporter's
+    cascade: cbox 0.5, q0 1.0, falloff 0.5) into a v1 file, wall clock of the
+    C call; beside it the reference writer appending the same volume's LOD0
+    atoms (the exporter's LOD0 share) on the host cores."""
+    import tempfile
+    d = tempfile.mkdtemp()
+    path = os.path.join(d, "export.vca")
+    try:
+        ts = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            assert pyvc.export_volume(dvol, path, dims=(n, n, n)) == 0
+            ts.append(time.perf_counter() - t0)
+        size = os.path.getsize(path)
+        out = {"workload": f"{n}^3 device-resident volume -> v1 archive (exporter cascade, cbox 0.5, q0 1.0, falloff 0.5)",
+               "s": round(min(ts), 3), "GBps": round(n ** 3 / min(ts) / 1e9, 3), "archive_bytes": size}
+        lib = None if no_cpu else _ref_lib()
+        if lib:
+            L = C.CDLL(os.path.join(ROOT, "oracle", "libcpubench.so"))
+            L.cpu_append_lod0.restype = C.c_double
+            L.cpu_append_lod0.argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                          C.c_float, C.c_int, C.POINTER(C.c_long)]
+            hv = pyvc.d2h(dvol, n ** 3)
+            bad = C.c_long()
+            t = L.cpu_append_lod0(lib.encode(), path.encode(), hv.ctypes.data, n, n, n, C.c_float(1.0), _threads(),
+                                  C.byref(bad))
+            assert t > 0 and bad.value == 0, (t, bad.value)
+            out["cpu_lod0_appends"] = {"s": round(t, 3), "GBps": round(n ** 3 / t / 1e9, 3), "cores": _threads(),
+                                       "kind": "reference", "lib": os.path.basename(lib),
+                                       "sample": f"every LOD0 atom of the {n}^3 volume at q 1.0 (the LOD0 share of "
+                                                 "the export), reference vc_append_atom under OpenMP"}
+    finally:
+        if os.path.exists(path):
+            os.unlink(path)
+        os.rmdir(d)
+    return out
+
+
 def ratio_section(pyvc, hip, o, dvol, dout, n, ratios, steps=10):
     """The same volume encoded at other target ratios: encode time, then
     back-to-back device LOD0 decodes (GB/s) and 16 sampled atoms against the
@@ -959,6 +998,10 @@ def main():
     if not args.profile:
         ds = ds_section(pyvc, hip, dvol, dout, n)
         log(f"downscale: {ds}")
+    exp = None
+    if not args.profile:
+        exp = export_section(pyvc, dvol, n, no_cpu=args.no_cpu)
+        log(f"export: {exp}")
     pyvc.dev_free(dout)
     pyvc.dev_free(dvol)
     dout = dvol = None
@@ -1022,6 +1065,7 @@ def main():
         "encode_s": round(t_enc, 3),
         "other_ratios": ratios,
         "downscale": ds,
+        "export": exp,
         "batched_encode": enc_batch,
         "transfers": xfer,
         "cpu_baseline": cpu,

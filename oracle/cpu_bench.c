@@ -117,3 +117,63 @@ double cpu_serve(const char *libpath, const uint8_t *arc, size_t len, size_t nre
     if (bad) *bad = nbad;
     return end;
 }
+
+/* The reference writer (vc_create / vc_set_base_q / vc_append_atom /
+ * vc_mark_zero_atom / vc_writer_close of the reference build) appending every
+ * LOD0 atom of an (nx, ny, nz) host volume at base_q q with an OpenMP
+ * parallel-for over atoms (its appends are thread-safe), all-zero atoms
+ * marked zero: the host-core cost of the exporter's LOD0 share.  Returns the
+ * seconds of the appends (negative on a load or create failure). */
+typedef struct { uint32_t nx, ny, nz; } dims3;
+typedef void *(*create_fn)(const char *, dims3, float);
+typedef int (*setq_fn)(void *, int, float);
+typedef int (*app_fn)(void *, int, uint32_t, uint32_t, uint32_t, const uint8_t *);
+typedef int (*zero_fn)(void *, int, uint32_t, uint32_t, uint32_t);
+typedef void (*wclose_fn)(void *);
+
+double cpu_append_lod0(const char *libpath, const char *path, const uint8_t *vol, uint32_t nx, uint32_t ny,
+                       uint32_t nz, float q, int nthreads, long *bad) {
+    void *h = dlopen(libpath, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return -1.0;
+    create_fn vcreate = (create_fn)dlsym(h, "vc_create");
+    setq_fn vsetq = (setq_fn)dlsym(h, "vc_set_base_q");
+    app_fn vapp = (app_fn)dlsym(h, "vc_append_atom");
+    zero_fn vzero = (zero_fn)dlsym(h, "vc_mark_zero_atom");
+    wclose_fn vclose = (wclose_fn)dlsym(h, "vc_writer_close");
+    if (!vcreate || !vsetq || !vapp || !vzero || !vclose) { dlclose(h); return -2.0; }
+    dims3 d = {nx, ny, nz};
+    void *w = vcreate(path, d, 1.0f);
+    if (!w) { dlclose(h); return -3.0; }
+    for (int l = 0; l < 8; ++l) vsetq(w, l, q);
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    const uint32_t acx = (nx + 31) / 32, acy = (ny + 31) / 32, acz = (nz + 31) / 32;
+    const long n = (long)acx * acy * acz;
+    long nbad = 0;
+    const double t0 = now_s();
+    #pragma omp parallel reduction(+:nbad)
+    {
+        uint8_t *a = (uint8_t *)malloc(32768);
+        #pragma omp for schedule(dynamic, 4)
+        for (long i = 0; i < n; ++i) {
+            const uint32_t ax = (uint32_t)(i % acx), ay = (uint32_t)((i / acx) % acy), az = (uint32_t)(i / ((long)acx * acy));
+            uint64_t any = 0;
+            for (uint32_t z = 0; z < 32; ++z)
+                for (uint32_t y = 0; y < 32; ++y) {
+                    uint8_t *row = a + (z * 32 + y) * 32;
+                    const uint32_t vz = az * 32 + z, vy = ay * 32 + y;
+                    for (uint32_t x = 0; x < 32; ++x) {
+                        const uint32_t vx = ax * 32 + x;
+                        row[x] = (vz < nz && vy < ny && vx < nx) ? vol[((uint64_t)vz * ny + vy) * nx + vx] : 0;
+                        any |= row[x];
+                    }
+                }
+            if ((any ? vapp(w, 0, az, ay, ax, a) : vzero(w, 0, az, ay, ax)) != 0) nbad++;
+        }
+        free(a);
+    }
+    const double t = now_s() - t0;
+    vclose(w);
+    dlclose(h);
+    if (bad) *bad = nbad;
+    return t;
+}
