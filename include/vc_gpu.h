@@ -73,10 +73,13 @@ vc_status vcx_decode_atoms(vc_archive *a, size_t n, const int32_t *lod_ax_ay_az,
 vc_status vcx_decode_atoms_noalloc(vc_archive *a, size_t n, const int32_t *quads_dev, uint8_t *out_dev,
                                    int32_t *status_dev, void *stream);
 
-/* Stage events: while set, each decode call records events[0] before its
- * entropy kernel, events[1] between the entropy and transform kernels and
- * events[2] after the transform kernel, on the calling thread's stream.
- * events are hipEvent_t handles owned by the caller; NULL disables. */
+/* Stage events: while set, each launch pair of the calling thread's decode
+ * calls (vc_decode_*, vcx_decode_*) records events[0] before its entropy
+ * kernel, events[1] between the entropy and transform kernels and events[2]
+ * after the transform kernel, on the stream those kernels run on (one of the
+ * thread's scratch-slot streams, not vcx_stream()).  Server batches (other
+ * thread, captured launches) record none.  events are hipEvent_t handles
+ * owned by the caller; NULL disables. */
 void vcx_set_stage_events(void *events[3]);
 
 /* Whole-volume encode with explicit per-LOD base_q (clamped to [0.05,4096]
@@ -131,8 +134,10 @@ vc_status vcx_synth_device(uint64_t seed, vc_dims dims, int nsheets, void *dev_o
  * voxel from its own 2x2x2 cell of dev_in, the odd boundary edge-replicated,
  * by method VCX_DS_BOX (mean) or VCX_DS_CBOX (the mean pushed by alpha, 0..1,
  * toward the cell's voxel of largest deviation); an all-zero cell gives 0.
- * Float arithmetic in source order, as the reference compiles it without
- * contraction.  sync: wait; otherwise ordered on vcx_stream().
+ * Float arithmetic in source order without contraction; the reference's
+ * own flags (-ffast-math -ffp-contract=fast) give the same bytes on every
+ * golden case (checked against builds at the baseline ISA and with FMA).
+ * sync: wait; otherwise ordered on vcx_stream().
  * VC_ERR_RANGE for NULL pointers, a zero dimension, an unknown method or a
  * NaN alpha. */
 enum { VCX_DS_BOX = 0, VCX_DS_CBOX = 1 };

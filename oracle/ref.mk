@@ -19,9 +19,11 @@
 #   libvcref_base.so               the reference flags with no -march (x86-64
 #                                  baseline, no FMA instructions available).
 #   vc_repack                      tools/vc_repack.c linked to libvcref_strict.so.
-#   libdsref_fast.so / _strict.so  tools/downscale.h (the exporter's within-cell
-#                                  downscale) through oracle/ds_ref_shim.c, with
-#                                  the reference flags and with strict float.
+#   libdsref_fast.so / _fast_v3.so / _strict.so  tools/downscale.h (the
+#                                  exporter's within-cell downscale) through
+#                                  oracle/ds_ref_shim.c, with the reference flags
+#                                  (baseline ISA, and x86-64-v3 with FMA) and
+#                                  with strict float.
 REF      ?= /root/reference
 OUT      ?= oracle/_ref
 SRC       = $(REF)/src/vc/vc.c
@@ -30,7 +32,8 @@ FASTF     = -O3 -ffp-contract=fast -ffast-math -funroll-all-loops -fopenmp
 LIBS      = -lm -lpthread
 
 all: $(OUT)/libvcref_fast_v3.so $(OUT)/libvcref_fast_v4.so $(OUT)/libvcref_strict.so \
-     $(OUT)/libvcref_base.so $(OUT)/vc_repack $(OUT)/libdsref_fast.so $(OUT)/libdsref_strict.so
+     $(OUT)/libvcref_base.so $(OUT)/vc_repack $(OUT)/libdsref_fast.so $(OUT)/libdsref_fast_v3.so \
+     $(OUT)/libdsref_strict.so
 
 $(OUT):
 	mkdir -p $(OUT)
@@ -53,6 +56,9 @@ $(OUT)/vc_repack: $(REF)/tools/vc_repack.c $(OUT)/libvcref_strict.so | $(OUT)
 
 $(OUT)/libdsref_fast.so: oracle/ds_ref_shim.c $(REF)/tools/downscale.h | $(OUT)
 	gcc -std=c2x -fPIC -shared $(FASTF) -I$(REF)/tools oracle/ds_ref_shim.c -o $@
+
+$(OUT)/libdsref_fast_v3.so: oracle/ds_ref_shim.c $(REF)/tools/downscale.h | $(OUT)
+	gcc -std=c2x -fPIC -shared $(FASTF) -march=x86-64-v3 -I$(REF)/tools oracle/ds_ref_shim.c -o $@
 
 $(OUT)/libdsref_strict.so: oracle/ds_ref_shim.c $(REF)/tools/downscale.h | $(OUT)
 	gcc -std=c2x -fPIC -shared -O2 -ffp-contract=off -I$(REF)/tools oracle/ds_ref_shim.c -o $@

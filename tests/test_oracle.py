@@ -400,10 +400,55 @@ def test_oracle_ds2x_golden(oracle):
     from make_golden import ds_input
     from pyoracle import ds_ref
     cases = json.load(open(os.path.join(GOLDEN, "ds.json")))
-    refs = [r for r in (ds_ref("strict"), ds_ref("fast")) if r]
+    refs = [r for r in (ds_ref("strict"), ds_ref("fast"), ds_ref("fast_v3")) if r]
     for c in cases:
         v = ds_input(tuple(c["shape_zyx"]), c["kind"], c["seed"])
         got = oracle.ds2x(v, c["method"], c["alpha"])
         assert sha(got) == c["sha"], c
         for r in refs:
             assert np.array_equal(got, r(v, c["method"], c["alpha"])), c
+
+
+def test_fast_math_builds_vs_strict(oracle, golden_containers, capsys):
+    """The reference built with its own flags (-ffast-math -ffp-contract=fast,
+    CMakeLists.txt:46) at three ISA levels against the strict build that pins
+    the quantizer's evaluation order: payloads of 64 atoms of the 128^3
+    synthetic volume (20 sheets) at the LOD0..2 base_q its rate control picks
+    for ratio 10, and at q = 5.  The strict build equals the oracle on every
+    atom; every build decodes the golden payloads to the same voxels (the
+    decoder has no float arithmetic); each fast build's agreement with strict
+    is recorded (FMA contraction can move a level across a rounding edge)."""
+    from pyoracle import Ref, host_has_avx512, ref_variants
+    v = ref_variants()
+    if "strict" not in v:
+        pytest.skip("reference build oracle/_ref not present (make -f oracle/ref.mk)")
+    vol = oracle.synth(1, 128, 128, 128, 20)
+    q0 = oracle.rate_control(vol, 10.0)
+    qs = [float(q) for q in oracle.lod_schedule(q0, 8)[:3]] + [5.0]
+    rng = np.random.default_rng(3)
+    atoms = []
+    for _ in range(64):
+        z, y, x = (int(t) for t in rng.integers(0, 4, 3))
+        atoms.append(vol[z * 32:z * 32 + 32, y * 32:y * 32 + 32, x * 32:x * 32 + 32].copy())
+    strict = Ref("strict")
+    base = {q: strict.payloads(atoms, q) for q in qs}
+    for q in qs:
+        assert base[q] == [oracle.encode_atom(a, q) for a in atoms], q
+    names = [n for n in ("fast_v4", "fast_v3", "base") if n in v and (n != "fast_v4" or host_has_avx512())]
+    table = {}
+    for n in names:
+        r = Ref(n)
+        table[n] = {f"{q:.4g}": sum(p == b for p, b in zip(r.payloads(atoms, q), base[q])) for q in qs}
+        for name, (m, b) in golden_containers.items():
+            ra, sa = r.open(b["v1"]), strict.open(b["v1"])
+            for lod in range(3):
+                st, dims = sa.lod_dims(lod)
+                if st:
+                    break
+                box = (0, 0, 0) + tuple(dims)
+                assert np.array_equal(ra.decode_region(lod, box)[1], sa.decode_region(lod, box)[1]), (n, name, lod)
+            ra.close()
+            sa.close()
+        assert all(c >= 0.8 * len(atoms) for c in table[n].values()), (n, table[n])
+    with capsys.disabled():
+        print(f"\nfast-math payload agreement with strict (of {len(atoms)} atoms): {table}")

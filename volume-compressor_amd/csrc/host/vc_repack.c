@@ -95,6 +95,9 @@ static vc_status plan_build(vc_archive *a, vcg_arc *d, rp_plan *p) {
             b++;
             if (np) {
                 const uint64_t blen = L2_BLOCK + bytes;
+                /* the directory keeps a blob's length in 32 bits (the tool
+                 * truncates it silently; vc_encode rejects the same case) */
+                if (blen > 0xFFFFFFFFull) { st = VC_ERR_RANGE; goto done; }
                 wr64(e, cursor);
                 wr32(e + 8, (uint32_t)blen);
                 flags = V2_RGN_DATA;
