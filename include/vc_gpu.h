@@ -239,7 +239,10 @@ size_t      vcx_server_capacity(const vcx_server *s);   /* atoms per batch slot 
  * time, transform workgroups one atom at a time, the batch size read from
  * device memory), captured once per batch slot, LOD class (LOD0 only / any
  * coarser LOD) and size bucket (up to 64, 512, 4096, 32768 atoms) as a
- * hipGraph and replayed per batch.  vcx_server_autotune waits until no batch
+ * hipGraph.  A batch replays its graph when the server is saturated (8 or
+ * more requests still queued), where replays of concurrent slots run one
+ * after another; otherwise it launches the same kernels directly, so a short
+ * LOD0 batch overlaps a long coarse one.  vcx_server_autotune waits until no batch
  * is in flight, then times every candidate launch shape per class and bucket
  * (entropy workgroups of 1, 2 or 4 waves, enough of them for all atoms at
  * once or for half, 256 / 512 / 1024 transform workgroups or one per atom)

@@ -13,6 +13,7 @@
 //               adds dc, clamps and stores u8 voxels to the destination.
 #include <hip/hip_runtime.h>
 #include <string.h>
+#include <stdlib.h>
 #include <mutex>
 #include "vcg_dev.hpp"
 #include "vcg_xform.hpp"
@@ -759,6 +760,10 @@ struct vcg_qgraph {
     uint32_t *lists = nullptr;           // the caller's scratch the graph was captured with
     vcg_item *items = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
+    // the captured launches' arguments (direct launches)
+    dec_job j;
+    vcg_qcfg cfg;
+    uint32_t *status = nullptr;
 };
 
 static_assert(sizeof(vcg_qhdr) == 16, "batch header");
@@ -795,6 +800,7 @@ extern "C" vcg_qgraph *vcg_qgraph_create(vcg_arc *a, int slot, const vcg_qcfg *c
     // the caller's scratch: cap list strides, then cap item records
     g->lists = (uint32_t *)scratch;
     g->items = (vcg_item *)((uint8_t *)scratch + cap * (size_t)VCG_LIST_STRIDE * 4);
+    g->j = j; g->cfg = *cfg; g->status = c->status_d;
     hipGraph_t graph = nullptr;
     VCG_CHECK(hipStreamBeginCapture(g->s, hipStreamCaptureModeThreadLocal));
     const dim3 ge(cfg->ent_groups), gt(cfg->idct_groups);
@@ -813,13 +819,26 @@ extern "C" vcg_qgraph *vcg_qgraph_create(vcg_arc *a, int slot, const vcg_qcfg *c
     return g;
 }
 
-extern "C" int vcg_qgraph_launch(vcg_qgraph *g, const uint8_t *qpin, size_t n, size_t nplace) {
+extern "C" int vcg_qgraph_launch(vcg_qgraph *g, const uint8_t *qpin, size_t n, size_t nplace, int direct) {
     if (!g || n == 0 || n > g->cap || nplace > g->cap) return -1;
     // header (n, claim counters at 0) and quads in one copy, placements in another
     VCG_CHECK(hipMemcpyAsync(g->qdev, qpin, 16 + n * 16, hipMemcpyHostToDevice, g->s));
     if (nplace)
         VCG_CHECK(hipMemcpyAsync(g->qdev + 16 + g->cap * 16, qpin + 16 + g->cap * 16, nplace * sizeof(vcg_place),
                                  hipMemcpyHostToDevice, g->s));
+    if (direct) {                        // the same two kernels, launched one by one
+        vcg_qhdr *hq = (vcg_qhdr *)g->qdev;
+        const dim3 ge(g->cfg.ent_groups), gt(g->cfg.idct_groups);
+        if (g->cfg.ent_waves == 1)
+            hipLaunchKernelGGL(k_entropy_p<1>, ge, dim3(64), 0, g->s, g->j, g_dtab, g->lists, g->items, g->status, hq);
+        else if (g->cfg.ent_waves == 2)
+            hipLaunchKernelGGL(k_entropy_p<2>, ge, dim3(128), 0, g->s, g->j, g_dtab, g->lists, g->items, g->status, hq);
+        else
+            hipLaunchKernelGGL(k_entropy_p<4>, ge, dim3(256), 0, g->s, g->j, g_dtab, g->lists, g->items, g->status, hq);
+        hipLaunchKernelGGL(k_idct_p, gt, dim3(IDCT_THREADS), LDS_BYTES, g->s, g->j, g_dtab, g->lists, g->items, hq);
+        VCG_CHECK(hipGetLastError());
+        return 0;
+    }
     VCG_CHECK(hipGraphLaunch(g->exec, g->s));
     return 0;
 }
@@ -828,7 +847,7 @@ extern "C" float vcg_qgraph_time(vcg_qgraph *g, const uint8_t *qpin, size_t n, s
     float best = 1e30f;
     for (int r = 0; r < reps; ++r) {
         VCG_CHECK(hipEventRecord(g->t0, g->s));
-        if (vcg_qgraph_launch(g, qpin, n, nplace) != 0) return -1.0f;
+        if (vcg_qgraph_launch(g, qpin, n, nplace, 0) != 0) return -1.0f;
         VCG_CHECK(hipEventRecord(g->t1, g->s));
         VCG_CHECK(hipEventSynchronize(g->t1));
         float ms = 0;

@@ -169,6 +169,12 @@ extern "C" int vcg_copy_async(void *dst, const void *src, size_t bytes, int kind
     return 0;
 }
 
+extern "C" int vcg_ptr_is_device(const void *p) {
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) { (void)hipGetLastError(); return 0; }
+    return at.type == hipMemoryTypeDevice ? 1 : 0;
+}
+
 extern "C" int vcg_mem_info(size_t *free_bytes, size_t *total_bytes) {
     vcg_get_ctx();                      // the library's device is current
     size_t f = 0, t = 0;

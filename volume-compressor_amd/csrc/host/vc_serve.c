@@ -46,6 +46,10 @@
 #define SRV_COARSE_SLOTS 1
 #endif
 #define SRV_MAX_BATCH 32768u
+/* queued requests from which batches replay their captured graphs */
+#ifndef SRV_REPLAY_BACKLOG
+#define SRV_REPLAY_BACKLOG 8
+#endif
 /* launch-shape table: one captured persistent launch per LOD class and batch
  * size bucket (a batch takes the smallest bucket holding it) */
 #define SRV_NB 4
@@ -61,6 +65,7 @@ typedef struct srv_req {
     vc_box box;
     uint8_t *out;
     size_t out_bytes;
+    int out_dev;                /* out is device memory */
     size_t first;               /* index of its first atom in the batch */
     struct srv_parent *parent;  /* a part of a request split over several batches, or NULL */
     struct srv_req *next;
@@ -181,8 +186,9 @@ static void drop_graphs(vcx_server *s) {
             }
 }
 
-/* worker: enqueue one admitted batch (no lock held) */
-static void launch(vcx_server *s, srv_slot *b) {
+/* worker: enqueue one admitted batch (no lock held).  replay: from its
+ * captured graph, else the same kernels launched directly */
+static void launch(vcx_server *s, srv_slot *b, int replay) {
     const size_t cap = s->cap_atoms;
     vcg_place *pl = (vcg_place *)(b->q_pin + 16 + cap * 16);
     uint32_t hdr[4] = {(uint32_t)b->n, 0, 0, 0};
@@ -207,12 +213,12 @@ static void launch(vcx_server *s, srv_slot *b) {
      * this slot's scratch: batches in other slots neither wait for it nor
      * delay it, and the slot's next batch follows its copies */
     const int k = (int)(b - s->slot);
-    if (vcg_qgraph_launch(slot_graph(s, b, b->cls, bucket_of(b->n)), b->q_pin, b->n, np) != 0)
+    if (vcg_qgraph_launch(slot_graph(s, b, b->cls, bucket_of(b->n)), b->q_pin, b->n, np, !replay) != 0)
         vc__panic("server batch launch rejected");
-    /* by pointer: pinned or pageable host memory, or device memory (a
-     * device destination costs no PCIe transfer) */
+    /* to host memory (pinned or pageable) or device memory (no PCIe
+     * transfer), with the copy kind of each destination */
     for (srv_req *r = b->reqs; r; r = r->next)
-        vcg_copy_async(r->out, b->out_dev + r->first * (size_t)VC_ATOM3, r->out_bytes, 0, k);
+        vcg_copy_async(r->out, b->out_dev + r->first * (size_t)VC_ATOM3, r->out_bytes, r->out_dev ? 3 : 2, k);
     vcg_copy_async(b->st_pin, b->st_dev, b->n * 4, 2, k);
     vcg_event_record(b->ev, k);
 }
@@ -389,8 +395,17 @@ static void *worker(void *arg) {
             fb->cls = cls;
             fb->busy = 1;
             s->inflight += 1;
+            /* Saturated (at least SRV_REPLAY_BACKLOG requests still queued
+             * after this admission), the batch replays its captured graph:
+             * graph replays of concurrent slots run one after another, which
+             * keeps a saturated device from splitting its SIMDs over every
+             * batch in flight.  Otherwise the same kernels go directly, so a
+             * short LOD0 batch overlaps a long coarse one. */
+            size_t backlog = 0;
+            for (srv_req *q = s->qhead; q && backlog < SRV_REPLAY_BACKLOG; q = q->next) backlog++;
+            const int replay = backlog >= SRV_REPLAY_BACKLOG;
             pthread_mutex_unlock(&s->mu);
-            launch(s, fb);
+            launch(s, fb, replay);
             pthread_mutex_lock(&s->mu);
             continue;
         }
@@ -525,6 +540,7 @@ int64_t vcx_server_submit_atoms(vcx_server *s, size_t n, const int32_t *quads, u
     for (size_t i = 0; i < n; ++i)
         if (!vc__atom_ok(s->a, quads[4 * i], quads[4 * i + 1], quads[4 * i + 2], quads[4 * i + 3])) return -1;
     const size_t cap = s->cap_atoms, np = (n + cap - 1) / cap;
+    const int dev = vcg_ptr_is_device(out);
     srv_req **parts = (srv_req **)calloc(np, sizeof *parts);
     srv_parent *par = np > 1 ? (srv_parent *)calloc(1, sizeof *par) : NULL;
     if (!parts || (np > 1 && !par)) { free(parts); free(par); return -1; }
@@ -535,6 +551,7 @@ int64_t vcx_server_submit_atoms(vcx_server *s, size_t n, const int32_t *quads, u
         parts[k] = r;
         memcpy(r->quads, quads + 4 * i0, m * 16);
         r->kind = 0; r->n = m; r->out = out + i0 * (size_t)VC_ATOM3; r->out_bytes = m * (size_t)VC_ATOM3;
+        r->out_dev = dev;
         for (size_t i = 0; i < m; ++i) if (r->quads[4 * i] != 0) r->cls = 1;
     }
     const int64_t id = enqueue(s, parts, np, par);
@@ -556,6 +573,7 @@ int64_t vcx_server_submit_region(vcx_server *s, int lod, vc_box box, uint8_t *ou
     const uint32_t per = (uint32_t)(cap / layer);                 /* atom layers per part */
     const size_t np = (az1 - az0 + per - 1) / per;
     const uint64_t plane = (uint64_t)(box.x1 - box.x0) * (box.y1 - box.y0);
+    const int dev = vcg_ptr_is_device(out);
     srv_req **parts = (srv_req **)calloc(np, sizeof *parts);
     srv_parent *par = np > 1 ? (srv_parent *)calloc(1, sizeof *par) : NULL;
     if (!parts || (np > 1 && !par)) { free(parts); free(par); return -1; }
@@ -569,6 +587,7 @@ int64_t vcx_server_submit_region(vcx_server *s, int lod, vc_box box, uint8_t *ou
         parts[k] = r;
         r->kind = 1; r->n = layer * (lb - la); r->lod = lod; r->box = b;
         r->out = out + (uint64_t)(b.z0 - box.z0) * plane; r->out_bytes = (uint64_t)(b.z1 - b.z0) * plane;
+        r->out_dev = dev;
         r->cls = lod != 0;
     }
     const int64_t id = enqueue(s, parts, np, par);

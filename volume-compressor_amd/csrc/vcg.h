@@ -56,6 +56,7 @@ int   vcg_sync(void);                       /* wait; returns accumulated vc stat
  * (slot -1) or on the stream of its scratch slot `slot` (0 .. VCG_SLOTS-1) */
 int   vcg_copy_async(void *dst, const void *src, size_t bytes, int kind, int slot); /* kind 0: by pointer */
 int   vcg_mem_info(size_t *free_bytes, size_t *total_bytes);   /* hipMemGetInfo; 0 / -1 */
+int   vcg_ptr_is_device(const void *p);     /* 1: device memory, 0: host (pinned or pageable) */
 void *vcg_event_create(void);
 void  vcg_event_destroy(void *e);
 void  vcg_event_record(void *e, int slot);
@@ -101,15 +102,15 @@ size_t vcg_slot_scratch_bytes(void);
  * to box placement p - 1, else whole to out_dev + item * 32768), cap
  * placements (vcg_place); status_dev gets 1 per corrupt atom, else 0.
  * scratch: cap x vcg_slot_scratch_bytes() bytes of device memory the graph
- * keeps using (its batches must not overlap).  launch: upload the
- * header, n quads and nplace placements, then replay the graph (enqueue
- * only).  time: reps launches, each timed by events on the slot stream
+ * keeps using (its batches must not overlap).  launch: upload the header, n
+ * quads and nplace placements, then replay the graph, or with direct launch
+ * the same two kernels one by one (enqueue only).  time: reps launches, each timed by events on the slot stream
  * (synchronous), the fastest in ms (the autotuner's measurement). */
 typedef struct { int32_t ent_waves; uint32_t ent_groups; uint32_t idct_groups; } vcg_qcfg;
 typedef struct vcg_qgraph vcg_qgraph;
 vcg_qgraph *vcg_qgraph_create(vcg_arc *a, int slot, const vcg_qcfg *cfg, uint8_t *qdev, size_t cap,
                               uint8_t *out_dev, int32_t *status_dev, void *scratch);
-int   vcg_qgraph_launch(vcg_qgraph *g, const uint8_t *qpin, size_t n, size_t nplace);
+int   vcg_qgraph_launch(vcg_qgraph *g, const uint8_t *qpin, size_t n, size_t nplace, int direct);
 float vcg_qgraph_time(vcg_qgraph *g, const uint8_t *qpin, size_t n, size_t nplace, int reps);
 void  vcg_qgraph_destroy(vcg_qgraph *g);
 /* Allocation-free list decode: at most VCG_NOALLOC_MAX quads in device
