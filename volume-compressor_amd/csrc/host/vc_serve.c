@@ -17,8 +17,12 @@
  * A batch runs as two persistent kernels (entropy waves claiming 64 atoms at
  * a time, transform workgroups one atom at a time, the batch size read from
  * device memory), captured once per slot, LOD class and size bucket as a
- * hipGraph and replayed; launch shapes come from a table that
- * vcx_server_autotune fills by timing candidates on the archive's own atoms.
+ * hipGraph.  A batch replays its graph while SRV_REPLAY_BACKLOG or more
+ * requests are queued (replays of concurrent slots run one after another, so
+ * the saturated loop has no launch gaps); otherwise it launches the same
+ * kernels directly, so a short batch overlaps the batches in flight.  Launch
+ * shapes come from a table that vcx_server_autotune fills by timing
+ * candidates on the archive's own atoms.
  *
  * Batches are LOD-aware.  Coarser LODs are quantized finer, so their atoms
  * carry several times the bins of an LOD0 atom, and a batch lasts as long as
