@@ -122,4 +122,96 @@ int kc_idct(const uint32_t *L, uint32_t cnt, int dc, const int32_t *st, const in
     free(t);
     return fast ? 1 : 2;
 }
+
+// v_mfma_i32_32x32x32_i8 over a wave's operands: D[l][r] = C[l][r] + sum
+// over both lane halves h2 and bytes e of A[row + 32 h2].e x B[col + 32 h2].e
+// (signed bytes), row = 8 (r / 4) + 4 (l / 32) + r % 4, col = l % 32; i32 wrap
+static void mfma_cpu(const uint32_t A[64][4], const uint32_t B[64][4], int32_t D[64][16]) {
+    for (int l = 0; l < 64; ++l)
+        for (int r = 0; r < 16; ++r) {
+            const int row = vcg_fwd1_k(r, l >> 5), col = l & 31;
+            uint32_t acc = (uint32_t)D[l][r];
+            for (int h2 = 0; h2 < 2; ++h2)
+                for (int e = 0; e < 16; ++e) {
+                    const int8_t a = (int8_t)((A[row + 32 * h2][e >> 2] >> (8 * (e & 3))) & 0xFFu);
+                    const int8_t b = (int8_t)((B[col + 32 * h2][e >> 2] >> (8 * (e & 3))) & 0xFFu);
+                    acc += (uint32_t)((int)a * (int)b);
+                }
+            D[l][r] = (int32_t)acc;
+        }
+}
+
+// k_idct_m's product chain for one block (d[l] = lane l's 16 w), as im_chain
+static void im_chain_cpu(const uint32_t d[64][8], const uint32_t B[3][64][4], int rnd, int32_t o[64][16]) {
+    static uint32_t hi[64][4], lo[64][4];
+    for (int l = 0; l < 64; ++l) vcg_im_planes(d[l], hi[l], lo[l]);
+    int32_t t[64][16];
+    memset(t, 0, sizeof t);
+    mfma_cpu(hi, B[0], t);
+    for (int l = 0; l < 64; ++l) for (int r = 0; r < 16; ++r) t[l][r] = (int32_t)((uint32_t)t[l][r] << 8);
+    mfma_cpu(hi, B[1], t);
+    mfma_cpu(lo, B[0], t);
+    for (int l = 0; l < 64; ++l) for (int r = 0; r < 16; ++r) t[l][r] = (int32_t)(((uint32_t)t[l][r] << 6) + (uint32_t)rnd);
+    mfma_cpu(lo, B[2], t);
+    for (int l = 0; l < 64; ++l) for (int r = 0; r < 16; ++r) o[l][r] = t[l][r] >> 14;
+}
+
+// k_idct_m on one atom's entries, block by block with the kernel's lane
+// layout: pass x reads and writes each plane in turn, passes y and z read
+// every block before writing any (the kernel's barriers).  Atoms past the
+// coefficient bound go to the i32 path like the kernel's status-4 items.
+// Returns 3 for the matrix-core path, else kc_idct's path.
+int kc_idct_m(const uint32_t *L, uint32_t cnt, int dc, const int32_t *st, const int32_t *bi,
+              const int32_t cmat[32][32], const uint16_t *scan, uint8_t *out) {
+    static int8_t ib[3][64][16];
+    vcg_im_tables(cmat, ib);
+    static uint32_t B[3][64][4];
+    memcpy(B, ib, sizeof B);
+    uint8_t *t = (uint8_t *)aligned_alloc(16, VCG_IM_TILE_BYTES);
+    for (int tid = 0; tid < NTI; ++tid) vcg_im_fill((uint32_t *)t, tid, NTI);
+    const int fmax = cnt ? vcg_freq(scan[vcg_ent_pos(L[cnt - 1])]) : -1;
+    uint64_t ss = 0;
+    uint32_t big = 0;
+    for (int tid = 0; tid < NTI; ++tid) vcg_im_scatter(t, L, cnt, scan, st, bi, tid, NTI, ss, big);
+    if (big || ss > (uint64_t)VCG_IM_L2 * VCG_IM_L2) {
+        free(t);
+        return kc_idct(L, cnt, dc, st, bi, cmat, scan, 0, out);
+    }
+    static uint32_t d[32][64][8];
+    static int32_t o[64][16];
+    for (int p = 0; p < 3; ++p) {
+        for (int Bk = 0; Bk < 32; ++Bk) {
+            if (vcg_im_skip(p, Bk, fmax)) continue;
+            for (int l = 0; l < 64; ++l) vcg_im_load(t, p, Bk, l & 31, l >> 5, d[Bk][l]);
+            if (p == 0) {                          // in place, plane by plane
+                im_chain_cpu(d[Bk], B, vcg_im_rnd(0, dc), o);
+                for (int l = 0; l < 64; ++l)
+                    for (int q = 0; q < 4; ++q) {
+                        uint32_t w[2];
+                        vcg_im_pack16(o[l] + 4 * q, w);
+                        memcpy(t + vcg_im_out_addr(0, Bk, l & 31, l >> 5, q), w, 8);
+                    }
+            }
+        }
+        if (p == 0) continue;
+        for (int Bk = 0; Bk < 32; ++Bk) {
+            if (vcg_im_skip(p, Bk, fmax)) continue;
+            im_chain_cpu(d[Bk], B, vcg_im_rnd(p, dc), o);
+            for (int l = 0; l < 64; ++l)
+                for (int q = 0; q < 4; ++q) {
+                    if (p == 1) {
+                        uint32_t w[2];
+                        vcg_im_pack16(o[l] + 4 * q, w);
+                        memcpy(t + vcg_im_out_addr(1, Bk, l & 31, l >> 5, q), w, 8);
+                    } else {
+                        const uint32_t w = vcg_im_pack8(o[l] + 4 * q);
+                        memcpy(t + vcg_im_vox_addr(Bk, l & 31, l >> 5, q), &w, 4);
+                    }
+                }
+        }
+    }
+    for (int z = 0; z < 32; ++z) memcpy(out + z * 1024, t + z * VCG_IM_ZS, 1024);
+    free(t);
+    return 3;
+}
 }

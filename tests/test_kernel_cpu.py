@@ -35,6 +35,7 @@ def kc():
     L.kc_fdct.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                           vp, vp, vp, C.c_float, vp, vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     L.kc_idct.argtypes = [vp, C.c_uint32, C.c_int, vp, vp, vp, vp, C.c_int, vp]
+    L.kc_idct_m.argtypes = [vp, C.c_uint32, C.c_int, vp, vp, vp, vp, vp]
     L.kc_quant.argtypes = [vp, C.c_size_t, C.c_float, C.c_float, vp]
     return L
 
@@ -138,6 +139,83 @@ def test_kernel_decode_paths_agree(kc, tabs, oracle, golden_atoms):
         st, a, path = k_decode(kc, tabs, pay, float(qs[qi]), oracle, 0)
         st2, b, _ = k_decode(kc, tabs, pay, float(qs[qi]), oracle, 2)
         assert np.array_equal(a, b)
+
+
+def _idct_both(kc, tabs, ent, dc, stp, bia):
+    cm, gs, scan, bs = tabs
+    ent = np.ascontiguousarray(ent, np.uint32)
+    a = np.zeros(32768, np.uint8)
+    b = np.zeros(32768, np.uint8)
+    path = kc.kc_idct_m(ent.ctypes.data, len(ent), dc, stp.ctypes.data, bia.ctypes.data, cm.ctypes.data,
+                        scan.ctypes.data, a.ctypes.data)
+    kc.kc_idct(ent.ctypes.data, len(ent), dc, stp.ctypes.data, bia.ctypes.data, cm.ctypes.data, scan.ctypes.data,
+               2, b.ctypes.data)
+    return path, a, b
+
+
+def test_kernel_idct_mfma_matches_reference(kc, tabs, oracle, golden_atoms):
+    """The matrix-core inverse (int8 products over an i16 tile, emulated lane
+    by lane) reproduces the reference's reconstructions of every golden atom,
+    all of them within its coefficient bound."""
+    cm, gs, scan, bs = tabs
+    qs = golden_atoms["qs"]
+    for ai, qi, pay, rsha in payloads_of(golden_atoms):
+        if not pay:
+            continue
+        rc = np.frombuffer(pay[1:], np.uint8) if len(pay) > 1 else np.zeros(1, np.uint8)
+        ent = np.zeros(32768, np.uint32)
+        cnt = C.c_uint32()
+        assert kc.kc_decode(rc.ctypes.data, len(pay) - 1, bs.ctypes.data, ent.ctypes.data, C.byref(cnt)) == 0
+        stp, bia = oracle.dq_tables(float(qs[qi]))
+        path, a, b = _idct_both(kc, tabs, ent[:cnt.value], pay[0], stp, bia)
+        assert path == 3, (ai, qi)
+        assert sha(a) == rsha and np.array_equal(a, b), (ai, qi)
+
+
+def _dequant(ent, scan, stp, bia):
+    """r per entry as the scatter computes it (src/vc/vc.c:327-338, i32 wrap)"""
+    e = ent.astype(np.int64)
+    t = (e + 11) & 0x1FFFF
+    m = (t >> 1) & 0xFFFF
+    lv = np.where(t & 1, -m, m).astype(np.int16).astype(np.int64)
+    idx = scan[(e >> 17)].astype(np.int64)
+    f = (idx >> 10) + ((idx >> 5) & 31) + (idx & 31)
+    mag = np.abs(lv)
+    r = ((mag * stp[f] + bia[f] + 128) & 0xFFFFFFFF).astype(np.uint32).view(np.int32) >> 8
+    return np.where(lv < 0, -r, r)
+
+
+def test_kernel_idct_mfma_bound_and_fallback(kc, tabs, oracle):
+    """Random level fields (sparse, dense, low-frequency only) scaled to
+    coefficient L2 norms up to and past the bound: inside it the matrix-core
+    path equals the i32 path byte for byte, past it the atom takes the i32
+    path; extreme levels (|lv| = 32767) included."""
+    cm, gs, scan, bs = tabs
+    rng = np.random.default_rng(17)
+    stp, bia = oracle.dq_tables(1.7)
+    seen = {3: 0, 2: 0, 1: 0}
+    for case in range(60):
+        kind = case % 4
+        nmax = {0: 40, 1: 4000, 2: 30000, 3: 300}[kind]
+        hi = 32768 if kind != 3 else 700           # kind 3: low frequencies only (pass pruning)
+        pos = np.unique(rng.integers(0, hi, int(rng.integers(1, nmax))))
+        mag = rng.integers(1, 40, len(pos)).astype(np.int64)
+        neg = rng.integers(0, 2, len(pos))
+        target = [500.0, 12000.0, 29000.0, 30900.0, 30999.0, 31001.0, 33000.0, 1e6][case % 8]
+        ent = ((pos.astype(np.int64) << 17) | ((2 * mag - 11 + neg) & 0x1FFFF)).astype(np.uint32)
+        l2 = np.sqrt((_dequant(ent, scan, stp, bia).astype(np.float64) ** 2).sum())
+        mag = np.clip(np.round(mag * target / max(l2, 1.0)), 1, 32767).astype(np.int64)
+        if case % 11 == 5:
+            mag[0] = 32767
+        ent = ((pos.astype(np.int64) << 17) | ((2 * mag - 11 + neg) & 0x1FFFF)).astype(np.uint32)
+        r = _dequant(ent, scan, stp, bia).astype(np.int64)
+        inside = np.abs(r).max() <= 31000 and (r * r).sum() <= 31000 ** 2
+        dc = int(rng.integers(0, 256))
+        path, a, b = _idct_both(kc, tabs, ent, dc, stp, bia)
+        assert (path == 3) == inside, (case, path, np.sqrt((r * r).sum()))
+        assert np.array_equal(a, b), (case, path)
+        seen[path] += 1
+    assert seen[3] >= 20 and seen[2] + seen[1] >= 10, seen
 
 
 def test_kernel_decode_damaged(kc, tabs, oracle, golden_corrupt):

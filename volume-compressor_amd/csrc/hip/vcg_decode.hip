@@ -239,9 +239,10 @@ __device__ __forceinline__ dst_view item_dst(const dec_job &j, uint32_t gi) {
 // rows (z, y) and (z, y+1).  Rows that lie wholly inside the box at a 16-byte
 // aligned destination go out as 16-byte stores, others byte by byte.
 __device__ __forceinline__ void store_atom(const dst_view &d, uint32_t ax, uint32_t ay, uint32_t az,
-                                           const uint8_t *stage, int tid) {
+                                           const uint8_t *stage, int tid, uint32_t zs = 1024) {
     const int z = tid >> 4, y0 = (tid & 15) * 2;
-    const uint4 *src = (const uint4 *)(stage + 64 * tid);
+    const uint8_t *row0 = stage + z * zs + y0 * 32;
+    const uint4 *src = (const uint4 *)row0;
     if (!d.box) {
         uint4 *dst = (uint4 *)(d.out + 64 * tid);
 #pragma unroll
@@ -260,7 +261,7 @@ __device__ __forceinline__ void store_atom(const dst_view &d, uint32_t ax, uint3
             dst[0] = src[2 * r];
             dst[1] = src[2 * r + 1];
         } else {
-            const uint8_t *b = stage + 64 * tid + 32 * r;
+            const uint8_t *b = row0 + 32 * r;
             for (int x = 0; x < 32; ++x)
                 if (vx + x >= d.bx0 && vx + x < d.bx1) row[vx + x - d.bx0] = b[x];
         }
@@ -308,7 +309,8 @@ __device__ __forceinline__ void idct_item(const dec_job &j, uint32_t i, const vc
     const vcg_item it = items[i];
     uint8_t *stage = (uint8_t *)buf;
     if (tid == 0 && j.status_out) j.status_out[gi] = it.status == 2 ? 1 : it.status == 3 ? 2 : 0;
-    if (it.status != 1) {
+    // status 4: an atom the matrix-core transform left to this full-range path
+    if (it.status != 1 && it.status != 4) {
         for (int v = tid; v < (int)VCG_A3 / 4; v += IDCT_THREADS) ((uint32_t *)stage)[v] = 0;
         __syncthreads();
         store_atom(item_dst(j, gi), ax, ay, az, stage, tid);
@@ -426,19 +428,282 @@ __global__ __launch_bounds__(IDCT_THREADS) void k_idct_p(dec_job j, const vcg_dt
 }
 
 // ---------------------------------------------------------------------------
+// inverse transform on the matrix cores (vcg_xform.hpp, "inverse on the
+// matrix cores"): an i16 tile of 64 KiB, so two transform workgroups share a
+// CU, each pass 32 int8 products per atom.  An atom whose coefficients
+// exceed the bound is marked status 4 and left to k_idct_ovf (the i32 path),
+// launched after it on the same stream.
+// ---------------------------------------------------------------------------
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+#define IM_THREADS 512
+// registers for four transform waves and one entropy wave per SIMD
+#ifndef IM_WAVES
+#define IM_WAVES 5
+#endif
+#define IM_REGS __attribute__((amdgpu_waves_per_eu(IM_WAVES)))
+static_assert(IM_THREADS == IDCT_THREADS, "the list prefetch and the store assume 512 threads");
+
+// one block's product chain (data planes hi / lo, basis planes Ch, 4 Cl, Cl):
+// o[r] = (sum_k C[k][n] v[k] + rnd) >> 14 for output register r
+__device__ __forceinline__ void im_chain(const uint32_t hi[4], const uint32_t lo[4], v4i bh, v4i b4l, v4i bl,
+                                         int rnd, int o[16]) {
+    const v4i ah = {(int)hi[0], (int)hi[1], (int)hi[2], (int)hi[3]};
+    const v4i al = {(int)lo[0], (int)lo[1], (int)lo[2], (int)lo[3]};
+    const v16i zero = {};
+    v16i t = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bh, zero, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) t[r] = (int)((uint32_t)t[r] << 8);
+    t = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, b4l, t, 0, 0, 0);
+    t = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh, t, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) t[r] = (int)(((uint32_t)t[r] << 6) + (uint32_t)rnd);
+    t = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl, t, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) o[r] = t[r] >> 14;
+}
+
+// block B of pass p (x or y): product and the 8-byte writes of this lane
+__device__ __forceinline__ void im_block_xy(uint8_t *tile, int p, int B, int jr, int hh, const uint32_t d[8],
+                                            v4i bh, v4i b4l, v4i bl) {
+    uint32_t hi[4], lo[4];
+    vcg_im_planes(d, hi, lo);
+    int o[16];
+    im_chain(hi, lo, bh, b4l, bl, vcg_im_rnd(p, 0), o);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        uint32_t w[2];
+        vcg_im_pack16(o + 4 * q, w);
+        *(uint2 *)(tile + vcg_im_out_addr(p, B, jr, hh, q)) = make_uint2(w[0], w[1]);
+    }
+}
+
+// One workgroup's item i: scatter into the i16 tile, the bound, three
+// product passes, the store.  Every thread calls it with the same i.
+__device__ __forceinline__ void idct_item_m(const dec_job &j, uint32_t i, const vcg_dtab *__restrict__ T,
+                                            const uint32_t *__restrict__ lists, vcg_item *__restrict__ items,
+                                            uint8_t *tile, unsigned long long *ssum, uint32_t *sbig,
+                                            int32_t *dqs) {
+    // per-lane addresses derive from tid; an opaque copy keeps the persistent
+    // kernel from hoisting them all out of its atom loop (and spilling them)
+    int tid = (int)threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const uint32_t gi = j.base + i;
+    int lod;
+    uint32_t ax, ay, az;
+    item_atom(j, gi, lod, ax, ay, az);
+    // the item, the dequant tables and the list head load together (as idct_item)
+    const uint32_t *L = lists + (size_t)i * VCG_LIST_STRIDE;
+    uint32_t pre[IDCT_PRE];
+#pragma unroll
+    for (int q = 0; q < IDCT_PRE; ++q) pre[q] = L[tid + q * IM_THREADS];
+    if (tid < VCG_NFREQ) {
+        const int o = j.mode == MODE_PAYLOAD ? 0 : lod * VCG_NFREQ;
+        dqs[tid] = j.dqs[o + tid];
+        dqs[VCG_NFREQ + tid] = j.dqb[o + tid];
+    }
+    const vcg_item it = items[i];
+    const int ln = tid & 63, wv = tid >> 6, jr = ln & 31, hh = ln >> 5;
+    const v4i bh = *(const v4i *)T->ib[0][ln], b4l = *(const v4i *)T->ib[1][ln], bl = *(const v4i *)T->ib[2][ln];
+    if (tid == 0 && j.status_out) j.status_out[gi] = it.status == 2 ? 1 : it.status == 3 ? 2 : 0;
+    if (it.status != 1) {
+        for (int v = tid; v < (int)VCG_A3 / 4; v += IM_THREADS) ((uint32_t *)tile)[v] = 0;
+        __syncthreads();
+        store_atom(item_dst(j, gi), ax, ay, az, tile, tid);
+        return;
+    }
+    const uint32_t cnt = it.cnt;
+    const int fmax = it.fmax, dc = (int)it.dc;
+    uint32_t sidx[IDCT_PRE];
+#pragma unroll
+    for (int q = 0; q < IDCT_PRE; ++q)
+        sidx[q] = tid + q * IM_THREADS < (int)cnt ? T->scan[vcg_ent_pos(pre[q])] : 0u;
+    vcg_im_fill((uint32_t *)tile, tid, IM_THREADS);
+    if (tid == 0) { *ssum = 0; *sbig = 0; }
+    __syncthreads();
+    const int32_t *st = dqs, *bi = dqs + VCG_NFREQ;
+    uint64_t ss = 0;
+    uint32_t big = 0;
+#pragma unroll
+    for (int q = 0; q < IDCT_PRE; ++q)
+        if (tid + q * IM_THREADS < (int)cnt) vcg_im_scatter1(tile, pre[q], sidx[q], st, bi, ss, big);
+    if (cnt > IDCT_PRE * IM_THREADS)
+        vcg_im_scatter(tile, L + IDCT_PRE * IM_THREADS, cnt - IDCT_PRE * IM_THREADS, T->scan, st, bi, tid,
+                       IM_THREADS, ss, big);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor((unsigned long long)ss, o, 64);
+    const bool wbig = __ballot(big != 0) != 0;
+    if (ln == 0) {
+        if (ss) atomicAdd(ssum, (unsigned long long)ss);
+        if (wbig) atomicOr(sbig, 1u);
+    }
+    __syncthreads();
+    if (*sbig || *ssum > (unsigned long long)VCG_IM_L2 * VCG_IM_L2) {
+        if (tid == 0) items[i].status = 4;
+        return;
+    }
+    // pass x: the wave's planes kz = wv + 8b, in place (it owns them)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int B = wv + 8 * b;
+        if (vcg_im_skip(0, B, fmax)) continue;
+        uint32_t d[8];
+        vcg_im_load(tile, 0, B, jr, hh, d);
+        im_block_xy(tile, 0, B, jr, hh, d, bh, b4l, bl);
+    }
+    __syncthreads();
+    // pass y: every wave reads its blocks before any writes
+    {
+        uint32_t d[4][8];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (!vcg_im_skip(1, wv + 8 * b, fmax)) vcg_im_load(tile, 1, wv + 8 * b, jr, hh, d[b]);
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (!vcg_im_skip(1, wv + 8 * b, fmax)) im_block_xy(tile, 1, wv + 8 * b, jr, hh, d[b], bh, b4l, bl);
+    }
+    __syncthreads();
+    // pass z: voxels (dc added, clamped) into the staging raster
+    {
+        uint32_t d[4][8];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) vcg_im_load(tile, 2, wv + 8 * b, jr, hh, d[b]);
+        __syncthreads();
+        const int rnd = vcg_im_rnd(2, dc);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            uint32_t hi[4], lo[4];
+            vcg_im_planes(d[b], hi, lo);
+            int o[16];
+            im_chain(hi, lo, bh, b4l, bl, rnd, o);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                *(uint32_t *)(tile + vcg_im_vox_addr(wv + 8 * b, jr, hh, q)) = vcg_im_pack8(o + 4 * q);
+        }
+    }
+    __syncthreads();
+    store_atom(item_dst(j, gi), ax, ay, az, tile, tid, VCG_IM_ZS);
+}
+
+__global__ __launch_bounds__(IM_THREADS) IM_REGS void k_idct_m(dec_job j, const vcg_dtab *__restrict__ T,
+                                                       const uint32_t *__restrict__ lists,
+                                                       vcg_item *__restrict__ items) {
+    extern __shared__ __align__(16) uint8_t tile[];
+    __shared__ unsigned long long ssum;
+    __shared__ uint32_t sbig;
+    __shared__ int32_t dqs[2 * VCG_NFREQ];
+    idct_item_m(j, xcd_item(blockIdx.x, gridDim.x), T, lists, items, tile, &ssum, &sbig, dqs);
+}
+
+// the items of chunk [c0, c0 + 512) the matrix-core transform marked, through
+// the i32 path
+__device__ __forceinline__ void ovf_chunk(const dec_job &j, uint32_t c0, uint32_t n, const vcg_dtab *__restrict__ T,
+                                          const uint32_t *__restrict__ lists, const vcg_item *__restrict__ items,
+                                          int *buf, unsigned long long *l1sum, int32_t *dqs, uint32_t *cnt,
+                                          uint32_t *list) {
+    __syncthreads();                     // the previous chunk is done with cnt and list
+    if (threadIdx.x == 0) *cnt = 0;
+    __syncthreads();
+    const uint32_t i = c0 + threadIdx.x;
+    if (i < n && items[i].status == 4) list[atomicAdd(cnt, 1u)] = i;
+    __syncthreads();
+    const uint32_t c = *cnt;
+    for (uint32_t k = 0; k < c; ++k) {
+        idct_item(j, list[k], T, lists, items, buf, l1sum, dqs);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(IDCT_THREADS) void k_idct_ovf(dec_job j, const vcg_dtab *__restrict__ T,
+                                                           const uint32_t *__restrict__ lists,
+                                                           const vcg_item *__restrict__ items) {
+    extern __shared__ int buf[];
+    __shared__ unsigned long long l1sum;
+    __shared__ int32_t dqs[2 * VCG_NFREQ];
+    __shared__ uint32_t cnt, list[IDCT_THREADS];
+    ovf_chunk(j, blockIdx.x * IDCT_THREADS, j.n, T, lists, items, buf, &l1sum, dqs, &cnt, list);
+}
+
+// persistent forms for the server's captured batches
+__global__ __launch_bounds__(IM_THREADS) IM_REGS void k_idct_mp(dec_job j, const vcg_dtab *__restrict__ T,
+                                                        const uint32_t *__restrict__ lists,
+                                                        vcg_item *__restrict__ items, vcg_qhdr *__restrict__ q) {
+    extern __shared__ __align__(16) uint8_t tile[];
+    __shared__ unsigned long long ssum;
+    __shared__ uint32_t sbig;
+    __shared__ int32_t dqs[2 * VCG_NFREQ];
+    __shared__ uint32_t cur;
+    const uint32_t n = q->n;
+#pragma clang loop unroll(disable)
+    for (;;) {
+        __syncthreads();                 // the previous atom is done with the tile and cur
+        if (threadIdx.x == 0) cur = atomicAdd(&q->idct, 1u);
+        __syncthreads();
+        const uint32_t i = __builtin_amdgcn_readfirstlane(cur);   // uniform: scalar addressing
+        if (i >= n) break;
+        idct_item_m(j, i, T, lists, items, tile, &ssum, &sbig, dqs);
+    }
+}
+
+__global__ __launch_bounds__(IDCT_THREADS) void k_idct_ovf_p(dec_job j, const vcg_dtab *__restrict__ T,
+                                                             const uint32_t *__restrict__ lists,
+                                                             const vcg_item *__restrict__ items,
+                                                             const vcg_qhdr *__restrict__ q) {
+    extern __shared__ int buf[];
+    __shared__ unsigned long long l1sum;
+    __shared__ int32_t dqs[2 * VCG_NFREQ];
+    __shared__ uint32_t cnt, list[IDCT_THREADS];
+    const uint32_t n = q->n;
+    for (uint32_t c0 = blockIdx.x * IDCT_THREADS; c0 < n; c0 += gridDim.x * IDCT_THREADS)
+        ovf_chunk(j, c0, n, T, lists, items, buf, &l1sum, dqs, &cnt, list);
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 static const size_t LDS_BYTES = (size_t)VCG_TILE_INTS * 4;
 static const uint32_t MAX_BATCH = 32768;   // items per launch pair (scratch: 4 GiB)
 static const uint64_t VCG_HOST_SLAB = 1ull << 30;   // host-destination staging per slab
 
+static const size_t LDS_M_BYTES = VCG_IM_TILE_BYTES;
+
 static void ensure_attr() {
-    static bool done = false;
-    if (!done) {
-        VCG_CHECK(hipFuncSetAttribute((const void *)k_idct, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)LDS_BYTES));
-        done = true;
-    }
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const void *big[] = {(const void *)k_idct, (const void *)k_idct_p, (const void *)k_idct_ovf,
+                             (const void *)k_idct_ovf_p};
+        for (const void *k : big)
+            VCG_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_BYTES));
+        const void *mk[] = {(const void *)k_idct_m, (const void *)k_idct_mp};
+        for (const void *k : mk)
+            VCG_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_M_BYTES));
+    });
+}
+
+// The transform of a launch's j.n items on stream s: the matrix-core kernel,
+// then the i32 kernel for the items it marked (a scan that ends at once when
+// none is).  VCG_IDCT_I32 builds the i32 kernel alone.
+static void launch_idct(const dec_job &j, uint32_t *lists, vcg_item *items, hipStream_t s) {
+#ifdef VCG_IDCT_I32
+    hipLaunchKernelGGL(k_idct, dim3(j.n), dim3(IDCT_THREADS), LDS_BYTES, s, j, g_dtab, lists, items);
+#else
+    hipLaunchKernelGGL(k_idct_m, dim3(j.n), dim3(IM_THREADS), LDS_M_BYTES, s, j, g_dtab, lists, items);
+    hipLaunchKernelGGL(k_idct_ovf, dim3((j.n + IDCT_THREADS - 1) / IDCT_THREADS), dim3(IDCT_THREADS), LDS_BYTES, s,
+                       j, g_dtab, lists, items);
+#endif
+}
+// the persistent form: gt transform workgroups, batches of up to cap items
+static void launch_idct_p(const dec_job &j, uint32_t gt, size_t cap, uint32_t *lists, vcg_item *items,
+                          vcg_qhdr *hq, hipStream_t s) {
+#ifdef VCG_IDCT_I32
+    hipLaunchKernelGGL(k_idct_p, dim3(gt), dim3(IDCT_THREADS), LDS_BYTES, s, j, g_dtab, lists, items, hq);
+#else
+    hipLaunchKernelGGL(k_idct_mp, dim3(gt), dim3(IM_THREADS), LDS_M_BYTES, s, j, g_dtab, lists, items, hq);
+    const uint32_t go = (uint32_t)((cap + IDCT_THREADS - 1) / IDCT_THREADS);
+    hipLaunchKernelGGL(k_idct_ovf_p, dim3(go < 64 ? go : 64), dim3(IDCT_THREADS), LDS_BYTES, s, j, g_dtab, lists,
+                       items, hq);
+#endif
 }
 
 // Launch the entropy / transform pairs for `total` items on the slot streams
@@ -521,7 +786,7 @@ static int run_batch(vcg_ctx *c, dec_job j, uint32_t total, bool inputs_on_s, co
         if (c->ev[1]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[1], sE));
         // transform: after its entropy pass (same stream) and earlier work on c->s
         if (!inputs_on_s) VCG_CHECK(hipStreamWaitEvent(sE, c->evF, 0));
-        hipLaunchKernelGGL(k_idct, dim3(j.n), dim3(IDCT_THREADS), LDS_BYTES, sE, j, g_dtab, blk->lists, blk->items);
+        launch_idct(j, blk->lists, blk->items, sE);
         VCG_CHECK(hipGetLastError());
         blk_give(blk, sE);
         if (c->ev[2]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[2], sE));
@@ -743,7 +1008,7 @@ extern "C" int vcg_list_launch_noalloc(vcg_arc *a, size_t n, const int32_t *quad
     hipLaunchKernelGGL(k_entropy, dim3((j.n + ENT_THREADS - 1) / ENT_THREADS), dim3(ENT_THREADS), 0, s, j, g_dtab,
                        lists, items, word);
     if (hipGetLastError() != hipSuccess) return -1;
-    hipLaunchKernelGGL(k_idct, dim3(j.n), dim3(IDCT_THREADS), LDS_BYTES, s, j, g_dtab, lists, items);
+    launch_idct(j, lists, items, s);
     if (hipGetLastError() != hipSuccess) return -1;
     return 0;
 }
@@ -776,12 +1041,6 @@ extern "C" vcg_qgraph *vcg_qgraph_create(vcg_arc *a, int slot, const vcg_qcfg *c
     if (!scratch) return nullptr;
     vcg_ctx *c = vcg_get_ctx();
     ensure_attr();
-    static bool attr_p = false;
-    if (!attr_p) {
-        VCG_CHECK(hipFuncSetAttribute((const void *)k_idct_p, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)LDS_BYTES));
-        attr_p = true;
-    }
     const vcg_geom *gd = vcg_arc_geom_dev(a);
     dec_job j;
     memset(&j, 0, sizeof j);
@@ -803,14 +1062,14 @@ extern "C" vcg_qgraph *vcg_qgraph_create(vcg_arc *a, int slot, const vcg_qcfg *c
     g->j = j; g->cfg = *cfg; g->status = c->status_d;
     hipGraph_t graph = nullptr;
     VCG_CHECK(hipStreamBeginCapture(g->s, hipStreamCaptureModeThreadLocal));
-    const dim3 ge(cfg->ent_groups), gt(cfg->idct_groups);
+    const dim3 ge(cfg->ent_groups);
     if (W == 1)
         hipLaunchKernelGGL(k_entropy_p<1>, ge, dim3(64), 0, g->s, j, g_dtab, g->lists, g->items, c->status_d, hq);
     else if (W == 2)
         hipLaunchKernelGGL(k_entropy_p<2>, ge, dim3(128), 0, g->s, j, g_dtab, g->lists, g->items, c->status_d, hq);
     else
         hipLaunchKernelGGL(k_entropy_p<4>, ge, dim3(256), 0, g->s, j, g_dtab, g->lists, g->items, c->status_d, hq);
-    hipLaunchKernelGGL(k_idct_p, gt, dim3(IDCT_THREADS), LDS_BYTES, g->s, j, g_dtab, g->lists, g->items, hq);
+    launch_idct_p(j, cfg->idct_groups, cap, g->lists, g->items, hq, g->s);
     VCG_CHECK(hipStreamEndCapture(g->s, &graph));
     VCG_CHECK(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0));
     VCG_CHECK(hipGraphDestroy(graph));
@@ -828,14 +1087,14 @@ extern "C" int vcg_qgraph_launch(vcg_qgraph *g, const uint8_t *qpin, size_t n, s
                                  hipMemcpyHostToDevice, g->s));
     if (direct) {                        // the same two kernels, launched one by one
         vcg_qhdr *hq = (vcg_qhdr *)g->qdev;
-        const dim3 ge(g->cfg.ent_groups), gt(g->cfg.idct_groups);
+        const dim3 ge(g->cfg.ent_groups);
         if (g->cfg.ent_waves == 1)
             hipLaunchKernelGGL(k_entropy_p<1>, ge, dim3(64), 0, g->s, g->j, g_dtab, g->lists, g->items, g->status, hq);
         else if (g->cfg.ent_waves == 2)
             hipLaunchKernelGGL(k_entropy_p<2>, ge, dim3(128), 0, g->s, g->j, g_dtab, g->lists, g->items, g->status, hq);
         else
             hipLaunchKernelGGL(k_entropy_p<4>, ge, dim3(256), 0, g->s, g->j, g_dtab, g->lists, g->items, g->status, hq);
-        hipLaunchKernelGGL(k_idct_p, gt, dim3(IDCT_THREADS), LDS_BYTES, g->s, g->j, g_dtab, g->lists, g->items, hq);
+        launch_idct_p(g->j, g->cfg.idct_groups, g->cap, g->lists, g->items, hq, g->s);
         VCG_CHECK(hipGetLastError());
         return 0;
     }

@@ -21,6 +21,7 @@ struct vcg_dtab {
     uint8_t  fscan[VCG_A3];   // frequency cz+cy+cx of scan position p (non-decreasing in p)
     int8_t   fa[2][64][16];   // forward pass 1 on MFMA: per-lane basis planes (vcg_fwd1_tables)
     int32_t  rowsum[32];      // basis row sums
+    int8_t   ib[3][64][16];   // inverse on MFMA: per-lane basis planes Ch, 4 Cl, Cl (vcg_im_tables)
 };
 extern vcg_dtab *g_dtab;  // device pointer
 

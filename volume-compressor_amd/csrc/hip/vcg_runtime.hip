@@ -63,6 +63,7 @@ extern "C" int vcg_init(const vcg_tables *t) {
         for (int q = 0; q < VCG_A3; ++q)
             h.fscan[q] = (uint8_t)((h.scan[q] >> 10) + ((h.scan[q] >> 5) & 31) + (h.scan[q] & 31));
         vcg_fwd1_tables(h.cmat, h.fa, h.rowsum);
+        vcg_im_tables(h.cmat, h.ib);
         // the allocation-free decode's static scratch: its device address
         if (vcg_noalloc_setup() != 0) vc__panic("hipGetSymbolAddress failed for the decoder's static scratch");
         VCG_CHECK(hipMalloc(&g_dtab, sizeof(vcg_dtab)));

@@ -478,6 +478,186 @@ VCG_HD int vcg_fwd1_k(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
 // corr = (128 - dc) rowsum[k] + 8192
 VCG_HD int vcg_fwd1_out(int dh, int dl, int corr) { return (dh * 64 + dl + corr) >> 14; }
 
+// ===========================================================================
+// inverse on the matrix cores (i16 tile)
+// ===========================================================================
+// Each inverse pass is, per line, out[n] = (sum_k C[k][n] v[k] + 8192) >> 14
+// (src/vc/vc.c:172-205; the reference's even/odd split only regroups the
+// integer sum).  Here a pass is a set of 32x32x32 int8 products
+// D[line][n] = sum_k A[line][k] B[k][n], A the data, B the basis, exact:
+//   - v = 256 vh + vl with vl in [-128, 127]: the tile holds w = v + 128 as
+//     i16, and then vh = byte 1 of w, vl = byte 0 of w xor 0x80;
+//   - C = 64 Ch + Cl with Cl = ((C + 32) & 63) - 32 (|C| <= 4091: Ch, Cl
+//     and 4 Cl all fit int8);
+//   - C v = 64 (256 Ch vh + 4Cl vh + Ch vl) + Cl vl, i.e. the chain
+//     T = Ch*vh; T <<= 8; T += 4Cl*vh; T += Ch*vl; T = (T << 6) + rnd;
+//     T += Cl*vl; out = T >> 14, every step exact mod 2^32.
+// With the atom's coefficient L2 norm at most VCG_IM_L2 (and every
+// coefficient at most that), every pass input fits: each 1-D pass has
+// sigma_max(C / 2^14) = 1.00008 and adds at most 0.5 of rounding per element
+// (L2 <= 0.5 sqrt(32768) = 90.6 per pass), so pass inputs stay below
+// 1.00008^2 31000 + 2 x 90.6 = 31186 in magnitude, inside the
+// [-32896, 32639] that w = v + 128 covers, and |sum C v| <= 2^14 x 1.00008 x
+// 31186 < 2^29, far from i32 overflow.  Atoms past the bound take the i32
+// path (vcg_inv_pass_*).
+//
+// Tile: 1024 lines of 32 i16 (64 bytes, four 16-byte chunks); chunk c of
+// line L sits at chunk position c ^ ((L >> 1) & 3) ^ ((L >> 5) & 3), which
+// keeps every group of eight lanes of a 16-byte line read on distinct banks
+// in all three passes.
+//
+// v_mfma_i32_32x32x32_i8 lane layout: lane l = j + 32 h supplies A row j
+// (k = 16h .. 16h+15, one 32-byte read) and B column j (the same k); output
+// register r of lane l is D[8 (r / 4) + 4 h + r % 4][l mod 32].
+//
+// Passes and their 32 blocks (a block is one product; a wave takes blocks
+// wv, wv + 8, wv + 16, wv + 24):
+//   x: block = plane kz, rows = ky; in [kz][ky][kx], out [kz][x][ky];
+//   y: block = (kz 4m..4m+3, x 8p..8p+7), B = 4m + p, row j = (kz = 4m +
+//      j % 4, x = 8p + j / 4); in [kz][x][ky], out [y][x][kz];
+//   z: block = plane y, rows = x; in [y][x][kz], out voxels (z, y, x) to the
+//      staging raster (z stride VCG_IM_ZS bytes).
+// Pass x skips planes kz > fmax and pass y kz groups past fmax: their inputs
+// are zero (w = 128), and so are the outputs at the positions they would
+// write (lines with kz + ky > fmax).
+#define VCG_IM_L2 31000u
+#define VCG_IM_TILE_BYTES 65536
+#define VCG_IM_ZS 1040                  // staging z stride: 16-byte aligned, banks spread
+#define VCG_IM_W0 0x00800080u           // two i16 w = 128 (v = 0)
+
+VCG_HD uint32_t vcg_perm(uint32_t hi, uint32_t lo, uint32_t sel) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_perm(hi, lo, sel);
+#else
+    // selector bytes 0..7 pick a byte of {hi, lo}, 0x0c gives 0x00, 0x0d 0xff
+    const uint64_t v = (uint64_t)hi << 32 | lo;
+    uint32_t r = 0;
+    for (int b = 0; b < 4; ++b) {
+        const uint32_t s = (sel >> (8 * b)) & 0xFFu;
+        const uint32_t byte = s < 8 ? (uint32_t)((v >> (8 * s)) & 0xFFu) : s == 0x0c ? 0u : 0xFFu;
+        r |= byte << (8 * b);
+    }
+    return r;
+#endif
+}
+
+// byte offset of chunk c (8 i16) of tile line L
+VCG_HD uint32_t vcg_im_chunk(uint32_t L, uint32_t c) {
+    return (L << 6) + ((c ^ ((L >> 1) & 3u) ^ ((L >> 5) & 3u)) << 4);
+}
+
+// per-lane B operands: lane l = n + 32 h, byte e = plane of C[16h + e][n];
+// plane 0 = Ch, 1 = 4 Cl, 2 = Cl
+VCG_HD void vcg_im_tables(const int32_t (*C)[32], int8_t ib[3][64][16]) {
+    for (int l = 0; l < 64; ++l)
+        for (int e = 0; e < 16; ++e) {
+            const int c = C[16 * (l >> 5) + e][l & 31];
+            const int lo = ((c + 32) & 63) - 32;
+            ib[0][l][e] = (int8_t)((c - lo) / 64);
+            ib[1][l][e] = (int8_t)(4 * lo);
+            ib[2][l][e] = (int8_t)lo;
+        }
+}
+
+VCG_HD void vcg_im_fill(uint32_t *t, int tid, int nthr) {
+    struct u4 { uint32_t a, b, c, d; };
+    u4 *q = (u4 *)t;
+    for (int k = tid; k < VCG_IM_TILE_BYTES / 16; k += nthr) q[k] = u4{VCG_IM_W0, VCG_IM_W0, VCG_IM_W0, VCG_IM_W0};
+}
+
+// Dequantize list entry v (coefficient at raster index idx) as vcg_scatter1
+// does, store w = r + 128 into the i16 tile; ss += r^2 and big |= |r| above
+// the bound (then ss is not needed).
+VCG_HD void vcg_im_scatter1(uint8_t *t, uint32_t v, uint32_t idx, const int32_t *st, const int32_t *bi, uint64_t &ss,
+                            uint32_t &big) {
+    const int lv = vcg_ent_level(v);
+    const int f = vcg_freq(idx);
+    const uint32_t mag = (uint32_t)(lv < 0 ? -lv : lv);
+    int r = (int)(mag * (uint32_t)st[f] + (uint32_t)bi[f] + 128u) >> 8;
+    r = lv < 0 ? (int)(0u - (uint32_t)r) : r;
+    const uint32_t a = r < 0 ? 0u - (uint32_t)r : (uint32_t)r;
+    big |= a > VCG_IM_L2 ? 1u : 0u;
+    ss += (uint64_t)a * a;
+    const uint32_t L = idx >> 5, x = idx & 31u;
+    *(int16_t *)(t + vcg_im_chunk(L, x >> 3) + 2 * (x & 7u)) = (int16_t)(uint16_t)((uint32_t)r + 128u);
+}
+
+VCG_HD void vcg_im_scatter(uint8_t *t, const uint32_t *L, uint32_t cnt, const uint16_t *scan, const int32_t *st,
+                           const int32_t *bi, int tid, int nthr, uint64_t &ss, uint32_t &big) {
+    const uint32_t un = (uint32_t)nthr;
+    for (uint32_t e0 = (uint32_t)tid; e0 < cnt; e0 += 8u * un) {
+        uint32_t v[8], idx[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = e0 + q * un < cnt ? L[e0 + q * un] : 0u;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) idx[q] = e0 + q * un < cnt ? scan[vcg_ent_pos(v[q])] : 0u;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (e0 + q * un >= cnt) break;
+            vcg_im_scatter1(t, v[q], idx[q], st, bi, ss, big);
+        }
+    }
+}
+
+// block B of pass p: skipped (all-zero input)?
+VCG_HD bool vcg_im_skip(int p, int B, int fmax) {
+    return p == 0 ? B > fmax : p == 1 ? 4 * (B >> 2) > fmax : false;
+}
+// the tile line of A row j of block B in pass p
+VCG_HD uint32_t vcg_im_in_line(int p, int B, int j) {
+    if (p == 1) return (uint32_t)((4 * (B >> 2) + (j & 3)) * 32 + 8 * (B & 3) + (j >> 2));
+    return (uint32_t)(B * 32 + j);
+}
+// lane (row j, half h) of block B in pass p: its 16 w of A, two 16-byte reads
+VCG_HD void vcg_im_load(const uint8_t *t, int p, int B, int j, int h, uint32_t d[8]) {
+    const uint32_t L = vcg_im_in_line(p, B, j);
+    const uint32_t *a = (const uint32_t *)(t + vcg_im_chunk(L, 2u * (uint32_t)h));
+    const uint32_t *b = (const uint32_t *)(t + vcg_im_chunk(L, 2u * (uint32_t)h + 1u));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { d[q] = a[q]; d[4 + q] = b[q]; }
+}
+// the two int8 planes of 16 w (element e of dword e / 2): hi = vh, lo = vl
+VCG_HD void vcg_im_planes(const uint32_t d[8], uint32_t hi[4], uint32_t lo[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        hi[q] = vcg_perm(d[2 * q + 1], d[2 * q], 0x07050301u);
+        lo[q] = vcg_perm(d[2 * q + 1], d[2 * q], 0x06040200u) ^ 0x80808080u;
+    }
+}
+// rounding constant of the chain (before >> 14): passes x, y store w = out
+// + 128; pass z adds the atom's dc
+VCG_HD int vcg_im_rnd(int p, int dc) { return 8192 + ((p == 2 ? dc : 128) << 14); }
+
+// pass x / y: lane (n = l, half h) writes its output registers 4q..4q+3
+// (four consecutive rows) as 8 bytes; returns the byte offset
+VCG_HD uint32_t vcg_im_out_addr(int p, int B, int l, int h, int q) {
+    if (p == 0) return vcg_im_chunk((uint32_t)(B * 32 + l), (uint32_t)q) + 8u * (uint32_t)h;
+    const int m = B >> 2;
+    const uint32_t L = (uint32_t)(l * 32 + 8 * (B & 3) + 2 * q + h);
+    return vcg_im_chunk(L, (uint32_t)(m >> 1)) + 8u * (uint32_t)(m & 1);
+}
+// four outputs (w, i16) packed little-endian into two dwords
+VCG_HD void vcg_im_pack16(const int o[4], uint32_t w[2]) {
+    w[0] = vcg_perm((uint32_t)o[1], (uint32_t)o[0], 0x05040100u);
+    w[1] = vcg_perm((uint32_t)o[3], (uint32_t)o[2], 0x05040100u);
+}
+// pass z: lane (z = l, half h), registers 4q..4q+3 = voxels x = 8q + 4h ..
+// +3 of row (z, y = B): clamped, packed, at this staging offset
+VCG_HD uint32_t vcg_im_vox_addr(int B, int l, int h, int q) {
+    return (uint32_t)(l * VCG_IM_ZS + B * 32 + 8 * q + 4 * h);
+}
+// The bytes are packed with permutes, not shifts and ors: hipcc fuses
+// "shift, clamp to u8, pack two" into v_ashr_pk_u8_i32, which writes only the
+// low 16 bits of its destination, and then ors the register's stale high
+// half into the upper bytes.
+VCG_HD uint32_t vcg_im_pack8(const int o[4]) {
+    uint32_t v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = (uint32_t)(o[i] < 0 ? 0 : o[i] > 255 ? 255 : o[i]);
+    const uint32_t p01 = vcg_perm(v[1], v[0], 0x0c0c0400u), p23 = vcg_perm(v[3], v[2], 0x0c0c0400u);
+    return vcg_perm(p23, p01, 0x05040100u);
+}
+
 // quantize the tile's coefficients in scan order into lv[p]; returns this
 // thread's max (p+1) over nonzero levels (the EOB candidate)
 VCG_HD int vcg_quant_scan(const int *t, const uint16_t *scan, const float *gstep, float q, int16_t *lv, int tid,
