@@ -48,50 +48,86 @@ void kc_quant(const int32_t *c, size_t n, float q, float g, int32_t *out) {
     for (size_t i = 0; i < n; ++i) out[i] = vcg_quant1(c[i], q, g);
 }
 
+static void mfma_cpu(const uint32_t A[64][4], const uint32_t B[64][4], int32_t D[64][16]);
+
 // k_fdct on one atom of an (nx,ny,nz) volume: returns 1 for an all-zero atom.
-// Pass 1 follows the kernel's matrix-core form lane by lane: each z-plane's
-// 64 lanes load their 16-voxel rows (vcg_fwd1_row), the voxels become int8
-// by flipping their top bit, and each output register (k, y) of the 32x32x32
-// product is the sum over both lane halves and all 16 bytes of A byte x B
-// byte, exactly what v_mfma_i32_32x32x32_i8 computes; then passes 2 and 3 as
-// the kernel runs them.
+// The three passes follow the kernel's matrix-core form lane by lane (each
+// product emulated as v_mfma_i32_32x32x32_i8 computes it): pass 1 from the
+// 64 lanes' 16-voxel rows (vcg_fwd1_row, top bit flipped) into the i16 tile,
+// passes 2 and 3 from the tile's byte planes, every block read before any
+// block is written (the kernel's barriers); pass 3 into the i32 tile.
 int kc_fdct(const uint8_t *vol, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t ax, uint32_t ay, uint32_t az,
             const int32_t cmat[32][32], const uint16_t *scan, const float *gstep, float q, int16_t *lv,
             int32_t *coef_raster, uint32_t *eob, uint32_t *dc_out) {
-    int *t = (int *)calloc(VCG_TILE_INTS, sizeof(int));
-    static int8_t fa[2][64][16];
+    int *t = (int *)aligned_alloc(16, VCG_TILE_INTS * sizeof(int));
+    uint8_t *tile = (uint8_t *)t;
+    static int8_t fa[3][64][16];
     static int32_t rowsum[32];
-    static uint32_t raw[32][64][4];
+    static uint32_t raw[32][64][4], F[3][64][4];
     vcg_fwd1_tables(cmat, fa, rowsum);
+    memcpy(F, fa, sizeof F);
     uint32_t sum = 0;
     for (int z = 0; z < 32; ++z)
         for (int l = 0; l < 64; ++l) {
             vcg_fwd1_row(vol, nx, ny, nz, ax, ay, az, z, l & 31, l >> 5, raw[z][l]);
             for (int e = 0; e < 16; ++e) sum += (raw[z][l][e >> 2] >> (8 * (e & 3))) & 0xFFu;
+            for (int w = 0; w < 4; ++w) raw[z][l][w] ^= 0x80808080u;
         }
     if (sum == 0) { free(t); *eob = 0; *dc_out = 0; return 1; }
     const int dc = (int)((sum + 16384u) / 32768u);
-    for (int z = 0; z < 32; ++z)
-        for (int l = 0; l < 64; ++l) {
-            const int y = l & 31, h = l >> 5;
-            for (int r = 0; r < 16; ++r) {
-                const int k = vcg_fwd1_k(r, h);
-                int d[2] = {0, 0};
-                for (int p = 0; p < 2; ++p)
-                    for (int h2 = 0; h2 < 2; ++h2)
-                        for (int e = 0; e < 16; ++e) {
-                            const int8_t b = (int8_t)(((raw[z][y + 32 * h2][e >> 2] ^ 0x80808080u) >> (8 * (e & 3))) & 0xFFu);
-                            d[p] += fa[p][k + 32 * h2][e] * b;
-                        }
-                t[vcg_tidx(z, y, k)] = vcg_fwd1_out(d[0], d[1], (128 - dc) * rowsum[k] + 8192);
+    static int32_t D[64][16];
+    // pass 1: D[y][kx] = (v-128) x Ch, << 6 + corr, + (v-128) x Cl
+    for (int z = 0; z < 32; ++z) {
+        memset(D, 0, sizeof D);
+        mfma_cpu(raw[z], F[0], D);
+        for (int l = 0; l < 64; ++l)
+            for (int r = 0; r < 16; ++r)
+                D[l][r] = (int32_t)(((uint32_t)D[l][r] << 6) + (uint32_t)vcg_fwd1_corr(dc, rowsum[l & 31]));
+        mfma_cpu(raw[z], F[1], D);
+        for (int l = 0; l < 64; ++l)
+            for (int qq = 0; qq < 4; ++qq) {
+                int o[4];
+                for (int i = 0; i < 4; ++i) o[i] = D[l][4 * qq + i] >> 14;
+                uint32_t w[2];
+                vcg_im_pack16(o, w);
+                memcpy(tile + vcg_im_out_addr(0, z, l & 31, l >> 5, qq), w, 8);
+            }
+    }
+    static uint32_t d[32][64][8], hi[64][4], lo[64][4];
+    for (int p = 1; p < 3; ++p) {
+        for (int B = 0; B < 32; ++B)
+            for (int l = 0; l < 64; ++l) vcg_im_load(tile, p, B, l & 31, l >> 5, d[B][l]);
+        for (int B = 0; B < 32; ++B) {
+            for (int l = 0; l < 64; ++l) vcg_im_planes(d[B][l], hi[l], lo[l]);
+            memset(D, 0, sizeof D);
+            // pass 2: data x basis (D[row][ky]); pass 3: basis x data (D[kz][kx])
+            if (p == 1) mfma_cpu(hi, F[0], D); else mfma_cpu(F[0], hi, D);
+            for (int l = 0; l < 64; ++l) for (int r = 0; r < 16; ++r) D[l][r] = (int32_t)((uint32_t)D[l][r] << 8);
+            if (p == 1) { mfma_cpu(hi, F[2], D); mfma_cpu(lo, F[0], D); } else { mfma_cpu(F[2], hi, D); mfma_cpu(F[0], lo, D); }
+            const uint32_t rnd = p == 1 ? (uint32_t)vcg_im_rnd(1, 0) : 8192u;
+            for (int l = 0; l < 64; ++l) for (int r = 0; r < 16; ++r) D[l][r] = (int32_t)(((uint32_t)D[l][r] << 6) + rnd);
+            if (p == 1) mfma_cpu(lo, F[1], D); else mfma_cpu(F[1], lo, D);
+            for (int l = 0; l < 64; ++l) {
+                if (p == 1) {
+                    for (int qq = 0; qq < 4; ++qq) {
+                        int o[4];
+                        for (int i = 0; i < 4; ++i) o[i] = D[l][4 * qq + i] >> 14;
+                        uint32_t w[2];
+                        vcg_im_pack16(o, w);
+                        memcpy(tile + vcg_im_out_addr(1, B, l & 31, l >> 5, qq), w, 8);
+                    }
+                } else {
+                    for (int r = 0; r < 16; ++r) t[vcg_tidx(vcg_fwd1_k(r, l >> 5), B, l & 31)] = D[l][r] >> 14;
+                }
             }
         }
-    for (int tid = 0; tid < NT; ++tid) vcg_fwd_pass_y(t, cmat, tid, NT);
-    for (int tid = 0; tid < NT; ++tid) vcg_fwd_pass_z(t, cmat, tid, NT);
+    }
     if (coef_raster)
         for (int k = 0; k < 32768; ++k) coef_raster[k] = t[vcg_tidx(k >> 10, (k >> 5) & 31, k & 31)];
     int last = 0;
-    for (int tid = 0; tid < NT; ++tid) { int l = vcg_quant_scan(t, scan, gstep, q, lv, tid, NT); if (l > last) last = l; }
+    float step[VCG_NFREQ], dz[VCG_NFREQ];
+    vcg_quant_tables(gstep, q, step, dz, 0, 1);
+    for (int tid = 0; tid < NT; ++tid) { int l = vcg_quant_scan(t, scan, step, dz, lv, tid, NT); if (l > last) last = l; }
     *eob = (uint32_t)last; *dc_out = (uint32_t)dc;
     free(t);
     return 0;

@@ -852,13 +852,19 @@ VCG_HD uint32_t vcg_rc_encode_atom_fsm(const int16_t *L, uint32_t eob, const uin
 
 // dead-zone quantizer, evaluated in source order with no contraction
 // (step = base_q*g; dz = 0.8f*step; level = (int)((a-dz)/step+1)).
-VCG_HD int vcg_quant1(int c, float base_q, float g) {
+// with the step and dead zone of the coefficient's frequency precomputed
+// (step = base_q * g, dz = 0.80f * step, as below)
+VCG_HD int vcg_quant1s(int c, float step, float dz) {
 #pragma clang fp contract(off)
-    float step = base_q * g;
-    float dz = 0.80f * step;
     float a = (float)(c < 0 ? -c : c);
     int level = 0;
     if (a >= dz) level = (int)((a - dz) / step + 1.0f);
     if (level > 32767) level = 32767;
     return c < 0 ? -level : level;
+}
+VCG_HD int vcg_quant1(int c, float base_q, float g) {
+#pragma clang fp contract(off)
+    float step = base_q * g;
+    float dz = 0.80f * step;
+    return vcg_quant1s(c, step, dz);
 }

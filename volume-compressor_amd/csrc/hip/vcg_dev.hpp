@@ -19,7 +19,7 @@ struct vcg_dtab {
     uint32_t band_start[9];
     uint16_t scan[VCG_A3];
     uint8_t  fscan[VCG_A3];   // frequency cz+cy+cx of scan position p (non-decreasing in p)
-    int8_t   fa[2][64][16];   // forward pass 1 on MFMA: per-lane basis planes (vcg_fwd1_tables)
+    int8_t   fa[3][64][16];   // forward transform on MFMA: per-lane basis planes Ch, Cl, 4 Cl (vcg_fwd1_tables)
     int32_t  rowsum[32];      // basis row sums
     int8_t   ib[3][64][16];   // inverse on MFMA: per-lane basis planes Ch, 4 Cl, Cl (vcg_im_tables)
 };

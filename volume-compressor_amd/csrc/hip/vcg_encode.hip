@@ -5,10 +5,10 @@
 //   k_fdct       : one 512-thread workgroup per atom, atoms of one or many
 //                  volumes per launch.  Gathers the 32^3 atom (zero padded),
 //                  sums it (zero atom -> no payload, dc = (sum+16384)/32768,
-//                  src/vc/vc.c:599-604), runs forward pass 1 (along x) as
-//                  exact int8 MFMA products straight from the voxel bytes,
-//                  passes 2 and 3 as partial butterflies in LDS
-//                  (src/vc/vc.c:142-171, 212-218) and either stores raster
+//                  src/vc/vc.c:599-604), runs the three forward passes as
+//                  exact int8 MFMA products (pass 1 straight from the voxel
+//                  bytes, passes 2 and 3 over an i16 tile in LDS;
+//                  src/vc/vc.c:142-171, 212-218) and either stores raster
 //                  coefficients (rate-control sample) or quantizes them
 //                  (src/vc/vc.c:293-306, strict IEEE in source order) and
 //                  stores the levels in scan order plus the EOB.
@@ -228,6 +228,7 @@ __global__ __launch_bounds__(FDCT_TX) void k_fdct(const enc_src *__restrict__ sr
     extern __shared__ int buf[];
     __shared__ uint32_t ssum;
     __shared__ int seob;
+    __shared__ float sstep[VCG_NFREQ], sdz[VCG_NFREQ];    // the quantizer's step and dead zone per frequency
     const int tid = threadIdx.x;
     const uint32_t i = blockIdx.x;
     const uint64_t flat = base + i;
@@ -239,10 +240,13 @@ __global__ __launch_bounds__(FDCT_TX) void k_fdct(const enc_src *__restrict__ sr
     enc_atom_of(s, store ? s.list : nullptr, gi, lod, ax, ay, az);
     enc_item *items = store ? s.sitems + gi : items_out + i;
     if (tid == 0) { ssum = 0; seob = 0; }
+    if (!store) vcg_quant_tables(T->gstep, s.q[lod], sstep, sdz, tid, FDCT_TX);
     __syncthreads();
     // pass 1 on the matrix cores (vcg_xform.hpp): wave w takes z-planes w,
     // w+8, w+16, w+24; lane (y, h) loads the 16 voxels x = 16h.. of row (z, y)
-    // of each, which also give the atom's sum (and so dc) before any product
+    // of each, which also give the atom's sum (and so dc) before any product.
+    // In the products below y = lane mod 32 is also the output column (kx in
+    // pass 1, ky in pass 2, kx in pass 3).
     const int wv = tid >> 6, ln = tid & 63, y = ln & 31, hh = ln >> 5;
     v4i raw[4];
     uint32_t part = 0;
@@ -264,36 +268,103 @@ __global__ __launch_bounds__(FDCT_TX) void k_fdct(const enc_src *__restrict__ sr
         return;
     }
     const int dc = (int)((sum + 16384u) / 32768u);
+    // the forward passes on the matrix cores (vcg_xform.hpp): passes 1 and 2
+    // into the i16 tile (the first 64 KiB of buf), pass 3 into the i32 tile
+    uint8_t *tile = (uint8_t *)buf;
+    const v4i bh = *(const v4i *)T->fa[0][ln], bl = *(const v4i *)T->fa[1][ln], b4l = *(const v4i *)T->fa[2][ln];
     {
-        const v4i ah = *(const v4i *)T->fa[0][ln], al = *(const v4i *)T->fa[1][ln];
-        int corr[16];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) corr[r] = (128 - dc) * T->rowsum[vcg_fwd1_k(r, hh)] + 8192;
+        const int corr = vcg_fwd1_corr(dc, T->rowsum[y]);      // output column kx = lane mod 32
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const v16i zero = {};
-            const v16i dh = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, raw[b], zero, 0, 0, 0);
-            const v16i dl = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, raw[b], zero, 0, 0, 0);
-            const int z = wv + 8 * b;
+            v16i t = __builtin_amdgcn_mfma_i32_32x32x32_i8(raw[b], bh, zero, 0, 0, 0);
 #pragma unroll
-            for (int r = 0; r < 16; ++r) buf[vcg_tidx(z, y, vcg_fwd1_k(r, hh))] = vcg_fwd1_out(dh[r], dl[r], corr[r]);
+            for (int r = 0; r < 16; ++r) t[r] = (int)(((uint32_t)t[r] << 6) + (uint32_t)corr);
+            t = __builtin_amdgcn_mfma_i32_32x32x32_i8(raw[b], bl, t, 0, 0, 0);
+            int o[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) o[r] = t[r] >> 14;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t w[2];
+                vcg_im_pack16(o + 4 * q, w);
+                *(uint2 *)(tile + vcg_im_out_addr(0, wv + 8 * b, y, hh, q)) = make_uint2(w[0], w[1]);
+            }
         }
     }
     __syncthreads();
-    vcg_fwd_pass_y(buf, T->cmat, tid, FDCT_TX);
+    {   // pass 2: every wave reads its blocks before any writes
+        uint32_t d[4][8];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) vcg_im_load(tile, 1, wv + 8 * b, y, hh, d[b]);
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            uint32_t hi[4], lo[4];
+            vcg_im_planes(d[b], hi, lo);
+            const v4i ah = {(int)hi[0], (int)hi[1], (int)hi[2], (int)hi[3]};
+            const v4i al = {(int)lo[0], (int)lo[1], (int)lo[2], (int)lo[3]};
+            const v16i zero = {};
+            v16i t = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bh, zero, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) t[r] = (int)((uint32_t)t[r] << 8);
+            t = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, b4l, t, 0, 0, 0);
+            t = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh, t, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) t[r] = (int)(((uint32_t)t[r] << 6) + (uint32_t)vcg_im_rnd(1, 0));
+            t = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl, t, 0, 0, 0);
+            int o[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) o[r] = t[r] >> 14;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t w[2];
+                vcg_im_pack16(o + 4 * q, w);
+                *(uint2 *)(tile + vcg_im_out_addr(1, wv + 8 * b, y, hh, q)) = make_uint2(w[0], w[1]);
+            }
+        }
+    }
     __syncthreads();
-    vcg_fwd_pass_z(buf, T->cmat, tid, FDCT_TX);
+    {   // pass 3: basis as A (rows kz), data line (ky, kx) as B (column kx)
+        uint32_t d[4][8];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) vcg_im_load(tile, 2, wv + 8 * b, y, hh, d[b]);
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            uint32_t hi[4], lo[4];
+            vcg_im_planes(d[b], hi, lo);
+            const v4i ah = {(int)hi[0], (int)hi[1], (int)hi[2], (int)hi[3]};
+            const v4i al = {(int)lo[0], (int)lo[1], (int)lo[2], (int)lo[3]};
+            const v16i zero = {};
+            v16i t = __builtin_amdgcn_mfma_i32_32x32x32_i8(bh, ah, zero, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) t[r] = (int)((uint32_t)t[r] << 8);
+            t = __builtin_amdgcn_mfma_i32_32x32x32_i8(b4l, ah, t, 0, 0, 0);
+            t = __builtin_amdgcn_mfma_i32_32x32x32_i8(bh, al, t, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) t[r] = (int)(((uint32_t)t[r] << 6) + 8192u);
+            t = __builtin_amdgcn_mfma_i32_32x32x32_i8(bl, al, t, 0, 0, 0);
+            const int ky = wv + 8 * b;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) buf[vcg_tidx(vcg_fwd1_k(r, hh), ky, y)] = t[r] >> 14;
+        }
+    }
     __syncthreads();
     if (store) {
         int32_t *co = s.coef + gi * VCG_A3;
-        for (int p = tid; p < 32768; p += FDCT_TX) {
-            const uint32_t idx = T->scan[p];
-            co[p] = buf[vcg_tidx((int)(idx >> 10), (int)((idx >> 5) & 31), (int)(idx & 31))];
+        for (int p0 = tid; p0 < 32768; p0 += 8 * FDCT_TX) {
+            uint32_t idx[8];                 // eight scan-table loads in flight
+#pragma unroll
+            for (int k = 0; k < 8; ++k) idx[k] = T->scan[p0 + k * FDCT_TX];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                co[p0 + k * FDCT_TX] = buf[vcg_tidx((int)(idx[k] >> 10), (int)((idx[k] >> 5) & 31), (int)(idx[k] & 31))];
         }
         if (tid == 0) { enc_item it = {0, (uint32_t)dc, 0, (uint32_t)lod}; *items = it; }
         return;
     }
-    const int last = vcg_quant_scan(buf, T->scan, T->gstep, s.q[lod], lv_out + (size_t)i * VCG_A3, tid, FDCT_TX);
+    const int last = vcg_quant_scan(buf, T->scan, sstep, sdz, lv_out + (size_t)i * VCG_A3, tid, FDCT_TX);
     if (last) atomicMax(&seob, last);
     __syncthreads();
     if (tid == 0) { enc_item it = {(uint32_t)seob, (uint32_t)dc, 0, (uint32_t)lod}; *items = it; }

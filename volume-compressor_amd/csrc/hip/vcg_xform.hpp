@@ -281,165 +281,39 @@ VCG_HD void vcg_inv_stage(uint8_t *stage, const uint8_t *vox, int tid) {
 // ===========================================================================
 // forward
 // ===========================================================================
-// Two forward lines (element n of line i at s_i[n*stride]), in place:
-//   s[n] = v[n]+v[31-n], d[n] = v[n]-v[31-n]  (n < 16)
-//   out[2j] = (sum_n C[2j][n] s[n] + 8192) >> 14, out[2j+1] likewise with d
-// (src/vc/vc.c:148-169), i32 wrap.  The even outputs are regrouped exactly
-// (integer sums, mod 2^32) by the even-row symmetries of the basis,
-// C[2j][15-n] = (-1)^j C[2j][n], C[4m][7-n] = (-1)^m C[4m][n], C[8p][3-n] =
-// (-1)^p C[8p][n] (checked on the table at device init), as the inverse does:
-// 344 multiply-adds per line instead of 512.  Operands stay below 2^23 in
-// magnitude (|v| <= 255 in pass 1, <= 2039 in pass 2, <= 16293 in pass 3, and
-// the regrouped sums at most 16 times that), so 24-bit multiplies are exact.
-// Output rows go four at a time: the two lines share each basis row (a
-// workgroup-uniform scalar load), and the eight sums are independent chains.
-VCG_HD void vcg_fwd_line2(const int32_t (*C)[32], int *s0, int *s1, int stride) {
-    struct row16 { int32_t v[16]; };
-    struct row8 { int32_t v[8]; };
-    struct row4 { int32_t v[4]; };
-    struct row2 { int32_t v[2]; };
-    int d0[16], d1[16], sd0[8], sd1[8], t0[4], t1[4], u0[4], u1[4];
-    {
-        int a0[16], a1[16];
-#pragma unroll
-        for (int n = 0; n < 16; ++n) {
-            const int p0 = s0[n * stride], q0 = s0[(31 - n) * stride];
-            const int p1 = s1[n * stride], q1 = s1[(31 - n) * stride];
-            a0[n] = p0 + q0; d0[n] = p0 - q0;
-            a1[n] = p1 + q1; d1[n] = p1 - q1;
-        }
-        int b0[8], b1[8];
-#pragma unroll
-        for (int n = 0; n < 8; ++n) {
-            b0[n] = a0[n] + a0[15 - n]; sd0[n] = a0[n] - a0[15 - n];       // sd: rows 4m+2
-            b1[n] = a1[n] + a1[15 - n]; sd1[n] = a1[n] - a1[15 - n];
-        }
-        int c0[4], c1[4];
-#pragma unroll
-        for (int n = 0; n < 4; ++n) {
-            c0[n] = b0[n] + b0[7 - n]; t0[n] = b0[n] - b0[7 - n];           // t: rows 8p+4
-            c1[n] = b1[n] + b1[7 - n]; t1[n] = b1[n] - b1[7 - n];
-        }
-#pragma unroll
-        for (int n = 0; n < 2; ++n) {
-            u0[n] = c0[n] + c0[3 - n]; u0[2 + n] = c0[n] - c0[3 - n];       // u: rows 16q | 16q+8
-            u1[n] = c1[n] + c1[3 - n]; u1[2 + n] = c1[n] - c1[3 - n];
-        }
-    }
-    // odd rows 2j+1: sum_{n<16} C[2j+1][n] d[n]
-#pragma unroll 1
-    for (int j0 = 0; j0 < 16; j0 += 4) {
-        int a0[4], a1[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const row16 r = *(const row16 *)C[2 * (j0 + q) + 1];
-            a0[q] = 8192; a1[q] = 8192;
-#pragma unroll
-            for (int n = 0; n < 16; ++n) {
-                a0[q] = vcg_acc<true>(a0[q], r.v[n], d0[n]);
-                a1[q] = vcg_acc<true>(a1[q], r.v[n], d1[n]);
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            s0[(2 * (j0 + q) + 1) * stride] = a0[q] >> 14;
-            s1[(2 * (j0 + q) + 1) * stride] = a1[q] >> 14;
-        }
-    }
-    // rows 4m+2: sum_{n<8} C[4m+2][n] sd[n]
-#pragma unroll 1
-    for (int m0 = 0; m0 < 8; m0 += 4) {
-        int a0[4], a1[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const row8 r = *(const row8 *)C[4 * (m0 + q) + 2];
-            a0[q] = 8192; a1[q] = 8192;
-#pragma unroll
-            for (int n = 0; n < 8; ++n) {
-                a0[q] = vcg_acc<true>(a0[q], r.v[n], sd0[n]);
-                a1[q] = vcg_acc<true>(a1[q], r.v[n], sd1[n]);
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            s0[(4 * (m0 + q) + 2) * stride] = a0[q] >> 14;
-            s1[(4 * (m0 + q) + 2) * stride] = a1[q] >> 14;
-        }
-    }
-    // rows 8p+4 from t (n < 4); rows 16q and 16q+8 from u (n < 2)
-    {
-        int a0[4], a1[4], g0[4], g1[4];
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            const row4 r = *(const row4 *)C[8 * p + 4];
-            a0[p] = 8192; a1[p] = 8192;
-#pragma unroll
-            for (int n = 0; n < 4; ++n) {
-                a0[p] = vcg_acc<true>(a0[p], r.v[n], t0[n]);
-                a1[p] = vcg_acc<true>(a1[p], r.v[n], t1[n]);
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {            // q < 2: row 16q from u[0..1]; else row 16(q-2)+8 from u[2..3]
-            const int k = q < 2 ? 16 * q : 16 * (q - 2) + 8, o = q < 2 ? 0 : 2;
-            const row2 r = *(const row2 *)C[k];
-            g0[q] = 8192; g1[q] = 8192;
-#pragma unroll
-            for (int n = 0; n < 2; ++n) {
-                g0[q] = vcg_acc<true>(g0[q], r.v[n], u0[o + n]);
-                g1[q] = vcg_acc<true>(g1[q], r.v[n], u1[o + n]);
-            }
-        }
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            s0[(8 * p + 4) * stride] = a0[p] >> 14;
-            s1[(8 * p + 4) * stride] = a1[p] >> 14;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int k = q < 2 ? 16 * q : 16 * (q - 2) + 8;
-            s0[k * stride] = g0[q] >> 14;
-            s1[k * stride] = g1[q] >> 14;
-        }
-    }
-}
-
-// Passes 2 and 3 (pass 1 runs on the matrix cores, below) give each thread
-// the line pairs (line, line + nthr) for line = tid, tid + 2*nthr, ... (nthr
-// divides 512).
-VCG_HD void vcg_fwd_pass_y(int *t, const int32_t (*C)[32], int tid, int nthr) {
-    for (int l = tid; l < 1024; l += 2 * nthr)
-        vcg_fwd_line2(C, t + vcg_tidx(l >> 5, 0, l & 31), t + vcg_tidx((l + nthr) >> 5, 0, (l + nthr) & 31), VCG_SY);
-}
-VCG_HD void vcg_fwd_pass_z(int *t, const int32_t (*C)[32], int tid, int nthr) {
-    for (int l = tid; l < 1024; l += 2 * nthr)
-        vcg_fwd_line2(C, t + vcg_tidx(0, l >> 5, l & 31), t + vcg_tidx(0, (l + nthr) >> 5, (l + nthr) & 31), VCG_SZ);
-}
-
 // ---------------------------------------------------------------------------
-// forward pass 1 on the matrix cores (v_mfma_i32_32x32x32_i8)
+// forward transform on the matrix cores (v_mfma_i32_32x32x32_i8)
 // ---------------------------------------------------------------------------
-// Pass 1 (along x) of the forward transform, out[k] = (sum_n C[k][n] (v[n] -
-// dc) + 8192) >> 14, is computed exactly as an int8 matrix product:
-//   v - dc = (v - 128) + (128 - dc), and v - 128 is int8 (the voxel's byte
-//   with its top bit flipped), so  sum_n C[k][n](v[n]-dc)
-//     = sum_n C[k][n](v[n]-128) + (128-dc) rowsum[k];
-//   |C| <= 4091 is not int8, but C = 64 Ch + Cl with Cl = ((C+32) mod 64) -
-//   32 in [-32, 31] and Ch in [-64, 64] is: two int8 products, i32 sums.
-// All terms stay far inside i32 (|sum| <= 32 * 4091 * 128 plus 127 * 92672),
-// so the result equals the reference's butterfly (src/vc/vc.c:142-171),
-// whose regrouping is exact, bit for bit.
+// Each forward pass is, per line, out[k] = (sum_n C[k][n] v[n] + 8192) >> 14
+// (src/vc/vc.c:142-171, whose even/odd regrouping only reorders the integer
+// sum), computed as exact int8 products over the inverse's i16 tile layout
+// (vcg_im_chunk, "inverse on the matrix cores"):
+//   - pass 1 (x) takes the voxels as bytes: v - dc = (v - 128) + (128 - dc),
+//     v - 128 is int8 (the byte with its top bit flipped), and
+//     sum_n C[k][n] (v[n] - dc) = sum_n C[k][n] (v[n] - 128) + (128 - dc)
+//     rowsum[k]; with C = 64 Ch + Cl two products: T = (v-128)*Ch; T = (T <<
+//     6) + corr; T += (v-128)*Cl; out = T >> 14;
+//   - passes 2 (y) and 3 (z) take w = v + 128 from the tile in two byte
+//     planes and run the inverse's four-product chain with the basis rows
+//     C[k][.] in place of the columns.
+// Every pass input fits the tile: a pass grows magnitudes by at most
+// max_k sum_n |C[k][n]| / 2^14 = 92672 / 16384 < 5.66, so |v| <= 1443 into
+// pass 2 and <= 8164 into pass 3 (|voxel - dc| <= 255); pass 3 writes i32
+// coefficients.
 //
-// One 32x32x32 product covers a z-plane: 32 output frequencies k (rows) by
-// 32 lines y (columns), K = the 32 voxels n of a line.  Lane l = j + 32 h of
-// a wave supplies 16 bytes of each operand, byte e pairing with byte e of
-// the other operand's lane of the same h: A = Ch or Cl row k = j, n = 16h +
-// e; B = line y = j, n = 16h + e, i.e. 16 consecutive voxels of the row,
-// one 16-byte load.  Output register r of lane l is D[k][y] with y = l mod
-// 32 and k = 8 (r / 4) + 4 (l / 32) + r mod 4.
+// Per-lane basis operand: lane c + 32 h, byte e = plane of C[c][16h + e]
+// (planes Ch, Cl, 4 Cl); as an A operand it supplies row c, as a B operand
+// column c.  Products (a wave takes blocks wv, wv + 8, wv + 16, wv + 24):
+//   x: block = plane z, A = voxel row y (16 voxels x = 16h.. per lane, one
+//      16-byte load), B = basis: D[y][kx] -> tile [z][kx][y];
+//   y: blocks and rows as the inverse's pass y (rows (z = 4m + j % 4, kx =
+//      8p + j / 4)), A = data, B = basis: D[row][ky] -> tile [ky][kx][z];
+//   z: block = plane ky, A = basis, B = data line (ky, kx = col):
+//      D[kz][kx] -> i32 tile (vcg_tidx) at (kz, ky, kx).
 
-// per-lane A operands (the two basis planes) and row sums, built once
-VCG_HD void vcg_fwd1_tables(const int32_t (*C)[32], int8_t fa[2][64][16], int32_t rowsum[32]) {
+// output register r of lane half h: row 8 (r / 4) + 4 h + r % 4
+// per-lane basis operands (planes Ch, Cl, 4 Cl) and row sums, built once
+VCG_HD void vcg_fwd1_tables(const int32_t (*C)[32], int8_t fa[3][64][16], int32_t rowsum[32]) {
     for (int k = 0; k < 32; ++k) {
         int32_t s = 0;
         for (int n = 0; n < 32; ++n) s += C[k][n];
@@ -451,6 +325,7 @@ VCG_HD void vcg_fwd1_tables(const int32_t (*C)[32], int8_t fa[2][64][16], int32_
             const int lo = ((c + 32) & 63) - 32;
             fa[0][l][e] = (int8_t)((c - lo) / 64);
             fa[1][l][e] = (int8_t)lo;
+            fa[2][l][e] = (int8_t)(4 * lo);
         }
 }
 
@@ -474,9 +349,8 @@ VCG_HD void vcg_fwd1_row(const uint8_t *vol, uint32_t nx, uint32_t ny, uint32_t 
 // output register r of lane half h: frequency row k
 VCG_HD int vcg_fwd1_k(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
 
-// combine the two plane products of output (k, y) into the pass-1 value;
-// corr = (128 - dc) rowsum[k] + 8192
-VCG_HD int vcg_fwd1_out(int dh, int dl, int corr) { return (dh * 64 + dl + corr) >> 14; }
+// pass-1 rounding term of output column k (the tile stores w = out + 128)
+VCG_HD int vcg_fwd1_corr(int dc, int32_t rowsum_k) { return (128 - dc) * rowsum_k + 8192 + (128 << 14); }
 
 // ===========================================================================
 // inverse on the matrix cores (i16 tile)
@@ -658,17 +532,38 @@ VCG_HD uint32_t vcg_im_pack8(const int o[4]) {
     return vcg_perm(p23, p01, 0x05040100u);
 }
 
+#ifndef VCG_NFREQ
+#define VCG_NFREQ 94                            // distinct frequencies cz + cy + cx
+#endif
 // quantize the tile's coefficients in scan order into lv[p]; returns this
 // thread's max (p+1) over nonzero levels (the EOB candidate)
-VCG_HD int vcg_quant_scan(const int *t, const uint16_t *scan, const float *gstep, float q, int16_t *lv, int tid,
-                          int nthr) {
+VCG_HD int vcg_quant_scan(const int *t, const uint16_t *scan, const float *step, const float *dz, int16_t *lv,
+                          int tid, int nthr) {
     int last = 0;
-    for (int p = tid; p < 32768; p += nthr) {
-        const uint32_t idx = scan[p];
-        const int c = t[vcg_tidx((int)(idx >> 10), (int)((idx >> 5) & 31), (int)(idx & 31))];
-        const int l = vcg_quant1(c, q, gstep[vcg_freq(idx)]);
-        lv[p] = (int16_t)l;
-        if (l) last = p + 1;
+    const int un = nthr;
+    for (int p0 = tid; p0 < 32768; p0 += 8 * un) {
+        uint32_t idx[8];                       // eight scan-table loads in flight
+#pragma unroll
+        for (int k = 0; k < 8; ++k) idx[k] = p0 + k * un < 32768 ? scan[p0 + k * un] : 0u;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int p = p0 + k * un;
+            if (p >= 32768) break;
+            const uint32_t id = idx[k];
+            const int f = vcg_freq(id);
+            const int l = vcg_quant1s(t[vcg_tidx((int)(id >> 10), (int)((id >> 5) & 31), (int)(id & 31))], step[f], dz[f]);
+            lv[p] = (int16_t)l;
+            if (l) last = p + 1;
+        }
     }
     return last;
+}
+// the step and dead zone of each frequency at base_q (vcg_quant1's own float
+// operations, so every level is unchanged)
+VCG_HD void vcg_quant_tables(const float *gstep, float q, float *step, float *dz, int tid, int nthr) {
+#pragma clang fp contract(off)
+    for (int f = tid; f < VCG_NFREQ; f += nthr) {
+        step[f] = q * gstep[f];
+        dz[f] = 0.80f * step[f];
+    }
 }
