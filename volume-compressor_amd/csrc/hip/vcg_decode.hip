@@ -668,6 +668,23 @@ static const uint64_t VCG_HOST_SLAB = 1ull << 30;   // host-destination staging 
 
 static const size_t LDS_M_BYTES = VCG_IM_TILE_BYTES;
 
+// k_entropy's launches carry unused dynamic LDS so that its workgroup holds
+// 84 KiB: more than half of a CU's 160 KiB, so a CU takes at most one entropy
+// workgroup (one wave per SIMD), and still room for one matrix-core transform
+// workgroup (64 KiB tile + tables) beside it.  Entropy launches of
+// concurrent decodes then spread over disjoint CUs instead of stacking two
+// waves on some SIMDs and leaving others idle (back-to-back LOD0 decode of
+// 1024^3 at ratio 10: 5.12 -> 4.33 ms per decode; a lone launch is
+// unchanged, it never places two workgroups on a CU).  The server's
+// four-wave persistent entropy launches carry the same padding (its
+// one- and two-wave shapes do not: they serve small batches).
+static const size_t ENT_LDS_STATIC = (size_t)ENT_WAVES * (VCG_CTX_SLOTS + VCG_RING) * 64u * 4u + 9u * 4u;
+static const size_t ENT_LDS_TOTAL = 84u * 1024u;
+static const size_t ENT_LDS_PAD = ENT_LDS_TOTAL > ENT_LDS_STATIC ? ENT_LDS_TOTAL - ENT_LDS_STATIC : 0;
+static_assert(2 * ENT_LDS_TOTAL > 160u * 1024u, "two entropy workgroups must not fit one CU");
+static_assert(ENT_LDS_TOTAL + VCG_IM_TILE_BYTES + 2048u <= 160u * 1024u,
+              "an entropy and a transform workgroup must fit one CU");
+
 static void ensure_attr() {
     static std::once_flag once;
     std::call_once(once, [] {
@@ -678,6 +695,10 @@ static void ensure_attr() {
         const void *mk[] = {(const void *)k_idct_m, (const void *)k_idct_mp};
         for (const void *k : mk)
             VCG_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_M_BYTES));
+        VCG_CHECK(hipFuncSetAttribute((const void *)k_entropy, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)ENT_LDS_PAD));
+        VCG_CHECK(hipFuncSetAttribute((const void *)k_entropy_p<4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)ENT_LDS_PAD));
     });
 }
 
@@ -780,8 +801,8 @@ static int run_batch(vcg_ctx *c, dec_job j, uint32_t total, bool inputs_on_s, co
         if (up_bytes && sl != first) VCG_CHECK(hipStreamWaitEvent(sE, c->evI[first], 0));
         scratch_blk *blk = blk_take(sE, maxb);
         if (c->ev[0]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[0], sE));
-        hipLaunchKernelGGL(k_entropy, dim3((j.n + ENT_THREADS - 1) / ENT_THREADS), dim3(ENT_THREADS), 0, sE, j, g_dtab,
-                           blk->lists, blk->items, c->status_d);
+        hipLaunchKernelGGL(k_entropy, dim3((j.n + ENT_THREADS - 1) / ENT_THREADS), dim3(ENT_THREADS), ENT_LDS_PAD, sE,
+                           j, g_dtab, blk->lists, blk->items, c->status_d);
         VCG_CHECK(hipGetLastError());
         if (c->ev[1]) VCG_CHECK(hipEventRecord((hipEvent_t)c->ev[1], sE));
         // transform: after its entropy pass (same stream) and earlier work on c->s
@@ -1005,8 +1026,8 @@ extern "C" int vcg_list_launch_noalloc(vcg_arc *a, size_t n, const int32_t *quad
     uint32_t *word = (uint32_t *)base;
     vcg_item *items = (vcg_item *)(base + NA_ITEMS_OFF);
     uint32_t *lists = (uint32_t *)(base + NA_LISTS_OFF);
-    hipLaunchKernelGGL(k_entropy, dim3((j.n + ENT_THREADS - 1) / ENT_THREADS), dim3(ENT_THREADS), 0, s, j, g_dtab,
-                       lists, items, word);
+    hipLaunchKernelGGL(k_entropy, dim3((j.n + ENT_THREADS - 1) / ENT_THREADS), dim3(ENT_THREADS), ENT_LDS_PAD, s, j,
+                       g_dtab, lists, items, word);
     if (hipGetLastError() != hipSuccess) return -1;
     launch_idct(j, lists, items, s);
     if (hipGetLastError() != hipSuccess) return -1;
@@ -1068,7 +1089,8 @@ extern "C" vcg_qgraph *vcg_qgraph_create(vcg_arc *a, int slot, const vcg_qcfg *c
     else if (W == 2)
         hipLaunchKernelGGL(k_entropy_p<2>, ge, dim3(128), 0, g->s, j, g_dtab, g->lists, g->items, c->status_d, hq);
     else
-        hipLaunchKernelGGL(k_entropy_p<4>, ge, dim3(256), 0, g->s, j, g_dtab, g->lists, g->items, c->status_d, hq);
+        hipLaunchKernelGGL(k_entropy_p<4>, ge, dim3(256), ENT_LDS_PAD, g->s, j, g_dtab, g->lists, g->items,
+                           c->status_d, hq);
     launch_idct_p(j, cfg->idct_groups, cap, g->lists, g->items, hq, g->s);
     VCG_CHECK(hipStreamEndCapture(g->s, &graph));
     VCG_CHECK(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0));
@@ -1093,7 +1115,8 @@ extern "C" int vcg_qgraph_launch(vcg_qgraph *g, const uint8_t *qpin, size_t n, s
         else if (g->cfg.ent_waves == 2)
             hipLaunchKernelGGL(k_entropy_p<2>, ge, dim3(128), 0, g->s, g->j, g_dtab, g->lists, g->items, g->status, hq);
         else
-            hipLaunchKernelGGL(k_entropy_p<4>, ge, dim3(256), 0, g->s, g->j, g_dtab, g->lists, g->items, g->status, hq);
+            hipLaunchKernelGGL(k_entropy_p<4>, ge, dim3(256), ENT_LDS_PAD, g->s, g->j, g_dtab, g->lists, g->items,
+                               g->status, hq);
         launch_idct_p(g->j, g->cfg.idct_groups, g->cap, g->lists, g->items, hq, g->s);
         VCG_CHECK(hipGetLastError());
         return 0;
