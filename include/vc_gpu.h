@@ -142,6 +142,12 @@ vc_status vcx_synth_device(uint64_t seed, vc_dims dims, int nsheets, void *dev_o
  * NaN alpha. */
 enum { VCX_DS_BOX = 0, VCX_DS_CBOX = 1 };
 vc_status vcx_downscale2x_device(const void *dev_in, vc_dims in, int method, float alpha, void *dev_out, int sync);
+/* Test hook, no counterpart in the reference: nonzero makes the 2x kernels
+ * (this downscale, vc_test_downsample2x and the encoder's pyramid) index
+ * their output with 64-bit integers, the path they otherwise take only past
+ * about 4.27e9 output voxels, so small volumes exercise it.  Process-wide;
+ * 0 restores the default. */
+void vcx_test_wide_index(int on);
 
 /* ---- the exporter's pyramid cascade ----
  * Replaces the compute path of tools/vc_export_stream.c (a command-line tool

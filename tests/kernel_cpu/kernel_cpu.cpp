@@ -44,6 +44,78 @@ int kc_decode_fsm(const uint8_t *s, uint32_t n, uint32_t mis, const uint32_t *ba
     free(buf);
     return r;
 }
+// One wavefront of k_entropy, its lanes in lockstep as on the GPU: lane k
+// decodes the range-coder bytes [offs[k] + 1, offs[k] + lens[k]) of buf, a
+// padded archive-like pool where the bytes past a stream belong to the next
+// payload.  Every lane, parked or not, runs every refill, flush and block
+// until no lane is active (k_entropy's loop), so what a parked lane does
+// with the bytes past its stream shows.  out: nl lists of 32768 + 64
+// entries; cnt, bad per lane.  variant 1 plants a refill that appends the
+// look-ahead bytes unmasked when no active lane is within 16 bytes of its
+// stream end (a variant once tried on the device): a parked lane then takes
+// in the next payload's bytes and decodes them as bins.
+static void wave_refill(vcg_fsm &f, vcg_look &L, const uint8_t *s, uint32_t n, bool mask) {
+    if (mask) { vcg_fsm_refill(f, L, s, n); return; }
+    const uint32_t o = f.E - L.e < 8u ? f.E - L.e : 8u;
+    const uint32_t a = 4u * o, b = 32u - a;
+    const uint64_t data = ((L.lo >> a) >> a) | ((L.hi << b) << b);
+    f.w |= f.wb < 64u ? data << f.wb : 0ull;
+    f.E += (64u - f.wb) >> 3;
+    f.wb = 64u;
+    vcg_look_load(L, s, n, f.E);
+}
+int kc_decode_wave(const uint8_t *buf, const uint32_t *offs, const uint32_t *lens, int nl,
+                   const uint32_t *band_start, uint32_t *out, uint32_t *cnt, int *bad, int variant) {
+    if (nl < 1 || nl > 64) return -1;
+    const size_t LS = 32768 + 64;
+    vcg_ctx_array ctx[64];
+    vcg_ring_array ring[64];
+    vcg_fsm f[64];
+    vcg_look A[64], B[64];
+    const uint8_t *s[64];
+    uint32_t n[64];
+    // a lane of the planted variant can go on emitting past any real list:
+    // its flushes then go to a sink and the lane reports bad = 3
+    static uint32_t sink[64];
+    int over[64] = {0};
+    auto lane_out = [&](int k) -> uint32_t * {
+        if (f[k].fl + VCG_FLUSH_N > LS) over[k] = 1;
+        if (over[k]) { f[k].fl = 0; return sink; }
+        return out + k * LS;
+    };
+    for (int k = 0; k < nl; ++k) {
+        for (uint32_t c = 0; c < VCG_CTX_SLOTS; ++c) ctx[k].v[c] = c == 13u || c == 15u ? VCG_P_BYPASS : 2048u;
+        for (uint32_t r = 0; r < VCG_RING; ++r) ring[k].v[r] = 0xDEADBEEFu;
+        s[k] = buf + offs[k] + 1;
+        n[k] = lens[k] - 1;
+        vcg_fsm_init(f[k], A[k], B[k], s[k], n[k], band_start);
+    }
+    auto any_active = [&] { for (int k = 0; k < nl; ++k) if (!vcg_fsm_parked(f[k])) return true; return false; };
+    auto any_near = [&] { for (int k = 0; k < nl; ++k) if (vcg_fsm_near(f[k], VCG_REFILL_STEPS)) return true; return false; };
+    auto masked = [&] {
+        if (variant != 1) return true;
+        for (int k = 0; k < nl; ++k)
+            if (!vcg_fsm_parked(f[k]) && f[k].E + 16u > n[k]) return true;
+        return false;
+    };
+    while (any_active()) {
+        bool m = masked();
+        for (int k = 0; k < nl; ++k) wave_refill(f[k], A[k], s[k], n[k], m);
+        for (int k = 0; k < nl; ++k) vcg_fsm_flush(f[k], ring[k], lane_out(k));
+        bool chk = any_near();
+        for (int k = 0; k < nl; ++k) vcg_fsm_block(f[k], ctx[k], ring[k], band_start, chk);
+        m = masked();
+        for (int k = 0; k < nl; ++k) wave_refill(f[k], B[k], s[k], n[k], m);
+        chk = any_near();
+        for (int k = 0; k < nl; ++k) vcg_fsm_block(f[k], ctx[k], ring[k], band_start, chk);
+    }
+    for (int k = 0; k < nl; ++k) {
+        vcg_fsm_flush(f[k], ring[k], lane_out(k));
+        bad[k] = over[k] ? 3 : (int)f[k].bad;
+        cnt[k] = bad[k] ? 0 : f[k].fl;
+    }
+    return 0;
+}
 void kc_quant(const int32_t *c, size_t n, float q, float g, int32_t *out) {
     for (size_t i = 0; i < n; ++i) out[i] = vcg_quant1(c[i], q, g);
 }

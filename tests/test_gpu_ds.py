@@ -37,6 +37,27 @@ def test_ds2x_large_matches_oracle(pyvc, oracle):
             assert st == 0 and np.array_equal(out, oracle.ds2x(vol, m, a)), (dims, m, a)
 
 
+def test_wide_index_paths_vs_oracle(pyvc, oracle):
+    """the 64-bit output-index bodies of the 2x kernels (taken on their own
+    only past ~4.27e9 output voxels), forced on small volumes: the downscale
+    in both thread shapes, the reference's test downsample and a whole encode
+    (its pyramid) against the oracle"""
+    pyvc.lib().vcx_test_wide_index(1)
+    try:
+        for dims in ((514, 300, 131), (256, 96, 64)):
+            vol = pyvc.synth(9, dims, 12)
+            for m, a in ((0, 0.0), (1, 0.5)):
+                st, out = pyvc.downscale2x(vol, m, a)
+                assert st == 0 and np.array_equal(out, oracle.ds2x(vol, m, a)), (dims, m, a)
+            assert np.array_equal(pyvc.test_downsample2x(vol), oracle.downsample2x(vol)), dims
+        vol = pyvc.synth(4, (97, 131, 75), 6)
+        st, arc = pyvc.encode(vol, 10.0)
+        ost, oarc = oracle.encode(vol, 10.0)
+        assert st == 0 and ost == 0 and arc == oarc
+    finally:
+        pyvc.lib().vcx_test_wide_index(0)
+
+
 def test_ds2x_argument_errors(pyvc):
     import ctypes as C
     L = pyvc.lib()

@@ -37,6 +37,7 @@ def kc():
     L.kc_idct.argtypes = [vp, C.c_uint32, C.c_int, vp, vp, vp, vp, C.c_int, vp]
     L.kc_idct_m.argtypes = [vp, C.c_uint32, C.c_int, vp, vp, vp, vp, vp]
     L.kc_quant.argtypes = [vp, C.c_size_t, C.c_float, C.c_float, vp]
+    L.kc_decode_wave.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, vp, C.c_int]
     return L
 
 
@@ -288,6 +289,51 @@ def test_fsm_decoder_matches(kc, tabs, golden_atoms, golden_corrupt):
         for mis in range(8):
             got = fsm_decode(kc, tabs, rc, mis, mis & 1)
             assert got[0] == ref[0] and np.array_equal(got[1], ref[1]), (len(rc), mis)
+
+
+def wave_decode(kc, tabs, pool, offs, lens, variant=0):
+    """k_entropy's loop over one wavefront, lanes in lockstep (kc_decode_wave):
+    per lane (bad, entries)"""
+    cm, gs, scan, bs = tabs
+    nl = len(offs)
+    buf = np.concatenate([np.frombuffer(pool, np.uint8), np.zeros(64, np.uint8)])   # the device copy's padding
+    o, n = np.array(offs, np.uint32), np.array(lens, np.uint32)
+    ent = np.zeros(nl * (32768 + 64), np.uint32)
+    cnt, bad = np.zeros(nl, np.uint32), np.zeros(nl, np.int32)
+    assert kc.kc_decode_wave(buf.ctypes.data, o.ctypes.data, n.ctypes.data, nl, bs.ctypes.data, ent.ctypes.data,
+                             cnt.ctypes.data, bad.ctypes.data, variant) == 0
+    return [(int(bad[k]), ent[k * (32768 + 64):k * (32768 + 64) + int(cnt[k])]) for k in range(nl)]
+
+
+def test_fsm_wave_lockstep(kc, tabs, golden_atoms, golden_corrupt):
+    """Lanes of a wavefront in lockstep over a pool of back-to-back payloads,
+    as k_entropy runs them: a parked lane keeps stepping until the whole wave
+    is done, on the bytes that follow its stream (the next payload's).  Every
+    lane's entries equal the straight-line decoder's.  The refill variant that
+    skips the stream-end mask when no active lane is near its end (it passed
+    the one-lane-at-a-time check, which stops a lane once it parks, and failed
+    on the GPU) fails here: a parked lane appends the next payload's bytes to
+    its empty window, its code turns nonzero and its zero bins start decoding
+    coefficients again."""
+    for g in (golden_atoms, golden_corrupt):
+        pool = g["blob"].tobytes()
+        recs = [(int(o), int(n)) for o, n in zip(g["offs"], g["lens"]) if int(n) > 1]
+        ref = [loop_decode(kc, tabs, pool[o + 1:o + n]) for o, n in recs]
+        planted_differs = False
+        for w in range(0, len(recs), 64):
+            offs = [o for o, _ in recs[w:w + 64]]
+            lens = [n for _, n in recs[w:w + 64]]
+            for variant in (0, 1):
+                got = wave_decode(kc, tabs, pool, offs, lens, variant)
+                for k, (bad, ent) in enumerate(got):
+                    r = ref[w + k]
+                    same = bool(bad) == bool(r[0]) and (bad or np.array_equal(ent, r[1]))
+                    if variant == 0:
+                        assert same, (w + k, bad, r[0])
+                    elif not same:
+                        planted_differs = True
+        if g is golden_atoms:
+            assert planted_differs, "the lockstep harness must catch the unmasked-refill variant"
 
 
 def test_fsm_decoder_level_fields(kc, tabs):

@@ -104,6 +104,10 @@ __device__ __forceinline__ void ds_split(IT o, uint32_t ox, uint32_t oy, uint32_
     y = (uint32_t)(row - (IT)z * oy);
 }
 __host__ __device__ __forceinline__ bool ds_small(uint64_t total) { return total + (1ull << 24) < (1ull << 32); }
+// test hook: nonzero makes the launches below take the 64-bit index path
+// whatever the output count, so small volumes exercise it
+static int g_ds_wide = 0;
+extern "C" void vcg_test_wide_index(int on) { g_ds_wide = on != 0; }
 
 template <typename IT>
 __device__ __forceinline__ void downsample_body(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny,
@@ -129,8 +133,8 @@ __device__ __forceinline__ void downsample_body(const uint8_t *__restrict__ in, 
     }
 }
 __global__ void k_downsample(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
-                             uint8_t *__restrict__ out) {
-    if (ds_small((uint64_t)((nx + 1) / 2) * ((ny + 1) / 2) * ((nz + 1) / 2)))
+                             uint8_t *__restrict__ out, int wide) {
+    if (!wide && ds_small((uint64_t)((nx + 1) / 2) * ((ny + 1) / 2) * ((nz + 1) / 2)))
         downsample_body<uint32_t>(in, nx, ny, nz, out);
     else
         downsample_body<uint64_t>(in, nx, ny, nz, out);
@@ -181,8 +185,8 @@ __device__ __forceinline__ void ds2x_v_body(const uint8_t *__restrict__ in, uint
     }
 }
 __global__ void k_ds2x_v(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz, int method,
-                         float alpha, uint8_t *__restrict__ out) {
-    if (ds_small((uint64_t)(nx / 8) * (ny / 2) * (nz / 2)))
+                         float alpha, uint8_t *__restrict__ out, int wide) {
+    if (!wide && ds_small((uint64_t)(nx / 8) * (ny / 2) * (nz / 2)))
         ds2x_v_body<uint32_t>(in, nx, ny, nz, method, alpha, out);
     else
         ds2x_v_body<uint64_t>(in, nx, ny, nz, method, alpha, out);
@@ -205,8 +209,8 @@ __device__ __forceinline__ void ds2x_body(const uint8_t *__restrict__ in, uint32
     }
 }
 __global__ void k_ds2x(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz, int method, float alpha,
-                       uint8_t *__restrict__ out) {
-    if (ds_small((uint64_t)((nx + 1) / 2) * ((ny + 1) / 2) * ((nz + 1) / 2)))
+                       uint8_t *__restrict__ out, int wide) {
+    if (!wide && ds_small((uint64_t)((nx + 1) / 2) * ((ny + 1) / 2) * ((nz + 1) / 2)))
         ds2x_body<uint32_t>(in, nx, ny, nz, method, alpha, out);
     else
         ds2x_body<uint64_t>(in, nx, ny, nz, method, alpha, out);
@@ -664,7 +668,7 @@ extern "C" int vcg_downsample(const void *in, const uint32_t d[3], void *out) {
     uint64_t blocks = (total + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_downsample, dim3((uint32_t)blocks), dim3(256), 0, c->s, (const uint8_t *)in, d[0], d[1],
-                       d[2], (uint8_t *)out);
+                       d[2], (uint8_t *)out, g_ds_wide);
     VCG_CHECK(hipGetLastError());
     return 0;
 }
@@ -679,10 +683,10 @@ extern "C" int vcg_ds2x(const void *in, const uint32_t d[3], int method, float a
     if (blocks > 65536) blocks = 65536;
     if (vec)
         hipLaunchKernelGGL(k_ds2x_v, dim3((uint32_t)blocks), dim3(256), 0, c->s, (const uint8_t *)in, d[0], d[1],
-                           d[2], method, alpha, (uint8_t *)out);
+                           d[2], method, alpha, (uint8_t *)out, g_ds_wide);
     else
         hipLaunchKernelGGL(k_ds2x, dim3((uint32_t)blocks), dim3(256), 0, c->s, (const uint8_t *)in, d[0], d[1], d[2],
-                           method, alpha, (uint8_t *)out);
+                           method, alpha, (uint8_t *)out, g_ds_wide);
     VCG_CHECK(hipGetLastError());
     if (sync) VCG_CHECK(hipStreamSynchronize(c->s));
     return 0;

@@ -195,6 +195,7 @@ extern "C" void vcg_event_record(void *e, int slot) {
     vcg_ctx *c = vcg_get_ctx();
     VCG_CHECK(hipEventRecord((hipEvent_t)e, slot_stream(c, slot)));
 }
+extern "C" void vcg_event_sync(void *e) { VCG_CHECK(hipEventSynchronize((hipEvent_t)e)); }
 extern "C" int vcg_event_done(void *e) {
     const hipError_t r = hipEventQuery((hipEvent_t)e);
     if (r == hipSuccess) return 1;

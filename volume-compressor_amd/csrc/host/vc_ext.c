@@ -93,6 +93,8 @@ vc_status vcx_downscale2x_device(const void *dev_in, vc_dims in, int method, flo
     return VC_OK;
 }
 
+void vcx_test_wide_index(int on) { vcg_test_wide_index(on); }
+
 /* ---- test hooks (src/vc/vc.c:1557-1591), run on the GPU ---- */
 uint8_t *vc_test_downsample2x(const uint8_t *vol, vc_dims in, vc_dims *out) {
     vc__require_gpu();

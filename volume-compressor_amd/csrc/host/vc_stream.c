@@ -25,6 +25,8 @@
 
 #define ARENA_DEFAULT (8ull << 30)
 
+#define SDEV_EVENTS 64
+
 typedef struct vc_sdev {
     pthread_mutex_t mu;            /* held from a call's residency check through its launches */
     vcg_arc *arc;
@@ -38,6 +40,11 @@ typedef struct vc_sdev {
     uint64_t *need; uint64_t nneed;
     uint8_t *stage; uint64_t stage_cap;   /* pinned */
     uint64_t reads, evictions;     /* statistics */
+    /* one event per recent call that launched on the arena, recorded on the
+     * calling thread's stream after its launches: an eviction waits for these
+     * (not for the whole device) before the arena is overwritten */
+    void *ev[SDEV_EVENTS];
+    uint32_t ev_next;
 } vc_sdev;
 
 static uint64_t align16(uint64_t x) { return (x + 15u) & ~15ull; }
@@ -45,6 +52,7 @@ static uint64_t align16(uint64_t x) { return (x + 15u) & ~15ull; }
 void vc__stream_free(vc_archive *a) {
     vc_sdev *s = (vc_sdev *)a->sdev;
     if (!s) return;
+    for (int k = 0; k < SDEV_EVENTS; ++k) if (s->ev[k]) vcg_event_destroy(s->ev[k]);
     if (s->arc) vcg_arc_destroy(s->arc);
     if (s->stage) vcg_host_free(s->stage);
     free(s->dir); free(s->dir0); free(s->res); free(s->irr); free(s->mark); free(s->need);
@@ -125,8 +133,13 @@ static void touch(vc_sdev *s, const vc_archive *a, int lod, uint32_t rz, uint32_
     s->need[s->nneed++] = ri;
 }
 
+/* every launch that read the arena has finished (called with s->mu held) */
+static void arena_idle(vc_sdev *s) {
+    for (int k = 0; k < SDEV_EVENTS; ++k) if (s->ev[k]) vcg_event_sync(s->ev[k]);
+}
+
 static void evict(vc_sdev *s) {
-    vcg_device_sync();                          /* no decode in flight still reads the arena */
+    arena_idle(s);                              /* no decode in flight still reads the arena */
     if (vcg_arc_put(s->arc, FILE_HDR, s->dir0, s->nreg * 16) != 0) vc__panic("stream arena reset");
     memset(s->res, 0, s->nreg * 8);
     s->cursor = s->head;
@@ -207,7 +220,15 @@ vcg_arc *vc__stream_begin(vc_archive *a, int lod, const vc_box *box, size_t n, c
 
 void vc__stream_end(vc_archive *a) {
     vc_sdev *s = (vc_sdev *)a->sdev;
-    if (s) pthread_mutex_unlock(&s->mu);
+    if (!s) return;
+    /* the slot's previous record (64 calls ago) must be done before the
+     * event is reused, or an eviction could miss that call's launches */
+    void **e = &s->ev[s->ev_next];
+    s->ev_next = (s->ev_next + 1) % SDEV_EVENTS;
+    if (!*e) *e = vcg_event_create();
+    else vcg_event_sync(*e);
+    vcg_event_record(*e, -1);
+    pthread_mutex_unlock(&s->mu);
 }
 
 vc_status vcx_stream_cache(vc_archive *a, size_t bytes) {
@@ -217,7 +238,7 @@ vc_status vcx_stream_cache(vc_archive *a, size_t bytes) {
     pthread_mutex_unlock(&a->mu);
     if (s) {
         pthread_mutex_lock(&s->mu);
-        vcg_device_sync();
+        arena_idle(s);
         pthread_mutex_unlock(&s->mu);
         vc__stream_free(a);
     }

@@ -61,6 +61,7 @@ void *vcg_event_create(void);
 void  vcg_event_destroy(void *e);
 void  vcg_event_record(void *e, int slot);
 int   vcg_event_done(void *e);              /* 1 when everything before the record finished */
+void  vcg_event_sync(void *e);              /* wait until everything before the record finished */
 void *vcg_host_alloc(size_t bytes);         /* pinned host memory */
 void  vcg_host_free(void *p);
 
@@ -195,6 +196,7 @@ int vcg_downsample(const void *dev_in, const uint32_t dims[3], void *dev_out);
 /* the exporter's within-cell downscale (tools/downscale.h), method 0 box,
  * 1 cbox; on the calling thread's stream (sync: wait) */
 int vcg_ds2x(const void *dev_in, const uint32_t dims[3], int method, float alpha, void *dev_out, int sync);
+void vcg_test_wide_index(int on);   /* test hook: 64-bit index path of the 2x kernels */
 
 /* ---- synthetic generator ---- */
 typedef struct { int32_t axis, c0, tu, tv, a1, a2, w1, w2, p1, p2, half, inten; } vcg_sheet;

@@ -74,6 +74,7 @@ def lib():
         "vcx_repack": (i32, [vp, P(vp), P(sz)]),
         "vcx_stream_cache": (i32, [vp, sz]),
         "vcx_downscale2x_device": (i32, [vp, Dims, i32, f32, vp, i32]),
+        "vcx_test_wide_index": (None, [i32]),
         "vcx_stream_stats": (i32, [vp, P(u64), P(u64)]),
         "vcx_repack_device": (i32, [vp, vp, sz, P(sz)]),
         "vcx_server_create": (vp, [vp, sz]),
