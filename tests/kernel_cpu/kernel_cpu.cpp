@@ -148,20 +148,18 @@ int kc_fdct(const uint8_t *vol, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t 
     if (sum == 0) { free(t); *eob = 0; *dc_out = 0; return 1; }
     const int dc = (int)((sum + 16384u) / 32768u);
     static int32_t D[64][16];
-    // pass 1: D[y][kx] = (v-128) x Ch, << 6 + corr, + (v-128) x Cl
+    // pass 1, scaled by 4: D[y][kx] = (v-128) x Ch, << 8 + 4 corr, + (v-128) x 4 Cl
     for (int z = 0; z < 32; ++z) {
         memset(D, 0, sizeof D);
         mfma_cpu(raw[z], F[0], D);
         for (int l = 0; l < 64; ++l)
             for (int r = 0; r < 16; ++r)
-                D[l][r] = (int32_t)(((uint32_t)D[l][r] << 6) + (uint32_t)vcg_fwd1_corr(dc, rowsum[l & 31]));
-        mfma_cpu(raw[z], F[1], D);
+                D[l][r] = (int32_t)(((uint32_t)D[l][r] << 8) + 4u * (uint32_t)vcg_fwd1_corr(dc, rowsum[l & 31]));
+        mfma_cpu(raw[z], F[2], D);
         for (int l = 0; l < 64; ++l)
             for (int qq = 0; qq < 4; ++qq) {
-                int o[4];
-                for (int i = 0; i < 4; ++i) o[i] = D[l][4 * qq + i] >> 14;
                 uint32_t w[2];
-                vcg_im_pack16(o, w);
+                vcg_im_pack16h(D[l] + 4 * qq, w);
                 memcpy(tile + vcg_im_out_addr(0, z, l & 31, l >> 5, qq), w, 8);
             }
     }
@@ -176,16 +174,16 @@ int kc_fdct(const uint8_t *vol, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t 
             if (p == 1) mfma_cpu(hi, F[0], D); else mfma_cpu(F[0], hi, D);
             for (int l = 0; l < 64; ++l) for (int r = 0; r < 16; ++r) D[l][r] = (int32_t)((uint32_t)D[l][r] << 8);
             if (p == 1) { mfma_cpu(hi, F[2], D); mfma_cpu(lo, F[0], D); } else { mfma_cpu(F[2], hi, D); mfma_cpu(F[0], lo, D); }
-            const uint32_t rnd = p == 1 ? (uint32_t)vcg_im_rnd(1, 0) : 8192u;
-            for (int l = 0; l < 64; ++l) for (int r = 0; r < 16; ++r) D[l][r] = (int32_t)(((uint32_t)D[l][r] << 6) + rnd);
-            if (p == 1) mfma_cpu(lo, F[1], D); else mfma_cpu(F[1], lo, D);
+            // pass 2 scaled by 4 (<< 8, 4 rnd, 4 Cl), pass 3 not (i32 output)
+            const uint32_t rnd = p == 1 ? (uint32_t)vcg_im_rnd4(1, 0) : 8192u;
+            const int sh = p == 1 ? 8 : 6;
+            for (int l = 0; l < 64; ++l) for (int r = 0; r < 16; ++r) D[l][r] = (int32_t)(((uint32_t)D[l][r] << sh) + rnd);
+            if (p == 1) mfma_cpu(lo, F[2], D); else mfma_cpu(F[1], lo, D);
             for (int l = 0; l < 64; ++l) {
                 if (p == 1) {
                     for (int qq = 0; qq < 4; ++qq) {
-                        int o[4];
-                        for (int i = 0; i < 4; ++i) o[i] = D[l][4 * qq + i] >> 14;
                         uint32_t w[2];
-                        vcg_im_pack16(o, w);
+                        vcg_im_pack16h(D[l] + 4 * qq, w);
                         memcpy(tile + vcg_im_out_addr(1, B, l & 31, l >> 5, qq), w, 8);
                     }
                 } else {
@@ -249,8 +247,9 @@ static void mfma_cpu(const uint32_t A[64][4], const uint32_t B[64][4], int32_t D
         }
 }
 
-// k_idct_m's product chain for one block (d[l] = lane l's 16 w), as im_chain
-static void im_chain_cpu(const uint32_t d[64][8], const uint32_t B[3][64][4], int rnd, int32_t o[64][16]) {
+// k_idct_m's product chain for one block (d[l] = lane l's 16 w), as im_chain:
+// scaled by 4, the output in bits 16..31 of o
+static void im_chain_cpu(const uint32_t d[64][8], const uint32_t B[3][64][4], int rnd4, int32_t o[64][16]) {
     static uint32_t hi[64][4], lo[64][4];
     for (int l = 0; l < 64; ++l) vcg_im_planes(d[l], hi[l], lo[l]);
     int32_t t[64][16];
@@ -259,9 +258,9 @@ static void im_chain_cpu(const uint32_t d[64][8], const uint32_t B[3][64][4], in
     for (int l = 0; l < 64; ++l) for (int r = 0; r < 16; ++r) t[l][r] = (int32_t)((uint32_t)t[l][r] << 8);
     mfma_cpu(hi, B[1], t);
     mfma_cpu(lo, B[0], t);
-    for (int l = 0; l < 64; ++l) for (int r = 0; r < 16; ++r) t[l][r] = (int32_t)(((uint32_t)t[l][r] << 6) + (uint32_t)rnd);
-    mfma_cpu(lo, B[2], t);
-    for (int l = 0; l < 64; ++l) for (int r = 0; r < 16; ++r) o[l][r] = t[l][r] >> 14;
+    for (int l = 0; l < 64; ++l) for (int r = 0; r < 16; ++r) t[l][r] = (int32_t)(((uint32_t)t[l][r] << 8) + (uint32_t)rnd4);
+    mfma_cpu(lo, B[1], t);
+    for (int l = 0; l < 64; ++l) for (int r = 0; r < 16; ++r) o[l][r] = t[l][r];
 }
 
 // k_idct_m on one atom's entries, block by block with the kernel's lane
@@ -292,11 +291,11 @@ int kc_idct_m(const uint32_t *L, uint32_t cnt, int dc, const int32_t *st, const 
             if (vcg_im_skip(p, Bk, fmax)) continue;
             for (int l = 0; l < 64; ++l) vcg_im_load(t, p, Bk, l & 31, l >> 5, d[Bk][l]);
             if (p == 0) {                          // in place, plane by plane
-                im_chain_cpu(d[Bk], B, vcg_im_rnd(0, dc), o);
+                im_chain_cpu(d[Bk], B, vcg_im_rnd4(0, dc), o);
                 for (int l = 0; l < 64; ++l)
                     for (int q = 0; q < 4; ++q) {
                         uint32_t w[2];
-                        vcg_im_pack16(o[l] + 4 * q, w);
+                        vcg_im_pack16h(o[l] + 4 * q, w);
                         memcpy(t + vcg_im_out_addr(0, Bk, l & 31, l >> 5, q), w, 8);
                     }
             }
@@ -304,15 +303,15 @@ int kc_idct_m(const uint32_t *L, uint32_t cnt, int dc, const int32_t *st, const 
         if (p == 0) continue;
         for (int Bk = 0; Bk < 32; ++Bk) {
             if (vcg_im_skip(p, Bk, fmax)) continue;
-            im_chain_cpu(d[Bk], B, vcg_im_rnd(p, dc), o);
+            im_chain_cpu(d[Bk], B, vcg_im_rnd4(p, dc), o);
             for (int l = 0; l < 64; ++l)
                 for (int q = 0; q < 4; ++q) {
                     if (p == 1) {
                         uint32_t w[2];
-                        vcg_im_pack16(o[l] + 4 * q, w);
+                        vcg_im_pack16h(o[l] + 4 * q, w);
                         memcpy(t + vcg_im_out_addr(1, Bk, l & 31, l >> 5, q), w, 8);
                     } else {
-                        const uint32_t w = vcg_im_pack8(o[l] + 4 * q);
+                        const uint32_t w = vcg_im_pack8h(o[l] + 4 * q);
                         memcpy(t + vcg_im_vox_addr(Bk, l & 31, l >> 5, q), &w, 4);
                     }
                 }

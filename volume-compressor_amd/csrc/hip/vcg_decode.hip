@@ -444,10 +444,11 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 #define IM_REGS __attribute__((amdgpu_waves_per_eu(IM_WAVES)))
 static_assert(IM_THREADS == IDCT_THREADS, "the list prefetch and the store assume 512 threads");
 
-// one block's product chain (data planes hi / lo, basis planes Ch, 4 Cl, Cl):
-// o[r] = (sum_k C[k][n] v[k] + rnd) >> 14 for output register r
-__device__ __forceinline__ void im_chain(const uint32_t hi[4], const uint32_t lo[4], v4i bh, v4i b4l, v4i bl,
-                                         int rnd, int o[16]) {
+// one block's product chain (data planes hi / lo, basis planes Ch and 4 Cl),
+// scaled by 4: o[r] = 4 (sum_k C[k][n] v[k] + rnd), rnd4 = 4 rnd, for output
+// register r; the output (sum + rnd) >> 14 is its bits 16..31
+__device__ __forceinline__ void im_chain(const uint32_t hi[4], const uint32_t lo[4], v4i bh, v4i b4l, int rnd4,
+                                         int o[16]) {
     const v4i ah = {(int)hi[0], (int)hi[1], (int)hi[2], (int)hi[3]};
     const v4i al = {(int)lo[0], (int)lo[1], (int)lo[2], (int)lo[3]};
     const v16i zero = {};
@@ -457,23 +458,23 @@ __device__ __forceinline__ void im_chain(const uint32_t hi[4], const uint32_t lo
     t = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, b4l, t, 0, 0, 0);
     t = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh, t, 0, 0, 0);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) t[r] = (int)(((uint32_t)t[r] << 6) + (uint32_t)rnd);
-    t = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl, t, 0, 0, 0);
+    for (int r = 0; r < 16; ++r) t[r] = (int)(((uint32_t)t[r] << 8) + (uint32_t)rnd4);
+    t = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, b4l, t, 0, 0, 0);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) o[r] = t[r] >> 14;
+    for (int r = 0; r < 16; ++r) o[r] = t[r];
 }
 
 // block B of pass p (x or y): product and the 8-byte writes of this lane
 __device__ __forceinline__ void im_block_xy(uint8_t *tile, int p, int B, int jr, int hh, const uint32_t d[8],
-                                            v4i bh, v4i b4l, v4i bl) {
+                                            v4i bh, v4i b4l) {
     uint32_t hi[4], lo[4];
     vcg_im_planes(d, hi, lo);
     int o[16];
-    im_chain(hi, lo, bh, b4l, bl, vcg_im_rnd(p, 0), o);
+    im_chain(hi, lo, bh, b4l, vcg_im_rnd4(p, 0), o);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         uint32_t w[2];
-        vcg_im_pack16(o + 4 * q, w);
+        vcg_im_pack16h(o + 4 * q, w);
         *(uint2 *)(tile + vcg_im_out_addr(p, B, jr, hh, q)) = make_uint2(w[0], w[1]);
     }
 }
@@ -504,7 +505,7 @@ __device__ __forceinline__ void idct_item_m(const dec_job &j, uint32_t i, const 
     }
     const vcg_item it = items[i];
     const int ln = tid & 63, wv = tid >> 6, jr = ln & 31, hh = ln >> 5;
-    const v4i bh = *(const v4i *)T->ib[0][ln], b4l = *(const v4i *)T->ib[1][ln], bl = *(const v4i *)T->ib[2][ln];
+    const v4i bh = *(const v4i *)T->ib[0][ln], b4l = *(const v4i *)T->ib[1][ln];
     if (tid == 0 && j.status_out) j.status_out[gi] = it.status == 2 ? 1 : it.status == 3 ? 2 : 0;
     if (it.status != 1) {
         for (int v = tid; v < (int)VCG_A3 / 4; v += IM_THREADS) ((uint32_t *)tile)[v] = 0;
@@ -549,7 +550,7 @@ __device__ __forceinline__ void idct_item_m(const dec_job &j, uint32_t i, const 
         if (vcg_im_skip(0, B, fmax)) continue;
         uint32_t d[8];
         vcg_im_load(tile, 0, B, jr, hh, d);
-        im_block_xy(tile, 0, B, jr, hh, d, bh, b4l, bl);
+        im_block_xy(tile, 0, B, jr, hh, d, bh, b4l);
     }
     __syncthreads();
     // pass y: every wave reads its blocks before any writes
@@ -561,7 +562,7 @@ __device__ __forceinline__ void idct_item_m(const dec_job &j, uint32_t i, const 
         __syncthreads();
 #pragma unroll
         for (int b = 0; b < 4; ++b)
-            if (!vcg_im_skip(1, wv + 8 * b, fmax)) im_block_xy(tile, 1, wv + 8 * b, jr, hh, d[b], bh, b4l, bl);
+            if (!vcg_im_skip(1, wv + 8 * b, fmax)) im_block_xy(tile, 1, wv + 8 * b, jr, hh, d[b], bh, b4l);
     }
     __syncthreads();
     // pass z: voxels (dc added, clamped) into the staging raster
@@ -570,16 +571,16 @@ __device__ __forceinline__ void idct_item_m(const dec_job &j, uint32_t i, const 
 #pragma unroll
         for (int b = 0; b < 4; ++b) vcg_im_load(tile, 2, wv + 8 * b, jr, hh, d[b]);
         __syncthreads();
-        const int rnd = vcg_im_rnd(2, dc);
+        const int rnd4 = vcg_im_rnd4(2, dc);
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             uint32_t hi[4], lo[4];
             vcg_im_planes(d[b], hi, lo);
             int o[16];
-            im_chain(hi, lo, bh, b4l, bl, rnd, o);
+            im_chain(hi, lo, bh, b4l, rnd4, o);
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                *(uint32_t *)(tile + vcg_im_vox_addr(wv + 8 * b, jr, hh, q)) = vcg_im_pack8(o + 4 * q);
+                *(uint32_t *)(tile + vcg_im_vox_addr(wv + 8 * b, jr, hh, q)) = vcg_im_pack8h(o + 4 * q);
         }
     }
     __syncthreads();

@@ -281,17 +281,16 @@ __global__ __launch_bounds__(FDCT_TX) void k_fdct(const enc_src *__restrict__ sr
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const v16i zero = {};
+            // scaled by 4 (vcg_im_pack16h): the second product takes 4 Cl
             v16i t = __builtin_amdgcn_mfma_i32_32x32x32_i8(raw[b], bh, zero, 0, 0, 0);
 #pragma unroll
-            for (int r = 0; r < 16; ++r) t[r] = (int)(((uint32_t)t[r] << 6) + (uint32_t)corr);
-            t = __builtin_amdgcn_mfma_i32_32x32x32_i8(raw[b], bl, t, 0, 0, 0);
-            int o[16];
-#pragma unroll
-            for (int r = 0; r < 16; ++r) o[r] = t[r] >> 14;
+            for (int r = 0; r < 16; ++r) t[r] = (int)(((uint32_t)t[r] << 8) + 4u * (uint32_t)corr);
+            t = __builtin_amdgcn_mfma_i32_32x32x32_i8(raw[b], b4l, t, 0, 0, 0);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 uint32_t w[2];
-                vcg_im_pack16(o + 4 * q, w);
+                const int o[4] = {t[4 * q], t[4 * q + 1], t[4 * q + 2], t[4 * q + 3]};
+                vcg_im_pack16h(o, w);
                 *(uint2 *)(tile + vcg_im_out_addr(0, wv + 8 * b, y, hh, q)) = make_uint2(w[0], w[1]);
             }
         }
@@ -315,15 +314,13 @@ __global__ __launch_bounds__(FDCT_TX) void k_fdct(const enc_src *__restrict__ sr
             t = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, b4l, t, 0, 0, 0);
             t = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh, t, 0, 0, 0);
 #pragma unroll
-            for (int r = 0; r < 16; ++r) t[r] = (int)(((uint32_t)t[r] << 6) + (uint32_t)vcg_im_rnd(1, 0));
-            t = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl, t, 0, 0, 0);
-            int o[16];
-#pragma unroll
-            for (int r = 0; r < 16; ++r) o[r] = t[r] >> 14;
+            for (int r = 0; r < 16; ++r) t[r] = (int)(((uint32_t)t[r] << 8) + (uint32_t)vcg_im_rnd4(1, 0));
+            t = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, b4l, t, 0, 0, 0);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 uint32_t w[2];
-                vcg_im_pack16(o + 4 * q, w);
+                const int o[4] = {t[4 * q], t[4 * q + 1], t[4 * q + 2], t[4 * q + 3]};
+                vcg_im_pack16h(o, w);
                 *(uint2 *)(tile + vcg_im_out_addr(1, wv + 8 * b, y, hh, q)) = make_uint2(w[0], w[1]);
             }
         }

@@ -502,6 +502,14 @@ VCG_HD void vcg_im_planes(const uint32_t d[8], uint32_t hi[4], uint32_t lo[4]) {
 // + 128; pass z adds the atom's dc
 VCG_HD int vcg_im_rnd(int p, int dc) { return 8192 + ((p == 2 ? dc : 128) << 14); }
 
+// The chains end scaled by 4: the last product takes 4 Cl in place of Cl and
+// the shift and rounding constant before it are 4 times theirs (<< 8,
+// vcg_im_rnd4), so a chain leaves T = 4 (sum + rnd) and the output
+// (sum + rnd) >> 14 is bits 16..31 of T, taken whole by byte permutes
+// instead of a shift per register.  Exact while |sum + rnd| < 2^29 (the
+// coefficient bound keeps it below 5.16e8).
+VCG_HD int vcg_im_rnd4(int p, int dc) { return (int)(4u * (uint32_t)vcg_im_rnd(p, dc)); }
+
 // pass x / y: lane (n = l, half h) writes its output registers 4q..4q+3
 // (four consecutive rows) as 8 bytes; returns the byte offset
 VCG_HD uint32_t vcg_im_out_addr(int p, int B, int l, int h, int q) {
@@ -510,25 +518,28 @@ VCG_HD uint32_t vcg_im_out_addr(int p, int B, int l, int h, int q) {
     const uint32_t L = (uint32_t)(l * 32 + 8 * (B & 3) + 2 * q + h);
     return vcg_im_chunk(L, (uint32_t)(m >> 1)) + 8u * (uint32_t)(m & 1);
 }
-// four outputs (w, i16) packed little-endian into two dwords
-VCG_HD void vcg_im_pack16(const int o[4], uint32_t w[2]) {
-    w[0] = vcg_perm((uint32_t)o[1], (uint32_t)o[0], 0x05040100u);
-    w[1] = vcg_perm((uint32_t)o[3], (uint32_t)o[2], 0x05040100u);
+// four outputs of a scaled chain (bits 16..31 of each register) packed
+// little-endian as i16
+VCG_HD void vcg_im_pack16h(const int t[4], uint32_t w[2]) {
+    w[0] = vcg_perm((uint32_t)t[1], (uint32_t)t[0], 0x07060302u);
+    w[1] = vcg_perm((uint32_t)t[3], (uint32_t)t[2], 0x07060302u);
 }
 // pass z: lane (z = l, half h), registers 4q..4q+3 = voxels x = 8q + 4h ..
 // +3 of row (z, y = B): clamped, packed, at this staging offset
 VCG_HD uint32_t vcg_im_vox_addr(int B, int l, int h, int q) {
     return (uint32_t)(l * VCG_IM_ZS + B * 32 + 8 * q + 4 * h);
 }
-// The bytes are packed with permutes, not shifts and ors: hipcc fuses
-// "shift, clamp to u8, pack two" into v_ashr_pk_u8_i32, which writes only the
-// low 16 bits of its destination, and then ors the register's stale high
-// half into the upper bytes.
-VCG_HD uint32_t vcg_im_pack8(const int o[4]) {
+
+// four voxels of a scaled chain: clamp(t >> 16, 0, 255) is byte 2 of t
+// clamped to [0, 2^24 - 1] (one v_med3 each), gathered by permutes.  (Bytes
+// are packed with permutes, not shifts and ors: hipcc fuses "shift, clamp to
+// u8, pack two" into v_ashr_pk_u8_i32, which writes only the low 16 bits of
+// its destination and leaves the register's stale high half in the word.)
+VCG_HD uint32_t vcg_im_pack8h(const int t[4]) {
     uint32_t v[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) v[i] = (uint32_t)(o[i] < 0 ? 0 : o[i] > 255 ? 255 : o[i]);
-    const uint32_t p01 = vcg_perm(v[1], v[0], 0x0c0c0400u), p23 = vcg_perm(v[3], v[2], 0x0c0c0400u);
+    for (int i = 0; i < 4; ++i) v[i] = (uint32_t)(t[i] < 0 ? 0 : t[i] > 0xFFFFFF ? 0xFFFFFF : t[i]);
+    const uint32_t p01 = vcg_perm(v[1], v[0], 0x0c0c0602u), p23 = vcg_perm(v[3], v[2], 0x0c0c0602u);
     return vcg_perm(p23, p01, 0x05040100u);
 }
 
