@@ -829,10 +829,15 @@ def main():
                     help="profiler run: only the container rewrite of the 2048^3 ratio-50 archive, its JSON on stdout")
     ap.add_argument("--encode-batch-only", action="store_true",
                     help="profiler run: only the batched-encode section (256 volumes), its JSON on stdout")
+    ap.add_argument("--decode-only", action="store_true",
+                    help="profiler run: only the LOD0 decode section, the timed calls back to back as in a "
+                         "full run (no per-atom calls, no other sections, no CPU leg)")
     ap.add_argument("--profile", action="store_true",
                     help="profiler run: only the LOD0 decode launches, each call synchronous (no per-atom "
                          "calls, no serving trace, no CPU leg), then one 1 GiB calibration copy kernel")
     args = ap.parse_args()
+    # sections beyond the LOD0 decode run only in a full run
+    args.quick = args.profile or args.decode_only
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the decode shards by atom "
               "with no inter-GPU traffic", file=sys.stderr)
@@ -968,7 +973,7 @@ def main():
     # ---- plain per-call path: vc_decode_atom latency (host wall clock) ----
     lat = []
     out = np.zeros(32768, np.uint8)
-    for i in range(0 if args.profile else 200):
+    for i in range(0 if args.quick else 200):
         az, ay, ax = (int(x) for x in rng.integers(0, nax, 3))
         t0 = time.perf_counter()
         L.vc_decode_atom(a.h, 0, ax, ay, az, out.ctypes.data)
@@ -977,12 +982,12 @@ def main():
     p50, p99 = (float(np.percentile(lat, 50)), float(np.percentile(lat, 99))) if lat.size else (None, None)
 
     # ---- host transfers and the host-destination drop-in call ----
-    xfer = None if args.profile else transfer_section(pyvc, a, dout, n)
+    xfer = None if args.quick else transfer_section(pyvc, a, dout, n)
     if xfer:
         log(f"transfers: {xfer}")
     # ---- vc_encode on the host cores (the oracle's OpenMP encoder) ----
     cpu_enc = None
-    if not (args.no_cpu or args.profile):
+    if not (args.no_cpu or args.quick):
         hv = pyvc.d2h(dvol, nvox).reshape(n, n, n)
         cpu_enc = cpu_encode_bench([hv], [args.ratio])
         cpu_enc["gpu_GBps"] = round(nvox / t_enc / 1e9, 3)
@@ -991,15 +996,15 @@ def main():
 
     # ---- the same volume at the other target ratios ----
     ratios = None
-    if not (args.no_ratios or args.profile):
+    if not (args.no_ratios or args.quick):
         ratios = ratio_section(pyvc, hip, o, dvol, dout, n, [r for r in (10.0, 50.0, 100.0) if r != args.ratio])
     # ---- the exporter's within-cell downscale of the same volume (HBM-bound) ----
     ds = None
-    if not args.profile:
+    if not args.quick:
         ds = ds_section(pyvc, hip, dvol, dout, n)
         log(f"downscale: {ds}")
     exp = None
-    if not args.profile:
+    if not args.quick:
         exp = export_section(pyvc, dvol, n, no_cpu=args.no_cpu)
         log(f"export: {exp}")
     pyvc.dev_free(dout)
@@ -1008,17 +1013,17 @@ def main():
 
     # ---- serving: continuous-batching request loop under Poisson traces ----
     served = None
-    if not (args.no_serve or args.profile):
+    if not (args.no_serve or args.quick):
         served = serving_section(pyvc, args)
 
     # ---- batched encode service ----
     enc_batch = None
-    if not (args.no_encode_batch or args.profile):
+    if not (args.no_encode_batch or args.quick):
         enc_batch = encode_batch_section(pyvc, with_cpu=not args.no_cpu)
 
     # ---- CPU baseline ----
     cpu = None
-    if not (args.no_cpu or args.profile):
+    if not (args.no_cpu or args.quick):
         zyx = np.array([(z, y, x) for z in range(nax) for y in range(nax) for x in range(nax)], np.int32)
         cpu = cpu_baseline(arc, zyx)
         log(f"cpu baseline: {cpu}")
