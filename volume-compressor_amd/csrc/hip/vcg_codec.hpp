@@ -657,15 +657,21 @@ VCG_HD void vcg_efsm_shift(vcg_efsm &f, uint8_t *out) {
     f.low = (uint64_t)(lo << 8);
 }
 
-// the five-shift flush (vc.c:399-401); the lane is finished afterwards
+// the five-shift flush (vc.c:399-401); the lane is finished afterwards (and
+// puts nothing more)
 template <bool WRITE>
 VCG_HD void vcg_efsm_finish(vcg_efsm &f, uint8_t *out) {
     if (WRITE) for (int k = 0; k < 5; ++k) vcg_efsm_shift<WRITE>(f, out);
     else f.pos += 5;                                  // count: one byte per shift
     f.len = f.pos;
     f.done = 1;
-    f.cap = 0;
     f.lim = 0xFFFFFFFFu;
+}
+
+// a flush's pending run: csize - 1 bytes 0xFF + carry after the cache byte
+template <bool WRITE>
+VCG_HD void vcg_efsm_run(vcg_efsm &f, uint8_t *out, uint32_t hi) {
+    for (uint32_t k = 1; k < f.csize; ++k) vcg_efsm_put<WRITE>(f, out, (0xFFu + hi) & 0xFFu);
 }
 
 VCG_HD vcg_blk16 vcg_efsm_load(const vcg_efsm &f, uint32_t k) { return f.blk[k < 4095u ? k : 4095u]; }
@@ -738,15 +744,20 @@ VCG_HD uint32_t vcg_efsm_step(vcg_efsm &f, Ctx &ctx, Lv &lv, uint8_t *out) {
     const uint32_t pn = ((bit ? 31u : 4096u) + vcg_mul24(31u, P)) >> 5;
     ctx.st(h, pn);
     f.psig = isSig ? pn : f.psig;
-    // renormalisation
+    // renormalisation.  A flush puts out the cache byte (plus, rarely, a
+    // pending run).  That byte is stored every step, unmasked, at out[pos], or
+    // at out[cap] (a spare byte past the capacity) once the lane is finished
+    // or full; pos advances only on a flush, so the store of a step without
+    // one is overwritten by the next.  out must hold cap + 1 bytes.
     const bool sh = f.range < (1u << 24);
     if (WRITE) {
         const uint32_t hi = (uint32_t)(f.low >> 32), lo = (uint32_t)f.low;
         const bool fl = sh && (hi != 0u || lo < 0xFF000000u);
-        if (fl) {
-            vcg_efsm_put<WRITE>(f, out, (f.cache + hi) & 0xFFu);
-            for (uint32_t k = 1; k < f.csize; ++k) vcg_efsm_put<WRITE>(f, out, (0xFFu + hi) & 0xFFu);
-        }
+        const bool put = fl && !f.done;
+        out[!f.done && f.pos < f.cap ? f.pos : f.cap] = (uint8_t)(f.cache + hi);
+        f.ovf = put && f.pos >= f.cap ? 1 : f.ovf;
+        f.pos += put ? 1u : 0u;
+        if (fl && f.csize > 1u) vcg_efsm_run<WRITE>(f, out, hi);
         f.cache = fl ? lo >> 24 : f.cache;
         f.csize = fl ? 1u : f.csize + (sh ? 1u : 0u);
         f.low = sh ? (uint64_t)(lo << 8) : f.low;

@@ -526,17 +526,20 @@ __global__ __launch_bounds__(64) void k_rc_encode(uint32_t n, const int16_t *__r
     const uint32_t r = live ? i : 0u;
     const enc_item it = items[r];
     const bool zero = it.zero != 0;
-    uint8_t *out = pay + (size_t)r * pay_cap;
-    if (live && !zero) out[0] = (uint8_t)it.dc;
-    const uint32_t cap = pay_cap - 1 < VCG_RC_CAP ? pay_cap - 1 : VCG_RC_CAP;
+    // lanes past the items and all-zero atoms run as finished empty encodes
+    // whose stores all go to byte 1 of their sink: their own (unused) slot, or
+    // slot n, the pool's spare one past the last item
+    const bool quiet = !live || zero;
+    uint8_t *out = pay + (size_t)(live ? i : n) * pay_cap;
+    if (!quiet) out[0] = (uint8_t)it.dc;
+    // the range-coder bytes start at out + 1; the slot keeps a spare byte past
+    // them (the encoder's store sink once a lane is finished or full)
+    const uint32_t cap = pay_cap - 2 < VCG_RC_CAP ? pay_cap - 2 : VCG_RC_CAP;
     enc_lds_ctx ctx = enc_ctx_init(ctxm);
     enc_lds_lv lr{lvm + threadIdx.x * ENC_LV_STRIDE};
     vcg_efsm f;
-    // lanes past the items and all-zero atoms run as finished empty encodes
-    // that write nothing
-    vcg_efsm_init<true>(f, lr, lv + (size_t)r * VCG_A3, live && !zero ? it.eob : 0u, bands, out + 1,
-                        live && !zero ? cap : 0u);
-    if (!live || zero) f.done = 1;
+    vcg_efsm_init<true>(f, lr, lv + (size_t)r * VCG_A3, quiet ? 0u : it.eob, bands, out + 1, quiet ? 0u : cap);
+    if (quiet) f.done = 1;
     while (__any(!f.done)) {
         vcg_efsm_refill(f, lr);
         vcg_efsm_block<true>(f, ctx, lr, bands, out + 1, __any(vcg_efsm_near(f, VCG_EFSM_REFILL)));
@@ -808,6 +811,8 @@ static void scratch_release() {
 
 // atoms per encode launch (levels + payload scratch: 192 KiB each)
 #define ENC_ROWS 65536u
+// payload slot of k_rc_encode: dc byte, the range-coder bytes, one spare byte
+#define ENC_SLOT (VCG_PAY_CAP + 1u)
 // rate-control rows (candidate x sample atom) per launch: VCG_RC_ROWS
 // (levels: 64 KiB each).  A round's rows share one launch up to this size:
 // the count lanes are latency-bound, so more waves per SIMD cost little.
@@ -1003,7 +1008,7 @@ extern "C" int vcg_enc_all(int nv, vcg_enc *const *es, const float (*qs)[VCG_NLO
     const uint32_t maxb = tot < ENC_ROWS ? (uint32_t)tot : ENC_ROWS;
     vcg_grow((void **)&S.lv, &S.lv_cap, maxb, (size_t)VCG_A3 * 2);
     vcg_grow((void **)&S.items, &S.items_cap, maxb, sizeof(enc_item));
-    vcg_grow((void **)&S.pay, &S.pay_cap, (size_t)maxb * VCG_PAY_CAP, 1);
+    vcg_grow((void **)&S.pay, &S.pay_cap, ((size_t)maxb + 1) * ENC_SLOT, 1);   // + k_rc_encode's spare slot
     vcg_grow((void **)&S.lens, &S.lens_cap, maxb, 4);
     vcg_grow((void **)&S.aux, &S.aux_cap, (size_t)maxb * 2, 4);
     uint64_t *ddst = (uint64_t *)S.aux;       // destination address of each payload
@@ -1018,7 +1023,7 @@ extern "C" int vcg_enc_all(int nv, vcg_enc *const *es, const float (*qs)[VCG_NLO
                            (uint32_t)(seg.size() - 1), b, (int)FDCT_QUANT, g_dtab, S.lv, S.items);
         VCG_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_rc_encode, dim3((n + 63) / 64), dim3(64), 0, c->s, n, S.lv, S.items, g_dtab, S.pay,
-                           (uint32_t)VCG_PAY_CAP, S.lens);
+                           (uint32_t)ENC_SLOT, S.lens);
         VCG_CHECK(hipGetLastError());
         VCG_CHECK(hipMemcpyAsync(hl.data(), S.lens, (size_t)n * 4, hipMemcpyDeviceToHost, c->s));
         VCG_CHECK(hipStreamSynchronize(c->s));
@@ -1038,7 +1043,7 @@ extern "C" int vcg_enc_all(int nv, vcg_enc *const *es, const float (*qs)[VCG_NLO
         for (uint32_t k = 0; k < n; ++k)
             dst[k] = (uint64_t)(uintptr_t)(es[vk[k]]->pool + es[vk[k]]->pool_off[ik[k]]);
         VCG_CHECK(hipMemcpyAsync(ddst, dst.data(), (size_t)n * 8, hipMemcpyHostToDevice, c->s));
-        hipLaunchKernelGGL(k_compact, dim3(n), dim3(256), 0, c->s, S.pay, (uint32_t)VCG_PAY_CAP, S.lens,
+        hipLaunchKernelGGL(k_compact, dim3(n), dim3(256), 0, c->s, S.pay, (uint32_t)ENC_SLOT, S.lens,
                            (const uint64_t *)ddst);
         VCG_CHECK(hipGetLastError());
         for (int v = 0; v < nv; ++v) { es[v]->pool_len += add[v]; add[v] = 0; }
