@@ -297,6 +297,15 @@ def serve_bench(pyvc, a, reqs, qps, seed=11, device_bytes=16 << 30, tune=True):
         assert srv.autotune() == 0
     ids, arrive = {}, np.zeros(nreq)
     done_t, bad = {}, 0
+    rec = {}                      # k -> (t_submit, t_admit, t_done, batch class, batch atoms)
+
+    def take(got):
+        nonlocal bad
+        for g, s_, cls, ts, ta, td, ba in got:
+            k = ids[int(g)]
+            done_t[k] = int(td)
+            rec[k] = (int(ts), int(ta), int(td), int(cls), int(ba))
+            bad += int(s_ != 0)
     try:
         t0 = pyvc.now_ns() + 2_000_000
         sched = t0 + np.cumsum(gaps) * 1e9
@@ -310,15 +319,9 @@ def serve_bench(pyvc, a, reqs, qps, seed=11, device_bytes=16 << 30, tune=True):
             assert i >= 0, (kind, x if kind == "region" else len(x))
             ids[int(i)] = k
             arrive[k] = sched[k]
-            got, st, t = srv.poll(wait_ms=0)
-            for g, s_, tt in zip(got, st, t):
-                done_t[ids[int(g)]] = int(tt)
-                bad += int(s_ != 0)
+            take(srv.poll_served(wait_ms=0))
         while len(done_t) < nreq:
-            got, st, t = srv.poll(wait_ms=100)
-            for g, s_, tt in zip(got, st, t):
-                done_t[ids[int(g)]] = int(tt)
-                bad += int(s_ != 0)
+            take(srv.poll_served(wait_ms=100))
     finally:
         srv.close()
     lat = np.array([(done_t[k] - arrive[k]) / 1e6 for k in range(nreq)])
@@ -346,6 +349,21 @@ def serve_bench(pyvc, a, reqs, qps, seed=11, device_bytes=16 << 30, tune=True):
         if m.any():
             out[f"{name}_p50"] = round(float(np.percentile(lat[m], 50)), 3)
             out[f"{name}_p99"] = round(float(np.percentile(lat[m], 99)), 3)
+    # where the slowest 1% of requests spent their time: submission lag
+    # (scheduled arrival to submit), queueing (submit to admission into a
+    # batch) and service (admission to completion: the batch's launches and
+    # result copy), with the class and size of the batch that completed them
+    slow = np.argsort(lat)[-max(1, nreq // 100):]
+    out["p99_breakdown_ms"] = {
+        "requests": int(slow.size),
+        "submit_lag": round(float(np.mean([(rec[k][0] - arrive[k]) / 1e6 for k in slow])), 3),
+        "queue": round(float(np.mean([(rec[k][1] - rec[k][0]) / 1e6 for k in slow])), 3),
+        "service": round(float(np.mean([(rec[k][2] - rec[k][1]) / 1e6 for k in slow])), 3),
+        "coarse_batches": int(sum(rec[k][3] for k in slow)),
+        "batch_atoms": int(np.mean([rec[k][4] for k in slow])),
+    }
+    out["queue_ms_p50"] = round(float(np.percentile([(rec[k][1] - rec[k][0]) / 1e6 for k in range(nreq)], 50)), 3)
+    out["service_ms_p50"] = round(float(np.percentile([(rec[k][2] - rec[k][1]) / 1e6 for k in range(nreq)], 50)), 3)
     if reqs[0][0] == "atoms":
         atoms = sum(len(x) for _, x in reqs)
         out["atoms_per_s"] = round(atoms / span, 1)
@@ -362,7 +380,7 @@ def qps_sweep(pyvc, a, reqs, rates):
     out = []
     for q in rates:
         r = serve_bench(pyvc, a, reqs, float(q))
-        out.append({k: r[k] for k in ("offered_qps", "achieved_rps", "p50", "p99", "errors")})
+        out.append({k: r[k] for k in ("offered_qps", "achieved_rps", "p50", "p99", "errors", "p99_breakdown_ms")})
         log(f"  sweep {q}: {out[-1]}")
     return out
 
@@ -893,6 +911,15 @@ def main():
             for ax in range(nax):
                 pay[(az * nax + ay) * nax + ax] = a.payload_range(0, az, ay, ax)[2]
     pay_bytes = int(pay.sum())
+    # encode's algorithmic bytes (input voxels, each pyramid level read and
+    # the next written, every payload byte, the archive bytes)
+    lvl = [a.lod_dims(l)[1] for l in range(lod_count(a))]
+    vox = [int(x) * int(y) * int(z) for x, y, z in lvl]
+    all_pay = pay_bytes
+    for l in range(1, len(lvl)):
+        gx, gy, gz = ((d + 31) // 32 for d in lvl[l])
+        all_pay += sum(a.payload_range(l, az, ay, ax)[2] for az in range(gz) for ay in range(gy) for ax in range(gx))
+    enc_bytes = vox[0] + sum(vox[l - 1] + vox[l] for l in range(1, len(vox))) + all_pay + len(arc)
     log(f"encode (ratio {args.ratio}): {t_enc:.2f} s, archive {len(arc) / 1e6:.1f} MB, "
         f"LOD0 payload {pay_bytes / 1e6:.1f} MB (ratio {nvox / pay_bytes:.2f})")
 
@@ -1070,6 +1097,12 @@ def main():
                                   if lat.size else None),
         "serving": served,
         "encode_s": round(t_enc, 3),
+        # the whole vc_encode call (rate control, pyramid, transforms, range
+        # coding, assembly) against HBM peak; its time is set by the serial
+        # range coding of the heaviest coarse-LOD atoms
+        "encode_roofline": {"bound": "hbm", "achieved": round(enc_bytes / t_enc / 1e9, 3), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(enc_bytes / t_enc / 1e9 / HBM_PEAK_GBS, 6),
+                            "algorithmic_bytes": enc_bytes, "s": round(t_enc, 4)},
         "other_ratios": ratios,
         "downscale": ds,
         "export": exp,

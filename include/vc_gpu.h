@@ -241,6 +241,21 @@ int64_t     vcx_server_submit_region(vcx_server *s, int lod, vc_box box, uint8_t
 size_t      vcx_server_poll(vcx_server *s, int64_t *ids, int32_t *status, uint64_t *t_done_ns, size_t max,
                             int wait_ms);
 size_t      vcx_server_capacity(const vcx_server *s);   /* atoms per batch slot */
+/* A completion with its timing, for the breakdown of a request's latency:
+ * submitted, admitted into a batch (for a request served in parts, its first
+ * batch) and completed, on CLOCK_MONOTONIC in ns (vcx_now_ns); the LOD class
+ * of the batch that completed it (0: LOD0 only, 1: a coarser LOD) and that
+ * batch's atom count.  vcx_server_poll_served takes completions from the same
+ * queue as vcx_server_poll, with these fields.  No counterpart in the
+ * reference (it has no server). */
+typedef struct {
+    int64_t id;
+    int32_t status;
+    int32_t cls;
+    uint64_t t_submit_ns, t_admit_ns, t_done_ns;
+    uint64_t batch_atoms;
+} vcx_served;
+size_t      vcx_server_poll_served(vcx_server *s, vcx_served *out, size_t max, int wait_ms);
 /* Batches run as persistent kernels (entropy waves claiming 64 atoms at a
  * time, transform workgroups one atom at a time, the batch size read from
  * device memory), captured once per batch slot, LOD class (LOD0 only / any

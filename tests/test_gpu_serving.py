@@ -170,3 +170,55 @@ def test_recycled_context_starts_clean(pyvc, golden_containers):
     t.start()
     t.join(timeout=60)
     assert got and got[0][0] == 0 == want[0] and np.array_equal(got[0][1], want[1])
+
+
+def test_server_poll_served_timing(pyvc, served):
+    """vcx_server_poll_served: every request completes once, with its status,
+    submit <= admit <= done on the vcx_now_ns clock, the class of the batch
+    that completed it (LOD0-only requests in class-0 batches, coarser ones in
+    class 1) and a batch at least as large as the request's atoms (for a
+    request served in parts, the batch of its last part)."""
+    a = served
+    rng = np.random.default_rng(8)
+    reqs = _random_requests(a, rng, 20, 6)
+    srv = pyvc.Server(a, 3 * 32 * pyvc.Server.atom_bytes())
+    bufs, ids, natoms = [], {}, {}
+    try:
+        cap = srv.capacity()
+        t0 = pyvc.now_ns()
+        for k, (kind, r) in enumerate(reqs):
+            if kind == "atoms":
+                b = pyvc.PinnedBuffer(len(r) * A3)
+                i = srv.submit_atoms(r, b.p)
+                natoms[k] = len(r)
+            else:
+                lod, (x0, y0, z0, x1, y1, z1) = r
+                b = pyvc.PinnedBuffer((x1 - x0) * (y1 - y0) * (z1 - z0))
+                i = srv.submit_region(lod, (x0, y0, z0, x1, y1, z1), b.p)
+                natoms[k] = ((x1 - 1) // 32 - x0 // 32 + 1) * ((y1 - 1) // 32 - y0 // 32 + 1) * \
+                    ((z1 - 1) // 32 - z0 // 32 + 1)
+            assert i >= 0
+            bufs.append(b)
+            ids[int(i)] = k
+        recs = {}
+        for _ in range(2000):
+            for rec in srv.poll_served(wait_ms=50):
+                k = ids[int(rec[0])]
+                assert k not in recs
+                recs[k] = rec
+            if len(recs) == len(reqs):
+                break
+        t1 = pyvc.now_ns()
+        assert len(recs) == len(reqs)
+        for k, (kind, r) in enumerate(reqs):
+            _, st, cls, ts, ta, td, ba = recs[k]
+            lod = int(r[0, 0]) if kind == "atoms" else r[0]
+            coarse = bool((r[:, 0] != 0).any()) if kind == "atoms" else lod != 0
+            assert st == 0 and cls == int(coarse), (k, cls, coarse)
+            assert t0 <= ts <= ta <= td <= t1, (k, ts, ta, td)
+            # a request served in parts reports the batch of its last part
+            assert (ba >= natoms[k] if natoms[k] <= cap else ba >= 1) and ba <= cap, (k, ba, natoms[k], cap)
+    finally:
+        srv.close()
+        for b in bufs:
+            b.free()

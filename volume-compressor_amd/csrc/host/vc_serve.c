@@ -71,6 +71,7 @@ typedef struct srv_req {
     size_t out_bytes;
     int out_dev;                /* out is device memory */
     size_t first;               /* index of its first atom in the batch */
+    uint64_t t_sub, t_adm;      /* submitted, admitted into a batch (vcx_now_ns) */
     struct srv_parent *parent;  /* a part of a request split over several batches, or NULL */
     struct srv_req *next;
 } srv_req;
@@ -81,6 +82,7 @@ typedef struct srv_parent {
     int64_t id;
     size_t left;                /* parts not yet retired */
     int32_t st;
+    uint64_t t_adm;             /* admission of its first part */
 } srv_parent;
 
 typedef struct {
@@ -108,8 +110,8 @@ struct vcx_server {
     srv_req *qhead, *qtail;
     int64_t next_id;
     size_t pending, inflight;
-    /* completions: ring of (id, status, t_done_ns) */
-    int64_t *d_id; int32_t *d_st; uint64_t *d_t;
+    /* completions, in retirement order */
+    vcx_served *done;
     size_t d_cap, d_head, d_len;
     srv_slot slot[SRV_SLOTS];
     /* launch shapes per (class, bucket): the built-in default until
@@ -126,21 +128,16 @@ uint64_t vcx_now_ns(void) {
     return (uint64_t)t.tv_sec * 1000000000ull + (uint64_t)t.tv_nsec;
 }
 
-static void push_done(vcx_server *s, int64_t id, int32_t st, uint64_t t) {
+static void push_done(vcx_server *s, const vcx_served *d) {
     if (s->d_len == s->d_cap) {
         size_t nc = s->d_cap ? 2 * s->d_cap : 1024;
-        int64_t *ni = (int64_t *)malloc(nc * 8); int32_t *ns = (int32_t *)malloc(nc * 4);
-        uint64_t *nt = (uint64_t *)malloc(nc * 8);
-        if (!ni || !ns || !nt) vc__panic("out of memory (server completions)");
-        for (size_t k = 0; k < s->d_len; ++k) {
-            size_t j = (s->d_head + k) % (s->d_cap ? s->d_cap : 1);
-            ni[k] = s->d_id[j]; ns[k] = s->d_st[j]; nt[k] = s->d_t[j];
-        }
-        free(s->d_id); free(s->d_st); free(s->d_t);
-        s->d_id = ni; s->d_st = ns; s->d_t = nt; s->d_cap = nc; s->d_head = 0;
+        vcx_served *nd = (vcx_served *)malloc(nc * sizeof *nd);
+        if (!nd) vc__panic("out of memory (server completions)");
+        for (size_t k = 0; k < s->d_len; ++k) nd[k] = s->done[(s->d_head + k) % s->d_cap];
+        free(s->done);
+        s->done = nd; s->d_cap = nc; s->d_head = 0;
     }
-    size_t j = (s->d_head + s->d_len) % s->d_cap;
-    s->d_id[j] = id; s->d_st[j] = st; s->d_t[j] = t;
+    s->done[(s->d_head + s->d_len) % s->d_cap] = *d;
     s->d_len++;
 }
 
@@ -307,15 +304,21 @@ static void retire(vcx_server *s, srv_slot *b, uint64_t now) {
     while (r) {
         int32_t st = 0;
         for (size_t k = 0; k < r->n; ++k) st |= b->st_pin[r->first + k];
+        vcx_served d;
+        d.id = r->id; d.cls = b->cls; d.t_submit_ns = r->t_sub; d.t_admit_ns = r->t_adm; d.t_done_ns = now;
+        d.batch_atoms = b->n;
         if (r->parent) {
             srv_parent *p = r->parent;
             p->st |= st;
             if (--p->left == 0) {
-                push_done(s, p->id, p->st ? VC_ERR_FORMAT : VC_OK, now);
+                d.status = p->st ? VC_ERR_FORMAT : VC_OK;
+                d.t_admit_ns = p->t_adm;
+                push_done(s, &d);
                 free(p);
             }
         } else {
-            push_done(s, r->id, st ? VC_ERR_FORMAT : VC_OK, now);
+            d.status = st ? VC_ERR_FORMAT : VC_OK;
+            push_done(s, &d);
         }
         srv_req *nx = r->next;
         free(r->quads);
@@ -381,11 +384,14 @@ static void *worker(void *arg) {
         }
         if (cls >= 0) {
             srv_req *tail = NULL, *prev = NULL, *r = s->qhead;
+            const uint64_t t_adm = vcx_now_ns();
             fb->n = 0;
             while (r) {
                 srv_req *nx = r->next;
                 if (r->cls != cls) { prev = r; r = nx; continue; }
                 if (fb->n + r->n > s->cap_atoms) break;
+                r->t_adm = t_adm;
+                if (r->parent && !r->parent->t_adm) r->parent->t_adm = t_adm;
                 /* unlink r (queue order of the rest is kept) */
                 if (prev) prev->next = nx; else s->qhead = nx;
                 if (s->qtail == r) s->qtail = prev;
@@ -500,7 +506,7 @@ void vcx_server_destroy(vcx_server *s) {
         vcg_host_free(b->q_pin); vcg_host_free(b->st_pin);
         vcg_free(b->q_dev); vcg_free(b->st_dev); vcg_free(b->out_dev); vcg_free(b->scratch);
     }
-    free(s->d_id); free(s->d_st); free(s->d_t);
+    free(s->done);
     pthread_cond_destroy(&s->cv_done);
     pthread_cond_destroy(&s->cv_work);
     pthread_mutex_destroy(&s->mu);
@@ -518,10 +524,12 @@ static int64_t enqueue(vcx_server *s, srv_req **parts, size_t np, srv_parent *pa
         return -1;
     }
     const int64_t id = s->next_id++;
+    const uint64_t t_sub = vcx_now_ns();
     if (par) { par->id = id; par->left = np; }
     for (size_t k = 0; k < np; ++k) {
         srv_req *r = parts[k];
         r->id = id;
+        r->t_sub = t_sub;
         r->parent = par;
         r->next = NULL;
         if (s->qtail) s->qtail->next = r; else s->qhead = r;
@@ -599,7 +607,10 @@ int64_t vcx_server_submit_region(vcx_server *s, int lod, vc_box box, uint8_t *ou
     return id;
 }
 
-size_t vcx_server_poll(vcx_server *s, int64_t *ids, int32_t *status, uint64_t *t_done_ns, size_t max, int wait_ms) {
+/* up to max completions in retirement order into the caller's arrays,
+ * waiting up to wait_ms for the first one */
+static size_t take_done(vcx_server *s, size_t max, int wait_ms, int64_t *ids, int32_t *status, uint64_t *t_done_ns,
+                        vcx_served *rec) {
     if (!s || !max) return 0;
     pthread_mutex_lock(&s->mu);
     if (s->d_len == 0 && wait_ms > 0) {
@@ -612,16 +623,25 @@ size_t vcx_server_poll(vcx_server *s, int64_t *ids, int32_t *status, uint64_t *t
     }
     size_t k = 0;
     while (k < max && s->d_len) {
-        size_t j = s->d_head;
-        if (ids) ids[k] = s->d_id[j];
-        if (status) status[k] = s->d_st[j];
-        if (t_done_ns) t_done_ns[k] = s->d_t[j];
+        const vcx_served *d = &s->done[s->d_head];
+        if (ids) ids[k] = d->id;
+        if (status) status[k] = d->status;
+        if (t_done_ns) t_done_ns[k] = d->t_done_ns;
+        if (rec) rec[k] = *d;
         s->d_head = (s->d_head + 1) % s->d_cap;
         s->d_len--;
         k++;
     }
     pthread_mutex_unlock(&s->mu);
     return k;
+}
+
+size_t vcx_server_poll(vcx_server *s, int64_t *ids, int32_t *status, uint64_t *t_done_ns, size_t max, int wait_ms) {
+    return take_done(s, max, wait_ms, ids, status, t_done_ns, NULL);
+}
+
+size_t vcx_server_poll_served(vcx_server *s, vcx_served *out, size_t max, int wait_ms) {
+    return take_done(s, out ? max : 0, wait_ms, NULL, NULL, NULL, out);
 }
 
 size_t vcx_server_capacity(const vcx_server *s) { return s ? s->cap_atoms : 0; }

@@ -82,6 +82,7 @@ def lib():
         "vcx_server_submit_atoms": (C.c_int64, [vp, sz, vp, vp]),
         "vcx_server_submit_region": (C.c_int64, [vp, i32, Box, vp]),
         "vcx_server_poll": (sz, [vp, vp, vp, vp, sz, i32]),
+        "vcx_server_poll_served": (sz, [vp, vp, sz, i32]),
         "vcx_server_capacity": (sz, [vp]),
         "vcx_server_autotune": (i32, [vp]),
         "vcx_server_tuning": (sz, [vp, vp, sz]),
@@ -413,6 +414,11 @@ class PinnedBuffer:
         self.free()
 
 
+class Served(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("cls", C.c_int32), ("t_submit_ns", C.c_uint64),
+                ("t_admit_ns", C.c_uint64), ("t_done_ns", C.c_uint64), ("batch_atoms", C.c_uint64)]
+
+
 class Server:
     """vcx_server_*: continuous-batching request loop over an opened archive."""
 
@@ -456,6 +462,13 @@ class Server:
 
     def submit_region(self, lod, box, out_ptr):
         return self.L.vcx_server_submit_region(self.h, lod, Box(*box), out_ptr)
+
+    def poll_served(self, max_n=1024, wait_ms=10):
+        """completions with their timing: [(id, status, cls, t_submit, t_admit,
+        t_done, batch_atoms)] (ns, vcx_now_ns clock)"""
+        rec = (Served * max_n)()
+        k = self.L.vcx_server_poll_served(self.h, rec, max_n, wait_ms)
+        return [(r.id, r.status, r.cls, r.t_submit_ns, r.t_admit_ns, r.t_done_ns, r.batch_atoms) for r in rec[:k]]
 
     def poll(self, max_n=1024, wait_ms=10):
         ids = np.zeros(max_n, np.int64)
