@@ -710,7 +710,7 @@ def export_section(pyvc, dvol, n, no_cpu=False):
     return out
 
 
-def ratio_section(pyvc, hip, o, dvol, dout, n, ratios, steps=10):
+def ratio_section(pyvc, hip, o, dvol, dout, n, ratios, steps=30):
     """The same volume encoded at other target ratios: encode time, then
     back-to-back device LOD0 decodes (GB/s) and 16 sampled atoms against the
     oracle."""
