@@ -277,9 +277,15 @@ int kc_idct_m(const uint32_t *L, uint32_t cnt, int dc, const int32_t *st, const 
     uint8_t *t = (uint8_t *)aligned_alloc(16, VCG_IM_TILE_BYTES);
     for (int tid = 0; tid < NTI; ++tid) vcg_im_fill((uint32_t *)t, tid, NTI);
     const int fmax = cnt ? vcg_freq(scan[vcg_ent_pos(L[cnt - 1])]) : -1;
+    // the kernel's tables: per scan position tile offset | frequency, and
+    // per frequency the (step, bias) pair
+    static uint32_t imtab[32768];
+    for (int q = 0; q < 32768; ++q) imtab[q] = vcg_im_tab(scan[q]);
+    int32_t sb[2 * VCG_NFREQ];
+    for (int f = 0; f < VCG_NFREQ; ++f) { sb[2 * f] = st[f]; sb[2 * f + 1] = bi[f]; }
     uint64_t ss = 0;
     uint32_t big = 0;
-    for (int tid = 0; tid < NTI; ++tid) vcg_im_scatter(t, L, cnt, scan, st, bi, tid, NTI, ss, big);
+    for (int tid = 0; tid < NTI; ++tid) vcg_im_scatter(t, L, cnt, imtab, sb, tid, NTI, ss, big);
     if (big || ss > (uint64_t)VCG_IM_L2 * VCG_IM_L2) {
         free(t);
         return kc_idct(L, cnt, dc, st, bi, cmat, scan, 0, out);

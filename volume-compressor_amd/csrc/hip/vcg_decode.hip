@@ -498,10 +498,10 @@ __device__ __forceinline__ void idct_item_m(const dec_job &j, uint32_t i, const 
     uint32_t pre[IDCT_PRE];
 #pragma unroll
     for (int q = 0; q < IDCT_PRE; ++q) pre[q] = L[tid + q * IM_THREADS];
-    if (tid < VCG_NFREQ) {
+    if (tid < VCG_NFREQ) {                   // step and bias of frequency tid as a pair
         const int o = j.mode == MODE_PAYLOAD ? 0 : lod * VCG_NFREQ;
-        dqs[tid] = j.dqs[o + tid];
-        dqs[VCG_NFREQ + tid] = j.dqb[o + tid];
+        dqs[2 * tid] = j.dqs[o + tid];
+        dqs[2 * tid + 1] = j.dqb[o + tid];
     }
     const vcg_item it = items[i];
     const int ln = tid & 63, wv = tid >> 6, jr = ln & 31, hh = ln >> 5;
@@ -515,22 +515,21 @@ __device__ __forceinline__ void idct_item_m(const dec_job &j, uint32_t i, const 
     }
     const uint32_t cnt = it.cnt;
     const int fmax = it.fmax, dc = (int)it.dc;
-    uint32_t sidx[IDCT_PRE];
+    uint32_t tab[IDCT_PRE];
 #pragma unroll
     for (int q = 0; q < IDCT_PRE; ++q)
-        sidx[q] = tid + q * IM_THREADS < (int)cnt ? T->scan[vcg_ent_pos(pre[q])] : 0u;
+        tab[q] = tid + q * IM_THREADS < (int)cnt ? T->imtab[vcg_ent_pos(pre[q])] : 0u;
     vcg_im_fill((uint32_t *)tile, tid, IM_THREADS);
     if (tid == 0) { *ssum = 0; *sbig = 0; }
     __syncthreads();
-    const int32_t *st = dqs, *bi = dqs + VCG_NFREQ;
     uint64_t ss = 0;
     uint32_t big = 0;
 #pragma unroll
     for (int q = 0; q < IDCT_PRE; ++q)
-        if (tid + q * IM_THREADS < (int)cnt) vcg_im_scatter1(tile, pre[q], sidx[q], st, bi, ss, big);
+        if (tid + q * IM_THREADS < (int)cnt) vcg_im_scatter1(tile, pre[q], tab[q], dqs, ss, big);
     if (cnt > IDCT_PRE * IM_THREADS)
-        vcg_im_scatter(tile, L + IDCT_PRE * IM_THREADS, cnt - IDCT_PRE * IM_THREADS, T->scan, st, bi, tid,
-                       IM_THREADS, ss, big);
+        vcg_im_scatter(tile, L + IDCT_PRE * IM_THREADS, cnt - IDCT_PRE * IM_THREADS, T->imtab, dqs, tid, IM_THREADS,
+                       ss, big);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor((unsigned long long)ss, o, 64);
     const bool wbig = __ballot(big != 0) != 0;

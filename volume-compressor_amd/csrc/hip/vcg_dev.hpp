@@ -22,6 +22,7 @@ struct vcg_dtab {
     int8_t   fa[3][64][16];   // forward transform on MFMA: per-lane basis planes Ch, Cl, 4 Cl (vcg_fwd1_tables)
     int32_t  rowsum[32];      // basis row sums
     int8_t   ib[3][64][16];   // inverse on MFMA: per-lane basis planes Ch, 4 Cl, Cl (vcg_im_tables)
+    uint32_t imtab[VCG_A3];   // scan position p: i16-tile byte offset | frequency << 16 (vcg_im_tab)
 };
 extern vcg_dtab *g_dtab;  // device pointer
 

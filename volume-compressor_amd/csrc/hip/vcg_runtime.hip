@@ -55,7 +55,9 @@ extern "C" int vcg_init(const vcg_tables *t) {
                 if ((k & 7) == 0 && n < 2) ok = ok && t->cmat[k][3 - n] == ((k & 8) ? -1 : 1) * t->cmat[k][n];
                 if (!ok) vc__panic("DCT basis lacks the symmetries the inverse transform relies on");
             }
-        vcg_dtab h;
+        // the table image is 240 KB: built in static storage, not on the
+        // calling thread's stack (this runs once)
+        static vcg_dtab h;
         memcpy(h.cmat, t->cmat, sizeof h.cmat);
         memcpy(h.gstep, t->gstep, sizeof h.gstep);
         memcpy(h.band_start, t->band_start, sizeof h.band_start);
@@ -64,6 +66,7 @@ extern "C" int vcg_init(const vcg_tables *t) {
             h.fscan[q] = (uint8_t)((h.scan[q] >> 10) + ((h.scan[q] >> 5) & 31) + (h.scan[q] & 31));
         vcg_fwd1_tables(h.cmat, h.fa, h.rowsum);
         vcg_im_tables(h.cmat, h.ib);
+        for (int q = 0; q < VCG_A3; ++q) h.imtab[q] = vcg_im_tab(h.scan[q]);
         // the allocation-free decode's static scratch: its device address
         if (vcg_noalloc_setup() != 0) vc__panic("hipGetSymbolAddress failed for the decoder's static scratch");
         VCG_CHECK(hipMalloc(&g_dtab, sizeof(vcg_dtab)));

@@ -439,36 +439,42 @@ VCG_HD void vcg_im_fill(uint32_t *t, int tid, int nthr) {
     for (int k = tid; k < VCG_IM_TILE_BYTES / 16; k += nthr) q[k] = u4{VCG_IM_W0, VCG_IM_W0, VCG_IM_W0, VCG_IM_W0};
 }
 
-// Dequantize list entry v (coefficient at raster index idx) as vcg_scatter1
-// does, store w = r + 128 into the i16 tile; ss += r^2 and big |= |r| above
-// the bound (then ss is not needed).
-VCG_HD void vcg_im_scatter1(uint8_t *t, uint32_t v, uint32_t idx, const int32_t *st, const int32_t *bi, uint64_t &ss,
-                            uint32_t &big) {
-    const int lv = vcg_ent_level(v);
-    const int f = vcg_freq(idx);
-    const uint32_t mag = (uint32_t)(lv < 0 ? -lv : lv);
-    int r = (int)(mag * (uint32_t)st[f] + (uint32_t)bi[f] + 128u) >> 8;
-    r = lv < 0 ? (int)(0u - (uint32_t)r) : r;
-    const uint32_t a = r < 0 ? 0u - (uint32_t)r : (uint32_t)r;
-    big |= a > VCG_IM_L2 ? 1u : 0u;
-    ss += (uint64_t)a * a;
+// per scan position: the i16-tile byte offset of its raster index idx and,
+// in bits 16.., its frequency cz + cy + cx
+VCG_HD uint32_t vcg_im_tab(uint32_t idx) {
     const uint32_t L = idx >> 5, x = idx & 31u;
-    *(int16_t *)(t + vcg_im_chunk(L, x >> 3) + 2 * (x & 7u)) = (int16_t)(uint16_t)((uint32_t)r + 128u);
+    return (vcg_im_chunk(L, x >> 3) + 2u * (x & 7u)) | (uint32_t)vcg_freq(idx) << 16;
 }
 
-VCG_HD void vcg_im_scatter(uint8_t *t, const uint32_t *L, uint32_t cnt, const uint16_t *scan, const int32_t *st,
-                           const int32_t *bi, int tid, int nthr, uint64_t &ss, uint32_t &big) {
+// Dequantize list entry v (tab = its position's vcg_im_tab) as vcg_scatter1
+// does (sb: per frequency the Q8 step and bias as a pair) and store w = r +
+// 128 into the i16 tile; ss += r^2 and big |= r above the bound (then ss is
+// not needed).  The magnitude's Q8 product is taken unsigned: one that
+// wrapped past 2^31 (pathological q) counts as above the bound, so such an
+// atom takes the i32 path, which is exact mod 2^32 like the reference.
+VCG_HD void vcg_im_scatter1(uint8_t *t, uint32_t v, uint32_t tab, const int32_t *sb, uint64_t &ss, uint32_t &big) {
+    const int lv = vcg_ent_level(v);
+    const uint32_t f = tab >> 16;
+    const uint32_t mag = (uint32_t)(lv < 0 ? -lv : lv);
+    const uint32_t r = (uint32_t)((int)(mag * (uint32_t)sb[2 * f] + (uint32_t)sb[2 * f + 1] + 128u) >> 8);
+    big |= r > VCG_IM_L2 ? 1u : 0u;
+    ss += (uint64_t)r * r;
+    *(int16_t *)(t + (tab & 0xFFFFu)) = (int16_t)(uint16_t)(lv < 0 ? 128u - r : 128u + r);
+}
+
+VCG_HD void vcg_im_scatter(uint8_t *t, const uint32_t *L, uint32_t cnt, const uint32_t *imtab, const int32_t *sb,
+                           int tid, int nthr, uint64_t &ss, uint32_t &big) {
     const uint32_t un = (uint32_t)nthr;
     for (uint32_t e0 = (uint32_t)tid; e0 < cnt; e0 += 8u * un) {
-        uint32_t v[8], idx[8];
+        uint32_t v[8], tab[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) v[q] = e0 + q * un < cnt ? L[e0 + q * un] : 0u;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) idx[q] = e0 + q * un < cnt ? scan[vcg_ent_pos(v[q])] : 0u;
+        for (int q = 0; q < 8; ++q) tab[q] = e0 + q * un < cnt ? imtab[vcg_ent_pos(v[q])] : 0u;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             if (e0 + q * un >= cnt) break;
-            vcg_im_scatter1(t, v[q], idx[q], st, bi, ss, big);
+            vcg_im_scatter1(t, v[q], tab[q], sb, ss, big);
         }
     }
 }
