@@ -59,9 +59,10 @@ static void wave_refill(vcg_fsm &f, vcg_look &L, const uint8_t *s, uint32_t n, b
     const uint32_t o = f.E - L.e < 8u ? f.E - L.e : 8u;
     const uint32_t a = 4u * o, b = 32u - a;
     const uint64_t data = ((L.lo >> a) >> a) | ((L.hi << b) << b);
-    f.w |= f.wb < 64u ? data << f.wb : 0ull;
-    f.E += (64u - f.wb) >> 3;
-    f.wb = 64u;
+    const uint32_t c = f.pc & 0xFFu;
+    f.w |= c ? data << (64u - c) : 0ull;
+    f.E += c >> 3;
+    f.pc &= ~0xFFu;
     vcg_look_load(L, s, n, f.E);
 }
 int kc_decode_wave(const uint8_t *buf, const uint32_t *offs, const uint32_t *lens, int nl,

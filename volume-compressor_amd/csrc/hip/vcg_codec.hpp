@@ -167,8 +167,8 @@ VCG_HD int vcg_rc_decode_atom(const uint8_t *s, uint32_t n, const uint32_t *band
 // test while every lane is more than k positions before its next edge and
 // below state 78 - k (vcg_fsm_near).
 //
-// Input: a 64-bit little-endian window of wb valid bits ending at stream byte
-// E.  Every 8 steps (a step consumes at most 8 bits: one renormalisation per
+// Input: a 64-bit little-endian window of 64 - c valid bits ending at stream
+// byte E, c the bits consumed since the last top-up.  Every 8 steps (a step consumes at most 8 bits: one renormalisation per
 // bin, see vcg_rd_bit) the window is topped up to 64 bits.  Refills alternate
 // between two 16-byte look-ahead buffers, each loaded at the window end of its
 // previous use, two refills (16 steps) before it is needed: E has moved by at
@@ -176,14 +176,15 @@ VCG_HD int vcg_rc_decode_atom(const uint8_t *s, uint32_t n, const uint32_t *band
 // s[n] are masked to zero (past-end bytes read as 0, vc.c:425); loads are
 // clamped to s + n, so the buffer must extend 16 bytes past s[n-1].
 //
-// Output: each step writes its would-be entry into slot cnt % 8 of the lane's
-// ring and advances cnt only on an emission; every 16 steps the caller copies
-// ring slots [flushed, flushed+6) to the list (at most 6 emissions fit in 16
-// steps: a coefficient takes at least 3 bins), so list stores are unmasked and
-// entries past cnt are overwritten by the next flush.  The position and cnt
-// share one register: position << 17 | cnt mod 64 << 8, so one add advances
-// both; the flush counts its new entries as cnt - flushed mod 8 and clears the
-// bits of cnt above its ring slot.
+// Output: each step writes its would-be entry into slot k of the lane's ring,
+// k the entries emitted since the last flush, and advances k only on an
+// emission; every 16 steps the caller copies ring slots [0, 6) to the list
+// (at most 6 emissions fit in 16 steps: a coefficient takes at least 3 bins)
+// and k restarts at 0, so the ring addresses of a flush are constants, list
+// stores are unmasked and entries past the count are overwritten by the next
+// flush.  One register holds position << 17 | k << 8 | c: an add of three
+// terms per step advances the position, k (an emission is also a new
+// position) and the consumed bits c; the top-up clears c and the flush k.
 // ---------------------------------------------------------------------------
 #ifndef VCG_ENTROPY_FSM
 #define VCG_ENTROPY_FSM 1
@@ -221,10 +222,10 @@ VCG_HD void vcg_look_load(vcg_look &L, const uint8_t *s, uint32_t n, uint32_t e)
 
 struct vcg_fsm {
     uint32_t code, range;
-    uint64_t w;                  // window, wb valid bits, ending at stream byte E
-    uint32_t wb, E;
+    uint64_t w;                  // window, 64 - c valid bits, ending at stream byte E
+    uint32_t E;
     uint32_t P, S, psig, acc;
-    uint32_t pc;                 // scan position << 17 (mod 2^32) | emitted count mod 64 << 8
+    uint32_t pc;                 // scan position << 17 (mod 2^32) | k << 8 | c (see above)
     uint32_t lim17;              // next band edge / EOB << 17 (mod 2^32)
     uint32_t eob;
     uint32_t fl;                 // entries flushed to the list
@@ -243,10 +244,11 @@ VCG_HD void vcg_fsm_refill(vcg_fsm &f, vcg_look &L, const uint8_t *s, uint32_t n
     const uint32_t o = f.E - L.e < 8u ? f.E - L.e : 8u;   // >= 8 only for a parked lane
     const uint32_t a = 4u * o, b = 32u - a;                 // two shifts each, all < 64
     const uint64_t data = ((L.lo >> a) >> a) | ((L.hi << b) << b);
-    const uint64_t app = f.wb < 64u ? (data & vcg_tail_mask(f.E, n)) << f.wb : 0ull;
+    const uint32_t c = f.pc & 0xFFu;                        // consumed bits, a multiple of 8, <= 64
+    const uint64_t app = c ? (data & vcg_tail_mask(f.E, n)) << (64u - c) : 0ull;
     f.w |= app;
-    f.E += (64u - f.wb) >> 3;
-    f.wb = 64u;
+    f.E += c >> 3;
+    f.pc &= ~0xFFu;
     vcg_look_load(L, s, n, f.E);
 }
 
@@ -268,25 +270,40 @@ VCG_HD uint32_t vcg_low_bits(uint32_t x, uint32_t width) {
 #endif
 }
 
-VCG_HD void vcg_fsm_renorm(vcg_fsm &f) {
-    const uint32_t sh = (uint32_t)__builtin_clz(f.range) & 8u;   // range >= 2^17 here
+// a + b + c as one v_add3_u32 (the compiler turns b + c of disjoint bits into
+// an or first, which costs a second instruction)
+VCG_HD uint32_t vcg_add3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t r;
+    asm("v_add3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+#else
+    return a + b + c;
+#endif
+}
+
+// renormalise (at most one byte, range >= 2^17 here); returns the bits taken
+// from the window, for the caller to add to the consumed count
+VCG_HD uint32_t vcg_fsm_renorm_sh(vcg_fsm &f) {
+    const uint32_t sh = (uint32_t)__builtin_clz(f.range) & 8u;
     f.code = (f.code << sh) | vcg_low_bits((uint32_t)f.w, sh);
     f.range <<= sh;
     f.w >>= sh;
-    f.wb -= sh;
+    return sh;
 }
+VCG_HD void vcg_fsm_renorm(vcg_fsm &f) { f.pc += vcg_fsm_renorm_sh(f); }
 
 // s: first range-coder byte; n: byte count.  Context memory slots 1..12 must
 // hold 2048 and slots 13 and 15 VCG_P_BYPASS on entry.
 VCG_HD void vcg_fsm_init(vcg_fsm &f, vcg_look &A, vcg_look &B, const uint8_t *s, uint32_t n,
                          const uint32_t *band_start) {
-    f.w = 0; f.wb = 0; f.E = 0;
+    f.w = 0; f.pc = 64u; f.E = 0;                     // an empty window: 64 bits consumed
     vcg_look_load(A, s, n, 0);
-    vcg_fsm_refill(f, A, s, n);                       // bytes [0, 8): wb = 64
+    vcg_fsm_refill(f, A, s, n);                       // bytes [0, 8): c = 0
     f.code = 0;
-    for (int k = 0; k < 5; ++k) { f.code = (f.code << 8) | ((uint32_t)f.w & 0xFFu); f.w >>= 8; f.wb -= 8; }   // vc.c:406-409
+    for (int k = 0; k < 5; ++k) { f.code = (f.code << 8) | ((uint32_t)f.w & 0xFFu); f.w >>= 8; f.pc += 8u; }   // vc.c:406-409
     f.range = 0xFFFFFFFFu;
-    vcg_fsm_refill(f, A, s, n);                       // A was reloaded at 8: bytes [8, 13), wb = 64;
+    vcg_fsm_refill(f, A, s, n);                       // A was reloaded at 8: bytes [8, 13), c = 0;
                                                       // the EOB renormalises at most twice
     uint32_t eob = 0;
     for (int k = 0; k < 16; ++k) {                    // vc.c:520-525
@@ -299,7 +316,7 @@ VCG_HD void vcg_fsm_init(vcg_fsm &f, vcg_look &A, vcg_look &B, const uint8_t *s,
     B = A;                                            // both at the window end (13)
     eob = eob > 32768u ? 32768u : eob;                // vc.c:549
     f.eob = eob;
-    f.P = 2048u; f.S = 0; f.psig = 2048u; f.acc = 0; f.pc = 0; f.fl = 0;
+    f.P = 2048u; f.S = 0; f.psig = 2048u; f.acc = 0; f.pc &= 0xFFu; f.fl = 0;
     f.band = 0; f.bad = 0;
     f.lim17 = (band_start[1] < eob ? band_start[1] : eob) << 17;   // EOB 32768 wraps to 0: met only at the end
     if (eob == 0) vcg_fsm_park(f);
@@ -333,16 +350,16 @@ VCG_HD uint32_t vcg_fsm_step(vcg_fsm &f, Ctx &ctx, Ring &ring) {
     const uint32_t pn = ((bit ? 31u : 4096u) + vcg_mul24(31u, P)) >> 5;
     ctx.st(a, pn);
     f.psig = isSig ? pn : f.psig;
-    vcg_fsm_renorm(f);
+    const uint32_t sh = vcg_fsm_renorm_sh(f);
     // symbol bookkeeping
     const bool tail = S >= VCG_S_TAIL;
     const bool sgn = S == VCG_S_SIGN;
     const uint32_t acc2 = 2u * f.acc + (bit ? 0u - 10u : 0u - 11u);
     ring.st(f.pc & ((VCG_RING - 1u) << 8), (f.pc & VCG_POS_MASK) | (acc2 & 0x1FFFFu));
     const bool adv = (isSig & nbit) | sgn;
-    // one add moves the position (every new position) and the ring slot (an
-    // emission, which is also a new position)
-    f.pc += sgn ? 0x20100u : adv ? 0x20000u : 0u;
+    // one add moves the position (every new position), the ring slot (an
+    // emission, which is also a new position) and the consumed bits
+    f.pc = vcg_add3(f.pc, sgn ? 0x20100u : adv ? 0x20000u : 0u, sh);
     f.acc = tail ? acc2 : (S < 12u ? S : 12u);
     const uint32_t up = (bit || tail) ? S + 1u : 268u - (S > VCG_S_PFX ? S : VCG_S_PFX);
     f.S = adv ? 0u : up;
@@ -382,17 +399,18 @@ VCG_HD void vcg_fsm_block(vcg_fsm &f, Ctx &ctx, Ring &ring, const uint32_t *band
     }
 }
 
-// copy ring slots [flushed, flushed + VCG_FLUSH_N) to out (entries past cnt
+// copy ring slots [0, VCG_FLUSH_N) to out + flushed (entries past the count
 // are placeholders that a later flush overwrites; out needs VCG_FLUSH_N - 1
-// slack entries past the list capacity).  Returns the number of real entries.
+// slack entries past the list capacity) and restart the ring.  Returns the
+// number of real entries (at most VCG_FLUSH_N).
 template <class Ring>
 VCG_HD uint32_t vcg_fsm_flush(vcg_fsm &f, Ring &ring, uint32_t *out) {
-    const uint32_t pending = ((f.pc >> 8) - f.fl) & (VCG_RING - 1u);    // at most VCG_FLUSH_N < VCG_RING
+    const uint32_t pending = (f.pc >> 8) & 0x1FFu;
     uint32_t *o = out + f.fl;
 #pragma unroll
-    for (uint32_t k = 0; k < VCG_FLUSH_N; ++k) o[k] = ring.ld(((f.fl + k) << 8) & ((VCG_RING - 1u) << 8));
+    for (uint32_t k = 0; k < VCG_FLUSH_N; ++k) o[k] = ring.ld(k << 8);
     f.fl += pending;
-    f.pc &= ~0x1F800u;                                // keep the emission count field below bit 17
+    f.pc &= ~0x1FF00u;
     return pending;
 }
 

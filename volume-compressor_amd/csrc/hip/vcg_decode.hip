@@ -167,8 +167,9 @@ __device__ __forceinline__ void ent_atom(const dec_job &j, uint32_t i, const vcg
     vcg_fsm_flush(f, ring, out);
     const int bad = (int)f.bad;
     const uint32_t cnt = f.fl;
-    // the last entry (highest scan position) is still in the ring
-    const int fmax = cnt ? vcg_freq(T->scan[vcg_ent_pos(ring.ld(((cnt - 1u) << 8) & ((VCG_RING - 1u) << 8)))]) : -1;
+    // the last entry (highest scan position), read back from the list this
+    // lane has just written
+    const int fmax = cnt ? vcg_freq(T->scan[vcg_ent_pos(out[cnt - 1u])]) : -1;
 #else
     (void)ctxm; (void)ringm; (void)bands; (void)lane; (void)wave;
     uint32_t cnt = 0;
