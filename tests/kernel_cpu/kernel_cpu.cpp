@@ -99,10 +99,11 @@ int kc_decode_wave(const uint8_t *buf, const uint32_t *offs, const uint32_t *len
             if (!vcg_fsm_parked(f[k]) && f[k].E + 16u > n[k]) return true;
         return false;
     };
-    while (any_active()) {
+    for (uint32_t it = 0; any_active(); ++it) {
         bool m = masked();
         for (int k = 0; k < nl; ++k) wave_refill(f[k], A[k], s[k], n[k], m);
-        for (int k = 0; k < nl; ++k) vcg_fsm_flush(f[k], ring[k], lane_out(k));
+        if (it % VCG_FLUSH_EVERY == VCG_FLUSH_EVERY - 1)
+            for (int k = 0; k < nl; ++k) vcg_fsm_flush(f[k], ring[k], lane_out(k));
         bool chk = any_near();
         for (int k = 0; k < nl; ++k) vcg_fsm_block(f[k], ctx[k], ring[k], band_start, chk);
         m = masked();

@@ -249,7 +249,7 @@ def test_kernel_quantizer_exhaustive(kc, tabs, oracle):
 def fsm_decode(kc, tabs, rc, mis, check=0):
     cm, gs, scan, bs = tabs
     b = np.frombuffer(rc, np.uint8) if len(rc) else np.zeros(1, np.uint8)
-    ent = np.zeros(32768 + 8, np.uint32)          # flushes write up to 5 placeholder entries past the end
+    ent = np.zeros(32768 + 64, np.uint32)         # the device list stride: flushes write up to 21 placeholders past the end
     cnt = C.c_uint32()
     st = kc.kc_decode_fsm(b.ctypes.data, len(rc), mis, bs.ctypes.data, ent.ctypes.data, C.byref(cnt), check)
     return st, ent[:cnt.value]

@@ -178,11 +178,14 @@ VCG_HD int vcg_rc_decode_atom(const uint8_t *s, uint32_t n, const uint32_t *band
 //
 // Output: each step writes its would-be entry into slot k of the lane's ring,
 // k the entries emitted since the last flush, and advances k only on an
-// emission; every 16 steps the caller copies ring slots [0, 6) to the list
-// (at most 6 emissions fit in 16 steps: a coefficient takes at least 3 bins)
-// and k restarts at 0, so the ring addresses of a flush are constants, list
-// stores are unmasked and entries past the count are overwritten by the next
-// flush.  One register holds position << 17 | k << 8 | c: an add of three
+// emission; every 64 steps the caller copies ring slots [0, 22) to the list
+// (at most 22 emissions fit in 64 steps: a coefficient takes at least 3
+// bins) and k restarts at 0, so the ring addresses of a flush are constants,
+// list stores are unmasked and entries past the count are overwritten by the
+// next flush.  Flushing every 64 steps rather than every 16 (a 32-slot ring
+// instead of 8) quarters the partly written list lines and the stores that
+// carry them; those stores, contending with the transform kernel's traffic,
+// held back the entropy waves that share their vector memory counter.  One register holds position << 17 | k << 8 | c: an add of three
 // terms per step advances the position, k (an emission is also a new
 // position) and the consumed bits c; the top-up clears c and the flush k.
 // ---------------------------------------------------------------------------
@@ -190,9 +193,9 @@ VCG_HD int vcg_rc_decode_atom(const uint8_t *s, uint32_t n, const uint32_t *band
 #define VCG_ENTROPY_FSM 1
 #endif
 #define VCG_REFILL_STEPS 8
-#define VCG_FLUSH_STEPS 16
-#define VCG_FLUSH_N 6u
-#define VCG_RING 8u
+#define VCG_FLUSH_EVERY 4                       // loop iterations (of 16 steps) per flush
+#define VCG_FLUSH_N 22u
+#define VCG_RING 32u
 #define VCG_PARK_E 0x40000000u
 #define VCG_P_BYPASS (2048u | 4096u)
 #define VCG_S_PFX 13u
@@ -328,8 +331,19 @@ VCG_HD bool vcg_fsm_near(const vcg_fsm &f, uint32_t k) {
     return (f.lim17 - (f.pc & VCG_POS_MASK) <= (k << 17)) | (f.S - (VCG_S_BAD - k) < VCG_S_TAIL - VCG_S_BAD + k);
 }
 
+// the state after a bin in state S with bit b (the transitions above); a new
+// position (after a zero significance bin or the sign bin) is state 0
+VCG_HD uint32_t vcg_fsm_next(uint32_t S, uint32_t b) {
+    if (S == VCG_S_SIGN) return 0u;
+    if (b || S >= VCG_S_TAIL) return S + 1u;
+    if (S == 0u) return 0u;
+    return 268u - (S > VCG_S_PFX ? S : VCG_S_PFX);
+}
+
 // one bin.  Ctx: h = at(S, S < 13) the handle of the slot a bin in state S
-// writes (S, or the sink 14), ld_next(h) the next slot's value, st(h, v);
+// writes (S, or the sink 14), ld_next(h) the next slot's value, st(h, v),
+// nxt(S + 256 b) = vcg_fsm_next(S, b) (a table: two byte loads that do not
+// wait on the bit replace the transition arithmetic);
 // Ring: st(slot << 8, v) on the lane's output ring.  With CHECK, returns nonzero
 // when pos reached a band edge or EOB, or the prefix passed its cap (the
 // caller then runs vcg_fsm_edge); without, the caller has ruled these out.
@@ -339,6 +353,7 @@ VCG_HD uint32_t vcg_fsm_step(vcg_fsm &f, Ctx &ctx, Ring &ring) {
     const bool isCtx = S < VCG_S_PFX;
     const uint32_t a = ctx.at(S, isCtx);
     const uint32_t R = ctx.ld_next(a);
+    const uint32_t n0 = ctx.nxt(S), n1 = ctx.nxt(S + 256u);
     // binary decision (vc.c:414-441)
     const uint32_t r0 = (uint32_t)(((uint64_t)f.range * (P << 20)) >> 32);
     const bool nbit = f.code < r0, bit = !nbit;
@@ -356,13 +371,12 @@ VCG_HD uint32_t vcg_fsm_step(vcg_fsm &f, Ctx &ctx, Ring &ring) {
     const bool sgn = S == VCG_S_SIGN;
     const uint32_t acc2 = 2u * f.acc + (bit ? 0u - 10u : 0u - 11u);
     ring.st(f.pc & ((VCG_RING - 1u) << 8), (f.pc & VCG_POS_MASK) | (acc2 & 0x1FFFFu));
-    const bool adv = (isSig & nbit) | sgn;
+    f.S = bit ? n1 : n0;
+    const bool adv = f.S == 0u;                     // a new position: after (sig, 0) or the sign bin
     // one add moves the position (every new position), the ring slot (an
     // emission, which is also a new position) and the consumed bits
     f.pc = vcg_add3(f.pc, sgn ? 0x20100u : adv ? 0x20000u : 0u, sh);
     f.acc = tail ? acc2 : (S < 12u ? S : 12u);
-    const uint32_t up = (bit || tail) ? S + 1u : 268u - (S > VCG_S_PFX ? S : VCG_S_PFX);
-    f.S = adv ? 0u : up;
     f.P = adv ? f.psig : (bit ? R : VCG_P_BYPASS);
     if (!CHECK) return 0;
     return ((f.pc & VCG_POS_MASK) == f.lim17) | (f.S - VCG_S_BAD < VCG_S_TAIL - VCG_S_BAD);
@@ -426,6 +440,7 @@ struct vcg_ctx_array {
     VCG_HDM uint32_t ld_next(uint32_t h) const { return v[h + 1u]; }
     VCG_HDM uint32_t ld(uint32_t k) const { return v[k]; }
     VCG_HDM void st(uint32_t h, uint32_t p) { v[h] = p; }
+    VCG_HDM uint32_t nxt(uint32_t i) const { return vcg_fsm_next(i & 255u, i >> 8); }
 };
 struct vcg_ring_array {
     uint32_t v[VCG_RING];
@@ -442,9 +457,9 @@ VCG_HD int vcg_rc_decode_atom_fsm(const uint8_t *s, uint32_t n, const uint32_t *
     vcg_look A, B;
     vcg_fsm_init(f, A, B, s, n, band_start);
     int over = 0;
-    while (!vcg_fsm_parked(f)) {
+    for (uint32_t it = 0; !vcg_fsm_parked(f); ++it) {
         vcg_fsm_refill(f, A, s, n);
-        if (vcg_fsm_flush(f, ring, out) > VCG_FLUSH_N) over = 1;
+        if (it % VCG_FLUSH_EVERY == VCG_FLUSH_EVERY - 1 && vcg_fsm_flush(f, ring, out) > VCG_FLUSH_N) over = 1;
         vcg_fsm_block(f, ctx, ring, band_start, always_check || vcg_fsm_near(f, VCG_REFILL_STEPS));
         vcg_fsm_refill(f, B, s, n);
         vcg_fsm_block(f, ctx, ring, band_start, always_check || vcg_fsm_near(f, VCG_REFILL_STEPS));

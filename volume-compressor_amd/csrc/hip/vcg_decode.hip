@@ -72,7 +72,7 @@ __device__ __forceinline__ bool item_atom(const dec_job &j, uint32_t gi, int &lo
     return true;
 }
 
-// per-lane LDS array (context memory: VCG_CTX_SLOTS slots; output ring: 8 slots),
+// per-lane LDS array (context memory: VCG_CTX_SLOTS slots; output ring: VCG_RING slots),
 // slot-major over the 64 lanes of the wave so a wave's access is conflict-free.
 // m is the workgroup's array (a fixed LDS address) and l4 the lane's byte
 // offset in it: the wave's part (a power of two in size) ORed with lane * 4,
@@ -82,8 +82,10 @@ __device__ __forceinline__ bool item_atom(const dec_job &j, uint32_t gi, int &lo
 // select after the shift-or.
 struct lds_ctx {
     uint32_t *m;
+    const uint8_t *nx;           // the workgroup's next-state table (vcg_fsm_next)
     uint32_t l4, sink;
     __device__ __forceinline__ uint32_t at(uint32_t S, bool isCtx) const { return isCtx ? (S << 8) | l4 : sink; }
+    __device__ __forceinline__ uint32_t nxt(uint32_t i) const { return nx[i]; }
     __device__ __forceinline__ uint32_t ld_next(uint32_t h) const {
         return *(const uint32_t *)((const char *)m + h + 256u);
     }
@@ -107,12 +109,12 @@ struct lds_ring {
 // One lane's atom (item i of the launch): resolve it through the device copy
 // of the archive's index, range-decode its payload and write its coefficient
 // list and item record.  ctxm / ringm: the workgroup's context and output-ring
-// LDS arrays, a wave's part at byte (wave << 12) and (wave << 11), so a lane's
-// offset ORs into the slot addressing.
+// LDS arrays, a wave's part at byte (wave << 12) and wave * VCG_RING * 256, so
+// a lane's offset ORs into the slot addressing.
 __device__ __forceinline__ void ent_atom(const dec_job &j, uint32_t i, const vcg_dtab *__restrict__ T,
                                          uint32_t *__restrict__ lists, vcg_item *__restrict__ items,
                                          uint32_t *__restrict__ status, uint32_t *ctxm, uint32_t *ringm,
-                                         const uint32_t *bands, uint32_t lane, uint32_t wave) {
+                                         const uint8_t *nxt, const uint32_t *bands, uint32_t lane, uint32_t wave) {
     const uint32_t gi = j.base + i;
     int lod = 0;
     uint64_t off = 0;
@@ -143,9 +145,9 @@ __device__ __forceinline__ void ent_atom(const dec_job &j, uint32_t i, const vcg
     const uint32_t cl4 = (wave << 12) | (lane * 4u);
     uint32_t sink = (14u << 8) | cl4;
     asm volatile("" : "+v"(sink));
-    lds_ctx ctx{ctxm, cl4, sink};
+    lds_ctx ctx{ctxm, nxt, cl4, sink};
     // opaque, so that the slot offset and the lane offset stay one and-or
-    uint32_t rl4 = (wave << 11) | (lane * 4u);
+    uint32_t rl4 = (wave * (VCG_RING << 8)) | (lane * 4u);
     asm volatile("" : "+v"(rl4));
     lds_ring ring{ringm, rl4};
     for (uint32_t k = 1; k < VCG_CTX_SLOTS; ++k)
@@ -157,9 +159,9 @@ __device__ __forceinline__ void ent_atom(const dec_job &j, uint32_t i, const vcg
     vcg_fsm_init(f, A, B, s, n, bands);
     // the edge test is needed in a block only if some lane of the wave can
     // meet an edge in it (a wave-uniform branch)
-    while (__any(!vcg_fsm_parked(f))) {
+    for (uint32_t it = 0; __any(!vcg_fsm_parked(f)); ++it) {
         vcg_fsm_refill(f, A, s, n);
-        vcg_fsm_flush(f, ring, out);
+        if (it % VCG_FLUSH_EVERY == VCG_FLUSH_EVERY - 1) vcg_fsm_flush(f, ring, out);
         vcg_fsm_block(f, ctx, ring, bands, __any(vcg_fsm_near(f, VCG_REFILL_STEPS)));
         vcg_fsm_refill(f, B, s, n);
         vcg_fsm_block(f, ctx, ring, bands, __any(vcg_fsm_near(f, VCG_REFILL_STEPS)));
@@ -171,7 +173,7 @@ __device__ __forceinline__ void ent_atom(const dec_job &j, uint32_t i, const vcg
     // lane has just written
     const int fmax = cnt ? vcg_freq(T->scan[vcg_ent_pos(out[cnt - 1u])]) : -1;
 #else
-    (void)ctxm; (void)ringm; (void)bands; (void)lane; (void)wave;
+    (void)ctxm; (void)ringm; (void)nxt; (void)bands; (void)lane; (void)wave;
     uint32_t cnt = 0;
     const int bad = vcg_rc_decode_atom(p + 1, len - 1, T->band_start, out, &cnt);
     const int fmax = cnt && !bad ? vcg_freq(T->scan[vcg_ent_pos(out[cnt - 1])]) : -1;
@@ -194,14 +196,17 @@ __global__ __launch_bounds__(ENT_THREADS) void k_entropy(dec_job j, const vcg_dt
     __builtin_amdgcn_s_setprio(3);
     __shared__ uint32_t ctxw[ENT_WAVES][VCG_CTX_SLOTS * 64];
     __shared__ uint32_t ringw[ENT_WAVES][VCG_RING * 64];
+    __shared__ uint8_t nxt[512];
     __shared__ uint32_t bands[9];
     // band table in LDS: a band edge then waits on LDS only, not on the
     // look-ahead loads in flight
     if (threadIdx.x < 9) bands[threadIdx.x] = T->band_start[threadIdx.x];
+    for (uint32_t t = threadIdx.x; t < 512u; t += ENT_THREADS) nxt[t] = (uint8_t)vcg_fsm_next(t & 255u, t >> 8);
     __syncthreads();
     const uint32_t i = blockIdx.x * (uint32_t)ENT_THREADS + threadIdx.x;
     if (i >= j.n) return;
-    ent_atom(j, i, T, lists, items, status, &ctxw[0][0], &ringw[0][0], bands, threadIdx.x & 63u, threadIdx.x >> 6);
+    ent_atom(j, i, T, lists, items, status, &ctxw[0][0], &ringw[0][0], nxt, bands, threadIdx.x & 63u,
+             threadIdx.x >> 6);
 }
 
 // ---------------------------------------------------------------------------
@@ -395,8 +400,10 @@ __global__ __launch_bounds__(64 * W) void k_entropy_p(dec_job j, const vcg_dtab 
     __builtin_amdgcn_s_setprio(3);
     __shared__ uint32_t ctxw[W][VCG_CTX_SLOTS * 64];
     __shared__ uint32_t ringw[W][VCG_RING * 64];
+    __shared__ uint8_t nxt[512];
     __shared__ uint32_t bands[9];
     if (threadIdx.x < 9) bands[threadIdx.x] = T->band_start[threadIdx.x];
+    for (uint32_t t = threadIdx.x; t < 512u; t += 64u * W) nxt[t] = (uint8_t)vcg_fsm_next(t & 255u, t >> 8);
     __syncthreads();
     const uint32_t n = q->n;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -406,7 +413,7 @@ __global__ __launch_bounds__(64 * W) void k_entropy_p(dec_job j, const vcg_dtab 
         base = __shfl(base, 0, 64);
         if (base >= n) break;
         const uint32_t i = base + lane;
-        if (i < n) ent_atom(j, i, T, lists, items, status, &ctxw[0][0], &ringw[0][0], bands, lane, wave);
+        if (i < n) ent_atom(j, i, T, lists, items, status, &ctxw[0][0], &ringw[0][0], nxt, bands, lane, wave);
     }
 }
 
@@ -679,7 +686,7 @@ static const size_t LDS_M_BYTES = VCG_IM_TILE_BYTES;
 // unchanged, it never places two workgroups on a CU).  The server's
 // four-wave persistent entropy launches carry the same padding (its
 // one- and two-wave shapes do not: they serve small batches).
-static const size_t ENT_LDS_STATIC = (size_t)ENT_WAVES * (VCG_CTX_SLOTS + VCG_RING) * 64u * 4u + 9u * 4u;
+static const size_t ENT_LDS_STATIC = (size_t)ENT_WAVES * (VCG_CTX_SLOTS + VCG_RING) * 64u * 4u + 512u + 9u * 4u;
 static const size_t ENT_LDS_TOTAL = 84u * 1024u;
 static const size_t ENT_LDS_PAD = ENT_LDS_TOTAL > ENT_LDS_STATIC ? ENT_LDS_TOTAL - ENT_LDS_STATIC : 0;
 static_assert(2 * ENT_LDS_TOTAL > 160u * 1024u, "two entropy workgroups must not fit one CU");
