@@ -196,6 +196,13 @@ VCG_HD int vcg_rc_decode_atom(const uint8_t *s, uint32_t n, const uint32_t *band
 #define VCG_FLUSH_EVERY 4                       // loop iterations (of 16 steps) per flush
 #define VCG_FLUSH_N 22u
 #define VCG_RING 32u
+// a flush covers 16 * VCG_FLUSH_EVERY steps, and a coefficient takes at
+// least 3 bins; the ring holds the would-be entry past the last emission too,
+// its slot index sits in bits 8..16 of the position register, and slot
+// offsets OR into the lane's offset (a power of two)
+static_assert(VCG_FLUSH_N >= (16u * VCG_FLUSH_EVERY + 2u) / 3u, "a flush must cover every emission of its steps");
+static_assert(VCG_RING > VCG_FLUSH_N && (VCG_RING & (VCG_RING - 1u)) == 0u && VCG_RING <= 256u,
+              "ring slots: a power of two above the flush's entries");
 #define VCG_PARK_E 0x40000000u
 #define VCG_P_BYPASS (2048u | 4096u)
 #define VCG_S_PFX 13u
