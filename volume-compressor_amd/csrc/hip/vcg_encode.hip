@@ -23,6 +23,10 @@
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
 #include <string.h>
+#include <atomic>
+#include <condition_variable>
+#include <mutex>
+#include <sys/mman.h>
 #include <thread>
 #include <vector>
 #include "vcg_dev.hpp"
@@ -783,20 +787,86 @@ struct enc_scratch {
 #define HSTAGE_BYTES (32u << 20)
 
 // memcpy into a freshly allocated (pageable) buffer: the first touch of its
-// pages dominates, so large copies are split over a few host threads
-static void par_copy(uint8_t *dst, const uint8_t *src, size_t n) {
-    const size_t part = 1u << 20;
-    size_t nt = n / part;
-    if (nt < 2) { memcpy(dst, src, n); return; }
-    if (nt > 8) nt = 8;
-    const size_t each = (n + nt - 1) / nt;
-    std::vector<std::thread> th;
-    for (size_t t = 1; t < nt; ++t) {
-        const size_t o = t * each, m = n - o < each ? n - o : each;
-        th.emplace_back([=] { memcpy(dst + o, src + o, m); });
+// pages dominates, so large copies are split in 1 MiB parts over the caller
+// and a pool of host threads (started once, kept until the process ends;
+// a batch of hundreds of archives would otherwise start thousands of
+// threads).  One copy at a time uses the pool; a concurrent caller copies
+// alone.  The destination is offered transparent huge pages first (fewer
+// first-touch faults where the kernel allows them).
+struct copy_pool {
+    std::mutex busy, m;
+    std::condition_variable go, fin;
+    uint64_t gen = 0;
+    int active = 0;                       // workers inside the current job (under m)
+    uint8_t *dst = nullptr;
+    const uint8_t *src = nullptr;
+    size_t n = 0, parts = 0;
+    std::atomic<size_t> next{0}, done{0};
+    int nthr = 0;
+    // parts are claimed from `next`; the job's fields change only while no
+    // worker is inside a job (active == 0, under m)
+    void run_parts() {
+        for (size_t k; (k = next.fetch_add(1)) < parts;) {
+            const size_t o = k << 20, len = n - o < (1u << 20) ? n - o : (1u << 20);
+            memcpy(dst + o, src + o, len);
+            done.fetch_add(1);
+        }
     }
-    memcpy(dst, src, each);
-    for (auto &t : th) t.join();
+    void worker() {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> l(m);
+                go.wait(l, [&] { return gen != seen; });
+                seen = gen;
+                ++active;
+            }
+            run_parts();
+            {
+                std::lock_guard<std::mutex> l(m);
+                --active;
+            }
+            fin.notify_all();
+        }
+    }
+};
+static copy_pool *pool_get() {
+    static copy_pool *p = [] {
+        copy_pool *q = new copy_pool;            // never destroyed: its threads live until exit
+        const unsigned hw = std::thread::hardware_concurrency();
+        q->nthr = hw >= 16 ? 7 : hw >= 4 ? (int)hw / 2 - 1 : 0;
+        for (int t = 0; t < q->nthr; ++t) std::thread([q] { q->worker(); }).detach();
+        return q;
+    }();
+    return p;
+}
+static void par_copy(uint8_t *dst, const uint8_t *src, size_t n) {
+    if (n >= (4u << 20)) {
+        const uintptr_t a0 = ((uintptr_t)dst + 4095u) & ~(uintptr_t)4095u;
+        const uintptr_t a1 = ((uintptr_t)dst + n) & ~(uintptr_t)4095u;
+        if (a1 > a0) (void)madvise((void *)a0, a1 - a0, MADV_HUGEPAGE);
+    }
+    copy_pool *p = n >= (2u << 20) ? pool_get() : nullptr;
+    if (!p || p->nthr == 0 || !p->busy.try_lock()) { memcpy(dst, src, n); return; }
+    {
+        // a worker that woke for the previous job after it ended leaves first,
+        // so the job's fields change only with no worker inside a job
+        std::unique_lock<std::mutex> l(p->m);
+        p->fin.wait(l, [&] { return p->active == 0; });
+        p->dst = dst; p->src = src; p->n = n; p->parts = (n + (1u << 20) - 1) >> 20;
+        p->done.store(0);
+        p->next.store(0);
+        ++p->gen;
+    }
+    p->go.notify_all();
+    p->run_parts();
+    {
+        // every part copied, and no worker still inside the job (one that
+        // joins late finds no part left, or counts as active until it leaves)
+        std::unique_lock<std::mutex> l(p->m);
+        p->fin.wait(l, [&] { return p->done.load() == p->parts && p->active == 0; });
+    }
+    p->busy.unlock();
 }
 static thread_local enc_scratch tl_es;
 
