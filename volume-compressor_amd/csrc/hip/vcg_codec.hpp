@@ -765,6 +765,7 @@ VCG_HD uint32_t vcg_efsm_step(vcg_efsm &f, Ctx &ctx, Lv &lv, uint8_t *out) {
     const bool isCtx = S < VCG_S_PFX;
     const uint32_t h = ctx.at(S, isCtx);
     const uint32_t R = ctx.ld_next(h);
+    const uint32_t n0 = ctx.nxt(S), n1 = ctx.nxt(S + 256u);   // vcg_fsm_next(S, 0 / 1)
     const int vn = lv.ld(f.p + 1u);                   // the next level, in case p advances
     const int v = f.v;
     const uint32_t a = (uint32_t)(v < 0 ? -v : v);
@@ -805,10 +806,9 @@ VCG_HD uint32_t vcg_efsm_step(vcg_efsm &f, Ctx &ctx, Lv &lv, uint8_t *out) {
         f.pos += sh ? 1u : 0u;                        // count: one byte per shift
     }
     f.range = sh ? f.range << 8 : f.range;
-    // symbol bookkeeping (the decoder's transitions)
-    const bool adv = (isSig & !bit) | (S == VCG_S_SIGN);
-    const uint32_t up = (bit || tail) ? S + 1u : 268u - (S > VCG_S_PFX ? S : VCG_S_PFX);
-    f.S = adv ? 0u : up;
+    // symbol bookkeeping (the decoder's transitions, from its table)
+    f.S = bit ? n1 : n0;
+    const bool adv = f.S == 0u;                       // a new position
     f.P = adv ? f.psig : (bit ? R : VCG_P_BYPASS);
     f.p += adv ? 1u : 0u;
     f.v = adv ? vn : v;

@@ -440,19 +440,22 @@ __global__ __launch_bounds__(FDCT_THREADS) void k_quant_groups(const samp_group 
 // picking a handle stays one select after the shift-or.
 struct enc_lds_ctx {
     uint32_t *m;
+    const uint8_t *nx;           // the next-state table (vcg_fsm_next)
     uint32_t l4, sink;
     __device__ __forceinline__ uint32_t at(uint32_t S, bool isCtx) const { return isCtx ? (S << 8) | l4 : sink; }
+    __device__ __forceinline__ uint32_t nxt(uint32_t i) const { return nx[i]; }
     __device__ __forceinline__ uint32_t ld_next(uint32_t h) const {
         return *(const uint32_t *)((const char *)m + h + 256u);
     }
     __device__ __forceinline__ void st(uint32_t h, uint32_t v) { *(uint32_t *)((char *)m + h) = v; }
 };
-__device__ __forceinline__ enc_lds_ctx enc_ctx_init(uint32_t *ctxm) {
+// (the next-state table is filled by the kernel before its barrier)
+__device__ __forceinline__ enc_lds_ctx enc_ctx_init(uint32_t *ctxm, const uint8_t *nx) {
     for (uint32_t k = 1; k < VCG_CTX_SLOTS; ++k)
         ctxm[k * 64u + threadIdx.x] = k == 13u || k == 15u ? VCG_P_BYPASS : 2048u;
     uint32_t sink = (14u << 8) | (threadIdx.x * 4u);
     asm volatile("" : "+v"(sink));
-    return enc_lds_ctx{ctxm, threadIdx.x * 4u, sink};
+    return enc_lds_ctx{ctxm, nx, threadIdx.x * 4u, sink};
 }
 // per-lane ring of four 8-level blocks (64 B), lanes 68 B apart so the lanes'
 // reads spread over the LDS banks
@@ -478,7 +481,9 @@ __global__ __launch_bounds__(64) void k_rc_count_groups(const samp_group *__rest
     __shared__ uint32_t ctxm[VCG_CTX_SLOTS * 64];
     __shared__ __align__(16) uint8_t lvm[64 * ENC_LV_STRIDE];
     __shared__ uint32_t bands[9];
+    __shared__ uint8_t nxt[512];
     if (threadIdx.x < 9) bands[threadIdx.x] = T->band_start[threadIdx.x];
+    for (uint32_t t = threadIdx.x; t < 512u; t += 64u) nxt[t] = (uint8_t)vcg_fsm_next(t & 255u, t >> 8);
     __syncthreads();
     const uint64_t row = (uint64_t)blockIdx.x * 64u + threadIdx.x;
     const bool live = row < rows;
@@ -486,7 +491,7 @@ __global__ __launch_bounds__(64) void k_rc_count_groups(const samp_group *__rest
     const uint32_t gi = group_of(G, ng, r);
     const samp_group &g = G[gi];
     const bool zero = g.items[r - g.off].zero != 0;
-    enc_lds_ctx ctx = enc_ctx_init(ctxm);
+    enc_lds_ctx ctx = enc_ctx_init(ctxm, nxt);
     enc_lds_lv lr{lvm + threadIdx.x * ENC_LV_STRIDE};
     vcg_efsm f;
     // lanes past the rows and all-zero atoms run as finished empty encodes
@@ -523,7 +528,9 @@ __global__ __launch_bounds__(64) void k_rc_encode(uint32_t n, const int16_t *__r
     __shared__ uint32_t ctxm[VCG_CTX_SLOTS * 64];
     __shared__ __align__(16) uint8_t lvm[64 * ENC_LV_STRIDE];
     __shared__ uint32_t bands[9];
+    __shared__ uint8_t nxt[512];
     if (threadIdx.x < 9) bands[threadIdx.x] = T->band_start[threadIdx.x];
+    for (uint32_t t = threadIdx.x; t < 512u; t += 64u) nxt[t] = (uint8_t)vcg_fsm_next(t & 255u, t >> 8);
     __syncthreads();
     const uint32_t i = blockIdx.x * 64u + threadIdx.x;
     const bool live = i < n;
@@ -539,7 +546,7 @@ __global__ __launch_bounds__(64) void k_rc_encode(uint32_t n, const int16_t *__r
     // the range-coder bytes start at out + 1; the slot keeps a spare byte past
     // them (the encoder's store sink once a lane is finished or full)
     const uint32_t cap = pay_cap - 2 < VCG_RC_CAP ? pay_cap - 2 : VCG_RC_CAP;
-    enc_lds_ctx ctx = enc_ctx_init(ctxm);
+    enc_lds_ctx ctx = enc_ctx_init(ctxm, nxt);
     enc_lds_lv lr{lvm + threadIdx.x * ENC_LV_STRIDE};
     vcg_efsm f;
     vcg_efsm_init<true>(f, lr, lv + (size_t)r * VCG_A3, quiet ? 0u : it.eob, bands, out + 1, quiet ? 0u : cap);
