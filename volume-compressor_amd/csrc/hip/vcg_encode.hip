@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
 #include <string.h>
+#include <algorithm>
 #include <atomic>
 #include <condition_variable>
 #include <mutex>
@@ -797,25 +798,42 @@ struct enc_scratch {
 // pages dominates, so large copies are split in 1 MiB parts over the caller
 // and a pool of host threads (started once, kept until the process ends;
 // a batch of hundreds of archives would otherwise start thousands of
-// threads).  One copy at a time uses the pool; a concurrent caller copies
+// threads).  One job at a time uses the pool; a concurrent caller copies
 // alone.  The destination is offered transparent huge pages first (fewer
-// first-touch faults where the kernel allows them).
+// first-touch faults where the kernel allows them).  The pool also takes a
+// background job that touches the pages of a list of buffers (the archives
+// an encode will fill), started before the device work and joined after it,
+// so that the faults overlap the encode instead of the copies.
 struct copy_pool {
     std::mutex busy, m;
     std::condition_variable go, fin;
     uint64_t gen = 0;
     int active = 0;                       // workers inside the current job (under m)
+    // the job: one copy (touch == false) or the pages of tb[i][0, tl[i])
+    bool touch = false;
     uint8_t *dst = nullptr;
     const uint8_t *src = nullptr;
     size_t n = 0, parts = 0;
+    std::vector<uint8_t *> tb;
+    std::vector<size_t> tl, tpre;         // tpre[i]: first part of buffer i
     std::atomic<size_t> next{0}, done{0};
     int nthr = 0;
+    void part(size_t k) {
+        if (!touch) {
+            const size_t o = k << 20, len = n - o < (1u << 20) ? n - o : (1u << 20);
+            memcpy(dst + o, src + o, len);
+            return;
+        }
+        const size_t i = (size_t)(std::upper_bound(tpre.begin(), tpre.end(), k) - tpre.begin()) - 1;
+        const size_t o = (k - tpre[i]) << 20, len = tl[i] - o < (1u << 20) ? tl[i] - o : (1u << 20);
+        volatile uint8_t *b = tb[i] + o;
+        for (size_t q = 0; q < len; q += 4096) b[q] = 0;
+    }
     // parts are claimed from `next`; the job's fields change only while no
     // worker is inside a job (active == 0, under m)
     void run_parts() {
         for (size_t k; (k = next.fetch_add(1)) < parts;) {
-            const size_t o = k << 20, len = n - o < (1u << 20) ? n - o : (1u << 20);
-            memcpy(dst + o, src + o, len);
+            part(k);
             done.fetch_add(1);
         }
     }
@@ -836,6 +854,28 @@ struct copy_pool {
             fin.notify_all();
         }
     }
+    // with busy held: publish a job whose fields the caller set through f
+    template <class F> void start(F f) {
+        {
+            // a worker that woke for the previous job after it ended leaves
+            // first, so the job's fields change only with no worker inside
+            std::unique_lock<std::mutex> l(m);
+            fin.wait(l, [&] { return active == 0; });
+            f();
+            done.store(0);
+            next.store(0);
+            ++gen;
+        }
+        go.notify_all();
+    }
+    // with busy held: take parts too, then wait until every part is done and
+    // no worker is still inside the job (one that joins late finds no part
+    // left, or counts as active until it leaves)
+    void join() {
+        run_parts();
+        std::unique_lock<std::mutex> l(m);
+        fin.wait(l, [&] { return done.load() == parts && active == 0; });
+    }
 };
 static copy_pool *pool_get() {
     static copy_pool *p = [] {
@@ -847,33 +887,48 @@ static copy_pool *pool_get() {
     }();
     return p;
 }
+static void thp_advise(uint8_t *dst, size_t n) {
+    if (n < (4u << 20)) return;
+    const uintptr_t a0 = ((uintptr_t)dst + 4095u) & ~(uintptr_t)4095u;
+    const uintptr_t a1 = ((uintptr_t)dst + n) & ~(uintptr_t)4095u;
+    if (a1 > a0) (void)madvise((void *)a0, a1 - a0, MADV_HUGEPAGE);
+}
+// the background touch job of this thread, if one runs (it holds busy)
+static thread_local bool tl_touching = false;
+
 static void par_copy(uint8_t *dst, const uint8_t *src, size_t n) {
-    if (n >= (4u << 20)) {
-        const uintptr_t a0 = ((uintptr_t)dst + 4095u) & ~(uintptr_t)4095u;
-        const uintptr_t a1 = ((uintptr_t)dst + n) & ~(uintptr_t)4095u;
-        if (a1 > a0) (void)madvise((void *)a0, a1 - a0, MADV_HUGEPAGE);
-    }
-    copy_pool *p = n >= (2u << 20) ? pool_get() : nullptr;
+    thp_advise(dst, n);
+    copy_pool *p = n >= (2u << 20) && !tl_touching ? pool_get() : nullptr;
     if (!p || p->nthr == 0 || !p->busy.try_lock()) { memcpy(dst, src, n); return; }
-    {
-        // a worker that woke for the previous job after it ended leaves first,
-        // so the job's fields change only with no worker inside a job
-        std::unique_lock<std::mutex> l(p->m);
-        p->fin.wait(l, [&] { return p->active == 0; });
-        p->dst = dst; p->src = src; p->n = n; p->parts = (n + (1u << 20) - 1) >> 20;
-        p->done.store(0);
-        p->next.store(0);
-        ++p->gen;
-    }
-    p->go.notify_all();
-    p->run_parts();
-    {
-        // every part copied, and no worker still inside the job (one that
-        // joins late finds no part left, or counts as active until it leaves)
-        std::unique_lock<std::mutex> l(p->m);
-        p->fin.wait(l, [&] { return p->done.load() == p->parts && p->active == 0; });
-    }
+    p->start([&] { p->touch = false; p->dst = dst; p->src = src; p->n = n; p->parts = (n + (1u << 20) - 1) >> 20; });
+    p->join();
     p->busy.unlock();
+}
+extern "C" void vcg_host_prefault_begin(uint8_t *const *bufs, const size_t *lens, int n) {
+    if (tl_touching) return;
+    copy_pool *p = pool_get();
+    if (p->nthr == 0 || !p->busy.try_lock()) return;
+    p->start([&] {
+        p->touch = true;
+        p->tb.clear(); p->tl.clear(); p->tpre.clear();
+        size_t parts = 0;
+        for (int i = 0; i < n; ++i) {
+            if (!bufs[i] || !lens[i]) continue;
+            thp_advise(bufs[i], lens[i]);
+            p->tb.push_back(bufs[i]); p->tl.push_back(lens[i]); p->tpre.push_back(parts);
+            parts += (lens[i] + (1u << 20) - 1) >> 20;
+        }
+        p->parts = parts;
+    });
+    tl_touching = true;
+}
+
+extern "C" void vcg_host_prefault_wait(void) {
+    if (!tl_touching) return;
+    copy_pool *p = pool_get();
+    p->join();
+    p->busy.unlock();
+    tl_touching = false;
 }
 static thread_local enc_scratch tl_es;
 

@@ -179,8 +179,11 @@ static void lod_schedule(float q0, int nlod, float qs[VC_NLOD]) {
 }
 
 /* ---- layout + assembly ---- */
+/* pre: a buffer of pre_len bytes allocated for the archive ahead of time
+ * (NULL if none); assemble owns it: the archive, or freed */
 static vc_status assemble(vcg_enc *e, vc_dims d0, int nlod, float target, const float qs[VC_NLOD],
-                          const uint32_t *lens, uint64_t total, uint8_t **out, size_t *out_len) {
+                          const uint32_t *lens, uint64_t total, uint8_t *pre, size_t pre_len,
+                          uint8_t **out, size_t *out_len) {
     uint64_t prefix[VC_NLOD + 1], apre[VC_NLOD + 1], nreg = 0, na = 0;
     uint32_t acx[VC_NLOD], acy[VC_NLOD], acz[VC_NLOD], nrx[VC_NLOD], nry[VC_NLOD], nrz[VC_NLOD];
     for (int l = 0; l < nlod; ++l) {
@@ -216,7 +219,7 @@ static vc_status assemble(vcg_enc *e, vc_dims d0, int nlod, float target, const 
             boff[ri] = cursor; cursor += blen[ri];
         }
     vc_status rc = VC_OK;
-    if (too_big) { rc = VC_ERR_RANGE; goto done; }
+    if (too_big) { free(pre); rc = VC_ERR_RANGE; goto done; }
     /* per-atom slot and payload offsets: payloads follow the slot array in slot order */
     for (int l = 0; l < nlod; ++l)
         for (uint32_t rz = 0; rz < nrz[l]; ++rz) for (uint32_t ry = 0; ry < nry[l]; ++ry) for (uint32_t rx = 0; rx < nrx[l]; ++rx) {
@@ -245,10 +248,15 @@ static vc_status assemble(vcg_enc *e, vc_dims d0, int nlod, float target, const 
             if (cls[ri] == 2) { wr64(en, boff[ri]); wr32(en + 8, (uint32_t)blen[ri]); wr32(en + 12, 2); }
             else if (cls[ri] == 1) wr32(en + 12, 1);
         }
-        uint8_t *img = (uint8_t *)malloc(cursor);
+        uint8_t *img = pre && pre_len >= cursor ? pre : NULL;
+        if (!img) { free(pre); pre = NULL; img = (uint8_t *)malloc(cursor); }
         if (!img) { free(head); rc = VC_ERR_OOM; goto done; }
         vcg_enc_assemble(e, head, head_len, pay_off, slot_off, lens, total, img, cursor);
         free(head);
+        if (img == pre && pre_len > cursor) {           /* give back the estimate's excess */
+            uint8_t *t = (uint8_t *)realloc(img, cursor);
+            if (t) img = t;
+        }
         *out = img; *out_len = (size_t)cursor;
     }
 done:
@@ -268,6 +276,26 @@ static int rc_depth(int nv, const vc_dims *dims) {
     int d = RC_MAX_DEPTH;
     while (d > 1 && ns * (uint64_t)((1 << d) - 1) > vcg_rc_rows()) --d;
     return d;
+}
+
+/* An archive size that the encode is expected to stay under: header and
+ * directory, every region's slot array, and twice the LOD0 payload the
+ * target ratio allows (all LODs together came to 1.3-1.8 times that on
+ * synthetic volumes of 40-512 voxels per axis).  0 when larger than
+ * PREFAULT_MAX: such an archive is not allocated ahead. */
+#define PREFAULT_MAX (4ull << 30)
+static uint64_t archive_estimate(vc_dims d0, int nlod, float target) {
+    uint64_t nreg = 0;
+    for (int l = 0; l < nlod; ++l) {
+        uint32_t rz, ry, rx;
+        vc__region_grid(d0, l, &rz, &ry, &rx);
+        nreg += (uint64_t)rz * ry * rx;
+    }
+    const double pay = 2.0 * (double)d0.nx * d0.ny * d0.nz / (double)target;
+    const uint64_t head = (FILE_HDR + nreg * 16 + 4095) & ~4095ull;
+    if (pay > (double)PREFAULT_MAX) return 0;
+    const uint64_t est = head + nreg * L2_BLOCK + (uint64_t)pay + (64u << 10);
+    return est > PREFAULT_MAX ? 0 : est;
 }
 
 static uint64_t atom_total(vc_dims dims, int nlod) {
@@ -290,12 +318,27 @@ static void encode_group(int nv, const uint8_t *const *vols, int on_dev, const v
     uint32_t **lens = (uint32_t **)calloc((size_t)nv, sizeof *lens);
     int *nlod = (int *)malloc((size_t)nv * sizeof *nlod), *ovf = (int *)malloc((size_t)nv * sizeof *ovf);
     uint64_t *total = (uint64_t *)malloc((size_t)nv * 8);
-    if (!r || !es || !qs || !lens || !nlod || !ovf || !total) vc__panic("out of memory");
+    uint8_t **pre = (uint8_t **)calloc((size_t)nv, sizeof *pre);
+    size_t *pre_len = (size_t *)calloc((size_t)nv, sizeof *pre_len);
+    if (!r || !es || !qs || !lens || !nlod || !ovf || !total || !pre || !pre_len) vc__panic("out of memory");
     for (int v = 0; v < nv; ++v) {
         nlod[v] = vc__nlod(dims[v]);
         uint32_t d[3] = {dims[v].nx, dims[v].ny, dims[v].nz};
         es[v] = vcg_enc_begin(vols[v], on_dev, d, nlod[v]);
         r[v].e = es[v]; r[v].d = dims[v]; r[v].target = targets[v];
+    }
+    if (!q0_only) {
+        /* the archives' host memory, allocated at its estimate and faulted in
+         * by host threads while the device runs rate control and the encode
+         * (fresh pages otherwise dominate the copy-out); at most 32 GiB */
+        uint64_t sum = 0;
+        for (int v = 0; v < nv; ++v) {
+            const uint64_t est = archive_estimate(dims[v], nlod[v], targets[v]);
+            if (!est || sum + est > (32ull << 30)) continue;
+            pre[v] = (uint8_t *)malloc((size_t)est);
+            if (pre[v]) { pre_len[v] = (size_t)est; sum += est; }
+        }
+        vcg_host_prefault_begin(pre, pre_len, nv);
     }
     if (qs_in) {
         for (int l = 0; l < VC_NLOD; ++l) qs[0][l] = l < nlod[0] ? vc__clamp_q(qs_in[l]) : 0.0f;
@@ -312,15 +355,16 @@ static void encode_group(int nv, const uint8_t *const *vols, int on_dev, const v
             if (!lens[v]) vc__panic("out of memory");
         }
         vcg_enc_all(nv, es, (const float (*)[VC_NLOD])qs, lens, ovf);
+        vcg_host_prefault_wait();
         for (int v = 0; v < nv; ++v) {
             outs[v] = NULL; out_lens[v] = 0;
-            if (ovf[v]) st[v] = VC_ERR_RANGE;        /* a payload exceeded the coder bound */
-            else st[v] = assemble(es[v], dims[v], nlod[v], targets[v], qs[v], lens[v], total[v], &outs[v],
-                                  &out_lens[v]);
+            if (ovf[v]) { free(pre[v]); st[v] = VC_ERR_RANGE; }   /* a payload exceeded the coder bound */
+            else st[v] = assemble(es[v], dims[v], nlod[v], targets[v], qs[v], lens[v], total[v], pre[v], pre_len[v],
+                                  &outs[v], &out_lens[v]);
         }
     }
     for (int v = 0; v < nv; ++v) { vcg_enc_end(es[v]); free(lens[v]); }
-    free(r); free(es); free(qs); free(lens); free(nlod); free(ovf); free(total);
+    free(r); free(es); free(qs); free(lens); free(nlod); free(ovf); free(total); free(pre); free(pre_len);
 }
 
 static vc_status encode_impl(const uint8_t *vol, int on_dev, vc_dims dims, float target, const float *qs_in,

@@ -187,6 +187,13 @@ int  vcg_enc_assemble(vcg_enc *e, const uint8_t *head, size_t head_len, const ui
                       const uint64_t *slot_off, const uint32_t *lens, uint64_t total_atoms,
                       uint8_t *out, uint64_t total_len);
 void vcg_enc_end(vcg_enc *e);
+/* Touch the pages of n host buffers (bufs[i], lens[i] bytes; NULL entries
+ * skipped) on the library's host threads in the background, so that their
+ * first-touch faults overlap device work.  The buffers must stay allocated
+ * until vcg_host_prefault_wait, which this thread calls before it writes
+ * them.  Does nothing when another job holds the threads. */
+void vcg_host_prefault_begin(uint8_t *const *bufs, const size_t *lens, int n);
+void vcg_host_prefault_wait(void);
 
 /* Batched atom encode without a container (vcx_encode_atoms). */
 int vcg_encode_atoms(size_t n, const uint8_t *vox, float base_q, uint8_t *out, size_t cap,
