@@ -396,20 +396,36 @@ __device__ __forceinline__ uint32_t group_of(const samp_group *G, uint32_t ng, u
     return lo;
 }
 
+// Rows of a rate-control launch are counted heaviest first: k_quant_groups
+// files each row under a bin of its estimated bin count (EOB + 3 x nonzero
+// levels, RC_BIN_SHIFT wide), and k_rc_count_groups hands its lanes the rows
+// of the heaviest bin first, so that the lanes of a wave finish together.
+// The byte totals do not depend on the order (each row's count is exact, and
+// a group passes its budget whatever order its rows are counted in).
+#define RC_BINS 32u
+#define RC_BIN_SHIFT 12u
+
 // quantize every candidate's sample atoms (coefficients in scan order):
-// block = one row, four consecutive positions per thread and step
+// block = one row, four consecutive positions per thread and step.  bins:
+// RC_BINS counters (zero on entry); row filed at perm[bin * pcap + k].
 __global__ __launch_bounds__(FDCT_THREADS) void k_quant_groups(const samp_group *__restrict__ G, uint32_t ng,
                                                                const vcg_dtab *__restrict__ T,
                                                                int16_t *__restrict__ lv_out,
-                                                               uint32_t *__restrict__ eobs) {
+                                                               uint32_t *__restrict__ eobs,
+                                                               uint32_t *__restrict__ bins,
+                                                               uint32_t *__restrict__ perm, uint32_t pcap) {
     __shared__ int seob;
+    __shared__ uint32_t snz;
     __shared__ float gs[VCG_NFREQ];
     const int tid = threadIdx.x;
     const uint64_t row = blockIdx.x;
     const samp_group &g = G[group_of(G, ng, row)];
     const uint32_t i = (uint32_t)(row - g.off);
-    if (g.items[i].zero) { if (tid == 0) eobs[row] = 0; return; }
-    if (tid == 0) seob = 0;
+    if (g.items[i].zero) {
+        if (tid == 0) { eobs[row] = 0; perm[atomicAdd(&bins[0], 1u)] = (uint32_t)row; }
+        return;
+    }
+    if (tid == 0) { seob = 0; snz = 0; }
     if (tid < VCG_NFREQ) gs[tid] = T->gstep[tid];
     __syncthreads();
     const float q = g.q;
@@ -417,6 +433,7 @@ __global__ __launch_bounds__(FDCT_THREADS) void k_quant_groups(const samp_group 
     const uint32_t *fq = (const uint32_t *)T->fscan;
     uint2 *lo = (uint2 *)(lv_out + row * VCG_A3);
     int last = 0;
+    uint32_t nz = 0;
     for (int k = tid; k < 32768 / 4; k += FDCT_THREADS) {
         const int4 c = co[k];
         const uint32_t f = fq[k];
@@ -428,10 +445,17 @@ __global__ __launch_bounds__(FDCT_THREADS) void k_quant_groups(const samp_group 
                            (uint32_t)(uint16_t)l2 | ((uint32_t)(uint16_t)l3 << 16));
         const int p = 4 * k;
         if (l3) last = p + 4; else if (l2) last = p + 3; else if (l1) last = p + 2; else if (l0) last = p + 1;
+        nz += (l0 != 0) + (l1 != 0) + (l2 != 0) + (l3 != 0);
     }
     if (last) atomicMax(&seob, last);
+    if (nz) atomicAdd(&snz, nz);
     __syncthreads();
-    if (tid == 0) eobs[row] = (uint32_t)seob;
+    if (tid == 0) {
+        const uint32_t key = (uint32_t)seob + 3u * snz;
+        const uint32_t b = key >> RC_BIN_SHIFT < RC_BINS - 1u ? key >> RC_BIN_SHIFT : RC_BINS - 1u;
+        perm[b * pcap + atomicAdd(&bins[b], 1u)] = (uint32_t)row;
+        eobs[row] = (uint32_t)seob;
+    }
 }
 
 // per-lane context memory (VCG_CTX_SLOTS slots), slot-major over the 64
@@ -474,21 +498,32 @@ struct enc_lds_lv {
 // mode), payload bytes not stored.  A group whose running total passes its
 // budget is abandoned (lanes report 0xFFFFFFFE); 0xFFFFFFFF = coder bound
 // exceeded.
+// bins, perm: the rows filed by k_quant_groups (bins[b] rows at perm + b * pcap)
 __global__ __launch_bounds__(64) void k_rc_count_groups(const samp_group *__restrict__ G, uint32_t ng,
                                                         uint64_t rows, const int16_t *__restrict__ lv,
                                                         const uint32_t *__restrict__ eobs,
                                                         const vcg_dtab *__restrict__ T, uint32_t *__restrict__ kb,
-                                                        uint32_t *__restrict__ lens) {
+                                                        uint32_t *__restrict__ lens,
+                                                        const uint32_t *__restrict__ bins,
+                                                        const uint32_t *__restrict__ perm, uint32_t pcap) {
     __shared__ uint32_t ctxm[VCG_CTX_SLOTS * 64];
     __shared__ __align__(16) uint8_t lvm[64 * ENC_LV_STRIDE];
     __shared__ uint32_t bands[9];
     __shared__ uint8_t nxt[512];
+    __shared__ uint32_t bpre[RC_BINS];
     if (threadIdx.x < 9) bands[threadIdx.x] = T->band_start[threadIdx.x];
     for (uint32_t t = threadIdx.x; t < 512u; t += 64u) nxt[t] = (uint8_t)vcg_fsm_next(t & 255u, t >> 8);
+    if (threadIdx.x == 0) {                               // lanes before bin b (heaviest bin first)
+        uint32_t a = 0;
+        for (uint32_t b = RC_BINS; b-- > 0;) { bpre[b] = a; a += bins[b]; }
+    }
     __syncthreads();
-    const uint64_t row = (uint64_t)blockIdx.x * 64u + threadIdx.x;
-    const bool live = row < rows;
-    const uint64_t r = live ? row : 0;
+    const uint64_t lane = (uint64_t)blockIdx.x * 64u + threadIdx.x;
+    const bool live = lane < rows;
+    uint32_t b = RC_BINS - 1u;
+    while (b > 0u && lane >= bpre[b - 1u]) --b;          // the lightest bin that starts at or before lane
+    const uint64_t row = live ? perm[b * pcap + (uint32_t)(lane - bpre[b])] : 0u;
+    const uint64_t r = row;
     const uint32_t gi = group_of(G, ng, r);
     const samp_group &g = G[gi];
     const bool zero = g.items[r - g.off].zero != 0;
@@ -1074,21 +1109,23 @@ extern "C" int vcg_enc_candidates(int ng, vcg_enc *const *es, const float *qs, c
     const size_t maxrows = tot < cap ? (size_t)tot : (size_t)cap;
     vcg_grow((void **)&es_.lv, &es_.lv_cap, maxrows, (size_t)VCG_A3 * 2);
     vcg_grow((void **)&es_.lens, &es_.lens_cap, tot, 4);
-    vcg_grow((void **)&es_.aux, &es_.aux_cap, maxrows + (size_t)ng, 4);
+    vcg_grow((void **)&es_.aux, &es_.aux_cap, maxrows * (1 + RC_BINS) + (size_t)ng + RC_BINS, 4);
     samp_group *dG = (samp_group *)desc_area((size_t)ng * sizeof(samp_group));
     VCG_CHECK(hipMemcpyAsync(dG, G.data(), (size_t)ng * sizeof(samp_group), hipMemcpyHostToDevice, c->s));
-    uint32_t *eobs = es_.aux, *kb = es_.aux + maxrows;
+    uint32_t *eobs = es_.aux, *kb = es_.aux + maxrows, *bins = kb + ng, *perm = bins + RC_BINS;
     hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(256), 0, c->s, kb, (uint64_t)ng);
     uint64_t done = 0;
     for (size_t k = 0; k + 1 < cut.size(); ++k) {
         const int g0 = cut[k], g1 = cut[k + 1];
         const uint64_t r = (uint64_t)G[g1 - 1].off + G[g1 - 1].n;
         if (r == 0) continue;
+        hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(64), 0, c->s, bins, (uint64_t)RC_BINS);
         hipLaunchKernelGGL(k_quant_groups, dim3((uint32_t)r), dim3(FDCT_THREADS), 0, c->s, dG + g0,
-                           (uint32_t)(g1 - g0), g_dtab, es_.lv, eobs);
+                           (uint32_t)(g1 - g0), g_dtab, es_.lv, eobs, bins, perm, (uint32_t)maxrows);
         VCG_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_rc_count_groups, dim3((uint32_t)((r + 63) / 64)), dim3(64), 0, c->s, dG + g0,
-                           (uint32_t)(g1 - g0), r, es_.lv, eobs, g_dtab, kb + g0, es_.lens + done);
+                           (uint32_t)(g1 - g0), r, es_.lv, eobs, g_dtab, kb + g0, es_.lens + done,
+                           bins, perm, (uint32_t)maxrows);
         VCG_CHECK(hipGetLastError());
         done += r;
     }
