@@ -498,6 +498,12 @@ struct enc_lds_lv {
 // mode), payload bytes not stored.  A group whose running total passes its
 // budget is abandoned (lanes report 0xFFFFFFFE); 0xFFFFFFFF = coder bound
 // exceeded.
+// A lane reports to its group's running total every RC_REPORT bytes it
+// counts.  The total only counts whole reports, so it passes a budget only if
+// the group's full total does: the report size changes how soon a group is
+// abandoned, never whether.
+#define RC_REPORT 256u
+
 // bins, perm: the rows filed by k_quant_groups (bins[b] rows at perm + b * pcap)
 __global__ __launch_bounds__(64) void k_rc_count_groups(const samp_group *__restrict__ G, uint32_t ng,
                                                         uint64_t rows, const int16_t *__restrict__ lv,
@@ -534,19 +540,19 @@ __global__ __launch_bounds__(64) void k_rc_count_groups(const samp_group *__rest
     vcg_efsm_init<false>(f, lr, lv + r * VCG_A3, live && !zero ? eobs[r] : 0u, bands, nullptr, VCG_RC_CAP);
     if (!live || zero) f.done = 1;
     int stop = 0;
-    uint32_t kbm = 1024u;
+    uint32_t kbm = RC_REPORT;
     uint32_t *kg = kb + gi;
     while (__any(!f.done)) {
         vcg_efsm_refill(f, lr);
         // edge tests only where some lane of the wave can meet an edge (a
         // wave-uniform branch)
         vcg_efsm_block<false>(f, ctx, lr, bands, nullptr, __any(vcg_efsm_near(f, VCG_EFSM_REFILL)));
-        // every 1024 bytes a lane adds to its group's total; past the budget
-        // the group is abandoned
+        // every RC_REPORT bytes a lane adds to its group's total; past the
+        // budget the group is abandoned
         if (!f.done && f.pos >= kbm) {
             while (!f.done && f.pos >= kbm) {
-                kbm += 1024u;
-                if (atomicAdd(kg, 1024u) + 1024u > g.budget) { stop = 1; f.done = 1; f.lim = 0xFFFFFFFFu; }
+                kbm += RC_REPORT;
+                if (atomicAdd(kg, RC_REPORT) + RC_REPORT > g.budget) { stop = 1; f.done = 1; f.lim = 0xFFFFFFFFu; }
             }
         }
     }
