@@ -221,6 +221,22 @@ def test_all_zero_volume(pyvc, oracle):
     assert not a.decode_lod(0)[1].any()
 
 
+def test_encode_archive_outgrows_estimate(pyvc, oracle):
+    """The library allocates each archive ahead at an estimate from the target
+    ratio; an explicit fine base_q with a high target makes the archive several
+    times that estimate, and the archive must still equal the oracle's."""
+    vol = oracle.synth(3, 200, 176, 160, 12)
+    nlod = oracle.nlod(200, 176, 160)
+    qs = [0.05] * nlod
+    say("encode_q past the estimate", vol.shape)
+    st, arc = pyvc.encode_q(vol, 1000.0, qs)
+    ost, oarc = oracle.encode_q(vol, 1000.0, qs)
+    assert st == ost == 0
+    est = 2 * vol.size // 1000 + 8 * 512 * 1024 + 4096 + 64 * 1024
+    assert len(arc) > 2 * est, (len(arc), est)
+    assert arc == oarc
+
+
 def test_writer_matches_reference_file(pyvc, oracle, golden_containers):
     """The v1 writer, driven in the same single-threaded order as the golden
     archives, writes byte-identical files to the reference writer."""
