@@ -114,6 +114,24 @@ __host__ __device__ __forceinline__ bool ds_small(uint64_t total) { return total
 static int g_ds_wide = 0;
 extern "C" void vcg_test_wide_index(int on) { g_ds_wide = on != 0; }
 
+// output voxel (x, y, z) of the 2x pyramid level: the rounded mean of its
+// in-bounds 2x2x2 cell
+__device__ __forceinline__ uint8_t ds_voxel(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
+                                            uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t acc = 0, n = 0;
+    for (uint32_t dz = 0; dz < 2; ++dz) {
+        uint32_t zz = 2 * z + dz; if (zz >= nz) continue;
+        for (uint32_t dy = 0; dy < 2; ++dy) {
+            uint32_t yy = 2 * y + dy; if (yy >= ny) continue;
+            const uint8_t *row = in + ((uint64_t)zz * ny + yy) * nx;
+            for (uint32_t dx = 0; dx < 2; ++dx) {
+                uint32_t xx = 2 * x + dx; if (xx >= nx) continue;
+                acc += row[xx]; n++;
+            }
+        }
+    }
+    return (uint8_t)((acc + n / 2) / n);
+}
 template <typename IT>
 __device__ __forceinline__ void downsample_body(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny,
                                                 uint32_t nz, uint8_t *__restrict__ out) {
@@ -122,19 +140,7 @@ __device__ __forceinline__ void downsample_body(const uint8_t *__restrict__ in, 
     for (IT o = (IT)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += (IT)gridDim.x * blockDim.x) {
         uint32_t x, y, z;
         ds_split<IT>(o, ox, oy, x, y, z);
-        uint32_t acc = 0, n = 0;
-        for (uint32_t dz = 0; dz < 2; ++dz) {
-            uint32_t zz = 2 * z + dz; if (zz >= nz) continue;
-            for (uint32_t dy = 0; dy < 2; ++dy) {
-                uint32_t yy = 2 * y + dy; if (yy >= ny) continue;
-                const uint8_t *row = in + ((uint64_t)zz * ny + yy) * nx;
-                for (uint32_t dx = 0; dx < 2; ++dx) {
-                    uint32_t xx = 2 * x + dx; if (xx >= nx) continue;
-                    acc += row[xx]; n++;
-                }
-            }
-        }
-        out[o] = (uint8_t)((acc + n / 2) / n);
+        out[o] = ds_voxel(in, nx, ny, nz, x, y, z);
     }
 }
 __global__ void k_downsample(const uint8_t *__restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
@@ -143,6 +149,30 @@ __global__ void k_downsample(const uint8_t *__restrict__ in, uint32_t nx, uint32
         downsample_body<uint32_t>(in, nx, ny, nz, out);
     else
         downsample_body<uint64_t>(in, nx, ny, nz, out);
+}
+
+// One pyramid level of many volumes in one launch: job j (input dims nx, ny,
+// nz) owns blocks [blk0, next job's blk0), one thread per output voxel;
+// output counts below 2^31.
+struct ds_job {
+    const uint8_t *in;
+    uint8_t *out;
+    uint32_t nx, ny, nz, blk0;
+};
+#define DS_JOB_THREADS 256u
+__global__ __launch_bounds__(DS_JOB_THREADS) void k_downsample_jobs(const ds_job *__restrict__ J, uint32_t nj) {
+    uint32_t lo = 0, hi = nj;
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) / 2;
+        if (J[m].blk0 <= blockIdx.x) lo = m; else hi = m;
+    }
+    const ds_job j = J[lo];
+    const uint32_t ox = (j.nx + 1) / 2, oy = (j.ny + 1) / 2, oz = (j.nz + 1) / 2;
+    const uint32_t o = (blockIdx.x - j.blk0) * DS_JOB_THREADS + threadIdx.x;
+    if (o >= ox * oy * oz) return;
+    uint32_t x, y, z;
+    ds_split<uint32_t>(o, ox, oy, x, y, z);
+    j.out[o] = ds_voxel(j.in, j.nx, j.ny, j.nz, x, y, z);
 }
 
 // The exporter's within-cell 2x downscale (tools/downscale.h:32-93): output
@@ -745,39 +775,102 @@ extern "C" int vcg_ds2x(const void *in, const uint32_t d[3], int method, float a
     return 0;
 }
 
-extern "C" vcg_enc *vcg_enc_begin(const uint8_t *vol, int on_dev, const uint32_t dims[3], int nlod) {
+static uint8_t *desc_area(size_t n);
+
+// Encoders of nv volumes: each volume uploaded (or adopted) and its pyramid
+// built, one launch per level across the volumes whose level outputs stay
+// below 2^31 voxels (the others, and every volume under the wide-index test
+// hook, take a launch of their own).
+extern "C" void vcg_enc_begin_many(int nv, const uint8_t *const *vols, int on_dev, const uint32_t (*dims)[3],
+                                   const int *nlods, vcg_enc **out) {
     vcg_ctx *c = vcg_get_ctx();
     enc_setup();
-    vcg_enc *e = new vcg_enc();
-    memset(&e->src, 0, sizeof e->src);
-    e->arena = tl_arena.open;
-    uint32_t x = dims[0], y = dims[1], z = dims[2];
-    uint64_t acc = 0;
-    for (int l = 0; l < nlod; ++l) {
-        e->src.nx[l] = x; e->src.ny[l] = y; e->src.nz[l] = z;
-        e->src.acx[l] = (x + 31) / 32; e->src.acy[l] = (y + 31) / 32;
-        e->src.prefix[l] = acc;
-        acc += (uint64_t)((x + 31) / 32) * ((y + 31) / 32) * ((z + 31) / 32);
-        x = (x + 1) / 2; y = (y + 1) / 2; z = (z + 1) / 2;
+    int maxl = 0;
+    for (int v = 0; v < nv; ++v) {
+        vcg_enc *e = new vcg_enc();
+        memset(&e->src, 0, sizeof e->src);
+        e->arena = tl_arena.open;
+        const int nlod = nlods[v];
+        uint32_t x = dims[v][0], y = dims[v][1], z = dims[v][2];
+        uint64_t acc = 0;
+        for (int l = 0; l < nlod; ++l) {
+            e->src.nx[l] = x; e->src.ny[l] = y; e->src.nz[l] = z;
+            e->src.acx[l] = (x + 31) / 32; e->src.acy[l] = (y + 31) / 32;
+            e->src.prefix[l] = acc;
+            acc += (uint64_t)((x + 31) / 32) * ((y + 31) / 32) * ((z + 31) / 32);
+            x = (x + 1) / 2; y = (y + 1) / 2; z = (z + 1) / 2;
+        }
+        e->src.prefix[nlod] = acc;
+        e->src.nlod = nlod;
+        e->total = acc;
+        const size_t n0 = (size_t)dims[v][0] * dims[v][1] * dims[v][2];
+        if (on_dev) {
+            e->src.lod[0] = vols[v];
+        } else {
+            e->lod_buf[0] = (uint8_t *)enc_alloc(e->arena, n0);
+            VCG_CHECK(hipMemcpyAsync(e->lod_buf[0], vols[v], n0, hipMemcpyHostToDevice, c->s));
+            e->src.lod[0] = e->lod_buf[0];
+        }
+        for (int l = 1; l < nlod; ++l) {
+            e->lod_buf[l] = (uint8_t *)enc_alloc(e->arena, (size_t)e->src.nx[l] * e->src.ny[l] * e->src.nz[l]);
+            e->src.lod[l] = e->lod_buf[l];
+        }
+        if (nlod > maxl) maxl = nlod;
+        out[v] = e;
     }
-    e->src.prefix[nlod] = acc;
-    e->src.nlod = nlod;
-    e->total = acc;
-    size_t n0 = (size_t)dims[0] * dims[1] * dims[2];
-    if (on_dev) {
-        e->src.lod[0] = vol;
-    } else {
-        e->lod_buf[0] = (uint8_t *)enc_alloc(e->arena, n0);
-        VCG_CHECK(hipMemcpyAsync(e->lod_buf[0], vol, n0, hipMemcpyHostToDevice, c->s));
-        e->src.lod[0] = e->lod_buf[0];
+    // level l from level l - 1: the batched jobs of every level in one upload.
+    // The table outlives the call (thread-local), so that the host need not
+    // wait for the upload; this thread's next call reuses it only after the
+    // encode that follows this one has synchronised its stream.
+    static thread_local std::vector<ds_job> jobs;
+    jobs.clear();
+    std::vector<size_t> lv0(VCG_NLOD + 1, 0);       // level l's jobs: [lv0[l], lv0[l + 1])
+    std::vector<uint32_t> lvblocks(VCG_NLOD + 1, 0);
+    std::vector<uint8_t> alone((size_t)nv * VCG_NLOD, 0);
+    for (int l = 1; l < maxl; ++l) {
+        lv0[l] = jobs.size();
+        uint64_t blocks = 0;
+        for (int v = 0; v < nv; ++v) {
+            const vcg_enc *e = out[v];
+            if (l >= e->src.nlod) continue;
+            const uint64_t total = (uint64_t)e->src.nx[l] * e->src.ny[l] * e->src.nz[l];
+            const uint64_t nb = (total + DS_JOB_THREADS - 1) / DS_JOB_THREADS;
+            if (g_ds_wide || total >= (1ull << 31) || blocks + nb >= (1ull << 31)) {
+                alone[(size_t)v * VCG_NLOD + l] = 1;
+                continue;
+            }
+            jobs.push_back(ds_job{e->src.lod[l - 1], e->lod_buf[l], e->src.nx[l - 1], e->src.ny[l - 1],
+                                  e->src.nz[l - 1], (uint32_t)blocks});
+            blocks += nb;
+        }
+        lvblocks[l] = (uint32_t)blocks;
     }
-    for (int l = 1; l < nlod; ++l) {
-        size_t nl = (size_t)e->src.nx[l] * e->src.ny[l] * e->src.nz[l];
-        e->lod_buf[l] = (uint8_t *)enc_alloc(e->arena, nl);
-        uint32_t d[3] = {e->src.nx[l - 1], e->src.ny[l - 1], e->src.nz[l - 1]};
-        vcg_downsample(e->src.lod[l - 1], d, e->lod_buf[l]);
-        e->src.lod[l] = e->lod_buf[l];
+    for (int l = maxl > 1 ? maxl : 1; l <= VCG_NLOD; ++l) lv0[l] = jobs.size();
+    ds_job *dj = nullptr;
+    if (!jobs.empty()) {
+        dj = (ds_job *)desc_area(jobs.size() * sizeof(ds_job));
+        VCG_CHECK(hipMemcpyAsync(dj, jobs.data(), jobs.size() * sizeof(ds_job), hipMemcpyHostToDevice, c->s));
     }
+    for (int l = 1; l < maxl; ++l) {
+        const size_t nj = lv0[l + 1] - lv0[l];
+        if (nj) {
+            hipLaunchKernelGGL(k_downsample_jobs, dim3(lvblocks[l]), dim3(DS_JOB_THREADS), 0, c->s, dj + lv0[l],
+                               (uint32_t)nj);
+            VCG_CHECK(hipGetLastError());
+        }
+        for (int v = 0; v < nv; ++v) {
+            if (!alone[(size_t)v * VCG_NLOD + l]) continue;
+            const vcg_enc *e = out[v];
+            const uint32_t d[3] = {e->src.nx[l - 1], e->src.ny[l - 1], e->src.nz[l - 1]};
+            vcg_downsample(e->src.lod[l - 1], d, e->lod_buf[l]);
+        }
+    }
+}
+
+extern "C" vcg_enc *vcg_enc_begin(const uint8_t *vol, int on_dev, const uint32_t dims[3], int nlod) {
+    vcg_enc *e = nullptr;
+    const uint32_t d[1][3] = {{dims[0], dims[1], dims[2]}};
+    vcg_enc_begin_many(1, &vol, on_dev, d, &nlod, &e);
     return e;
 }
 

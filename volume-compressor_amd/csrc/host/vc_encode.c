@@ -321,12 +321,15 @@ static void encode_group(int nv, const uint8_t *const *vols, int on_dev, const v
     uint8_t **pre = (uint8_t **)calloc((size_t)nv, sizeof *pre);
     size_t *pre_len = (size_t *)calloc((size_t)nv, sizeof *pre_len);
     if (!r || !es || !qs || !lens || !nlod || !ovf || !total || !pre || !pre_len) vc__panic("out of memory");
+    uint32_t (*d3)[3] = (uint32_t (*)[3])malloc((size_t)nv * sizeof *d3);
+    if (!d3) vc__panic("out of memory");
     for (int v = 0; v < nv; ++v) {
         nlod[v] = vc__nlod(dims[v]);
-        uint32_t d[3] = {dims[v].nx, dims[v].ny, dims[v].nz};
-        es[v] = vcg_enc_begin(vols[v], on_dev, d, nlod[v]);
-        r[v].e = es[v]; r[v].d = dims[v]; r[v].target = targets[v];
+        d3[v][0] = dims[v].nx; d3[v][1] = dims[v].ny; d3[v][2] = dims[v].nz;
     }
+    vcg_enc_begin_many(nv, vols, on_dev, (const uint32_t (*)[3])d3, nlod, es);
+    free(d3);
+    for (int v = 0; v < nv; ++v) { r[v].e = es[v]; r[v].d = dims[v]; r[v].target = targets[v]; }
     if (!q0_only) {
         /* the archives' host memory, allocated at its estimate and faulted in
          * by host threads while the device runs rate control and the encode

@@ -151,6 +151,10 @@ void vcg_enc_arena_open(void);
 void vcg_enc_arena_close(void);
 /* Upload (or adopt) a LOD0 volume and build the nlod-level pyramid on device. */
 vcg_enc *vcg_enc_begin(const uint8_t *vol, int vol_on_device, const uint32_t dims[3], int nlod);
+/* vcg_enc_begin for nv volumes (volume v: dims[v], nlods[v] levels) into
+ * out[v], their pyramids built one launch per level across the volumes. */
+void vcg_enc_begin_many(int nv, const uint8_t *const *vols, int vol_on_device, const uint32_t (*dims)[3],
+                        const int *nlods, vcg_enc **out);
 /* An encoder over nlod level volumes already on the device (dims[l] = nx,
  * ny, nz of level l), encoded by vcg_enc_all like a pyramid. */
 vcg_enc *vcg_enc_begin_levels(const uint8_t *const *lods, const uint32_t (*dims)[3], int nlod);
