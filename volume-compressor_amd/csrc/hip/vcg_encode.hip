@@ -923,6 +923,11 @@ struct enc_scratch {
     uint32_t *aux = nullptr; size_t aux_cap = 0;      // rate-control candidates: eobs, counters
     uint8_t *desc = nullptr; size_t desc_cap = 0;     // per-call descriptors (sources, groups, addresses)
     uint8_t *img = nullptr; size_t img_cap = 0;       // archive image
+    // the coarse atoms' launch on its own stream (vcg_enc_all)
+    int16_t *clv = nullptr; size_t clv_cap = 0;
+    enc_item *citems = nullptr; size_t citems_cap = 0;
+    uint8_t *cpay = nullptr; size_t cpay_cap = 0;
+    uint32_t *clens = nullptr; size_t clens_cap = 0;
     uint8_t *hstage[2] = {nullptr, nullptr};          // pinned halves of one allocation
     hipEvent_t hev[2] = {nullptr, nullptr};
 };
@@ -1068,7 +1073,7 @@ static thread_local enc_scratch tl_es;
 
 static void scratch_release() {
     enc_scratch &S = tl_es;
-    void *bufs[] = {S.lv, S.items, S.pay, S.lens, S.aux, S.desc, S.img};
+    void *bufs[] = {S.lv, S.items, S.pay, S.lens, S.aux, S.desc, S.img, S.clv, S.citems, S.cpay, S.clens};
     for (void *b : bufs) if (b) VCG_CHECK(hipFree(b));
     if (S.hstage[0]) VCG_CHECK(hipHostFree(S.hstage[0]));
     for (int k = 0; k < 2; ++k) if (S.hev[k]) VCG_CHECK(hipEventDestroy(S.hev[k]));
@@ -1258,6 +1263,19 @@ static void pool_reserve(vcg_ctx *c, vcg_enc *e, uint64_t need) {
 // launches of up to ENC_ROWS atoms taken across volumes, coarsest LOD first.
 // lens[v][i]: payload length of encoder v's atom i (0 = all-zero atom);
 // ovf[v] = 1 if one of its payloads exceeded the coder bound.
+// The coarse atoms (LOD >= ENC_COARSE_LOD, first in the flat order) carry
+// the most bins, and the launch that holds them runs as long as the heaviest
+// one.  When the atoms need more than one launch and the coarse ones number
+// at most ENC_COARSE_MAX, they go in a launch of their own on a second stream
+// (with their own scratch), beside the launches of the rest.
+#define ENC_COARSE_LOD 2
+#define ENC_COARSE_MAX 16384u
+struct enc_side {
+    hipStream_t s = nullptr;
+    hipEvent_t up = nullptr, done = nullptr;
+};
+static thread_local enc_side tl_side;          // kept until the thread ends
+
 extern "C" int vcg_enc_all(int nv, vcg_enc *const *es, const float (*qs)[VCG_NLOD], uint32_t *const *lens,
                            int *ovf) {
     vcg_ctx *c = vcg_get_ctx();
@@ -1273,28 +1291,53 @@ extern "C" int vcg_enc_all(int nv, vcg_enc *const *es, const float (*qs)[VCG_NLO
         ovf[v] = 0;
     }
     seg.push_back(enc_seg{tot, 0, 0, 0});           // sentinel
-    const uint32_t maxb = tot < ENC_ROWS ? (uint32_t)tot : ENC_ROWS;
+    uint64_t nc = 0;                                // coarse atoms: flat [0, nc)
+    for (int l = VCG_NLOD - 1; l >= ENC_COARSE_LOD; --l)
+        for (int v = 0; v < nv; ++v)
+            if (l < es[v]->src.nlod) nc += es[v]->src.prefix[l + 1] - es[v]->src.prefix[l];
+    const bool side = tot > ENC_ROWS && nc > 0 && nc <= ENC_COARSE_MAX && nc < tot;
+    const uint64_t b0 = side ? nc : 0;              // the main stream's atoms: flat [b0, tot)
+    const uint32_t maxb = tot - b0 < ENC_ROWS ? (uint32_t)(tot - b0) : ENC_ROWS;
     vcg_grow((void **)&S.lv, &S.lv_cap, maxb, (size_t)VCG_A3 * 2);
     vcg_grow((void **)&S.items, &S.items_cap, maxb, sizeof(enc_item));
     vcg_grow((void **)&S.pay, &S.pay_cap, ((size_t)maxb + 1) * ENC_SLOT, 1);   // + k_rc_encode's spare slot
     vcg_grow((void **)&S.lens, &S.lens_cap, maxb, 4);
-    vcg_grow((void **)&S.aux, &S.aux_cap, (size_t)maxb * 2, 4);
+    const uint32_t maxc = maxb > nc ? maxb : (uint32_t)nc;
+    vcg_grow((void **)&S.aux, &S.aux_cap, (size_t)maxc * 2, 4);
     uint64_t *ddst = (uint64_t *)S.aux;       // destination address of each payload
-    std::vector<uint32_t> hl(maxb);
-    std::vector<uint64_t> dst(maxb), add(nv);
-    std::vector<uint32_t> vk(maxb);           // volume of each atom of the launch
-    std::vector<uint64_t> ik(maxb);           // its index in that volume
-    size_t sg = 0;
-    for (uint64_t b = 0; b < tot; b += maxb) {
-        const uint32_t n = (uint32_t)(tot - b < maxb ? tot - b : maxb);
-        hipLaunchKernelGGL(k_fdct, dim3(n), dim3(FDCT_TX), LDS_BYTES, c->s, dsrc, dseg,
-                           (uint32_t)(seg.size() - 1), b, (int)FDCT_QUANT, g_dtab, S.lv, S.items);
+    std::vector<uint32_t> hl(maxc);
+    std::vector<uint64_t> dst(maxc), add(nv);
+    std::vector<uint32_t> vk(maxc);           // volume of each atom of the launch
+    std::vector<uint64_t> ik(maxc);           // its index in that volume
+    enc_side &E = tl_side;
+    if (side) {
+        vcg_grow((void **)&S.clv, &S.clv_cap, nc, (size_t)VCG_A3 * 2);
+        vcg_grow((void **)&S.citems, &S.citems_cap, nc, sizeof(enc_item));
+        vcg_grow((void **)&S.cpay, &S.cpay_cap, ((size_t)nc + 1) * ENC_SLOT, 1);
+        vcg_grow((void **)&S.clens, &S.clens_cap, nc, 4);
+        if (!E.s) {
+            VCG_CHECK(hipStreamCreate(&E.s));
+            VCG_CHECK(hipEventCreateWithFlags(&E.up, hipEventDisableTiming));
+            VCG_CHECK(hipEventCreateWithFlags(&E.done, hipEventDisableTiming));
+        }
+        // after the descriptor upload (and the scratch it may have waited on)
+        VCG_CHECK(hipEventRecord(E.up, c->s));
+        VCG_CHECK(hipStreamWaitEvent(E.s, E.up, 0));
+        hipLaunchKernelGGL(k_fdct, dim3((uint32_t)nc), dim3(FDCT_TX), LDS_BYTES, E.s, dsrc, dseg,
+                           (uint32_t)(seg.size() - 1), (uint64_t)0, (int)FDCT_QUANT, g_dtab, S.clv, S.citems);
         VCG_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_rc_encode, dim3((n + 63) / 64), dim3(64), 0, c->s, n, S.lv, S.items, g_dtab, S.pay,
-                           (uint32_t)ENC_SLOT, S.lens);
+        hipLaunchKernelGGL(k_rc_encode, dim3((uint32_t)((nc + 63) / 64)), dim3(64), 0, E.s, (uint32_t)nc, S.clv,
+                           S.citems, g_dtab, S.cpay, (uint32_t)ENC_SLOT, S.clens);
         VCG_CHECK(hipGetLastError());
-        VCG_CHECK(hipMemcpyAsync(hl.data(), S.lens, (size_t)n * 4, hipMemcpyDeviceToHost, c->s));
+        VCG_CHECK(hipEventRecord(E.done, E.s));
+    }
+    // payload lengths of flat atoms [b, b + n) (at dlens, payloads at dpay) to
+    // the volumes' pools
+    auto place = [&](uint64_t b, uint32_t n, const uint32_t *dlens, const uint8_t *dpay) {
+        VCG_CHECK(hipMemcpyAsync(hl.data(), dlens, (size_t)n * 4, hipMemcpyDeviceToHost, c->s));
         VCG_CHECK(hipStreamSynchronize(c->s));
+        size_t sg = 0;
+        while (b >= seg[sg + 1].base) ++sg;
         for (uint32_t k = 0; k < n; ++k) {
             while (b + k >= seg[sg + 1].base) ++sg;
             const uint32_t v = seg[sg].v;
@@ -1311,11 +1354,25 @@ extern "C" int vcg_enc_all(int nv, vcg_enc *const *es, const float (*qs)[VCG_NLO
         for (uint32_t k = 0; k < n; ++k)
             dst[k] = (uint64_t)(uintptr_t)(es[vk[k]]->pool + es[vk[k]]->pool_off[ik[k]]);
         VCG_CHECK(hipMemcpyAsync(ddst, dst.data(), (size_t)n * 8, hipMemcpyHostToDevice, c->s));
-        hipLaunchKernelGGL(k_compact, dim3(n), dim3(256), 0, c->s, S.pay, (uint32_t)ENC_SLOT, S.lens,
+        hipLaunchKernelGGL(k_compact, dim3(n), dim3(256), 0, c->s, dpay, (uint32_t)ENC_SLOT, dlens,
                            (const uint64_t *)ddst);
         VCG_CHECK(hipGetLastError());
         for (int v = 0; v < nv; ++v) { es[v]->pool_len += add[v]; add[v] = 0; }
         VCG_CHECK(hipStreamSynchronize(c->s));
+    };
+    for (uint64_t b = b0; b < tot; b += maxb) {
+        const uint32_t n = (uint32_t)(tot - b < maxb ? tot - b : maxb);
+        hipLaunchKernelGGL(k_fdct, dim3(n), dim3(FDCT_TX), LDS_BYTES, c->s, dsrc, dseg,
+                           (uint32_t)(seg.size() - 1), b, (int)FDCT_QUANT, g_dtab, S.lv, S.items);
+        VCG_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_rc_encode, dim3((n + 63) / 64), dim3(64), 0, c->s, n, S.lv, S.items, g_dtab, S.pay,
+                           (uint32_t)ENC_SLOT, S.lens);
+        VCG_CHECK(hipGetLastError());
+        place(b, n, S.lens, S.pay);
+    }
+    if (side) {
+        VCG_CHECK(hipStreamWaitEvent(c->s, E.done, 0));
+        place(0, (uint32_t)nc, S.clens, S.cpay);
     }
     return 0;
 }
