@@ -1265,11 +1265,12 @@ static void pool_reserve(vcg_ctx *c, vcg_enc *e, uint64_t need) {
 // ovf[v] = 1 if one of its payloads exceeded the coder bound.
 // The coarse atoms (LOD >= ENC_COARSE_LOD, first in the flat order) carry
 // the most bins, and the launch that holds them runs as long as the heaviest
-// one.  When the atoms need more than one launch and the coarse ones number
+// one.  When there are at least ENC_SIDE_MIN atoms and the coarse ones number
 // at most ENC_COARSE_MAX, they go in a launch of their own on a second stream
-// (with their own scratch), beside the launches of the rest.
+// (with their own scratch), beside the transform and launches of the rest.
 #define ENC_COARSE_LOD 2
 #define ENC_COARSE_MAX 16384u
+#define ENC_SIDE_MIN 8192u
 struct enc_side {
     hipStream_t s = nullptr;
     hipEvent_t up = nullptr, done = nullptr;
@@ -1295,7 +1296,7 @@ extern "C" int vcg_enc_all(int nv, vcg_enc *const *es, const float (*qs)[VCG_NLO
     for (int l = VCG_NLOD - 1; l >= ENC_COARSE_LOD; --l)
         for (int v = 0; v < nv; ++v)
             if (l < es[v]->src.nlod) nc += es[v]->src.prefix[l + 1] - es[v]->src.prefix[l];
-    const bool side = tot > ENC_ROWS && nc > 0 && nc <= ENC_COARSE_MAX && nc < tot;
+    const bool side = tot >= ENC_SIDE_MIN && nc > 0 && nc <= ENC_COARSE_MAX && nc < tot;
     const uint64_t b0 = side ? nc : 0;              // the main stream's atoms: flat [b0, tot)
     const uint32_t maxb = tot - b0 < ENC_ROWS ? (uint32_t)(tot - b0) : ENC_ROWS;
     vcg_grow((void **)&S.lv, &S.lv_cap, maxb, (size_t)VCG_A3 * 2);
