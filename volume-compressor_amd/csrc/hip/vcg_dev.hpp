@@ -66,6 +66,9 @@ struct vcg_ctx {
     uint32_t *status_d = nullptr;
     uint32_t *status_h = nullptr;  // pinned
     void     *ev[3] = {nullptr, nullptr, nullptr};
+    // encode: the coarse atoms' stream, its start (descriptor upload) and end events
+    hipStream_t sC = nullptr;
+    hipEvent_t evCup = nullptr, evCdone = nullptr;
 };
 vcg_ctx *vcg_get_ctx();
 void vcg_grow(void **p, size_t *cap, size_t need, size_t elem);

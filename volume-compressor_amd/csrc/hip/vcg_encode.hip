@@ -1271,11 +1271,6 @@ static void pool_reserve(vcg_ctx *c, vcg_enc *e, uint64_t need) {
 #define ENC_COARSE_LOD 2
 #define ENC_COARSE_MAX 16384u
 #define ENC_SIDE_MIN 8192u
-struct enc_side {
-    hipStream_t s = nullptr;
-    hipEvent_t up = nullptr, done = nullptr;
-};
-static thread_local enc_side tl_side;          // kept until the thread ends
 
 extern "C" int vcg_enc_all(int nv, vcg_enc *const *es, const float (*qs)[VCG_NLOD], uint32_t *const *lens,
                            int *ovf) {
@@ -1310,27 +1305,26 @@ extern "C" int vcg_enc_all(int nv, vcg_enc *const *es, const float (*qs)[VCG_NLO
     std::vector<uint64_t> dst(maxc), add(nv);
     std::vector<uint32_t> vk(maxc);           // volume of each atom of the launch
     std::vector<uint64_t> ik(maxc);           // its index in that volume
-    enc_side &E = tl_side;
     if (side) {
         vcg_grow((void **)&S.clv, &S.clv_cap, nc, (size_t)VCG_A3 * 2);
         vcg_grow((void **)&S.citems, &S.citems_cap, nc, sizeof(enc_item));
         vcg_grow((void **)&S.cpay, &S.cpay_cap, ((size_t)nc + 1) * ENC_SLOT, 1);
         vcg_grow((void **)&S.clens, &S.clens_cap, nc, 4);
-        if (!E.s) {
-            VCG_CHECK(hipStreamCreate(&E.s));
-            VCG_CHECK(hipEventCreateWithFlags(&E.up, hipEventDisableTiming));
-            VCG_CHECK(hipEventCreateWithFlags(&E.done, hipEventDisableTiming));
+        if (!c->sC) {                               // kept with the thread's context
+            VCG_CHECK(hipStreamCreateWithFlags(&c->sC, hipStreamNonBlocking));
+            VCG_CHECK(hipEventCreateWithFlags(&c->evCup, hipEventDisableTiming));
+            VCG_CHECK(hipEventCreateWithFlags(&c->evCdone, hipEventDisableTiming));
         }
         // after the descriptor upload (and the scratch it may have waited on)
-        VCG_CHECK(hipEventRecord(E.up, c->s));
-        VCG_CHECK(hipStreamWaitEvent(E.s, E.up, 0));
-        hipLaunchKernelGGL(k_fdct, dim3((uint32_t)nc), dim3(FDCT_TX), LDS_BYTES, E.s, dsrc, dseg,
+        VCG_CHECK(hipEventRecord(c->evCup, c->s));
+        VCG_CHECK(hipStreamWaitEvent(c->sC, c->evCup, 0));
+        hipLaunchKernelGGL(k_fdct, dim3((uint32_t)nc), dim3(FDCT_TX), LDS_BYTES, c->sC, dsrc, dseg,
                            (uint32_t)(seg.size() - 1), (uint64_t)0, (int)FDCT_QUANT, g_dtab, S.clv, S.citems);
         VCG_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_rc_encode, dim3((uint32_t)((nc + 63) / 64)), dim3(64), 0, E.s, (uint32_t)nc, S.clv,
+        hipLaunchKernelGGL(k_rc_encode, dim3((uint32_t)((nc + 63) / 64)), dim3(64), 0, c->sC, (uint32_t)nc, S.clv,
                            S.citems, g_dtab, S.cpay, (uint32_t)ENC_SLOT, S.clens);
         VCG_CHECK(hipGetLastError());
-        VCG_CHECK(hipEventRecord(E.done, E.s));
+        VCG_CHECK(hipEventRecord(c->evCdone, c->sC));
     }
     // payload lengths of flat atoms [b, b + n) (at dlens, payloads at dpay) to
     // the volumes' pools
@@ -1372,7 +1366,7 @@ extern "C" int vcg_enc_all(int nv, vcg_enc *const *es, const float (*qs)[VCG_NLO
         place(b, n, S.lens, S.pay);
     }
     if (side) {
-        VCG_CHECK(hipStreamWaitEvent(c->s, E.done, 0));
+        VCG_CHECK(hipStreamWaitEvent(c->s, c->evCdone, 0));
         place(0, (uint32_t)nc, S.clens, S.cpay);
     }
     return 0;

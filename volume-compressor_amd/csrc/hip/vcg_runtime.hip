@@ -105,6 +105,7 @@ vcg_ctx *vcg_get_ctx() {
         // launches report per item and never collect the word), and the
         // owner's stage events (which it may have destroyed) are dropped
         for (int k = 0; k < VCG_SLOTS; ++k) VCG_CHECK(hipStreamSynchronize(r->sE[k]));
+        if (r->sC) VCG_CHECK(hipStreamSynchronize(r->sC));
         for (int k = 0; k < 3; ++k) r->ev[k] = nullptr;
         hipLaunchKernelGGL(k_reset_word, dim3(1), dim3(1), 0, r->s, r->status_d);
         VCG_CHECK(hipGetLastError());
