@@ -931,6 +931,9 @@ def main():
     assert L.vcx_sync() == 0
 
     # ---- bit-exactness: sampled atoms vs the CPU oracle ----
+    if args.warmup == 0:                      # the check reads a decode's output
+        assert a.decode_lod_device(0, dout, 0) == 0
+        assert L.vcx_sync() == 0
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from pyoracle import Oracle
     o = Oracle()
