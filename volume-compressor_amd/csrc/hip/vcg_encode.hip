@@ -669,6 +669,35 @@ __global__ void k_assemble(const uint8_t *__restrict__ pool, const uint64_t *__r
     }
 }
 
+// k_assemble for many archives in one image: atom i's payload (len[i] bytes
+// at device address src[i]) to img + dst[i], its 16-byte slot (file offset
+// poff[i], or the zero-atom flag) to img + sdst[i] unless that is ~0
+__global__ void k_assemble_many(const uint64_t *__restrict__ src, const uint32_t *__restrict__ len_,
+                                const uint64_t *__restrict__ dst, const uint64_t *__restrict__ sdst,
+                                const uint64_t *__restrict__ poff, uint8_t *__restrict__ img) {
+    const uint64_t i = blockIdx.x;
+    const uint32_t len = len_[i];
+    const uint64_t so = sdst[i];
+    const uint8_t *s0 = (const uint8_t *)(uintptr_t)src[i];
+    if (threadIdx.x == 0 && so != ~0ull) {
+        uint8_t s[16];
+        for (int b = 0; b < 16; ++b) s[b] = 0;
+        if (len) {
+            const uint64_t po = poff[i];
+            for (int b = 0; b < 8; ++b) s[b] = (uint8_t)(po >> (8 * b));
+            for (int b = 0; b < 4; ++b) s[8 + b] = (uint8_t)(len >> (8 * b));
+            s[12] = 1; s[13] = s0[0];
+        } else {
+            s[12] = 2;
+        }
+        for (int b = 0; b < 16; ++b) img[so + b] = s[b];
+    }
+    if (len) {
+        uint8_t *d = img + dst[i];
+        for (uint32_t b = threadIdx.x; b < len; b += blockDim.x) d[b] = s0[b];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -1419,6 +1448,80 @@ extern "C" int vcg_enc_assemble(vcg_enc *e, const uint8_t *head, size_t head_len
         VCG_CHECK(hipEventSynchronize(S.hev[k & 1]));
         const uint64_t off = k * HSTAGE_BYTES, n = total_len - off < HSTAGE_BYTES ? total_len - off : HSTAGE_BYTES;
         par_copy(out + off, S.hstage[k & 1], n);
+    }
+    return 0;
+}
+
+extern "C" int vcg_enc_assemble_many(int n, const vcg_asm_job *J) {
+    vcg_ctx *c = vcg_get_ctx();
+    enc_scratch &S = tl_es;
+    if (n <= 0) return 0;
+    // the image: archive k at base[k] (256-byte aligned); the header of each
+    // is written on the host, the rest comes back from the image
+    std::vector<uint64_t> base(n + 1);
+    uint64_t A = 0;
+    for (int k = 0; k < n; ++k) {
+        base[k + 1] = base[k] + ((J[k].total_len + 255) & ~255ull);
+        A += J[k].total_atoms;
+    }
+    const uint64_t isz = base[n];
+    // per-atom arrays in one upload: src, dst, sdst, poff (u64), len (u32)
+    std::vector<uint8_t> h((size_t)A * 36);
+    uint64_t *hs = (uint64_t *)h.data(), *hd = hs + A, *hsd = hd + A, *hp = hsd + A;
+    uint32_t *hl = (uint32_t *)(hp + A);
+    for (int k = 0, a = 0; k < n; ++k) {
+        const vcg_asm_job &j = J[k];
+        for (uint64_t i = 0; i < j.total_atoms; ++i, ++a) {
+            hs[a] = (uint64_t)(uintptr_t)(j.e->pool + j.e->pool_off[i]);
+            hd[a] = base[k] + j.pay_off[i];
+            hsd[a] = j.slot_off[i] ? base[k] + j.slot_off[i] : ~0ull;
+            hp[a] = j.pay_off[i];
+            hl[a] = j.lens[i];
+        }
+    }
+    uint8_t *d = desc_area(h.size());
+    VCG_CHECK(hipMemcpyAsync(d, h.data(), h.size(), hipMemcpyHostToDevice, c->s));
+    vcg_grow((void **)&S.img, &S.img_cap, isz, 1);
+    uint8_t *img = S.img;
+    hipLaunchKernelGGL(k_reset_words, dim3(4096), dim3(256), 0, c->s, (uint32_t *)img, isz / 4);
+    const uint64_t *ds = (const uint64_t *)d;
+    for (uint64_t b = 0; b < A; b += 1u << 20) {
+        const uint64_t m = A - b < (1u << 20) ? A - b : (1u << 20);
+        hipLaunchKernelGGL(k_assemble_many, dim3((uint32_t)m), dim3(256), 0, c->s, ds + b, (const uint32_t *)(ds + 4 * A) + b,
+                           ds + A + b, ds + 2 * A + b, ds + 3 * A + b, img);
+        VCG_CHECK(hipGetLastError());
+    }
+    for (int k = 0; k < n; ++k) memcpy(J[k].out, J[k].head, J[k].head_len);
+    // back through two pinned halves: the copies out of one half overlap the
+    // transfer into the other
+    if (!S.hstage[0]) {
+        if (hipHostMalloc((void **)&S.hstage[0], 2 * (size_t)HSTAGE_BYTES, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            vc__panic("out of pinned host memory (encode)");
+        }
+        S.hstage[1] = S.hstage[0] + HSTAGE_BYTES;
+        for (int k = 0; k < 2; ++k)
+            if (!S.hev[k]) VCG_CHECK(hipEventCreateWithFlags(&S.hev[k], hipEventDisableTiming));
+    }
+    const uint64_t nch = (isz + HSTAGE_BYTES - 1) / HSTAGE_BYTES;
+    auto chunk = [&](uint64_t q) {
+        const uint64_t off = q * HSTAGE_BYTES, len = isz - off < HSTAGE_BYTES ? isz - off : HSTAGE_BYTES;
+        VCG_CHECK(hipMemcpyAsync(S.hstage[q & 1], img + off, len, hipMemcpyDeviceToHost, c->s));
+        VCG_CHECK(hipEventRecord(S.hev[q & 1], c->s));
+    };
+    chunk(0);
+    int k0 = 0;                                       // first archive not yet done
+    for (uint64_t q = 0; q < nch; ++q) {
+        if (q + 1 < nch) chunk(q + 1);
+        VCG_CHECK(hipEventSynchronize(S.hev[q & 1]));
+        const uint64_t c0 = q * HSTAGE_BYTES, c1 = c0 + (isz - c0 < HSTAGE_BYTES ? isz - c0 : HSTAGE_BYTES);
+        for (int k = k0; k < n && base[k] < c1; ++k) {
+            // archive k's bytes past its header, within this chunk
+            const uint64_t a0 = base[k] + J[k].head_len, a1 = base[k] + J[k].total_len;
+            const uint64_t lo = a0 > c0 ? a0 : c0, hi = a1 < c1 ? a1 : c1;
+            if (lo < hi) par_copy(J[k].out + (lo - base[k]), S.hstage[q & 1] + (lo - c0), (size_t)(hi - lo));
+            if (a1 <= c1) k0 = k + 1;
+        }
     }
     return 0;
 }

@@ -179,11 +179,21 @@ static void lod_schedule(float q0, int nlod, float qs[VC_NLOD]) {
 }
 
 /* ---- layout + assembly ---- */
+/* An archive laid out on the host, for vcg_enc_assemble_many to fill: its
+ * header and directory, each atom's payload and slot offsets, its length and
+ * its host buffer. */
+typedef struct {
+    uint8_t *head, *img;
+    size_t head_len;
+    uint64_t *pay_off, *slot_off;
+    uint64_t len;
+} asm_plan;
+
 /* pre: a buffer of pre_len bytes allocated for the archive ahead of time
- * (NULL if none); assemble owns it: the archive, or freed */
-static vc_status assemble(vcg_enc *e, vc_dims d0, int nlod, float target, const float qs[VC_NLOD],
-                          const uint32_t *lens, uint64_t total, uint8_t *pre, size_t pre_len,
-                          uint8_t **out, size_t *out_len) {
+ * (NULL if none); layout owns it: the plan's buffer, or freed.  On VC_OK the
+ * caller assembles the plan and then releases it with plan_done. */
+static vc_status layout(vc_dims d0, int nlod, float target, const float qs[VC_NLOD],
+                        const uint32_t *lens, uint64_t total, uint8_t *pre, size_t pre_len, asm_plan *P) {
     uint64_t prefix[VC_NLOD + 1], apre[VC_NLOD + 1], nreg = 0, na = 0;
     uint32_t acx[VC_NLOD], acy[VC_NLOD], acz[VC_NLOD], nrx[VC_NLOD], nry[VC_NLOD], nrz[VC_NLOD];
     for (int l = 0; l < nlod; ++l) {
@@ -219,6 +229,7 @@ static vc_status assemble(vcg_enc *e, vc_dims d0, int nlod, float target, const 
             boff[ri] = cursor; cursor += blen[ri];
         }
     vc_status rc = VC_OK;
+    memset(P, 0, sizeof *P);
     if (too_big) { free(pre); rc = VC_ERR_RANGE; goto done; }
     /* per-atom slot and payload offsets: payloads follow the slot array in slot order */
     for (int l = 0; l < nlod; ++l)
@@ -251,17 +262,25 @@ static vc_status assemble(vcg_enc *e, vc_dims d0, int nlod, float target, const 
         uint8_t *img = pre && pre_len >= cursor ? pre : NULL;
         if (!img) { free(pre); pre = NULL; img = (uint8_t *)malloc(cursor); }
         if (!img) { free(head); rc = VC_ERR_OOM; goto done; }
-        vcg_enc_assemble(e, head, head_len, pay_off, slot_off, lens, total, img, cursor);
-        free(head);
-        if (img == pre && pre_len > cursor) {           /* give back the estimate's excess */
-            uint8_t *t = (uint8_t *)realloc(img, cursor);
-            if (t) img = t;
-        }
-        *out = img; *out_len = (size_t)cursor;
+        P->head = head; P->head_len = head_len; P->img = img; P->len = cursor;
+        P->pay_off = pay_off; P->slot_off = slot_off;
+        pay_off = slot_off = NULL;                     /* the plan's now */
     }
 done:
     free(cls); free(blen); free(boff); free(pay_off); free(slot_off);
     return rc;
+}
+
+/* after the assembly: the archive (trimmed to its length) to *out */
+static void plan_done(asm_plan *P, size_t pre_len, uint8_t **out, size_t *out_len) {
+    uint8_t *img = P->img;
+    if (pre_len > P->len) {                             /* give back the estimate's excess */
+        uint8_t *t = (uint8_t *)realloc(img, P->len);
+        if (t) img = t;
+    }
+    *out = img; *out_len = (size_t)P->len;
+    free(P->head); free(P->pay_off); free(P->slot_off);
+    memset(P, 0, sizeof *P);
 }
 
 /* Bisection levels per launch for a group: the most whose rounds (2^depth - 1
@@ -359,12 +378,23 @@ static void encode_group(int nv, const uint8_t *const *vols, int on_dev, const v
         }
         vcg_enc_all(nv, es, (const float (*)[VC_NLOD])qs, lens, ovf);
         vcg_host_prefault_wait();
+        /* every archive laid out, then all assembled in one device pass */
+        asm_plan *P = (asm_plan *)calloc((size_t)nv, sizeof *P);
+        vcg_asm_job *J = (vcg_asm_job *)calloc((size_t)nv, sizeof *J);
+        if (!P || !J) vc__panic("out of memory");
+        int nj = 0;
         for (int v = 0; v < nv; ++v) {
             outs[v] = NULL; out_lens[v] = 0;
-            if (ovf[v]) { free(pre[v]); st[v] = VC_ERR_RANGE; }   /* a payload exceeded the coder bound */
-            else st[v] = assemble(es[v], dims[v], nlod[v], targets[v], qs[v], lens[v], total[v], pre[v], pre_len[v],
-                                  &outs[v], &out_lens[v]);
+            if (ovf[v]) { free(pre[v]); st[v] = VC_ERR_RANGE; continue; }   /* a payload exceeded the coder bound */
+            st[v] = layout(dims[v], nlod[v], targets[v], qs[v], lens[v], total[v], pre[v], pre_len[v], &P[v]);
+            if (st[v] != VC_OK) continue;
+            J[nj++] = (vcg_asm_job){es[v], P[v].head, P[v].head_len, P[v].pay_off, P[v].slot_off, lens[v],
+                                    total[v], P[v].img, P[v].len};
         }
+        if (nj) vcg_enc_assemble_many(nj, J);
+        for (int v = 0; v < nv; ++v)
+            if (st[v] == VC_OK) plan_done(&P[v], P[v].img == pre[v] ? pre_len[v] : 0, &outs[v], &out_lens[v]);
+        free(P); free(J);
     }
     for (int v = 0; v < nv; ++v) { vcg_enc_end(es[v]); free(lens[v]); }
     free(r); free(es); free(qs); free(lens); free(nlod); free(ovf); free(total); free(pre); free(pre_len);

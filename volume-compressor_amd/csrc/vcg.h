@@ -190,6 +190,20 @@ int  vcg_enc_all(int nv, vcg_enc *const *es, const float (*qs)[VCG_NLOD], uint32
 int  vcg_enc_assemble(vcg_enc *e, const uint8_t *head, size_t head_len, const uint64_t *pay_off,
                       const uint64_t *slot_off, const uint32_t *lens, uint64_t total_atoms,
                       uint8_t *out, uint64_t total_len);
+/* vcg_enc_assemble for many archives at once: one device image for all, one
+ * upload of the per-atom offsets, one launch, and one pass of transfers back
+ * into the archives' host buffers (out, total_len bytes each). */
+typedef struct {
+    vcg_enc *e;
+    const uint8_t *head;
+    size_t head_len;
+    const uint64_t *pay_off, *slot_off;
+    const uint32_t *lens;
+    uint64_t total_atoms;
+    uint8_t *out;
+    uint64_t total_len;
+} vcg_asm_job;
+int  vcg_enc_assemble_many(int n, const vcg_asm_job *jobs);
 void vcg_enc_end(vcg_enc *e);
 /* Touch the pages of n host buffers (bufs[i], lens[i] bytes; NULL entries
  * skipped) on the library's host threads in the background, so that their
