@@ -896,13 +896,6 @@ extern "C" void vcg_enc_begin_many(int nv, const uint8_t *const *vols, int on_de
     }
 }
 
-extern "C" vcg_enc *vcg_enc_begin(const uint8_t *vol, int on_dev, const uint32_t dims[3], int nlod) {
-    vcg_enc *e = nullptr;
-    const uint32_t d[1][3] = {{dims[0], dims[1], dims[2]}};
-    vcg_enc_begin_many(1, &vol, on_dev, d, &nlod, &e);
-    return e;
-}
-
 // An encoder over level volumes the caller already built (device memory,
 // dims[l] = nx, ny, nz of level l): atom numbering and gathers as for a
 // pyramid, nothing computed or freed here.

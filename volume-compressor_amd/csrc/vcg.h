@@ -149,10 +149,9 @@ typedef struct vcg_enc vcg_enc;
  * vcg_enc_end. */
 void vcg_enc_arena_open(void);
 void vcg_enc_arena_close(void);
-/* Upload (or adopt) a LOD0 volume and build the nlod-level pyramid on device. */
-vcg_enc *vcg_enc_begin(const uint8_t *vol, int vol_on_device, const uint32_t dims[3], int nlod);
-/* vcg_enc_begin for nv volumes (volume v: dims[v], nlods[v] levels) into
- * out[v], their pyramids built one launch per level across the volumes. */
+/* Encoders of nv LOD0 volumes (volume v: dims[v], nlods[v] levels) into
+ * out[v]: each volume uploaded (or adopted) and its pyramid built on device,
+ * one launch per level across the volumes. */
 void vcg_enc_begin_many(int nv, const uint8_t *const *vols, int vol_on_device, const uint32_t (*dims)[3],
                         const int *nlods, vcg_enc **out);
 /* An encoder over nlod level volumes already on the device (dims[l] = nx,
