@@ -1175,31 +1175,58 @@ extern "C" int vcg_enc_samples(int nv, vcg_enc *const *es, const uint32_t *const
                                uint8_t *const *zero_masks) {
     vcg_ctx *c = vcg_get_ctx();
     if (nv <= 0) return 0;
-    for (int v = 0; v < nv; ++v) {
+    // encoders in an arena (a batch) share one list block and one item
+    // block, so that the lists go up and the items come back in one copy
+    // each; an arena is released whole, so the shared blocks need no owner
+    bool shared = nv > 1;
+    for (int v = 0; v < nv; ++v) shared = shared && es[v]->arena;
+    uint64_t tot = 0;
+    for (int v = 0; v < nv; ++v) tot += (uint64_t)(ns[v] > 0 ? ns[v] : 0);
+    uint32_t *lblk = nullptr;
+    enc_item *iblk = nullptr;
+    std::vector<uint32_t> hl;
+    if (shared && tot) {
+        lblk = (uint32_t *)enc_alloc(true, (size_t)tot * 12);
+        iblk = (enc_item *)enc_alloc(true, (size_t)tot * sizeof(enc_item));
+        hl.resize((size_t)tot * 3);
+    }
+    for (int v = 0, k = 0; v < nv; ++v) {
         vcg_enc *e = es[v];
         if (e->sample_coef) vc__panic("encode sample prepared twice");
         const int n = ns[v] > 0 ? ns[v] : 0;
         e->sample_n = n;
-        e->sample_list = (uint32_t *)enc_alloc(e->arena, (size_t)n * 12);
         e->sample_coef = (int32_t *)enc_alloc(e->arena, (size_t)n * VCG_A3 * 4);
-        e->sample_items = (enc_item *)enc_alloc(e->arena, (size_t)n * sizeof(enc_item));
-        if (n) VCG_CHECK(hipMemcpyAsync(e->sample_list, lists[v], (size_t)n * 12, hipMemcpyHostToDevice, c->s));
+        if (lblk) {
+            e->sample_list = lblk + (size_t)k * 3;
+            e->sample_items = iblk + k;
+            if (n) memcpy(hl.data() + (size_t)k * 3, lists[v], (size_t)n * 12);
+        } else {
+            e->sample_list = (uint32_t *)enc_alloc(e->arena, (size_t)n * 12);
+            e->sample_items = (enc_item *)enc_alloc(e->arena, (size_t)n * sizeof(enc_item));
+            if (n) VCG_CHECK(hipMemcpyAsync(e->sample_list, lists[v], (size_t)n * 12, hipMemcpyHostToDevice, c->s));
+        }
+        k += n;
     }
+    if (lblk) VCG_CHECK(hipMemcpyAsync(lblk, hl.data(), (size_t)tot * 12, hipMemcpyHostToDevice, c->s));
     std::vector<uint8_t> host;
     std::vector<enc_seg> seg;
     enc_src *dsrc; enc_seg *dseg;
-    const uint64_t tot = upload_sources(c, nv, es, nullptr, true, host, seg, &dsrc, &dseg);
-    if (tot) {
-        hipLaunchKernelGGL(k_fdct, dim3((uint32_t)tot), dim3(FDCT_TX), LDS_BYTES, c->s, dsrc, dseg,
+    const uint64_t ns_tot = upload_sources(c, nv, es, nullptr, true, host, seg, &dsrc, &dseg);
+    if (ns_tot) {
+        hipLaunchKernelGGL(k_fdct, dim3((uint32_t)ns_tot), dim3(FDCT_TX), LDS_BYTES, c->s, dsrc, dseg,
                            (uint32_t)seg.size(), (uint64_t)0, (int)FDCT_STORE_COEF, g_dtab,
                            (int16_t *)nullptr, (enc_item *)nullptr);
         VCG_CHECK(hipGetLastError());
     }
-    std::vector<enc_item> its(tot);
-    for (int v = 0, k = 0; v < nv; k += es[v]->sample_n, ++v)
-        if (es[v]->sample_n)
-            VCG_CHECK(hipMemcpyAsync(its.data() + k, es[v]->sample_items, (size_t)es[v]->sample_n * sizeof(enc_item),
-                                     hipMemcpyDeviceToHost, c->s));
+    std::vector<enc_item> its(ns_tot);
+    if (iblk) {
+        if (ns_tot) VCG_CHECK(hipMemcpyAsync(its.data(), iblk, (size_t)ns_tot * sizeof(enc_item), hipMemcpyDeviceToHost, c->s));
+    } else {
+        for (int v = 0, k = 0; v < nv; k += es[v]->sample_n, ++v)
+            if (es[v]->sample_n)
+                VCG_CHECK(hipMemcpyAsync(its.data() + k, es[v]->sample_items,
+                                         (size_t)es[v]->sample_n * sizeof(enc_item), hipMemcpyDeviceToHost, c->s));
+    }
     VCG_CHECK(hipStreamSynchronize(c->s));
     for (int v = 0, k = 0; v < nv; k += es[v]->sample_n, ++v)
         for (int i = 0; i < es[v]->sample_n; ++i) zero_masks[v][i] = (uint8_t)its[k + i].zero;
